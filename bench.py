@@ -1,0 +1,185 @@
+"""Benchmark of the coupled lattice-Boltzmann / swimmer / tracer step on one GPU.
+
+Workload (the 256^3 precision configuration): a 256^3 periodic D3Q15 MRT
+lattice, tau_s = tau_b = 0.25, rho = 1 + 1e-3 N(0,1), u = 1e-3 N(0,1);
+32768 swimmers (uniform positions, isotropic orientations, P = 1e-3,
+l = 0.1, alpha = 0.01) and 1e6 uniform tracers, seed 1234.  One "step" is one
+System.Step: ZeroForce, AddForces, collide+stream, CalcHydro, swimmer Move,
+tracer Move.  All state is resident in HBM before the timed region.
+
+`value` is MLUPS (lattice sites x steps / s / 1e6) of the FP64 drop-in form;
+the FP32-storage form is timed on the same inputs and reported beside it.
+Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "gpu-swimmers_amd"), os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+# algorithmic bytes per site of one collide+stream launch: 15 f read + 15 f
+# written + 3 force read, at the storage width of each form
+COLLIDE_BYTES = {"f64": 30 * 8 + 3 * 8, "f32": 30 * 4 + 3 * 4}
+# whole coupled step per site: Step + CalcHydro (15 f + 3 F read, rho + 3 u
+# written) + ZeroForce (3 F written)
+STEP_BYTES = {"f64": 464, "f32": 232}
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def make_workload(size, seed=1234):
+    from cases import make_case
+    return make_case((size, size, size), 0, 0.25, ns=32768, nt=1_000_000, alpha=0.01, seed=seed)
+
+
+def time_gpu(case, storage, steps, warmup):
+    from cases import build_lbswim
+    import lbswim
+
+    lat, system, sw, tr = build_lbswim(case, storage)
+    system.Step(warmup)
+    lbswim.synchronize()
+    lat.set_timing(True)
+    lbswim.synchronize()
+    t0 = time.perf_counter()
+    system.Step(steps)
+    lbswim.synchronize()
+    t1 = time.perf_counter()
+    times = lat.kernel_times()
+    lat.set_timing(False)
+    n = lat.addr.n
+    ms_step = (t1 - t0) * 1e3 / steps
+    mlups = n * steps / (t1 - t0) / 1e6
+    res = {"ms_per_step": ms_step, "mlups": mlups, "kernels": {}}
+    for name, (ms, cnt) in times.items():
+        if cnt:
+            res["kernels"][name] = {"avg_ms": ms / cnt, "launches": cnt}
+    coll = res["kernels"]["collide_stream"]["avg_ms"]
+    res["collide_gbs"] = COLLIDE_BYTES[storage] * n / (coll * 1e-3) / 1e9
+    res["step_gbs"] = STEP_BYTES[storage] * n / (ms_step * 1e-3) / 1e9
+    res["tumbles"] = sw.tumbles()
+    del lat, system, sw, tr
+    return res
+
+
+def cpu_baseline(case, steps=3):
+    """The FP64 oracle (OpenMP over the host cores) on the same workload."""
+    import lbswim_oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    cores = min(cores, os.cpu_count() or cores)
+    O.set_threads(cores)
+    lat = O.Lattice(*case.size, case.tau_s, case.tau_b)
+    lat.rho[:] = case.rho
+    lat.u[:] = case.u
+    lat.InitFromHydro()
+    sw = O.Swimmers(case.sw_r.shape[1], O.Common(**case.common))
+    sw.r[:] = case.sw_r
+    sw.n[:] = case.sw_n
+    tr = O.Tracers(case.tr_r.shape[1])
+    tr.r[:] = case.tr_r
+    O.run(lat, 1, sw, tr)  # warm-up
+    t0 = time.perf_counter()
+    O.run(lat, steps, sw, tr)
+    dt = time.perf_counter() - t0
+    return {"value": lat.n * steps / dt / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": "%d coupled steps (after 1 warm-up) of the same 256^3 workload, FP64 oracle, "
+                      "OpenMP over %d threads, %.1f s" % (steps, cores, dt)}
+
+
+def load_traffic(storage):
+    """HBM bytes per collide launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "collide_stream_%s_256cubed_pmc.json" % storage)
+    try:
+        with open(path) as fh:
+            return json.load(fh).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--forms", default="f64,f32")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    log("building workload %d^3" % a.size)
+    case = make_workload(a.size)
+    forms = [f for f in a.forms.split(",") if f]
+    res = {}
+    for form in forms:
+        log("timing %s: %d warm-up + %d steps" % (form, a.warmup, a.steps))
+        res[form] = time_gpu(case, form, a.steps, a.warmup)
+        log("%s: %.3f ms/step, %.0f MLUPS, collide %.0f GB/s" % (form, res[form]["ms_per_step"],
+                                                                 res[form]["mlups"], res[form]["collide_gbs"]))
+    main_form = "f64" if "f64" in res else forms[0]
+    m = res[main_form]
+    n = a.size ** 3
+    coll = m["kernels"]["collide_stream"]
+    traffic = load_traffic(main_form)
+    out = {
+        "metric": "MLUPS of the coupled step (ZeroForce+AddForces+collide/stream+CalcHydro+Move) at %d^3, "
+                  "FP64 drop-in" % a.size,
+        "value": round(m["mlups"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(m["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG64 inputs of the configuration's shape)",
+        "config": {"workload": "256^3 D3Q15 MRT, tau 0.25, 32768 swimmers (alpha 0.01), 1e6 tracers"
+                   if a.size == 256 else "%d^3 reduced" % a.size,
+                   "lattice": [a.size] * 3, "swimmers": 32768, "tracers": 1_000_000, "parallelism": "single GPU"},
+        "roofline": {
+            "kernel": "collide_stream (%s)" % main_form,
+            "bound": "hbm",
+            "achieved": round(m["collide_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(m["collide_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": COLLIDE_BYTES[main_form] * n,
+            "avg_launch_ms": round(coll["avg_ms"], 4),
+        },
+        "step_hbm_frac": round(m["step_gbs"] / HBM_PEAK_GBS, 4),
+        "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in m["kernels"].items()},
+    }
+    if "f32" in res and main_form != "f32":
+        f = res["f32"]
+        out["fp32_store"] = {"mlups": round(f["mlups"], 2), "ms_per_step": round(f["ms_per_step"], 4),
+                             "speedup_vs_fp64": round(f["mlups"] / m["mlups"], 3),
+                             "collide_gbs": round(f["collide_gbs"], 1),
+                             "collide_frac": round(f["collide_gbs"] / HBM_PEAK_GBS, 4),
+                             "step_hbm_frac": round(f["step_gbs"] / HBM_PEAK_GBS, 4),
+                             "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in f["kernels"].items()}}
+    if not a.no_cpu:
+        log("CPU baseline (oracle)")
+        out["cpu_baseline"] = cpu_baseline(case)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,609 @@
+// Lattice kernels for gfx950: collide+stream, CalcHydro, InitFromHydro,
+// ZeroForce and FP64 conservation totals.
+//
+// Layout (unchanged from the reference, lb.cu:49-57,117): site
+// ijk = i*ny*nz + j*nz + k, distributions f[p*n + ijk], vectors u[a*n + ijk].
+// One lane per site over the linear site index, so the 64 lanes of a
+// wavefront read and write 64 consecutive sites of each plane p: fully
+// coalesced for every lattice shape (the reference maps threadIdx.x to x, the
+// slowest axis, lb.cu:75-88).  Everything is bandwidth-bound, so the D3Q15
+// transforms are unrolled against compile-time tables and stay in registers.
+//
+// Two storage forms:
+//   double -- the reference's arithmetic, operation for operation, so that
+//             results match a CPU restatement bit for bit.  The only changes
+//             are dropped terms whose coefficient is exactly zero and the
+//             ghost modes that the reference zeroes before use.  Build with
+//             -ffp-contract=off.
+//   float  -- distributions stored as deviations g_p = f_p - w_p from the
+//             rest state (rho=1, u=0), density as rho-1, collided in FP32
+//             with an algebraically simplified transform.
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "d3q15.hpp"
+#include "internal.hpp"
+
+namespace lbs {
+
+Geom make_geom(int nx, int ny, int nz) {
+  Geom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.n = uint32_t(nx) * uint32_t(ny) * uint32_t(nz);
+  g.nyz = uint32_t(ny) * uint32_t(nz);
+  g.div_nz = make_fastdiv(uint32_t(nz));
+  g.div_ny = make_fastdiv(uint32_t(ny));
+  return g;
+}
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+struct Site {
+  uint32_t i, j, k;
+};
+
+__device__ __forceinline__ Site decompose(uint32_t s, const Geom& g) {
+  const uint32_t t = fastdiv(s, g.div_nz);
+  const uint32_t k = s - t * uint32_t(g.nz);
+  const uint32_t i = fastdiv(t, g.div_ny);
+  const uint32_t j = t - i * uint32_t(g.ny);
+  return Site{i, j, k};
+}
+
+// Destination of velocity p pushed from site s, with periodic wrap
+// (lb.cu:206-216).  ox/oy/oz hold the wrapped neighbour offsets for c = -1,0,1.
+struct Dest {
+  uint32_t ox[3], oy[3], oz[3];
+};
+
+__device__ __forceinline__ Dest dest_offsets(Site s, const Geom& g) {
+  Dest d;
+  const uint32_t im = s.i == 0 ? uint32_t(g.nx) - 1 : s.i - 1;
+  const uint32_t ip = s.i + 1 == uint32_t(g.nx) ? 0 : s.i + 1;
+  const uint32_t jm = s.j == 0 ? uint32_t(g.ny) - 1 : s.j - 1;
+  const uint32_t jp = s.j + 1 == uint32_t(g.ny) ? 0 : s.j + 1;
+  const uint32_t km = s.k == 0 ? uint32_t(g.nz) - 1 : s.k - 1;
+  const uint32_t kp = s.k + 1 == uint32_t(g.nz) ? 0 : s.k + 1;
+  d.ox[0] = im * g.nyz;
+  d.ox[1] = s.i * g.nyz;
+  d.ox[2] = ip * g.nyz;
+  d.oy[0] = jm * uint32_t(g.nz);
+  d.oy[1] = s.j * uint32_t(g.nz);
+  d.oy[2] = jp * uint32_t(g.nz);
+  d.oz[0] = km;
+  d.oz[1] = s.k;
+  d.oz[2] = kp;
+  return d;
+}
+
+template <int P>
+__device__ __forceinline__ uint32_t dest_of(const Dest& d) {
+  return d.ox[CI[P][0] + 1] + d.oy[CI[P][1] + 1] + d.oz[CI[P][2] + 1];
+}
+
+template <typename T>
+__device__ __forceinline__ void store_stream(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
+// ------------------------------------------------------------------------
+// FP64: the reference's arithmetic.
+// ------------------------------------------------------------------------
+
+// mode[M] = sum_p f[p]*mm[M][p] in p order, zero terms skipped (lb.cu:114-119).
+template <int M, int P>
+__device__ __forceinline__ void fwd_term(double& acc, const double (&f)[Q]) {
+  constexpr double c = TAB.mm[M][P];
+  if constexpr (c != 0.0) {
+    if constexpr (P == first_nz_mm(M))
+      acc = f[P] * c;
+    else
+      acc = acc + f[P] * c;
+  }
+}
+template <int M, int... P>
+__device__ __forceinline__ double fwd_row(const double (&f)[Q], std::integer_sequence<int, P...>) {
+  double acc = 0.0;
+  (fwd_term<M, P>(acc, f), ...);
+  return acc;
+}
+template <int M>
+__device__ __forceinline__ double fwd(const double (&f)[Q]) {
+  return fwd_row<M>(f, std::make_integer_sequence<int, Q>{});
+}
+
+// f[P] = sum_m mode[m]*mmi[P][m] in m order over the kept modes (lb.cu:198-203).
+template <int P, int M>
+__device__ __forceinline__ void inv_term(double& acc, const double (&mode)[NKEEP]) {
+  constexpr double c = TAB.mmi[P][M];
+  if constexpr (c != 0.0) {
+    if constexpr (M == first_nz_mmi(P))
+      acc = mode[M] * c;
+    else
+      acc = acc + mode[M] * c;
+  }
+}
+template <int P, int... M>
+__device__ __forceinline__ double inv_row(const double (&mode)[NKEEP], std::integer_sequence<int, M...>) {
+  double acc = 0.0;
+  (inv_term<P, M>(acc, mode), ...);
+  return acc;
+}
+template <int P>
+__device__ __forceinline__ double inv(const double (&mode)[NKEEP]) {
+  return inv_row<P>(mode, std::make_integer_sequence<int, NKEEP>{});
+}
+
+template <int... M>
+__device__ __forceinline__ void fwd_all(const double (&f)[Q], double (&mode)[NKEEP], std::integer_sequence<int, M...>) {
+  ((mode[M] = fwd<M>(f)), ...);
+}
+
+// MRT relaxation with forcing, lb.cu:127-195, in the same operation order.
+// Terms multiplied by an off-diagonal DQ_delta (exactly 0) are dropped:
+// x - usq*0.0 and x - 2*uF*0.0 equal x.
+__device__ __forceinline__ void relax_f64(double (&mode)[NKEEP], const double (&F)[3], double tau_s,
+                                          double tau_b, double omega_s, double omega_b) {
+  double u[3];
+  double usq = 0., uF = 0.;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    u[a] = (mode[1 + a] + 0.5 * F[a]) / mode[M_RHO];
+    mode[1 + a] += F[a];
+    usq += u[a] * u[a];
+    uF += u[a] * F[a];
+  }
+  double Sxx = mode[M_SXX], Sxy = mode[M_SXY], Sxz = mode[M_SXZ];
+  double Syy = mode[M_SYY], Syz = mode[M_SYZ], Szz = mode[M_SZZ];
+  double TrS = 0.;
+  TrS += Sxx;
+  TrS += Syy;
+  TrS += Szz;
+  Sxx -= TrS / 3;
+  Syy -= TrS / 3;
+  Szz -= TrS / 3;
+  TrS -= omega_b * (TrS - mode[M_RHO] * usq);
+  TrS += 2. * omega_b * tau_b * uF;
+  const double rho = mode[M_RHO];
+  const double cf = 2. * omega_s * tau_s;
+  // a = x
+  Sxx -= omega_s * (Sxx - rho * (u[0] * u[0] - usq));
+  Sxx += cf * (u[0] * F[0] + F[0] * u[0] - 2. * uF);
+  Sxy -= omega_s * (Sxy - rho * (u[0] * u[1]));
+  Sxy += cf * (u[0] * F[1] + F[0] * u[1]);
+  Sxz -= omega_s * (Sxz - rho * (u[0] * u[2]));
+  Sxz += cf * (u[0] * F[2] + F[0] * u[2]);
+  Sxx += TrS / 3;
+  // a = y (the yx element is recomputed by the reference but never used)
+  Syy -= omega_s * (Syy - rho * (u[1] * u[1] - usq));
+  Syy += cf * (u[1] * F[1] + F[1] * u[1] - 2. * uF);
+  Syz -= omega_s * (Syz - rho * (u[1] * u[2]));
+  Syz += cf * (u[1] * F[2] + F[1] * u[2]);
+  Syy += TrS / 3;
+  // a = z
+  Szz -= omega_s * (Szz - rho * (u[2] * u[2] - usq));
+  Szz += cf * (u[2] * F[2] + F[2] * u[2] - 2. * uF);
+  Szz += TrS / 3;
+  mode[M_SXX] = Sxx;
+  mode[M_SXY] = Sxy;
+  mode[M_SXZ] = Sxz;
+  mode[M_SYY] = Syy;
+  mode[M_SYZ] = Syz;
+  mode[M_SZZ] = Szz;
+}
+
+template <int... P>
+__device__ __forceinline__ void load_all(const double* __restrict__ f, uint32_t n, uint32_t s,
+                                         double (&out)[Q], std::integer_sequence<int, P...>) {
+  ((out[P] = f[size_t(P) * n + s]), ...);
+}
+
+template <int... P>
+__device__ __forceinline__ void push_all(double* __restrict__ fNew, uint32_t n, const Dest& d,
+                                         const double (&mode)[NKEEP], std::integer_sequence<int, P...>) {
+  (store_stream(&fNew[size_t(P) * n + dest_of<P>(d)], inv<P>(mode)), ...);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_collide_stream_f64(const double* __restrict__ fOld,
+                                                              const double* __restrict__ force,
+                                                              double* __restrict__ fNew, Geom g,
+                                                              double tau_s, double tau_b, double omega_s,
+                                                              double omega_b) {
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= g.n) return;
+  const uint32_t n = g.n;
+  double f[Q];
+  load_all(fOld, n, s, f, std::make_integer_sequence<int, Q>{});
+  const double F[3] = {force[s], force[size_t(n) + s], force[2 * size_t(n) + s]};
+  const Dest d = dest_offsets(decompose(s, g), g);
+  double mode[NKEEP];
+  fwd_all(f, mode, std::make_integer_sequence<int, NKEEP>{});
+  relax_f64(mode, F, tau_s, tau_b, omega_s, omega_b);
+  push_all(fNew, n, d, mode, std::make_integer_sequence<int, Q>{});
+}
+
+// lb.cu:250-295: only the density and momentum modes are needed.
+__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f64(const double* __restrict__ f,
+                                                          double* __restrict__ force,
+                                                          double* __restrict__ rho,
+                                                          double* __restrict__ u, Geom g, int zero_force) {
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= g.n) return;
+  const uint32_t n = g.n;
+  double fl[Q];
+  load_all(f, n, s, fl, std::make_integer_sequence<int, Q>{});
+  const double m0 = fwd<0>(fl);
+  const double m1 = fwd<1>(fl);
+  const double m2 = fwd<2>(fl);
+  const double m3 = fwd<3>(fl);
+  const double F0 = force[s], F1 = force[size_t(n) + s], F2 = force[2 * size_t(n) + s];
+  rho[s] = m0;
+  u[s] = (m1 + 0.5 * F0) / m0;
+  u[size_t(n) + s] = (m2 + 0.5 * F1) / m0;
+  u[2 * size_t(n) + s] = (m3 + 0.5 * F2) / m0;
+  if (zero_force) {
+    force[s] = 0.0;
+    force[size_t(n) + s] = 0.0;
+    force[2 * size_t(n) + s] = 0.0;
+  }
+}
+
+// lb.cu:314-371, including the reference's SXZ = rho*ux*uy (lb.cu:356).
+__global__ __launch_bounds__(BLOCK) void k_init_from_hydro_f64(const double* __restrict__ rho,
+                                                               const double* __restrict__ u,
+                                                               double* __restrict__ f, Geom g) {
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= g.n) return;
+  const uint32_t n = g.n;
+  const double r = rho[s];
+  const double ux = u[s], uy = u[size_t(n) + s], uz = u[2 * size_t(n) + s];
+  double mode[NKEEP];
+  mode[0] = r;
+  mode[1] = r * ux;
+  mode[2] = r * uy;
+  mode[3] = r * uz;
+  mode[M_SXX] = r * ux * ux;
+  mode[M_SXY] = r * ux * uy;
+  mode[M_SXZ] = r * ux * uy;
+  mode[M_SYY] = r * uy * uy;
+  mode[M_SYZ] = r * uy * uz;
+  mode[M_SZZ] = r * uz * uz;
+  [&]<int... P>(std::integer_sequence<int, P...>) {
+    ((f[size_t(P) * n + s] = inv<P>(mode)), ...);
+  }(std::make_integer_sequence<int, Q>{});
+}
+
+// ------------------------------------------------------------------------
+// FP32 deviation form.
+// ------------------------------------------------------------------------
+
+struct Moments32 {
+  float drho, j[3], S[6];  // S: xx, xy, xz, yy, yz, zz
+};
+
+__device__ __forceinline__ Moments32 moments_dev32(const float (&g)[Q]) {
+  Moments32 m;
+  const float cpx = (g[7] + g[8]) + (g[9] + g[10]);     // cube velocities with c_x = +1
+  const float cmx = (g[11] + g[12]) + (g[13] + g[14]);  // c_x = -1
+  const float cube = cpx + cmx;
+  m.drho = ((g[0] + (g[1] + g[2])) + ((g[3] + g[4]) + (g[5] + g[6]))) + cube;
+  m.j[0] = (g[1] - g[2]) + (cpx - cmx);
+  m.j[1] = (g[3] - g[4]) + (((g[7] + g[8]) + (g[11] + g[12])) - ((g[9] + g[10]) + (g[13] + g[14])));
+  m.j[2] = (g[5] - g[6]) + (((g[7] + g[9]) + (g[11] + g[13])) - ((g[8] + g[10]) + (g[12] + g[14])));
+  const float third = m.drho * (1.0f / 3.0f);
+  m.S[0] = ((g[1] + g[2]) + cube) - third;
+  m.S[3] = ((g[3] + g[4]) + cube) - third;
+  m.S[5] = ((g[5] + g[6]) + cube) - third;
+  m.S[1] = ((g[7] + g[8]) + (g[13] + g[14])) - ((g[9] + g[10]) + (g[11] + g[12]));
+  m.S[2] = ((g[7] + g[9]) + (g[12] + g[14])) - ((g[8] + g[10]) + (g[11] + g[13]));
+  m.S[4] = ((g[7] + g[10]) + (g[11] + g[14])) - ((g[8] + g[9]) + (g[12] + g[13]));
+  return m;
+}
+
+// Same relaxation as relax_f64 (lb.cu:127-195, including its usq and 2*uF
+// diagonal terms), on rho = 1 + drho, with fused multiply-adds.
+__device__ __forceinline__ void relax_dev32(Moments32& m, const float (&F)[3], float tau_s, float tau_b,
+                                            float omega_s, float omega_b) {
+  const float rho = 1.0f + m.drho;
+  const float inv_rho = 1.0f / rho;
+  float u[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    u[a] = __builtin_fmaf(0.5f, F[a], m.j[a]) * inv_rho;
+    m.j[a] += F[a];
+  }
+  const float usq = __builtin_fmaf(u[0], u[0], __builtin_fmaf(u[1], u[1], u[2] * u[2]));
+  const float uF = __builtin_fmaf(u[0], F[0], __builtin_fmaf(u[1], F[1], u[2] * F[2]));
+  float* S = m.S;
+  float TrS = (S[0] + S[3]) + S[5];
+  const float t3 = TrS * (1.0f / 3.0f);
+  S[0] -= t3;
+  S[3] -= t3;
+  S[5] -= t3;
+  TrS = __builtin_fmaf(-omega_b, TrS - rho * usq, TrS);
+  TrS = __builtin_fmaf(2.0f * omega_b * tau_b, uF, TrS);
+  const float cf = 2.0f * omega_s * tau_s;
+  const float nt3 = TrS * (1.0f / 3.0f);
+  constexpr int DI[3] = {0, 3, 5};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float& s = S[DI[a]];
+    s = __builtin_fmaf(-omega_s, s - rho * __builtin_fmaf(u[a], u[a], -usq), s);
+    s = __builtin_fmaf(cf, 2.0f * __builtin_fmaf(u[a], F[a], -uF), s);
+    s += nt3;
+  }
+  constexpr int OFF[3][3] = {{1, 0, 1}, {2, 0, 2}, {4, 1, 2}};  // slot, a, b
+#pragma unroll
+  for (int o = 0; o < 3; ++o) {
+    float& s = S[OFF[o][0]];
+    const int a = OFF[o][1], b = OFF[o][2];
+    s = __builtin_fmaf(-omega_s, s - rho * (u[a] * u[b]), s);
+    s = __builtin_fmaf(cf, __builtin_fmaf(u[a], F[b], F[a] * u[b]), s);
+  }
+}
+
+// g_p = w_p (drho - 3/2 T + 3 c.j + 9/2 (sum_a c_a^2 S_aa + 2 sum_{a<b} c_a c_b S_ab)),
+// the kept-mode part of mmi (d3q15.c:171-185) with mode 0 replaced by drho.
+__device__ __forceinline__ void inverse_dev32(const Moments32& m, float (&g)[Q]) {
+  constexpr float W0 = 2.0f / 9.0f, W1 = 1.0f / 9.0f, W2 = 1.0f / 72.0f;
+  const float* S = m.S;
+  const float T = (S[0] + S[3]) + S[5];
+  const float base = __builtin_fmaf(-1.5f, T, m.drho);
+  g[0] = W0 * base;
+  const float bx = __builtin_fmaf(4.5f, S[0], base), by = __builtin_fmaf(4.5f, S[3], base),
+              bz = __builtin_fmaf(4.5f, S[5], base);
+  const float jx3 = 3.0f * m.j[0], jy3 = 3.0f * m.j[1], jz3 = 3.0f * m.j[2];
+  g[1] = W1 * (bx + jx3);
+  g[2] = W1 * (bx - jx3);
+  g[3] = W1 * (by + jy3);
+  g[4] = W1 * (by - jy3);
+  g[5] = W1 * (bz + jz3);
+  g[6] = W1 * (bz - jz3);
+  const float A = __builtin_fmaf(3.0f, T, m.drho);
+  const float sxy = 9.0f * S[1], sxz = 9.0f * S[2], syz = 9.0f * S[4];
+  // pairs (p, complement p): 7/14 (+,+,+), 8/13 (+,+,-), 9/12 (+,-,+), 10/11 (+,-,-)
+  const float s7 = A + (sxy + sxz + syz), t7 = jx3 + jy3 + jz3;
+  const float s8 = A + (sxy - sxz - syz), t8 = jx3 + jy3 - jz3;
+  const float s9 = A + (-sxy + sxz - syz), t9 = jx3 - jy3 + jz3;
+  const float s10 = A + (-sxy - sxz + syz), t10 = jx3 - jy3 - jz3;
+  g[7] = W2 * (s7 + t7);
+  g[14] = W2 * (s7 - t7);
+  g[8] = W2 * (s8 + t8);
+  g[13] = W2 * (s8 - t8);
+  g[9] = W2 * (s9 + t9);
+  g[12] = W2 * (s9 - t9);
+  g[10] = W2 * (s10 + t10);
+  g[11] = W2 * (s10 - t10);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_collide_stream_f32(const float* __restrict__ fOld,
+                                                              const float* __restrict__ force,
+                                                              float* __restrict__ fNew, Geom g,
+                                                              float tau_s, float tau_b, float omega_s,
+                                                              float omega_b) {
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= g.n) return;
+  const uint32_t n = g.n;
+  float gl[Q];
+#pragma unroll
+  for (int p = 0; p < Q; ++p) gl[p] = fOld[size_t(p) * n + s];
+  const float F[3] = {force[s], force[size_t(n) + s], force[2 * size_t(n) + s]};
+  const Dest d = dest_offsets(decompose(s, g), g);
+  Moments32 m = moments_dev32(gl);
+  relax_dev32(m, F, tau_s, tau_b, omega_s, omega_b);
+  float out[Q];
+  inverse_dev32(m, out);
+  [&]<int... P>(std::integer_sequence<int, P...>) {
+    (store_stream(&fNew[size_t(P) * n + dest_of<P>(d)], out[P]), ...);
+  }(std::make_integer_sequence<int, Q>{});
+}
+
+__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f32(const float* __restrict__ f,
+                                                          float* __restrict__ force,
+                                                          float* __restrict__ drho_out,
+                                                          float* __restrict__ u, Geom g, int zero_force) {
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= g.n) return;
+  const uint32_t n = g.n;
+  float gl[Q];
+#pragma unroll
+  for (int p = 0; p < Q; ++p) gl[p] = f[size_t(p) * n + s];
+  const Moments32 m = moments_dev32(gl);
+  const float inv_rho = 1.0f / (1.0f + m.drho);
+  drho_out[s] = m.drho;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) u[size_t(a) * n + s] = __builtin_fmaf(0.5f, force[size_t(a) * n + s], m.j[a]) * inv_rho;
+  if (zero_force) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) force[size_t(a) * n + s] = 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_init_from_hydro_f32(const float* __restrict__ drho,
+                                                               const float* __restrict__ u,
+                                                               float* __restrict__ f, Geom g) {
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= g.n) return;
+  const uint32_t n = g.n;
+  Moments32 m;
+  m.drho = drho[s];
+  const float r = 1.0f + m.drho;
+  const float ux = u[s], uy = u[size_t(n) + s], uz = u[2 * size_t(n) + s];
+  m.j[0] = r * ux;
+  m.j[1] = r * uy;
+  m.j[2] = r * uz;
+  // The stress modes as the reference sets them, SXZ = rho*ux*uy included
+  // (lb.cu:354-361); the rest state has zero stress, so no offset applies.
+  const float sxx = r * ux * ux, syy = r * uy * uy, szz = r * uz * uz;
+  m.S[0] = sxx;
+  m.S[1] = r * ux * uy;
+  m.S[2] = r * ux * uy;
+  m.S[3] = syy;
+  m.S[4] = r * uy * uz;
+  m.S[5] = szz;
+  float out[Q];
+  inverse_dev32(m, out);
+#pragma unroll
+  for (int p = 0; p < Q; ++p) f[size_t(p) * n + s] = out[p];
+}
+
+// ------------------------------------------------------------------------
+// Utilities
+// ------------------------------------------------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_fill_zero(uint4* __restrict__ p, size_t n16) {
+  const size_t stride = size_t(gridDim.x) * BLOCK;
+  for (size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+}
+__global__ void k_fill_zero_tail(unsigned char* __restrict__ p, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <typename T>
+__device__ __forceinline__ double rest_weight(int p) {
+  if constexpr (sizeof(T) == 4) return TAB.w[p];
+  return 0.0;
+}
+
+// Block-partial FP64 sums of mass and momentum (deterministic order).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_totals_partial(const T* __restrict__ f, Geom g,
+                                                          double* __restrict__ partial) {
+  __shared__ double red[4][BLOCK];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const uint32_t n = g.n;
+  for (uint32_t s = blockIdx.x * BLOCK + threadIdx.x; s < n; s += gridDim.x * BLOCK) {
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+      const double fp = double(f[size_t(p) * n + s]) + rest_weight<T>(p);
+      acc[0] += fp;
+      acc[1] += CI[p][0] * fp;
+      acc[2] += CI[p][1] * fp;
+      acc[3] += CI[p][2] * fp;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) partial[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_totals_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+  if (threadIdx.x < 4) {
+    double a = 0.0;
+    for (int b = 0; b < nblocks; ++b) a += partial[b * 4 + threadIdx.x];
+    out[threadIdx.x] = a;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_copy(const double* __restrict__ src, double* __restrict__ dst, size_t n) {
+  const size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
+}
+
+inline unsigned grid_for(uint32_t n) { return (n + BLOCK - 1) / BLOCK; }
+
+}  // namespace
+
+// ---- launchers -------------------------------------------------------------
+
+template <>
+hipError_t launch_collide_stream<double>(const double* fOld, const double* force, double* fNew,
+                                         const Geom& g, double tau_s, double tau_b, hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  // omega = 1/(tau + 1/2) as lb.cu:107-108 computes it, once per launch.
+  const double omega_s = 1.0 / (tau_s + 0.5), omega_b = 1.0 / (tau_b + 0.5);
+  hipLaunchKernelGGL(k_collide_stream_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, fOld, force, fNew, g,
+                     tau_s, tau_b, omega_s, omega_b);
+  return hipGetLastError();
+}
+
+template <>
+hipError_t launch_collide_stream<float>(const float* fOld, const float* force, float* fNew, const Geom& g,
+                                        double tau_s, double tau_b, hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  const float os = float(1.0 / (tau_s + 0.5)), ob = float(1.0 / (tau_b + 0.5));
+  hipLaunchKernelGGL(k_collide_stream_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, fOld, force, fNew, g,
+                     float(tau_s), float(tau_b), os, ob);
+  return hipGetLastError();
+}
+
+template <>
+hipError_t launch_calc_hydro<double>(const double* f, double* force, double* rho, double* u, const Geom& g,
+                                     bool zero_force, hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_calc_hydro_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g,
+                     int(zero_force));
+  return hipGetLastError();
+}
+
+template <>
+hipError_t launch_calc_hydro<float>(const float* f, float* force, float* rho, float* u, const Geom& g,
+                                    bool zero_force, hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_calc_hydro_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g,
+                     int(zero_force));
+  return hipGetLastError();
+}
+
+template <>
+hipError_t launch_init_from_hydro<double>(const double* rho, const double* u, double* f, const Geom& g,
+                                          hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_init_from_hydro_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, rho, u, f, g);
+  return hipGetLastError();
+}
+
+template <>
+hipError_t launch_init_from_hydro<float>(const float* rho, const float* u, float* f, const Geom& g,
+                                         hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_init_from_hydro_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, rho, u, f, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  const size_t n16 = bytes / 16;
+  if (n16) {
+    const size_t blocks = (n16 + BLOCK - 1) / BLOCK;
+    const unsigned grid = unsigned(blocks < 65536 ? blocks : 65536);
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid), dim3(BLOCK), 0, s, static_cast<uint4*>(p), n16);
+  }
+  const size_t tail = bytes - n16 * 16;
+  if (tail) {
+    hipLaunchKernelGGL(k_fill_zero_tail, dim3(1), dim3(64), 0, s, static_cast<unsigned char*>(p) + n16 * 16,
+                       tail);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_totals(const T* f, const Geom& g, double* partial, int nblocks, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_totals_partial<T>, dim3(nblocks), dim3(BLOCK), 0, s, f, g, partial);
+  hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(64), 0, s, partial, nblocks, out);
+  return hipGetLastError();
+}
+template hipError_t launch_totals<double>(const double*, const Geom&, double*, int, double*, hipStream_t);
+template hipError_t launch_totals<float>(const float*, const Geom&, double*, int, double*, hipStream_t);
+
+hipError_t launch_copy(const double* src, double* dst, size_t count, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy, dim3(unsigned((count + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, src, dst, count);
+  return hipGetLastError();
+}
+
+}  // namespace lbs

@@ -1,0 +1,371 @@
+// Particle kernels for gfx950: Peskin 4-point delta, velocity interpolation,
+// singular-force spreading, swimmer and tracer moves.
+//
+// Particle state is FP64 in both storage forms; u and force are read/written
+// in the lattice's storage type T.  The arithmetic of the reference is kept
+// operation for operation (delta.cu, interp.cu, SwimmerArray.cu,
+// TracerArray.cu), so with FP64 storage a swimmer or tracer moves exactly as a
+// CPU restatement does.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "d3q15.hpp"
+#include "internal.hpp"
+
+namespace lbs {
+namespace {
+
+constexpr int BLOCK = 256;
+
+// Peskin (2002) four-point regularised delta, delta.cu:6-25.
+__device__ __forceinline__ double peskin(double x) {
+  const double abs_x = fabs(x);
+  double root = -4. * x * x;
+  double phi = -2. * abs_x;
+  if (abs_x >= 2.0) return 0.;
+  if (abs_x >= 1.0) {
+    root += 12. * abs_x - 7.;
+    phi += 5.;
+    phi -= sqrt(root);
+  } else {
+    root += 4. * abs_x + 1;
+    phi += 3.;
+    phi += sqrt(root);
+  }
+  return 0.125 * phi;
+}
+
+// ((int)x + n) % n of interp.cu:26 for x >= -n, extended to a true periodic
+// wrap (and NaN mapped to 0) so that no input can index outside the lattice.
+__device__ __forceinline__ uint32_t wrap_index(double x, int n) {
+  if (!(x > -2147483648.0 && x < 2147483647.0)) x = 0.0;
+  int v = int(x) % n;
+  return uint32_t(v < 0 ? v + n : v);
+}
+
+// InterpVelocity, interp.cu:8-47: v_d = sum_{i,j,k} dx_i dy_j dz_k u_d(site),
+// summed in i, j, k order.
+template <typename T>
+__device__ __forceinline__ void interp(const T* __restrict__ u, const Geom& g, const double (&r)[3],
+                                       double (&v)[3]) {
+  const int size[3] = {g.nx, g.ny, g.nz};
+  uint32_t idx[3][4];
+  double del[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double x0 = ceil(r[d] - 2.);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double x = x0 + i;
+      idx[d][i] = wrap_index(x, size[d]);
+      del[d][i] = peskin(r[d] - x);
+    }
+  }
+  const size_t n = g.n;
+  v[0] = v[1] = v[2] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double dxy = del[0][i] * del[1][j];
+      const uint32_t row = idx[0][i] * g.nyz + idx[1][j] * uint32_t(g.nz);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double delta3d = dxy * del[2][k];
+        const uint32_t ijk = row + idx[2][k];
+        v[0] += delta3d * double(u[ijk]);
+        v[1] += delta3d * double(u[n + ijk]);
+        v[2] += delta3d * double(u[2 * n + ijk]);
+      }
+    }
+  }
+}
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ----
+struct U4 {
+  uint32_t x[4];
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c.x[0], hi0 = __umulhi(0xD2511F53u, c.x[0]);
+    const uint32_t lo1 = 0xCD9E8D57u * c.x[2], hi1 = __umulhi(0xCD9E8D57u, c.x[2]);
+    U4 o;
+    o.x[0] = hi1 ^ c.x[1] ^ k0;
+    o.x[1] = lo1;
+    o.x[2] = hi0 ^ c.x[3] ^ k1;
+    o.x[3] = lo0;
+    c = o;
+  }
+  return c;
+}
+
+// Block `tag` of swimmer `is` at draw counter `ctr`, keyed by the seed.
+__device__ __forceinline__ U4 draw(unsigned long long seed, int is, uint64_t ctr, uint32_t tag) {
+  U4 c;
+  c.x[0] = uint32_t(is);
+  c.x[1] = uint32_t(ctr);
+  c.x[2] = uint32_t(ctr >> 32);
+  c.x[3] = tag;
+  return philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
+}
+
+// float in (0,1] from the top 24 bits (the range of curand_uniform).
+__device__ __forceinline__ float u01f(uint32_t x) { return float((x >> 8) + 1u) * 0x1p-24f; }
+// double in (0,1) from 53 bits.
+__device__ __forceinline__ double u01d(uint32_t a, uint32_t b) {
+  uint64_t m = (uint64_t(a) << 21) ^ uint64_t(b >> 11);
+  m &= (uint64_t(1) << 53) - 1;
+  return (double(m) + 0.5) * 0x1p-53;
+}
+
+// Three N(0,1) normals by Box-Muller from draw blocks 1 and 2.
+__device__ __forceinline__ void normals3(unsigned long long seed, int is, uint64_t ctr, double (&g)[3]) {
+  const U4 b1 = draw(seed, is, ctr, 1u), b2 = draw(seed, is, ctr, 2u);
+  const double u1 = u01d(b1.x[0], b1.x[1]), u2 = u01d(b1.x[2], b1.x[3]);
+  const double u3 = u01d(b2.x[0], b2.x[1]), u4 = u01d(b2.x[2], b2.x[3]);
+  double rad = sqrt(-2.0 * log(u1));
+  g[0] = rad * cos(2.0 * M_PI * u2);
+  g[1] = rad * sin(2.0 * M_PI * u2);
+  rad = sqrt(-2.0 * log(u3));
+  g[2] = rad * cos(2.0 * M_PI * u4);
+}
+
+// ---- force spreading -------------------------------------------------------
+// DoSwimmerArrayAddForces + AccumulateDeltaForce (SwimmerArray.cu:42-112).
+// One wavefront per swimmer.  The tail stencil (at r - l n, force -P n) and
+// the head stencil (at r, force +P n) usually share their 4x4x4 box; each
+// lane takes one site of the union box and adds tail + head with one atomic
+// per component, so a site seen only by this swimmer gets 0 + t + h exactly
+// as the reference's two atomics give.  Boxes wider than 5 fall back to two
+// separate 4x4x4 passes.
+template <typename T>
+__device__ __forceinline__ void spread_atomic(T* addr, double val) {
+  unsafeAtomicAdd(addr, T(val));
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_swimmers_add_forces(int ns, double P, double l,
+                                                               const double* __restrict__ r,
+                                                               const double* __restrict__ nv,
+                                                               T* __restrict__ force, Geom g) {
+  const int lane = threadIdx.x & 63;
+  const int is = int((blockIdx.x * BLOCK + threadIdx.x) >> 6);
+  if (is >= ns) return;
+  const size_t n = g.n;
+  const int size[3] = {g.nx, g.ny, g.nz};
+  double rt[3], rh[3], Ft[3], Fh[3], x0t[3], x0h[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double ra = r[size_t(ns) * a + is], na = nv[size_t(ns) * a + is];
+    rt[a] = ra - na * l;
+    Ft[a] = -P * na;
+    rh[a] = ra;
+    Fh[a] = Ft[a] * -1.0;
+    x0t[a] = ceil(rt[a] - 2.);
+    x0h[a] = ceil(rh[a] - 2.);
+  }
+  double lo[3];
+  int ext[3];
+  bool fused = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = fmin(x0t[a], x0h[a]);
+    const double span = fmax(x0t[a], x0h[a]) - lo[a];
+    fused = fused && (span <= 1.0);
+    ext[a] = span <= 1.0 ? int(span) + 4 : 4;
+  }
+  if (fused) {
+    const int vol = ext[0] * ext[1] * ext[2];
+    for (int c = lane; c < vol; c += 64) {
+      const int cz = c % ext[2], t = c / ext[2];
+      const int cy = t % ext[1], cx = t / ext[1];
+      const double X = lo[0] + cx, Y = lo[1] + cy, Z = lo[2] + cz;
+      const double wt = peskin(rt[0] - X) * peskin(rt[1] - Y) * peskin(rt[2] - Z);
+      const double wh = peskin(rh[0] - X) * peskin(rh[1] - Y) * peskin(rh[2] - Z);
+      if (wt == 0.0 && wh == 0.0) continue;
+      const uint32_t ijk = wrap_index(X, size[0]) * g.nyz + wrap_index(Y, size[1]) * uint32_t(g.nz) +
+                           wrap_index(Z, size[2]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) spread_atomic(&force[size_t(d) * n + ijk], wt * Ft[d] + wh * Fh[d]);
+    }
+  } else {
+    const int ci = lane >> 4, cj = (lane >> 2) & 3, ck = lane & 3;
+    for (int end = 0; end < 2; ++end) {
+      const double* rp = end ? rh : rt;
+      const double* x0 = end ? x0h : x0t;
+      const double* Fp = end ? Fh : Ft;
+      const double X = x0[0] + ci, Y = x0[1] + cj, Z = x0[2] + ck;
+      const double w = peskin(rp[0] - X) * peskin(rp[1] - Y) * peskin(rp[2] - Z);
+      const uint32_t ijk = wrap_index(X, size[0]) * g.nyz + wrap_index(Y, size[1]) * uint32_t(g.nz) +
+                           wrap_index(Z, size[2]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) spread_atomic(&force[size_t(d) * n + ijk], w * Fp[d]);
+    }
+  }
+}
+
+// DoSwimmerArrayMove, SwimmerArray.cu:123-210 (norm starts at 0; with
+// rotation off and no tumble, n is kept).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_swimmers_move(int ns, SwimParams cp, double* __restrict__ r,
+                                                         double* __restrict__ v, double* __restrict__ nv,
+                                                         uint64_t* __restrict__ prng, const T* __restrict__ u,
+                                                         Geom g, unsigned long long* __restrict__ tumbles) {
+  const int is = int(blockIdx.x * BLOCK + threadIdx.x);
+  if (is >= ns) return;
+  const int size[3] = {g.nx, g.ny, g.nz};
+  double R[3], N[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    R[d] = r[size_t(ns) * d + is];
+    N[d] = nv[size_t(ns) * d + is];
+  }
+  double vel[3];
+  interp(u, g, R, vel);
+  double rDot[3], rMinus[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    rDot[d] = cp.mobility * cp.P * N[d];
+    if (!cp.translational_advection_off) rDot[d] += vel[d];
+    rMinus[d] = R[d] - N[d] * cp.l;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    v[size_t(ns) * d + is] = rDot[d];
+    R[d] += rDot[d];
+    r[size_t(ns) * d + is] = fmod(R[d] + size[d], double(size[d]));
+  }
+  const uint64_t ctr = prng[is];
+  const float U = u01f(draw(cp.seed, is, ctr, 0u).x[0]);
+  double newn[3];
+  double norm = 0.0;
+  bool turned = true;
+  if (double(U) < cp.alpha) {
+    normals3(cp.seed, is, ctr, newn);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) norm += newn[d] * newn[d];
+    atomicAdd(tumbles, 1ull);
+  } else if (!cp.rotational_advection_off) {
+    double vMinus[3];
+    interp(u, g, rMinus, vMinus);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double nDot = vel[d] - vMinus[d];
+      nDot /= cp.l;
+      newn[d] = N[d] + nDot;
+      norm += newn[d] * newn[d];
+    }
+  } else {
+    turned = false;
+  }
+  if (turned) {
+    norm = sqrt(norm);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) nv[size_t(ns) * d + is] = newn[d] / norm;
+  }
+  prng[is] = ctr + 1;
+}
+
+// DoTracerArrayMove, TracerArray.cu:12-43 (s updated per particle).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
+                                                        double* __restrict__ v, const T* __restrict__ u, Geom g) {
+  const int it = int(blockIdx.x * BLOCK + threadIdx.x);
+  if (it >= nt) return;
+  const int size[3] = {g.nx, g.ny, g.nz};
+  double R[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) R[d] = r[size_t(nt) * d + it];
+  double vel[3];
+  interp(u, g, R, vel);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    v[size_t(nt) * d + it] = vel[d];
+    R[d] += vel[d];
+    r[size_t(nt) * d + it] = fmod(R[d] + size[d], double(size[d]));
+    sd[size_t(nt) * d + it] += vel[d];
+  }
+}
+
+// ---- device-math probes used by the parity tests ----
+__global__ void k_peskin_probe(const double* __restrict__ x, double* __restrict__ out, int count) {
+  const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < count) out[i] = peskin(x[i]);
+}
+
+__global__ void k_interp_probe(const double* __restrict__ u, const double* __restrict__ r,
+                               double* __restrict__ v, int count, Geom g) {
+  const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= count) return;
+  const double R[3] = {r[i], r[size_t(count) + i], r[2 * size_t(count) + i]};
+  double out[3];
+  interp(u, g, R, out);
+  v[i] = out[0];
+  v[size_t(count) + i] = out[1];
+  v[2 * size_t(count) + i] = out[2];
+}
+
+inline unsigned blocks_for(size_t threads) { return unsigned((threads + BLOCK - 1) / BLOCK); }
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_swimmers_add_forces(int ns, const SwimParams& p, const double* r, const double* n, T* force,
+                                      const Geom& g, hipStream_t s) {
+  if (ns <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_swimmers_add_forces<T>, dim3(blocks_for(size_t(ns) * 64)), dim3(BLOCK), 0, s, ns, p.P,
+                     p.l, r, n, force, g);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_swimmers_move(int ns, const SwimParams& p, double* r, double* v, double* n, uint64_t* prng,
+                                const T* u, const Geom& g, unsigned long long* tumbles, hipStream_t s) {
+  if (ns <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_swimmers_move<T>, dim3(blocks_for(size_t(ns))), dim3(BLOCK), 0, s, ns, p, r, v, n, prng,
+                     u, g, tumbles);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_tracers_move(int nt, double* r, double* sd, double* v, const T* u, const Geom& g,
+                               hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tracers_move<T>, dim3(blocks_for(size_t(nt))), dim3(BLOCK), 0, s, nt, r, sd, v, u, g);
+  return hipGetLastError();
+}
+
+template hipError_t launch_swimmers_add_forces<double>(int, const SwimParams&, const double*, const double*,
+                                                       double*, const Geom&, hipStream_t);
+template hipError_t launch_swimmers_add_forces<float>(int, const SwimParams&, const double*, const double*,
+                                                      float*, const Geom&, hipStream_t);
+template hipError_t launch_swimmers_move<double>(int, const SwimParams&, double*, double*, double*, uint64_t*,
+                                                 const double*, const Geom&, unsigned long long*, hipStream_t);
+template hipError_t launch_swimmers_move<float>(int, const SwimParams&, double*, double*, double*, uint64_t*,
+                                                const float*, const Geom&, unsigned long long*, hipStream_t);
+template hipError_t launch_tracers_move<double>(int, double*, double*, double*, const double*, const Geom&,
+                                                hipStream_t);
+template hipError_t launch_tracers_move<float>(int, double*, double*, double*, const float*, const Geom&,
+                                               hipStream_t);
+
+hipError_t launch_peskin_probe(const double* x, double* out, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_peskin_probe, dim3(blocks_for(size_t(count))), dim3(BLOCK), 0, s, x, out, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_interp_probe(const double* u, const double* r, double* v, int count, const Geom& g,
+                               hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_interp_probe, dim3(blocks_for(size_t(count))), dim3(BLOCK), 0, s, u, r, v, count, g);
+  return hipGetLastError();
+}
+
+}  // namespace lbs

@@ -1,0 +1,224 @@
+/*
+ * lbswim C ABI -- the drop-in boundary of the MI355X-native coupled
+ * lattice-Boltzmann / swimmer / tracer step.
+ *
+ * The reference exposes its library to Python through Cython `cdef extern`
+ * declarations of C++ classes (lbswim/_lb.pxd:3-49, lbswim/_shared.pxd:1-13,
+ * lbswim/SwimmerArray.pxd:4-26, lbswim/TracerArray.pxd:4-11).  Each entry
+ * point below replaces one member of that surface; the comment on each names
+ * it.  Plain C types only: opaque handles, pointers, sizes and int status codes.
+ *
+ * Conventions
+ *   - Every function returning int returns LBSWIM_OK (0) on success or a
+ *     positive LBSWIM_E* code; lbswim_last_error() then describes the failure
+ *     (per thread).  The reference prints and calls exit() instead
+ *     (lbswim/liblbswim/cucall.h:5-14).
+ *   - Data layout is the reference's: lattice site ijk = i*ny*nz + j*nz + k;
+ *     fields element-slowest, f[p*n + ijk], u[a*n + ijk]; particle vectors
+ *     r[a*N + i] (README.md:39; lb.cu:49-57,117).
+ *   - Every object owns a host mirror and a device copy of each field
+ *     (SharedArray, lbswim/liblbswim/Shared.h:4-25).  Host mirrors are FP64
+ *     (the particle PRNG mirror is uint64) and zero-initialised; *_h2d and
+ *     *_d2h copy a whole field and return when the copy is complete.
+ *   - Device work of all objects is ordered on one library stream, as the
+ *     reference's default stream orders it; lbswim_synchronize() waits for it.
+ *     Lattice/particle calls only enqueue (the reference's Step also waits,
+ *     lb.cu:239).  Calls are not thread-safe.
+ */
+#ifndef LBSWIM_H
+#define LBSWIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBSWIM_Q 15 /* DQ_q, lbswim/liblbswim/d3q15.h:5 */
+#define LBSWIM_D 3  /* DQ_d, lbswim/liblbswim/d3q15.h:4 */
+
+/* status codes */
+#define LBSWIM_OK 0
+#define LBSWIM_EINVAL 1 /* invalid argument (size, tau, count, field id, NULL) */
+#define LBSWIM_ENOMEM 2 /* host or device allocation failed */
+#define LBSWIM_EHIP 3   /* HIP runtime error (launch, copy, stream) */
+#define LBSWIM_ENODEV 4 /* no usable GPU */
+
+/* storage forms of the lattice fields */
+#define LBSWIM_STORE_F64 0    /* FP64 everywhere: the reference's arithmetic */
+#define LBSWIM_STORE_F32DEV 1 /* FP32 f as deviation from the rest state, FP32 rho/u/force */
+
+/* lattice fields (LatticeData members, lbswim/liblbswim/Lattice.h:45-49) */
+#define LBSWIM_RHO 0
+#define LBSWIM_U 1
+#define LBSWIM_FORCE 2
+#define LBSWIM_FOLD 3
+#define LBSWIM_FNEW 4
+
+/* particle fields (SwimmerArray.h:24-28, TracerArray.h:10-13) */
+#define LBSWIM_PART_R 0
+#define LBSWIM_PART_V 1
+#define LBSWIM_PART_N 2    /* swimmers only */
+#define LBSWIM_PART_S 3    /* tracers only */
+#define LBSWIM_PART_PRNG 4 /* swimmers only: uint64 draw counter per swimmer */
+
+/* kernels that lbswim_lattice_kernel_time() reports */
+#define LBSWIM_K_COLLIDE 0 /* DoStep */
+#define LBSWIM_K_HYDRO 1   /* DoCalcHydro */
+#define LBSWIM_K_ZERO 2    /* DoLatticeZeroForce */
+#define LBSWIM_K_ADDF 3    /* DoSwimmerArrayAddForces */
+#define LBSWIM_K_SMOVE 4   /* DoSwimmerArrayMove */
+#define LBSWIM_K_TMOVE 5   /* DoTracerArrayMove */
+#define LBSWIM_K_INIT 6    /* DoInitFromHydro */
+#define LBSWIM_K_COUNT 7
+
+/* Same members, order and types as LBParams (lbswim/liblbswim/LBParams.h:4-30). */
+typedef struct lbswim_params {
+  double cs2;
+  double w[LBSWIM_Q];
+  double xi[LBSWIM_Q][LBSWIM_D];
+  int ci[LBSWIM_Q][LBSWIM_D];
+  double Q[LBSWIM_Q][LBSWIM_D][LBSWIM_D];
+  int complement[LBSWIM_Q];
+  double norms[LBSWIM_Q];
+  double mm[LBSWIM_Q][LBSWIM_Q];
+  double mmi[LBSWIM_Q][LBSWIM_Q];
+  double tau_s;
+  double tau_b;
+} lbswim_params;
+
+/* Same as LatticeAddressing (lbswim/liblbswim/Lattice.h:8-12). */
+typedef struct lbswim_addressing {
+  int size[LBSWIM_D];
+  int strides[LBSWIM_D];
+  int n;
+} lbswim_addressing;
+
+/* Same members, order and layout as CommonParams (lbswim/liblbswim/SwimmerArray.h:8-16). */
+typedef struct lbswim_common_params {
+  double P;
+  double l;
+  double alpha;
+  double mobility;
+  unsigned char translational_advection_off;
+  unsigned char rotational_advection_off;
+  unsigned long long seed;
+} lbswim_common_params;
+
+typedef struct lbswim_lattice lbswim_lattice;
+typedef struct lbswim_swimmers lbswim_swimmers;
+typedef struct lbswim_tracers lbswim_tracers;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* Message of the last failure on this thread ("" if none). No reference counterpart. */
+const char* lbswim_last_error(void);
+/* Wait for all enqueued device work (the reference waits inside Step, lb.cu:239). */
+int lbswim_synchronize(void);
+
+/* ---- Lattice (lbswim/liblbswim/Lattice.h:53-67; lbswim/_lb.pxd:22-30) ---- */
+
+/* Lattice::Lattice(nx, ny, nz, tau_s, tau_b) (lb.cu:40-63) plus the storage form.
+ * Sizes >= 1 with 15*nx*ny*nz < 2^31; tau > 0. Device fields start at zero. */
+int lbswim_lattice_create(int nx, int ny, int nz, double tau_s, double tau_b, int storage,
+                          lbswim_lattice** out);
+/* Lattice::~Lattice (lb.cu:66-67). NULL is ignored. */
+void lbswim_lattice_destroy(lbswim_lattice* lat);
+/* Lattice::Step: collide + push-stream, swap fOld/fNew, time_step++ (lb.cu:220-247). */
+int lbswim_lattice_step(lbswim_lattice* lat);
+/* Lattice::CalcHydro: rho, u from fOld and force (lb.cu:297-312). */
+int lbswim_lattice_calc_hydro(lbswim_lattice* lat);
+/* Lattice::InitFromHydro: equilibrium fOld from rho, u (lb.cu:373-388). */
+int lbswim_lattice_init_from_hydro(lbswim_lattice* lat);
+/* Lattice::ZeroForce (lb.cu:413-428). */
+int lbswim_lattice_zero_force(lbswim_lattice* lat);
+/* Lattice::params.host -- the D3Q15 tables and tau (Lattice.h:61; d3q15.c:2-186). */
+int lbswim_lattice_params(lbswim_lattice* lat, const lbswim_params** host);
+/* Lattice::addr.host (Lattice.h:62; lb.cu:49-57). */
+int lbswim_lattice_addressing(lbswim_lattice* lat, const lbswim_addressing** host);
+/* Lattice::time_step read / write (Lattice.h:65; lb.pyx:167-169,240). */
+int lbswim_lattice_get_time_step(lbswim_lattice* lat, int* out);
+int lbswim_lattice_set_time_step(lbswim_lattice* lat, int ts);
+/* Storage form given at creation. No reference counterpart. */
+int lbswim_lattice_storage(lbswim_lattice* lat, int* out);
+/* LatticeData::{rho,u,force,fOld,fNew} as SharedArray<double>{host,device,size}
+ * (Lattice.h:45-49; Shared.h:4-10). *host is the FP64 mirror (stable for the
+ * lattice's life), *device the current device buffer in the storage type,
+ * *size the element count (n, 3n or 15n). Any output pointer may be NULL. */
+int lbswim_lattice_field(lbswim_lattice* lat, int field, double** host, void** device, size_t* size);
+/* SharedArray::H2D / D2H of one field (Shared.cu:15-26). For fOld/fNew the
+ * device side is the buffer currently holding that role, as after the
+ * reference's pointer swap (lb.cu:242-244). */
+int lbswim_lattice_h2d(lbswim_lattice* lat, int field);
+int lbswim_lattice_d2h(lbswim_lattice* lat, int field);
+
+/* ---- SwimmerArray (lbswim/liblbswim/SwimmerArray.h:18-29; SwimmerArray.pxd:16-26) ---- */
+
+/* SwimmerArray::SwimmerArray(n, const CommonParams*) (SwimmerArray.cu:16-25).
+ * n >= 0. r, v, n start at zero and every swimmer's draw counter at 0. */
+int lbswim_swimmers_create(int n, const lbswim_common_params* p, lbswim_swimmers** out);
+void lbswim_swimmers_destroy(lbswim_swimmers* sw);
+/* SwimmerArray::num. */
+int lbswim_swimmers_num(const lbswim_swimmers* sw, int* out);
+/* SwimmerArray::common.host (SwimmerArray.h:25); edit it, then common_h2d. */
+int lbswim_swimmers_common(lbswim_swimmers* sw, lbswim_common_params** host);
+/* SharedItem<CommonParams>::H2D / D2H (Shared.cu:43-58; swimmers.pyx:21-68). */
+int lbswim_swimmers_common_h2d(lbswim_swimmers* sw);
+int lbswim_swimmers_common_d2h(lbswim_swimmers* sw);
+/* SwimmerArray::AddForces(Lattice*) const (SwimmerArray.cu:114-121). */
+int lbswim_swimmers_add_forces(lbswim_swimmers* sw, lbswim_lattice* lat);
+/* SwimmerArray::Move(Lattice*) (SwimmerArray.cu:212-220). */
+int lbswim_swimmers_move(lbswim_swimmers* sw, lbswim_lattice* lat);
+/* SwimmerArray::{r,v,n,prng} as SharedArray{host,device,size}; *elem_size is
+ * 8 for all four (double, double, double, uint64 counter). */
+int lbswim_swimmers_field(lbswim_swimmers* sw, int field, void** host, void** device, size_t* count,
+                          size_t* elem_size);
+int lbswim_swimmers_h2d(lbswim_swimmers* sw, int field);
+int lbswim_swimmers_d2h(lbswim_swimmers* sw, int field);
+/* Number of tumbles since creation (diagnostic; no reference counterpart). */
+int lbswim_swimmers_tumbles(lbswim_swimmers* sw, unsigned long long* out);
+
+/* ---- TracerArray (lbswim/liblbswim/TracerArray.h:6-14; TracerArray.pxd:4-11) ---- */
+
+/* TracerArray::TracerArray(n) (TracerArray.cu:5-10). n >= 0; fields start at zero. */
+int lbswim_tracers_create(int n, lbswim_tracers** out);
+void lbswim_tracers_destroy(lbswim_tracers* tr);
+int lbswim_tracers_num(const lbswim_tracers* tr, int* out);
+/* TracerArray::Move(Lattice*) (TracerArray.cu:45-52). */
+int lbswim_tracers_move(lbswim_tracers* tr, lbswim_lattice* lat);
+/* TracerArray::{r,s,v} (TracerArray.h:10-13). */
+int lbswim_tracers_field(lbswim_tracers* tr, int field, void** host, void** device, size_t* count);
+int lbswim_tracers_h2d(lbswim_tracers* tr, int field);
+int lbswim_tracers_d2h(lbswim_tracers* tr, int field);
+
+/* ---- coupled step ------------------------------------------------------ */
+
+/* `nsteps` repetitions of System.Step (lbswim/System.py:9-16): ZeroForce;
+ * AddForces of every swimmer array; Step; CalcHydro; Move of every swimmer
+ * array, then of every tracer array. */
+int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
+                       int ntr, int nsteps);
+
+/* ---- diagnostics (no reference counterpart) ---------------------------- */
+
+/* FP64 totals of the current distributions: out[0] = total mass, out[1..3] = total momentum. */
+int lbswim_lattice_totals(lbswim_lattice* lat, double out[4]);
+/* Per-kernel timing with HIP events on the library stream; on = 1 starts
+ * recording (and clears earlier records), 0 stops. */
+int lbswim_lattice_set_timing(lbswim_lattice* lat, int on);
+/* Sum of recorded durations (ms) and launch count for one LBSWIM_K_* kernel. */
+int lbswim_lattice_kernel_time(lbswim_lattice* lat, int kernel, double* total_ms, long* count);
+/* Copy `count` doubles device->device with a plain streaming kernel `reps` times;
+ * *ms is the average per copy (bandwidth probe). */
+int lbswim_diag_copy(size_t count, int reps, double* ms);
+/* Evaluate the device Peskin delta at x[0..count) (host arrays). */
+int lbswim_diag_peskin(const double* x, double* out, int count);
+/* Interpolate a host FP64 u[3][nx*ny*nz] at host positions r[3][count] on the device. */
+int lbswim_diag_interp(int nx, int ny, int nz, const double* u, const double* r, double* v, int count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBSWIM_H */

@@ -1,0 +1,282 @@
+"""GPU parity: the HIP path (through the lbswim package and the C ABI) against
+the CPU oracle on the same seeded inputs.
+
+Bar: the FP64 storage form keeps the reference's arithmetic, so lattice
+fields must agree bit for bit on single operations and within 1e-12
+norm-wise relative (||a-b||_inf/||b||_inf) after long coupled runs, where
+only the order of concurrent force atomics may differ.  The FP32 storage form
+must stay within 1e-5 relative of the FP64 run in u.
+"""
+import os
+import pickle
+import sys
+
+import numpy as np
+import pytest
+
+import lbswim_oracle as O
+from cases import build_lbswim, collect_lbswim, config1, config2, make_case, run_lbswim, run_oracle
+from util import rel_inf
+
+pytestmark = pytest.mark.gpu
+
+PARITY = 1e-12  # FP64 coupled-run tolerance (norm-wise relative)
+F32_U_TOL = 1e-5  # FP32-storage u error against FP64
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def oracle_threads():
+    O.set_threads(min(16, os.cpu_count() or 1))
+
+
+def compare(gpu, ref, keys, tol, label):
+    worst = {}
+    for k in keys:
+        if ref[k].size == 0:
+            continue
+        worst[k] = rel_inf(gpu[k], ref[k])
+    say("%s: %s" % (label, {k: "%.2e" % v for k, v in worst.items()}))
+    bad = {k: v for k, v in worst.items() if not v <= tol}
+    assert not bad, bad
+
+
+# ------------------------------------------------------------- device math
+
+def test_peskin_device_bitwise():
+    from lbswim import _core
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-2.5, 2.5, 200000), [0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 0.5, 1.5, 1e-300,
+                                                         np.nextafter(1.0, 0), np.nextafter(2.0, 0)]])
+    say("peskin probe: %d points" % x.size)
+    got = _core.diag_peskin(x)
+    ref = np.array([O.peskin_delta(v) for v in x])
+    assert np.array_equal(got, ref)
+
+
+def test_interp_device_bitwise():
+    from lbswim import _core
+    size = (9, 7, 11)
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(3 * 9 * 7 * 11)
+    r = np.vstack([rng.uniform(-1.5, s + 0.5, 3000) for s in size])
+    say("interp probe: lattice %s, %d points" % (size, r.shape[1]))
+    got = _core.diag_interp(size, u, r)
+    lat = O.Lattice(*size, 0.25, 0.25)
+    lat.u[:] = u
+    ref = np.stack([lat.interp(r[:, i]) for i in range(r.shape[1])], axis=1)
+    assert np.array_equal(got, ref)
+
+
+# ------------------------------------------------------------- lattice ops
+
+LATTICE_SIZES = [(4, 5, 6), (7, 5, 3), (1, 1, 1), (2, 1, 2), (16, 16, 16), (3, 70, 65), (5, 1, 130)]
+
+
+@pytest.mark.parametrize("size", LATTICE_SIZES)
+def test_lattice_ops_bitwise(size):
+    from lbswim import lb
+    say("lattice ops bitwise: size %s" % (size,))
+    rng = np.random.default_rng(7)
+    n = size[0] * size[1] * size[2]
+    rho, u = 1.0 + 0.05 * rng.standard_normal(n), 0.05 * rng.standard_normal(3 * n)
+    F = 1e-3 * rng.standard_normal(3 * n)
+    ref = O.Lattice(*size, 0.3, 0.8)
+    ref.rho[:], ref.u[:] = rho, u
+    ref.InitFromHydro()
+    gl = lb.Lattice(*size, 0.3, 0.8)
+    gl.rho.data.ravel()[:] = rho
+    gl.u.data.ravel()[:] = u
+    gl.rho.H2D()
+    gl.u.H2D()
+    gl.InitFromHydro()
+    gl.fOld.D2H()
+    assert np.array_equal(gl.fOld.data.ravel(), ref.fOld), "InitFromHydro"
+    for step in range(3):
+        ref.force[:] = F * (step + 1)
+        gl.force.data.ravel()[:] = F * (step + 1)
+        gl.force.H2D()
+        ref.Step()
+        gl.Step()
+        ref.CalcHydro()
+        gl.CalcHydro()
+        for name in ("fOld", "rho", "u"):
+            getattr(gl, name).D2H()
+            assert np.array_equal(getattr(gl, name).data.ravel(), getattr(ref, name)), (name, step)
+    assert gl.time_step == 3
+    gl.ZeroForce()
+    gl.force.D2H()
+    assert not gl.force.data.any()
+    # fNew holds the previous distributions after the swap (lb.cu:242-244)
+    gl.fNew.D2H()
+    assert np.array_equal(gl.fNew.data.ravel(), ref.fNew)
+
+
+def test_system_step_single_ops_bitwise():
+    """One coupled step with swimmers whose stencils cross the periodic faces."""
+    case = make_case((12, 9, 10), 1, 0.3, 0.6, ns=40, nt=50, alpha=0.0, seed=5, drho=1e-2, du=1e-2, P=1e-2, l=0.7)
+    case.sw_r[:, :4] = np.array([[0.1, 11.9, 0.0, 5.0], [8.95, 0.2, 0.0, 4.0], [9.99, 0.01, 0.0, 3.0]])
+    say("system step bitwise: %s ns=40 nt=50" % (case.size,))
+    ref, _ = run_oracle(O, case)
+    got, _ = run_lbswim(case)
+    # with 40 swimmers some sites get >2 force contributions whose atomic order
+    # is free; everything else is exact
+    compare(got, ref, ("fOld", "rho", "u", "force", "sw_r", "sw_n", "sw_v", "tr_r", "tr_s", "tr_v"), 1e-15,
+            "1 coupled step")
+    assert np.array_equal(got["sw_prng"], ref["sw_prng"])
+
+
+# ------------------------------------------------------------- coupled runs
+
+def test_config1_alpha0_1000_steps():
+    case = config1(1000, alpha=0.0)
+    say("config1 alpha=0: 16^3, 1 swimmer, 4096 tracers, 1000 steps")
+    ref, _ = run_oracle(O, case)
+    got, _ = run_lbswim(case)
+    compare(got, ref, ("fOld", "rho", "u", "force", "sw_r", "sw_n", "sw_v", "tr_r", "tr_v", "tr_s"), PARITY,
+            "config1 alpha=0 after 1000 steps")
+
+
+def test_config1_alpha_tumbling_1000_steps():
+    alpha = 0.01
+    case = config1(1000, alpha=alpha)
+    say("config1 alpha=0.01: 1000 steps")
+    ref, t_ref = run_oracle(O, case)
+    got, t_gpu = run_lbswim(case)
+    # same counter-based stream on both sides: identical tumble decisions
+    assert t_gpu == t_ref
+    assert np.array_equal(got["sw_prng"], ref["sw_prng"])
+    compare(got, ref, ("fOld", "rho", "u", "sw_r", "sw_n", "tr_r"), 1e-10, "config1 alpha=0.01")
+
+
+def test_tumble_rate_binomial_bound():
+    from lbswim import lb, swimmers
+    alpha, ns, steps = 0.01, 20000, 50
+    say("tumble statistics: %d swimmers x %d steps, alpha=%g" % (ns, steps, alpha))
+    lat = lb.Lattice(16, 16, 16, 0.25, 0.25)
+    lat.rho.data[:] = 1.0
+    lat.rho.H2D()
+    lat.InitFromHydro()
+    lat.CalcHydro()
+    cp = swimmers.CommonParams()
+    cp.P, cp.l, cp.alpha, cp.seed = 0.0, 0.1, alpha, 2024
+    sw = swimmers.Array(ns, cp)
+    rng = np.random.default_rng(3)
+    sw.r[:] = rng.uniform(0, 16, (3, ns))
+    sw.n[:] = np.array([0.0, 0.0, 1.0])[:, None]
+    sw.H2D()
+    for _ in range(steps):
+        sw.Move(lat)
+    N = ns * steps
+    t = sw.tumbles()
+    assert abs(t - alpha * N) < 4.0 * np.sqrt(N * alpha * (1 - alpha)), t
+    sw.D2H()
+    turned = sw.n.data[2] != 1.0
+    nn = sw.n.data[:, turned]
+    assert np.abs(nn.mean(axis=1)).max() < 4.0 / np.sqrt(3 * nn.shape[1])
+    np.testing.assert_allclose(np.linalg.norm(sw.n.data, axis=0), 1.0, atol=1e-15)
+
+
+def test_config2_full_size_short_run():
+    """128^3 with 4096 swimmers and 1e5 tracers (the FP64 drop-in config) for
+    10 steps against the oracle."""
+    case = config2(steps=10)
+    say("config2: 128^3, 4096 swimmers, 1e5 tracers, 10 steps")
+    ref, _ = run_oracle(O, case)
+    got, _ = run_lbswim(case)
+    compare(got, ref, ("fOld", "rho", "u", "force", "sw_r", "sw_n", "sw_v", "tr_r", "tr_v"), PARITY,
+            "config2 after 10 steps")
+
+
+def test_config2_reduced_500_steps():
+    """The config2 recipe on a 32^3 lattice for the full 500 steps."""
+    case = config2(steps=500, size=32, ns=64, nt=2000)
+    say("config2 recipe at 32^3: 64 swimmers, 2000 tracers, 500 steps")
+    ref, _ = run_oracle(O, case)
+    got, _ = run_lbswim(case)
+    compare(got, ref, ("fOld", "rho", "u", "sw_r", "sw_n", "sw_v", "tr_r", "tr_v"), PARITY,
+            "config2@32^3 after 500 steps")
+
+
+# ------------------------------------------------------------- FP32 storage
+
+@pytest.mark.parametrize("size,steps", [((32, 32, 32), 200), ((64, 48, 40), 100)])
+def test_f32_storage_error_against_f64(size, steps):
+    case = make_case(size, steps, 0.25, ns=64, nt=500, alpha=0.0, seed=99)
+    say("f32 vs f64: %s, %d steps" % (size, steps))
+    a, _ = run_lbswim(case, "f64")
+    b, _ = run_lbswim(case, "f32")
+    errs = {k: rel_inf(b[k], a[k]) for k in ("u", "rho", "sw_r", "tr_r", "fOld")}
+    du = rel_inf(b["rho"] - 1.0, a["rho"] - 1.0)
+    say("f32 errors: %s  drho %.2e" % ({k: "%.2e" % v for k, v in errs.items()}, du))
+    assert errs["u"] <= F32_U_TOL
+    assert errs["sw_r"] <= 1e-9 and errs["tr_r"] <= 1e-9
+
+
+# ------------------------------------------------------------- properties
+
+def test_conservation_at_256_cubed():
+    """Total mass is conserved and total momentum unchanged (force dipoles sum
+    to zero) at the full 256^3 size, FP64 storage."""
+    case = make_case((256, 256, 256), 0, 0.25, ns=32768, nt=0, alpha=0.0, seed=11)
+    say("conservation: 256^3, 32768 swimmers, 20 steps")
+    lat, system, sw, tr = build_lbswim(case, "f64")
+    t0 = lat.totals()
+    system.Step(20)
+    t1 = lat.totals()
+    say("totals before %s after %s" % (t0, t1))
+    assert abs(t1[0] - t0[0]) / t0[0] < 1e-13
+    assert np.abs(t1[1:] - t0[1:]).max() < 1e-9
+
+
+def test_pickle_continuation_identical():
+    case = make_case((10, 12, 8), 0, 0.25, ns=5, nt=20, alpha=0.0, seed=4)
+    say("pickle continuation")
+    lat, system, sw, tr = build_lbswim(case, "f64")
+    system.Step(7)
+    blob = pickle.dumps((lat, sw, tr))
+    lat2, sw2, tr2 = pickle.loads(blob)
+    lat2.CalcHydro()
+    from lbswim.System import System
+    s2 = System(lat2)
+    s2.AddThing(sw2)
+    s2.AddThing(tr2)
+    system.Step(5)
+    s2.Step(5)
+    a, b = collect_lbswim(lat, sw, tr), collect_lbswim(lat2, sw2, tr2)
+    assert lat2.time_step == lat.time_step == 12
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_empty_particle_arrays_and_python_driver():
+    from lbswim import lb, swimmers, tracers
+    from lbswim.System import System
+    say("empty arrays")
+    lat = lb.Lattice(8, 8, 8, 0.25, 0.25)
+    lat.rho.data[:] = 1.0
+    lat.rho.H2D()
+    lat.InitFromHydro()
+    cp = swimmers.CommonParams()
+    sw = swimmers.Array(0, cp)
+    tr = tracers.Array(0)
+
+    class Passive:  # a non-native "thing" forces the Python driver
+        def AddForces(self, lat):
+            pass
+
+        def Move(self, lat):
+            pass
+
+    s = System(lat)
+    for th in (sw, tr, Passive()):
+        s.AddThing(th)
+    s.Step(3)
+    assert lat.time_step == 3
+    sw.D2H()
+    tr.D2H()
+    assert sw.r.data.shape == (3, 0)

@@ -114,12 +114,21 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="f64,f32")
+    ap.add_argument("--probe", choices=["copy"], help="run the streaming-copy bandwidth probe instead")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if a.probe == "copy":
+        from lbswim import _core
+        count = 1 << 28  # 2 GiB per buffer, 4 GiB of traffic per copy
+        ms = _core.diag_copy(count, 20)
+        gbs = 2 * 8 * count / (ms * 1e-3) / 1e9
+        print(json.dumps({"probe": "copy", "doubles": count, "bytes_per_copy": 16 * count, "avg_ms": round(ms, 4),
+                          "GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+        return
     log("building workload %d^3" % a.size)
     case = make_workload(a.size)
     forms = [f for f in a.forms.split(",") if f]

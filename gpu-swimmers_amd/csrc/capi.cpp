@@ -10,7 +10,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "internal.hpp"
@@ -146,26 +148,49 @@ struct lbswim_lattice {
   bool timing = false;
   std::vector<TimeRec> recs;
   std::vector<hipEvent_t> pool;
+  // workspace of the deterministic spreading in lbswim_system_step
+  SpreadGeom sg{};
+  uint32_t* sp_counts = nullptr;  // ntiles
+  uint32_t* sp_off = nullptr;     // ntiles + 1
+  uint32_t* sp_cursor = nullptr;  // ntiles
+  size_t sp_pts_cap = 0;
+  void* sp_pts = nullptr;  // wc, del, F for sp_pts_cap points
+  size_t sp_list_cap = 0;
+  uint32_t* sp_list = nullptr;
+  uint32_t* sp_sorted = nullptr;
 };
 
-struct lbswim_swimmers {
+// Device storage of one particle array in slot order (particle_order.hip):
+// field f of slot s, component c is at d[f][c*num + s]; perm[s] is the user's
+// index of the particle in slot s.  Every field buffer and the staging buffer
+// hold 3*num 8-byte words, so buffers can trade places after a re-order.
+struct PStore {
   int num = 0;
-  lbswim_common_params common_host{};
-  lbswim_common_params common_dev{};  // the copy kernels see (SharedItem::device)
-  double* d_r = nullptr;
-  double* d_v = nullptr;
-  double* d_n = nullptr;
-  uint64_t* d_prng = nullptr;
-  unsigned long long* d_tumbles = nullptr;
+  uint64_t* d[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool has[5] = {false, false, false, false, false};
+  uint64_t* stage = nullptr;
+  int* perm = nullptr;
+  int* perm_alt = nullptr;
+  uint32_t* keys = nullptr;
+  uint32_t* keys_alt = nullptr;
+  int* idx = nullptr;
+  int* order = nullptr;
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  bool dirty = true;  // positions written from the host since the last ordering
+  int moves_since_order = 0;
   void* h[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
+struct lbswim_swimmers {
+  lbswim_common_params common_host{};
+  lbswim_common_params common_dev{};  // the copy kernels see (SharedItem::device)
+  PStore st;
+  unsigned long long* d_tumbles = nullptr;
+};
+
 struct lbswim_tracers {
-  int num = 0;
-  double* d_r = nullptr;
-  double* d_s = nullptr;
-  double* d_v = nullptr;
-  void* h[4] = {nullptr, nullptr, nullptr, nullptr};
+  PStore st;
 };
 
 namespace {
@@ -329,6 +354,12 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->d_f[0]);
   dev_free(lat->d_f[1]);
   dev_free(lat->d_partial);
+  dev_free(lat->sp_counts);
+  dev_free(lat->sp_off);
+  dev_free(lat->sp_cursor);
+  dev_free(lat->sp_pts);
+  dev_free(lat->sp_list);
+  dev_free(lat->sp_sorted);
   for (double* h : lat->h) std::free(h);
   for (auto& r : lat->recs) {
     (void)hipEventDestroy(r.start);
@@ -483,6 +514,106 @@ int lbswim_lattice_d2h(lbswim_lattice* lat, int field) {
   return LBSWIM_OK;
 }
 
+// ---- particle storage ----------------------------------------------------------
+
+namespace {
+
+// Moves between re-orderings of a particle array's storage.
+constexpr int REORDER_INTERVAL = 64;
+
+void pstore_free(PStore& st) {
+  for (auto*& p : st.d) dev_free(p), p = nullptr;
+  dev_free(st.stage);
+  dev_free(st.perm);
+  dev_free(st.perm_alt);
+  dev_free(st.keys);
+  dev_free(st.keys_alt);
+  dev_free(st.idx);
+  dev_free(st.order);
+  dev_free(st.sort_tmp);
+  for (void* h : st.h) std::free(h);
+}
+
+int pstore_alloc(PStore& st, int num, std::initializer_list<int> fields) {
+  st.num = num;
+  const size_t words = 3 * size_t(num);
+  int rc;
+  for (int f : fields) {
+    st.has[f] = true;
+    if ((rc = dev_alloc_zero(reinterpret_cast<void**>(&st.d[f]), 8 * words))) return rc;
+  }
+  if ((rc = dev_alloc_zero(reinterpret_cast<void**>(&st.stage), 8 * words)) ||
+      (rc = dev_alloc_zero(reinterpret_cast<void**>(&st.perm), 4 * size_t(num))) ||
+      (rc = dev_alloc_zero(reinterpret_cast<void**>(&st.perm_alt), 4 * size_t(num))) ||
+      (rc = dev_alloc_zero(reinterpret_cast<void**>(&st.keys), 4 * size_t(num))) ||
+      (rc = dev_alloc_zero(reinterpret_cast<void**>(&st.keys_alt), 4 * size_t(num))) ||
+      (rc = dev_alloc_zero(reinterpret_cast<void**>(&st.idx), 4 * size_t(num))) ||
+      (rc = dev_alloc_zero(reinterpret_cast<void**>(&st.order), 4 * size_t(num))))
+    return rc;
+  if (num > 0) {
+    HIP_TRY(sort_scratch_bytes(num, &st.sort_tmp_bytes));
+    if ((rc = dev_alloc_zero(&st.sort_tmp, st.sort_tmp_bytes ? st.sort_tmp_bytes : 1))) return rc;
+    HIP_TRY(launch_iota(num, st.perm, stream()));
+  }
+  return LBSWIM_OK;
+}
+
+int pstore_ncomp(int field) { return field == LBSWIM_PART_PRNG ? 1 : 3; }
+
+int pstore_host(PStore& st, int field, void** out) {
+  if (!st.h[field]) {
+    const size_t cnt = size_t(pstore_ncomp(field)) * st.num;
+    st.h[field] = std::calloc(cnt ? cnt : 1, 8);
+    if (!st.h[field]) return fail(LBSWIM_ENOMEM, "host mirror allocation failed");
+  }
+  *out = st.h[field];
+  return LBSWIM_OK;
+}
+
+// Host (user order) -> device (slot order).
+int pstore_h2d(PStore& st, int field) {
+  void* h = nullptr;
+  if (int rc = pstore_host(st, field, &h)) return rc;
+  const int nc = pstore_ncomp(field);
+  if (int rc = copy_h2d(st.stage, h, 8 * size_t(nc) * st.num)) return rc;
+  HIP_TRY(launch_gather<uint64_t>(st.num, nc, st.perm, st.stage, st.d[field], stream()));
+  HIP_TRY(hipStreamSynchronize(stream()));
+  if (field == LBSWIM_PART_R) st.dirty = true;
+  return LBSWIM_OK;
+}
+
+// Device (slot order) -> host (user order).
+int pstore_d2h(PStore& st, int field) {
+  void* h = nullptr;
+  if (int rc = pstore_host(st, field, &h)) return rc;
+  const int nc = pstore_ncomp(field);
+  HIP_TRY(launch_scatter<uint64_t>(st.num, nc, st.perm, st.d[field], st.stage, stream()));
+  return copy_d2h(h, st.stage, 8 * size_t(nc) * st.num);
+}
+
+// Re-order the storage by the Morton code of each particle's site when the
+// positions came from the host or REORDER_INTERVAL moves have passed.
+int pstore_maybe_reorder(PStore& st, const Geom& g) {
+  if (st.num <= 1 || (!st.dirty && st.moves_since_order < REORDER_INTERVAL)) return LBSWIM_OK;
+  const double* r = reinterpret_cast<const double*>(st.d[LBSWIM_PART_R]);
+  HIP_TRY(launch_spatial_order(st.num, r, g, st.keys, st.keys_alt, st.idx, st.order, st.sort_tmp,
+                               st.sort_tmp_bytes, stream()));
+  for (int f = 0; f < 5; ++f) {
+    if (!st.has[f]) continue;
+    HIP_TRY(launch_gather<uint64_t>(st.num, pstore_ncomp(f), st.order, st.d[f], st.stage, stream()));
+    std::swap(st.d[f], st.stage);
+  }
+  HIP_TRY(launch_gather<int>(st.num, 1, st.order, st.perm, st.perm_alt, stream()));
+  std::swap(st.perm, st.perm_alt);
+  st.dirty = false;
+  st.moves_since_order = 0;
+  return LBSWIM_OK;
+}
+
+double* fld(PStore& st, int f) { return reinterpret_cast<double*>(st.d[f]); }
+
+}  // namespace
+
 // ---- swimmers ---------------------------------------------------------------
 
 int lbswim_swimmers_create(int n, const lbswim_common_params* p, lbswim_swimmers** out) {
@@ -494,20 +625,16 @@ int lbswim_swimmers_create(int n, const lbswim_common_params* p, lbswim_swimmers
   if (rc) return rc;
   lbswim_swimmers* sw = new (std::nothrow) lbswim_swimmers();
   if (!sw) return fail(LBSWIM_ENOMEM, "swimmer array allocation failed");
-  sw->num = n;
   sw->common_host = *p;
   sw->common_dev = *p;
-  const size_t v3 = 3 * sizeof(double) * size_t(n);
-  if ((rc = dev_alloc_zero(reinterpret_cast<void**>(&sw->d_r), v3)) ||
-      (rc = dev_alloc_zero(reinterpret_cast<void**>(&sw->d_v), v3)) ||
-      (rc = dev_alloc_zero(reinterpret_cast<void**>(&sw->d_n), v3)) ||
-      (rc = dev_alloc_zero(reinterpret_cast<void**>(&sw->d_prng), sizeof(uint64_t) * size_t(n))) ||
+  if ((rc = pstore_alloc(sw->st, n, {LBSWIM_PART_R, LBSWIM_PART_V, LBSWIM_PART_N, LBSWIM_PART_PRNG})) ||
       (rc = dev_alloc_zero(reinterpret_cast<void**>(&sw->d_tumbles), sizeof(unsigned long long)))) {
     std::string msg = g_err;
     lbswim_swimmers_destroy(sw);
     g_err = msg;
     return rc;
   }
+  HIP_TRY(hipStreamSynchronize(stream()));
   *out = sw;
   return LBSWIM_OK;
 }
@@ -515,18 +642,14 @@ int lbswim_swimmers_create(int n, const lbswim_common_params* p, lbswim_swimmers
 void lbswim_swimmers_destroy(lbswim_swimmers* sw) {
   if (!sw) return;
   if (g_rt.ready) (void)hipStreamSynchronize(stream());
-  dev_free(sw->d_r);
-  dev_free(sw->d_v);
-  dev_free(sw->d_n);
-  dev_free(sw->d_prng);
+  pstore_free(sw->st);
   dev_free(sw->d_tumbles);
-  for (void* h : sw->h) std::free(h);
   delete sw;
 }
 
 int lbswim_swimmers_num(const lbswim_swimmers* sw, int* out) {
   CHECK_ARG(sw && out, "NULL argument");
-  *out = sw->num;
+  *out = sw->st.num;
   return LBSWIM_OK;
 }
 
@@ -553,43 +676,44 @@ int lbswim_swimmers_add_forces(lbswim_swimmers* sw, lbswim_lattice* lat) {
   if (int rc = check_lattice(lat)) return rc;
   KernelTimer t(lat, LBSWIM_K_ADDF);
   const SwimParams p = to_kernel(sw->common_dev);
+  PStore& st = sw->st;
   hipError_t e;
   if (lat->storage == LBSWIM_STORE_F64)
-    e = launch_swimmers_add_forces<double>(sw->num, p, sw->d_r, sw->d_n, static_cast<double*>(lat->d_force),
-                                           lat->geom, stream());
+    e = launch_swimmers_add_forces<double>(st.num, p, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_N),
+                                           static_cast<double*>(lat->d_force), lat->geom, stream());
   else
-    e = launch_swimmers_add_forces<float>(sw->num, p, sw->d_r, sw->d_n, static_cast<float*>(lat->d_force),
-                                          lat->geom, stream());
+    e = launch_swimmers_add_forces<float>(st.num, p, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_N),
+                                          static_cast<float*>(lat->d_force), lat->geom, stream());
   t.done();
   HIP_TRY(e);
+  return LBSWIM_OK;
+}
+
+static int swimmers_move_impl(lbswim_swimmers* sw, lbswim_lattice* lat) {
+  PStore& st = sw->st;
+  if (int rc = pstore_maybe_reorder(st, lat->geom)) return rc;
+  KernelTimer t(lat, LBSWIM_K_SMOVE);
+  const SwimParams p = to_kernel(sw->common_dev);
+  uint64_t* prng = st.d[LBSWIM_PART_PRNG];
+  hipError_t e;
+  if (lat->storage == LBSWIM_STORE_F64)
+    e = launch_swimmers_move<double>(st.num, p, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_V),
+                                     fld(st, LBSWIM_PART_N), prng, st.perm, static_cast<const double*>(lat->d_u),
+                                     lat->geom, sw->d_tumbles, stream());
+  else
+    e = launch_swimmers_move<float>(st.num, p, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_V),
+                                    fld(st, LBSWIM_PART_N), prng, st.perm, static_cast<const float*>(lat->d_u),
+                                    lat->geom, sw->d_tumbles, stream());
+  t.done();
+  HIP_TRY(e);
+  st.moves_since_order++;
   return LBSWIM_OK;
 }
 
 int lbswim_swimmers_move(lbswim_swimmers* sw, lbswim_lattice* lat) {
   CHECK_ARG(sw, "swimmer handle is NULL");
   if (int rc = check_lattice(lat)) return rc;
-  KernelTimer t(lat, LBSWIM_K_SMOVE);
-  const SwimParams p = to_kernel(sw->common_dev);
-  hipError_t e;
-  if (lat->storage == LBSWIM_STORE_F64)
-    e = launch_swimmers_move<double>(sw->num, p, sw->d_r, sw->d_v, sw->d_n, sw->d_prng,
-                                     static_cast<const double*>(lat->d_u), lat->geom, sw->d_tumbles, stream());
-  else
-    e = launch_swimmers_move<float>(sw->num, p, sw->d_r, sw->d_v, sw->d_n, sw->d_prng,
-                                    static_cast<const float*>(lat->d_u), lat->geom, sw->d_tumbles, stream());
-  t.done();
-  HIP_TRY(e);
-  return LBSWIM_OK;
-}
-
-static void* swimmer_dev(lbswim_swimmers* sw, int field) {
-  switch (field) {
-    case LBSWIM_PART_R: return sw->d_r;
-    case LBSWIM_PART_V: return sw->d_v;
-    case LBSWIM_PART_N: return sw->d_n;
-    case LBSWIM_PART_PRNG: return sw->d_prng;
-    default: return nullptr;
-  }
+  return swimmers_move_impl(sw, lat);
 }
 
 static int swimmer_check_field(int field) {
@@ -599,29 +723,15 @@ static int swimmer_check_field(int field) {
   return LBSWIM_OK;
 }
 
-static size_t swimmer_count(const lbswim_swimmers* sw, int field) {
-  return field == LBSWIM_PART_PRNG ? size_t(sw->num) : 3 * size_t(sw->num);
-}
-
-static int swimmer_host(lbswim_swimmers* sw, int field, void** out) {
-  if (!sw->h[field]) {
-    const size_t cnt = swimmer_count(sw, field);
-    sw->h[field] = std::calloc(cnt ? cnt : 1, 8);
-    if (!sw->h[field]) return fail(LBSWIM_ENOMEM, "host mirror allocation failed");
-  }
-  *out = sw->h[field];
-  return LBSWIM_OK;
-}
-
 int lbswim_swimmers_field(lbswim_swimmers* sw, int field, void** host, void** device, size_t* count,
                           size_t* elem_size) {
   CHECK_ARG(sw, "swimmer handle is NULL");
   if (int rc = swimmer_check_field(field)) return rc;
   if (host) {
-    if (int rc = swimmer_host(sw, field, host)) return rc;
+    if (int rc = pstore_host(sw->st, field, host)) return rc;
   }
-  if (device) *device = swimmer_dev(sw, field);
-  if (count) *count = swimmer_count(sw, field);
+  if (device) *device = sw->st.d[field];
+  if (count) *count = size_t(pstore_ncomp(field)) * sw->st.num;
   if (elem_size) *elem_size = 8;
   return LBSWIM_OK;
 }
@@ -629,17 +739,13 @@ int lbswim_swimmers_field(lbswim_swimmers* sw, int field, void** host, void** de
 int lbswim_swimmers_h2d(lbswim_swimmers* sw, int field) {
   CHECK_ARG(sw, "swimmer handle is NULL");
   if (int rc = swimmer_check_field(field)) return rc;
-  void* h = nullptr;
-  if (int rc = swimmer_host(sw, field, &h)) return rc;
-  return copy_h2d(swimmer_dev(sw, field), h, 8 * swimmer_count(sw, field));
+  return pstore_h2d(sw->st, field);
 }
 
 int lbswim_swimmers_d2h(lbswim_swimmers* sw, int field) {
   CHECK_ARG(sw, "swimmer handle is NULL");
   if (int rc = swimmer_check_field(field)) return rc;
-  void* h = nullptr;
-  if (int rc = swimmer_host(sw, field, &h)) return rc;
-  return copy_d2h(h, swimmer_dev(sw, field), 8 * swimmer_count(sw, field));
+  return pstore_d2h(sw->st, field);
 }
 
 int lbswim_swimmers_tumbles(lbswim_swimmers* sw, unsigned long long* out) {
@@ -657,16 +763,13 @@ int lbswim_tracers_create(int n, lbswim_tracers** out) {
   if (rc) return rc;
   lbswim_tracers* tr = new (std::nothrow) lbswim_tracers();
   if (!tr) return fail(LBSWIM_ENOMEM, "tracer array allocation failed");
-  tr->num = n;
-  const size_t v3 = 3 * sizeof(double) * size_t(n);
-  if ((rc = dev_alloc_zero(reinterpret_cast<void**>(&tr->d_r), v3)) ||
-      (rc = dev_alloc_zero(reinterpret_cast<void**>(&tr->d_s), v3)) ||
-      (rc = dev_alloc_zero(reinterpret_cast<void**>(&tr->d_v), v3))) {
+  if ((rc = pstore_alloc(tr->st, n, {LBSWIM_PART_R, LBSWIM_PART_S, LBSWIM_PART_V}))) {
     std::string msg = g_err;
     lbswim_tracers_destroy(tr);
     g_err = msg;
     return rc;
   }
+  HIP_TRY(hipStreamSynchronize(stream()));
   *out = tr;
   return LBSWIM_OK;
 }
@@ -674,81 +777,141 @@ int lbswim_tracers_create(int n, lbswim_tracers** out) {
 void lbswim_tracers_destroy(lbswim_tracers* tr) {
   if (!tr) return;
   if (g_rt.ready) (void)hipStreamSynchronize(stream());
-  dev_free(tr->d_r);
-  dev_free(tr->d_s);
-  dev_free(tr->d_v);
-  for (void* h : tr->h) std::free(h);
+  pstore_free(tr->st);
   delete tr;
 }
 
 int lbswim_tracers_num(const lbswim_tracers* tr, int* out) {
   CHECK_ARG(tr && out, "NULL argument");
-  *out = tr->num;
+  *out = tr->st.num;
+  return LBSWIM_OK;
+}
+
+static int tracers_move_impl(lbswim_tracers* tr, lbswim_lattice* lat) {
+  PStore& st = tr->st;
+  if (int rc = pstore_maybe_reorder(st, lat->geom)) return rc;
+  KernelTimer t(lat, LBSWIM_K_TMOVE);
+  hipError_t e;
+  if (lat->storage == LBSWIM_STORE_F64)
+    e = launch_tracers_move<double>(st.num, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_S), fld(st, LBSWIM_PART_V),
+                                    static_cast<const double*>(lat->d_u), lat->geom, stream());
+  else
+    e = launch_tracers_move<float>(st.num, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_S), fld(st, LBSWIM_PART_V),
+                                   static_cast<const float*>(lat->d_u), lat->geom, stream());
+  t.done();
+  HIP_TRY(e);
+  st.moves_since_order++;
   return LBSWIM_OK;
 }
 
 int lbswim_tracers_move(lbswim_tracers* tr, lbswim_lattice* lat) {
   CHECK_ARG(tr, "tracer handle is NULL");
   if (int rc = check_lattice(lat)) return rc;
-  KernelTimer t(lat, LBSWIM_K_TMOVE);
-  hipError_t e;
-  if (lat->storage == LBSWIM_STORE_F64)
-    e = launch_tracers_move<double>(tr->num, tr->d_r, tr->d_s, tr->d_v, static_cast<const double*>(lat->d_u),
-                                    lat->geom, stream());
-  else
-    e = launch_tracers_move<float>(tr->num, tr->d_r, tr->d_s, tr->d_v, static_cast<const float*>(lat->d_u),
-                                   lat->geom, stream());
-  t.done();
-  HIP_TRY(e);
-  return LBSWIM_OK;
+  return tracers_move_impl(tr, lat);
 }
 
-static double* tracer_dev(lbswim_tracers* tr, int field) {
-  switch (field) {
-    case LBSWIM_PART_R: return tr->d_r;
-    case LBSWIM_PART_S: return tr->d_s;
-    case LBSWIM_PART_V: return tr->d_v;
-    default: return nullptr;
-  }
-}
-
-static int tracer_host(lbswim_tracers* tr, int field, void** out) {
+static int tracer_check_field(int field) {
   CHECK_ARG(field == LBSWIM_PART_R || field == LBSWIM_PART_S || field == LBSWIM_PART_V, "unknown tracer field %d",
             field);
-  if (!tr->h[field]) {
-    const size_t cnt = 3 * size_t(tr->num);
-    tr->h[field] = std::calloc(cnt ? cnt : 1, sizeof(double));
-    if (!tr->h[field]) return fail(LBSWIM_ENOMEM, "host mirror allocation failed");
-  }
-  *out = tr->h[field];
   return LBSWIM_OK;
 }
 
 int lbswim_tracers_field(lbswim_tracers* tr, int field, void** host, void** device, size_t* count) {
   CHECK_ARG(tr, "tracer handle is NULL");
-  void* h = nullptr;
-  if (int rc = tracer_host(tr, field, &h)) return rc;
-  if (host) *host = h;
-  if (device) *device = tracer_dev(tr, field);
-  if (count) *count = 3 * size_t(tr->num);
+  if (int rc = tracer_check_field(field)) return rc;
+  if (host) {
+    if (int rc = pstore_host(tr->st, field, host)) return rc;
+  }
+  if (device) *device = tr->st.d[field];
+  if (count) *count = 3 * size_t(tr->st.num);
   return LBSWIM_OK;
 }
 
 int lbswim_tracers_h2d(lbswim_tracers* tr, int field) {
   CHECK_ARG(tr, "tracer handle is NULL");
-  void* h = nullptr;
-  if (int rc = tracer_host(tr, field, &h)) return rc;
-  return copy_h2d(tracer_dev(tr, field), h, 3 * sizeof(double) * size_t(tr->num));
+  if (int rc = tracer_check_field(field)) return rc;
+  return pstore_h2d(tr->st, field);
 }
 
 int lbswim_tracers_d2h(lbswim_tracers* tr, int field) {
   CHECK_ARG(tr, "tracer handle is NULL");
-  void* h = nullptr;
-  if (int rc = tracer_host(tr, field, &h)) return rc;
-  return copy_d2h(h, tracer_dev(tr, field), 3 * sizeof(double) * size_t(tr->num));
+  if (int rc = tracer_check_field(field)) return rc;
+  return pstore_d2h(tr->st, field);
 }
 
 // ---- coupled step -------------------------------------------------------------
+
+namespace {
+
+int grow(void** p, size_t* cap, size_t need, size_t elem) {
+  if (*cap >= need) return LBSWIM_OK;
+  HIP_TRY(hipStreamSynchronize(stream()));
+  dev_free(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (int rc = dev_alloc_zero(p, elem * need)) return rc;
+  *cap = need;
+  return LBSWIM_OK;
+}
+
+// ZeroForce + AddForces of every swimmer array, as one deterministic gather.
+int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
+  size_t npts = 0;
+  for (int i = 0; i < nsw; ++i) npts += 2 * size_t(sw[i]->st.num);
+  if (npts == 0) return lattice_zero_force_impl(lat);
+  CHECK_ARG(npts <= size_t(INT_MAX), "too many swimmers in one system");
+  if (!lat->sp_counts) {
+    lat->sg = make_spread_geom(lat->geom);
+    const size_t nt = size_t(lat->sg.ntiles);
+    int rc;
+    if ((rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_counts), 4 * nt)) ||
+        (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_off), 4 * (nt + 1))) ||
+        (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_cursor), 4 * nt)))
+      return rc;
+  }
+  // one allocation for wc (12 ints), del (12 doubles) and F (3 doubles) per point
+  const size_t per_pt = 12 * 4 + 12 * 8 + 3 * 8;
+  size_t cap_bytes = lat->sp_pts_cap * per_pt;
+  if (lat->sp_pts_cap < npts) {
+    if (int rc = grow(&lat->sp_pts, &cap_bytes, npts * per_pt, 1)) return rc;
+    lat->sp_pts_cap = npts;
+  }
+  const size_t lcap = spread_list_capacity(lat->sg, npts);
+  if (lat->sp_list_cap < lcap) {
+    size_t c1 = lat->sp_list_cap, c2 = lat->sp_list_cap;
+    if (int rc = grow(reinterpret_cast<void**>(&lat->sp_list), &c1, lcap, 4)) return rc;
+    if (int rc = grow(reinterpret_cast<void**>(&lat->sp_sorted), &c2, lcap, 4)) return rc;
+    lat->sp_list_cap = lcap;
+  }
+  PointBuf pb;
+  pb.cap = lat->sp_pts_cap;
+  char* base = static_cast<char*>(lat->sp_pts);
+  pb.del = reinterpret_cast<double*>(base);
+  pb.F = reinterpret_cast<double*>(base + 12 * 8 * pb.cap);
+  pb.wc = reinterpret_cast<int*>(base + 15 * 8 * pb.cap);
+
+  KernelTimer t(lat, LBSWIM_K_ADDF);
+  HIP_TRY(launch_fill_zero(lat->sp_counts, 4 * size_t(lat->sg.ntiles), stream()));
+  int offset = 0;
+  for (int i = 0; i < nsw; ++i) {
+    PStore& st = sw[i]->st;
+    HIP_TRY(launch_points(st.num, offset, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R),
+                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, lat->sg, pb, lat->sp_counts, stream()));
+    offset += st.num;
+  }
+  hipError_t e;
+  if (lat->storage == LBSWIM_STORE_F64)
+    e = launch_spread<double>(int(npts), lat->geom, lat->sg, pb, lat->sp_counts, lat->sp_off, lat->sp_cursor,
+                              lat->sp_list, lat->sp_sorted, static_cast<double*>(lat->d_force), stream());
+  else
+    e = launch_spread<float>(int(npts), lat->geom, lat->sg, pb, lat->sp_counts, lat->sp_off, lat->sp_cursor,
+                             lat->sp_list, lat->sp_sorted, static_cast<float*>(lat->d_force), stream());
+  t.done();
+  HIP_TRY(e);
+  return LBSWIM_OK;
+}
+
+}  // namespace
 
 int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr, int ntr,
                        int nsteps) {
@@ -760,15 +923,13 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
   for (int i = 0; i < ntr; ++i) CHECK_ARG(tr[i] != nullptr, "tracer handle %d is NULL", i);
   for (int s = 0; s < nsteps; ++s) {
     int rc;
-    if ((rc = lattice_zero_force_impl(lat))) return rc;
-    for (int i = 0; i < nsw; ++i)
-      if ((rc = lbswim_swimmers_add_forces(sw[i], lat))) return rc;
+    if ((rc = spread_forces(lat, sw, nsw))) return rc;
     if ((rc = lattice_step_impl(lat))) return rc;
     if ((rc = lattice_hydro_impl(lat, false))) return rc;
     for (int i = 0; i < nsw; ++i)
-      if ((rc = lbswim_swimmers_move(sw[i], lat))) return rc;
+      if ((rc = swimmers_move_impl(sw[i], lat))) return rc;
     for (int i = 0; i < ntr; ++i)
-      if ((rc = lbswim_tracers_move(tr[i], lat))) return rc;
+      if ((rc = tracers_move_impl(tr[i], lat))) return rc;
   }
   return LBSWIM_OK;
 }

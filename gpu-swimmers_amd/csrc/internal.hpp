@@ -49,11 +49,51 @@ hipError_t launch_swimmers_add_forces(int ns, const SwimParams& p, const double*
                                       T* force, const Geom& g, hipStream_t s);
 template <typename T>
 hipError_t launch_swimmers_move(int ns, const SwimParams& p, double* r, double* v, double* n,
-                                uint64_t* prng, const T* u, const Geom& g,
+                                uint64_t* prng, const int* perm, const T* u, const Geom& g,
                                 unsigned long long* tumbles, hipStream_t s);
 template <typename T>
 hipError_t launch_tracers_move(int nt, double* r, double* sdisp, double* v, const T* u, const Geom& g,
                                hipStream_t s);
+
+// ---- deterministic spreading (ZeroForce + AddForces of all swimmer arrays)
+// Lattice tiles of tx*ty*tz <= 512 sites; each stencil touches at most
+// min(4, tiles) tiles per dimension.
+struct SpreadGeom {
+  int tx, ty, tz;
+  int ntx, nty, ntz;
+  int ntiles;
+};
+// Per-point stencil data by point id (capacity `cap`): wrapped site
+// coordinate wc[(d*4+k)*cap + id], Peskin weight del[(d*4+k)*cap + id], force
+// F[d*cap + id].  Point id = 2*(array base + user index) + (0 tail, 1 head).
+struct PointBuf {
+  int* wc;
+  double* del;
+  double* F;
+  size_t cap;
+};
+SpreadGeom make_spread_geom(const Geom& g);
+size_t spread_list_capacity(const SpreadGeom& sg, size_t npts);
+// counts[ntiles] must be zero before the first launch_points of a step.
+hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
+                         const Geom& g, const SpreadGeom& sg, const PointBuf& pb, uint32_t* counts, hipStream_t s);
+// off[ntiles+1], cursor[ntiles], list/sorted[spread_list_capacity]
+template <typename T>
+hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const PointBuf& pb, uint32_t* counts,
+                         uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, T* force, hipStream_t s);
+
+// ---- spatial ordering of particle storage (particle_order.hip)
+hipError_t launch_iota(int np, int* p, hipStream_t s);
+hipError_t sort_scratch_bytes(int np, size_t* bytes);
+// order_out[t] = slot that moves to position t (stable Morton order of floor(r))
+hipError_t launch_spatial_order(int np, const double* r, const Geom& g, uint32_t* keys, uint32_t* keys_alt, int* idx,
+                                int* order_out, void* scratch, size_t scratch_bytes, hipStream_t s);
+// dst[c*np + t] = src[c*np + order[t]]
+template <typename T>
+hipError_t launch_gather(int np, int ncomp, const int* order, const T* src, T* dst, hipStream_t s);
+// dst[c*np + order[t]] = src[c*np + t]
+template <typename T>
+hipError_t launch_scatter(int np, int ncomp, const int* order, const T* src, T* dst, hipStream_t s);
 
 // ---- diagnostics
 hipError_t launch_copy(const double* src, double* dst, size_t count, hipStream_t s);

@@ -216,10 +216,12 @@ __global__ __launch_bounds__(BLOCK) void k_swimmers_add_forces(int ns, double P,
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_swimmers_move(int ns, SwimParams cp, double* __restrict__ r,
                                                          double* __restrict__ v, double* __restrict__ nv,
-                                                         uint64_t* __restrict__ prng, const T* __restrict__ u,
-                                                         Geom g, unsigned long long* __restrict__ tumbles) {
+                                                         uint64_t* __restrict__ prng, const int* __restrict__ perm,
+                                                         const T* __restrict__ u, Geom g,
+                                                         unsigned long long* __restrict__ tumbles) {
   const int is = int(blockIdx.x * BLOCK + threadIdx.x);
   if (is >= ns) return;
+  const int user = perm[is];  // the random stream belongs to the swimmer, not its storage slot
   const int size[3] = {g.nx, g.ny, g.nz};
   double R[3], N[3];
 #pragma unroll
@@ -243,12 +245,12 @@ __global__ __launch_bounds__(BLOCK) void k_swimmers_move(int ns, SwimParams cp, 
     r[size_t(ns) * d + is] = fmod(R[d] + size[d], double(size[d]));
   }
   const uint64_t ctr = prng[is];
-  const float U = u01f(draw(cp.seed, is, ctr, 0u).x[0]);
+  const float U = u01f(draw(cp.seed, user, ctr, 0u).x[0]);
   double newn[3];
   double norm = 0.0;
   bool turned = true;
   if (double(U) < cp.alpha) {
-    normals3(cp.seed, is, ctr, newn);
+    normals3(cp.seed, user, ctr, newn);
 #pragma unroll
     for (int d = 0; d < 3; ++d) norm += newn[d] * newn[d];
     atomicAdd(tumbles, 1ull);
@@ -294,6 +296,219 @@ __global__ __launch_bounds__(BLOCK) void k_tracers_move(int nt, double* __restri
   }
 }
 
+// ---- deterministic force spreading -----------------------------------------
+// ZeroForce followed by AddForces of every swimmer array, computed as a gather
+// per lattice tile so that each site adds its contributions in swimmer order,
+// tail before head -- the order a sequential restatement of
+// SwimmerArray.cu:42-112 uses -- without atomics on the force field.
+//   1. k_points: per swimmer, the tail and head stencils (wrapped site
+//      coordinates and Peskin weights, exactly as AccumulateDeltaForce forms
+//      them) are stored by point id = 2*(array base + user index) + end, and
+//      every tile a stencil touches is counted;
+//   2. k_scan_tiles: exclusive scan of the counts (one workgroup);
+//   3. k_fill_lists: point ids appended to their tiles' lists;
+//   4. k_sort_lists: each list ordered by point id (rank sort);
+//   5. k_spread_tiles: one workgroup per tile sums, for each site, the
+//      contributions of its tile's points in id order and writes all three
+//      force components of every site once (zeros where no point reaches).
+
+__device__ __forceinline__ int tiles_of(const int (&wc)[4], int tdim, int ntd, int (&out)[4]) {
+  int m = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = wc[k] / tdim;
+    bool seen = false;
+    for (int q = 0; q < m; ++q) seen = seen || (out[q] == t);
+    if (!seen) out[m++] = t;
+  }
+  (void)ntd;
+  return m;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_points(int ns, int base, double P, double l, const double* __restrict__ r,
+                                                  const double* __restrict__ nv, const int* __restrict__ perm,
+                                                  Geom g, SpreadGeom sg, PointBuf pb, uint32_t* __restrict__ counts) {
+  const int slot = int(blockIdx.x * BLOCK + threadIdx.x);
+  if (slot >= ns) return;
+  const int size[3] = {g.nx, g.ny, g.nz};
+  const int tdim[3] = {sg.tx, sg.ty, sg.tz};
+  const int ntd[3] = {sg.ntx, sg.nty, sg.ntz};
+  const int user = perm[slot];
+  double ra[3], na[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    ra[a] = r[size_t(ns) * a + slot];
+    na[a] = nv[size_t(ns) * a + slot];
+  }
+  double Ft[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) Ft[a] = -P * na[a];
+  for (int end = 0; end < 2; ++end) {
+    const size_t id = 2 * (size_t(base) + size_t(user)) + end;
+    double rp[3], Fp[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      rp[a] = end ? ra[a] : ra[a] - na[a] * l;
+      Fp[a] = end ? Ft[a] * -1.0 : Ft[a];
+    }
+    int tl[3][4], nt[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double x0 = ceil(rp[d] - 2.);
+      int wc[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double x = x0 + k;
+        wc[k] = int(wrap_index(x, size[d]));
+        pb.wc[size_t(d * 4 + k) * pb.cap + id] = wc[k];
+        pb.del[size_t(d * 4 + k) * pb.cap + id] = peskin(rp[d] - x);
+      }
+      nt[d] = tiles_of(wc, tdim[d], ntd[d], tl[d]);
+      pb.F[size_t(d) * pb.cap + id] = Fp[d];
+    }
+    for (int a = 0; a < nt[0]; ++a)
+      for (int b = 0; b < nt[1]; ++b)
+        for (int c = 0; c < nt[2]; ++c)
+          atomicAdd(&counts[(size_t(tl[0][a]) * sg.nty + tl[1][b]) * sg.ntz + tl[2][c]], 1u);
+  }
+}
+
+// Exclusive scan of counts[0..nt) into off[0..nt], one workgroup; also clears cursor.
+__global__ __launch_bounds__(1024) void k_scan_tiles(int nt, const uint32_t* __restrict__ counts,
+                                                     uint32_t* __restrict__ off, uint32_t* __restrict__ cursor) {
+  __shared__ uint32_t part[1024];
+  const int t = int(threadIdx.x);
+  const int chunk = (nt + 1023) / 1024;
+  const int lo = t * chunk, hi = min(nt, lo + chunk);
+  uint32_t sum = 0;
+  for (int i = lo; i < hi; ++i) sum += counts[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int w = 1; w < 1024; w <<= 1) {
+    const uint32_t v = t >= w ? part[t - w] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+  for (int i = lo; i < hi; ++i) {
+    off[i] = run;
+    run += counts[i];
+    cursor[i] = 0u;
+  }
+  if (t == 1023) off[nt] = part[1023];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_lists(int npts, SpreadGeom sg, PointBuf pb,
+                                                      const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
+                                                      uint32_t* __restrict__ list) {
+  const int id = int(blockIdx.x * BLOCK + threadIdx.x);
+  if (id >= npts) return;
+  const int tdim[3] = {sg.tx, sg.ty, sg.tz};
+  const int ntd[3] = {sg.ntx, sg.nty, sg.ntz};
+  int tl[3][4], nt[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int wc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wc[k] = pb.wc[size_t(d * 4 + k) * pb.cap + id];
+    nt[d] = tiles_of(wc, tdim[d], ntd[d], tl[d]);
+  }
+  for (int a = 0; a < nt[0]; ++a)
+    for (int b = 0; b < nt[1]; ++b)
+      for (int c = 0; c < nt[2]; ++c) {
+        const size_t tile = (size_t(tl[0][a]) * sg.nty + tl[1][b]) * sg.ntz + tl[2][c];
+        const uint32_t pos = atomicAdd(&cursor[tile], 1u);
+        list[off[tile] + pos] = uint32_t(id);
+      }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sort_lists(const uint32_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ list,
+                                                      uint32_t* __restrict__ sorted) {
+  const uint32_t o = off[blockIdx.x], cnt = off[blockIdx.x + 1] - o;
+  for (uint32_t e = threadIdx.x; e < cnt; e += BLOCK) {
+    const uint32_t id = list[o + e];
+    uint32_t rank = 0;
+    for (uint32_t k = 0; k < cnt; ++k) rank += list[o + k] < id ? 1u : 0u;
+    sorted[o + rank] = id;
+  }
+}
+
+constexpr int SPREAD_CHUNK = 128;
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_spread_tiles(Geom g, SpreadGeom sg, PointBuf pb,
+                                                        const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ sorted, T* __restrict__ force) {
+  __shared__ int swc[SPREAD_CHUNK][12];
+  __shared__ double sdel[SPREAD_CHUNK][12];
+  __shared__ double sF[SPREAD_CHUNK][3];
+  const int tile = int(blockIdx.x);
+  const int tk = tile % sg.ntz, tj = (tile / sg.ntz) % sg.nty, ti = tile / (sg.ntz * sg.nty);
+  const uint32_t o = off[tile], cnt = off[tile + 1] - o;
+  const int tsites = sg.tx * sg.ty * sg.tz;
+  constexpr int PER = 2;  // sites per thread; tiles hold at most 512 sites
+  int X[PER], Y[PER], Z[PER];
+  bool ok[PER];
+  double acc[PER][3];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int L = int(threadIdx.x) + q * BLOCK;
+    const int lz = L % sg.tz, ly = (L / sg.tz) % sg.ty, lx = L / (sg.tz * sg.ty);
+    X[q] = ti * sg.tx + lx;
+    Y[q] = tj * sg.ty + ly;
+    Z[q] = tk * sg.tz + lz;
+    ok[q] = L < tsites && X[q] < g.nx && Y[q] < g.ny && Z[q] < g.nz;
+    acc[q][0] = acc[q][1] = acc[q][2] = 0.0;
+  }
+  for (uint32_t base = 0; base < cnt; base += SPREAD_CHUNK) {
+    const int m = int(min(uint32_t(SPREAD_CHUNK), cnt - base));
+    __syncthreads();
+    for (int e = int(threadIdx.x); e < m; e += BLOCK) {
+      const size_t id = sorted[o + base + e];
+#pragma unroll
+      for (int c = 0; c < 12; ++c) {
+        swc[e][c] = pb.wc[size_t(c) * pb.cap + id];
+        sdel[e][c] = pb.del[size_t(c) * pb.cap + id];
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sF[e][d] = pb.F[size_t(d) * pb.cap + id];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      if (!ok[q]) continue;
+      for (int e = 0; e < m; ++e) {
+        // stencil points (i, j, k) that land on this site, in loop order
+        for (int i = 0; i < 4; ++i) {
+          if (swc[e][i] != X[q]) continue;
+          for (int j = 0; j < 4; ++j) {
+            if (swc[e][4 + j] != Y[q]) continue;
+            const double dxy = sdel[e][i] * sdel[e][4 + j];
+            for (int k = 0; k < 4; ++k) {
+              if (swc[e][8 + k] != Z[q]) continue;
+              const double delta3d = dxy * sdel[e][8 + k];
+              acc[q][0] += delta3d * sF[e][0];
+              acc[q][1] += delta3d * sF[e][1];
+              acc[q][2] += delta3d * sF[e][2];
+            }
+          }
+        }
+      }
+    }
+  }
+  const size_t n = g.n;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    if (!ok[q]) continue;
+    const size_t site = (size_t(X[q]) * g.ny + Y[q]) * g.nz + Z[q];
+    force[site] = T(acc[q][0]);
+    force[n + site] = T(acc[q][1]);
+    force[2 * n + site] = T(acc[q][2]);
+  }
+}
+
 // ---- device-math probes used by the parity tests ----
 __global__ void k_peskin_probe(const double* __restrict__ x, double* __restrict__ out, int count) {
   const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
@@ -327,10 +542,11 @@ hipError_t launch_swimmers_add_forces(int ns, const SwimParams& p, const double*
 
 template <typename T>
 hipError_t launch_swimmers_move(int ns, const SwimParams& p, double* r, double* v, double* n, uint64_t* prng,
-                                const T* u, const Geom& g, unsigned long long* tumbles, hipStream_t s) {
+                                const int* perm, const T* u, const Geom& g, unsigned long long* tumbles,
+                                hipStream_t s) {
   if (ns <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_swimmers_move<T>, dim3(blocks_for(size_t(ns))), dim3(BLOCK), 0, s, ns, p, r, v, n, prng,
-                     u, g, tumbles);
+                     perm, u, g, tumbles);
   return hipGetLastError();
 }
 
@@ -347,13 +563,59 @@ template hipError_t launch_swimmers_add_forces<double>(int, const SwimParams&, c
 template hipError_t launch_swimmers_add_forces<float>(int, const SwimParams&, const double*, const double*,
                                                       float*, const Geom&, hipStream_t);
 template hipError_t launch_swimmers_move<double>(int, const SwimParams&, double*, double*, double*, uint64_t*,
-                                                 const double*, const Geom&, unsigned long long*, hipStream_t);
+                                                 const int*, const double*, const Geom&, unsigned long long*,
+                                                 hipStream_t);
 template hipError_t launch_swimmers_move<float>(int, const SwimParams&, double*, double*, double*, uint64_t*,
-                                                const float*, const Geom&, unsigned long long*, hipStream_t);
+                                                const int*, const float*, const Geom&, unsigned long long*,
+                                                hipStream_t);
 template hipError_t launch_tracers_move<double>(int, double*, double*, double*, const double*, const Geom&,
                                                 hipStream_t);
 template hipError_t launch_tracers_move<float>(int, double*, double*, double*, const float*, const Geom&,
                                                hipStream_t);
+
+
+SpreadGeom make_spread_geom(const Geom& g) {
+  SpreadGeom sg;
+  int tz = 1;
+  while (tz < g.nz && tz < 32) tz <<= 1;
+  sg.tz = tz;
+  sg.ty = 4;
+  sg.tx = 512 / (sg.ty * sg.tz);
+  sg.ntx = (g.nx + sg.tx - 1) / sg.tx;
+  sg.nty = (g.ny + sg.ty - 1) / sg.ty;
+  sg.ntz = (g.nz + sg.tz - 1) / sg.tz;
+  sg.ntiles = sg.ntx * sg.nty * sg.ntz;
+  return sg;
+}
+
+size_t spread_list_capacity(const SpreadGeom& sg, size_t npts) {
+  const size_t px = sg.ntx < 4 ? sg.ntx : 4, py = sg.nty < 4 ? sg.nty : 4, pz = sg.ntz < 4 ? sg.ntz : 4;
+  return npts * px * py * pz;
+}
+
+hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
+                         const Geom& g, const SpreadGeom& sg, const PointBuf& pb, uint32_t* counts, hipStream_t s) {
+  if (ns <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_points, dim3(blocks_for(size_t(ns))), dim3(BLOCK), 0, s, ns, base, p.P, p.l, r, n, perm, g,
+                     sg, pb, counts);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const PointBuf& pb, uint32_t* counts,
+                         uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, T* force, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, sg.ntiles, counts, off, cursor);
+  if (npts > 0)
+    hipLaunchKernelGGL(k_fill_lists, dim3(blocks_for(size_t(npts))), dim3(BLOCK), 0, s, npts, sg, pb, off, cursor,
+                       list);
+  hipLaunchKernelGGL(k_sort_lists, dim3(sg.ntiles), dim3(BLOCK), 0, s, off, list, sorted);
+  hipLaunchKernelGGL(k_spread_tiles<T>, dim3(sg.ntiles), dim3(BLOCK), 0, s, g, sg, pb, off, sorted, force);
+  return hipGetLastError();
+}
+template hipError_t launch_spread<double>(int, const Geom&, const SpreadGeom&, const PointBuf&, uint32_t*, uint32_t*,
+                                          uint32_t*, uint32_t*, uint32_t*, double*, hipStream_t);
+template hipError_t launch_spread<float>(int, const Geom&, const SpreadGeom&, const PointBuf&, uint32_t*, uint32_t*,
+                                         uint32_t*, uint32_t*, uint32_t*, float*, hipStream_t);
 
 hipError_t launch_peskin_probe(const double* x, double* out, int count, hipStream_t s) {
   if (count <= 0) return hipSuccess;

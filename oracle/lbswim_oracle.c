@@ -31,7 +31,10 @@
  *   - the orientation norm starts at 0 (SwimmerArray.cu:171 leaves it
  *     uninitialised), and with rotation off and no tumble n is left as is;
  *   - tracer s is updated per particle, s[d*N+i] += v_d
- *     (TracerArray.cu:41 races on s[0..2]).
+ *     (TracerArray.cu:41 races on s[0..2]);
+ *   - stencil indices ((int)x + n) % n (interp.cu:26) are wrapped once more
+ *     when negative, which only happens for x < -n where the reference reads
+ *     out of bounds.
  * Force spreading is done in swimmer order, tail then head, which is one of
  * the orders the reference's atomics may produce (SwimmerArray.cu:71).
  */
@@ -339,7 +342,9 @@ static void orc_stencil(const int size[3], const double r[3], int idx[3][4], dou
     double x0 = ceil(r[d] - 2.);
     for (i = 0; i < 4; i++) {
       double x = x0 + i;
-      idx[d][i] = ((int)x + size[d]) % size[d];
+      int v = ((int)x + size[d]) % size[d];
+      /* the reference indexes out of bounds for x < -n; wrap instead */
+      idx[d][i] = v < 0 ? v + size[d] : v;
       del[d][i] = orc_peskin_delta(r[d] - x);
     }
   }

@@ -154,7 +154,7 @@ struct lbswim_lattice {
   uint32_t* sp_off = nullptr;     // ntiles + 1
   uint32_t* sp_cursor = nullptr;  // ntiles
   size_t sp_pts_cap = 0;
-  void* sp_pts = nullptr;  // wc, del, F for sp_pts_cap points
+  void* sp_pts = nullptr;  // SpreadPoint records for sp_pts_cap points
   size_t sp_list_cap = 0;
   uint32_t* sp_list = nullptr;
   uint32_t* sp_sorted = nullptr;
@@ -282,6 +282,33 @@ int check_lattice(const lbswim_lattice* lat) {
   if (!lat) return fail(LBSWIM_EINVAL, "lattice handle is NULL");
   return LBSWIM_OK;
 }
+
+}  // namespace
+
+namespace {
+
+// The device keeps post-collision distributions g (pull streaming); the API's
+// fOld/fNew are post-stream, f_p(x) = g_p(x - c_p).  Calls fn(x, x - c_p) for
+// every site x of plane p.
+template <typename Fn>
+void for_each_upstream(const lbswim_lattice* lat, int p, Fn&& fn) {
+  const int nx = lat->addr.size[0], ny = lat->addr.size[1], nz = lat->addr.size[2];
+  const int* c = lat->params.ci[p];
+  size_t x = 0;
+  for (int i = 0; i < nx; ++i) {
+    const int ui = (i - c[0] + nx) % nx;
+    for (int j = 0; j < ny; ++j) {
+      const int uj = (j - c[1] + ny) % ny;
+      const size_t row = (size_t(ui) * ny + uj) * nz;
+      for (int k = 0; k < nz; ++k, ++x) {
+        const int uk = (k - c[2] + nz) % nz;
+        fn(x, row + uk);
+      }
+    }
+  }
+}
+
+bool is_dist(int field) { return field == LBSWIM_FOLD || field == LBSWIM_FNEW; }
 
 }  // namespace
 
@@ -494,9 +521,23 @@ int lbswim_lattice_h2d(lbswim_lattice* lat, int field) {
   if (int rc = lattice_host(lat, field, &h)) return rc;
   const size_t cnt = field_count(lat, field);
   void* dev = field_device(lat, field);
-  if (lat->storage == LBSWIM_STORE_F64) return copy_h2d(dev, h, cnt * sizeof(double));
+  const size_t n = lat->n;
+  if (lat->storage == LBSWIM_STORE_F64) {
+    if (!is_dist(field)) return copy_h2d(dev, h, cnt * sizeof(double));
+    std::vector<double> stage(cnt);
+    for (int p = 0; p < Q; ++p)
+      for_each_upstream(lat, p, [&](size_t x, size_t up) { stage[p * n + up] = h[p * n + x]; });
+    return copy_h2d(dev, stage.data(), cnt * sizeof(double));
+  }
   std::vector<float> stage(cnt);
-  for (size_t i = 0; i < cnt; ++i) stage[i] = float(h[i] - f32_offset(lat, field, i));
+  if (!is_dist(field)) {
+    for (size_t i = 0; i < cnt; ++i) stage[i] = float(h[i] - f32_offset(lat, field, i));
+  } else {
+    for (int p = 0; p < Q; ++p) {
+      const double w = TAB.w[p];
+      for_each_upstream(lat, p, [&](size_t x, size_t up) { stage[p * n + up] = float(h[p * n + x] - w); });
+    }
+  }
   return copy_h2d(dev, stage.data(), cnt * sizeof(float));
 }
 
@@ -507,10 +548,25 @@ int lbswim_lattice_d2h(lbswim_lattice* lat, int field) {
   if (int rc = lattice_host(lat, field, &h)) return rc;
   const size_t cnt = field_count(lat, field);
   const void* dev = field_device(lat, field);
-  if (lat->storage == LBSWIM_STORE_F64) return copy_d2h(h, dev, cnt * sizeof(double));
+  const size_t n = lat->n;
+  if (lat->storage == LBSWIM_STORE_F64) {
+    if (!is_dist(field)) return copy_d2h(h, dev, cnt * sizeof(double));
+    std::vector<double> stage(cnt);
+    if (int rc = copy_d2h(stage.data(), dev, cnt * sizeof(double))) return rc;
+    for (int p = 0; p < Q; ++p)
+      for_each_upstream(lat, p, [&](size_t x, size_t up) { h[p * n + x] = stage[p * n + up]; });
+    return LBSWIM_OK;
+  }
   std::vector<float> stage(cnt);
   if (int rc = copy_d2h(stage.data(), dev, cnt * sizeof(float))) return rc;
-  for (size_t i = 0; i < cnt; ++i) h[i] = double(stage[i]) + f32_offset(lat, field, i);
+  if (!is_dist(field)) {
+    for (size_t i = 0; i < cnt; ++i) h[i] = double(stage[i]) + f32_offset(lat, field, i);
+  } else {
+    for (int p = 0; p < Q; ++p) {
+      const double w = TAB.w[p];
+      for_each_upstream(lat, p, [&](size_t x, size_t up) { h[p * n + x] = double(stage[p * n + up]) + w; });
+    }
+  }
   return LBSWIM_OK;
 }
 
@@ -869,11 +925,9 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_cursor), 4 * nt)))
       return rc;
   }
-  // one allocation for wc (12 ints), del (12 doubles) and F (3 doubles) per point
-  const size_t per_pt = 12 * 4 + 12 * 8 + 3 * 8;
-  size_t cap_bytes = lat->sp_pts_cap * per_pt;
+  size_t cap_bytes = lat->sp_pts_cap * sizeof(SpreadPoint);
   if (lat->sp_pts_cap < npts) {
-    if (int rc = grow(&lat->sp_pts, &cap_bytes, npts * per_pt, 1)) return rc;
+    if (int rc = grow(&lat->sp_pts, &cap_bytes, npts * sizeof(SpreadPoint), 1)) return rc;
     lat->sp_pts_cap = npts;
   }
   const size_t lcap = spread_list_capacity(lat->sg, npts);
@@ -883,12 +937,7 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     if (int rc = grow(reinterpret_cast<void**>(&lat->sp_sorted), &c2, lcap, 4)) return rc;
     lat->sp_list_cap = lcap;
   }
-  PointBuf pb;
-  pb.cap = lat->sp_pts_cap;
-  char* base = static_cast<char*>(lat->sp_pts);
-  pb.del = reinterpret_cast<double*>(base);
-  pb.F = reinterpret_cast<double*>(base + 12 * 8 * pb.cap);
-  pb.wc = reinterpret_cast<int*>(base + 15 * 8 * pb.cap);
+  SpreadPoint* pts = static_cast<SpreadPoint*>(lat->sp_pts);
 
   KernelTimer t(lat, LBSWIM_K_ADDF);
   HIP_TRY(launch_fill_zero(lat->sp_counts, 4 * size_t(lat->sg.ntiles), stream()));
@@ -896,15 +945,15 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
   for (int i = 0; i < nsw; ++i) {
     PStore& st = sw[i]->st;
     HIP_TRY(launch_points(st.num, offset, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R),
-                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, lat->sg, pb, lat->sp_counts, stream()));
+                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, lat->sg, pts, lat->sp_counts, stream()));
     offset += st.num;
   }
   hipError_t e;
   if (lat->storage == LBSWIM_STORE_F64)
-    e = launch_spread<double>(int(npts), lat->geom, lat->sg, pb, lat->sp_counts, lat->sp_off, lat->sp_cursor,
+    e = launch_spread<double>(int(npts), lat->geom, lat->sg, pts, lat->sp_counts, lat->sp_off, lat->sp_cursor,
                               lat->sp_list, lat->sp_sorted, static_cast<double*>(lat->d_force), stream());
   else
-    e = launch_spread<float>(int(npts), lat->geom, lat->sg, pb, lat->sp_counts, lat->sp_off, lat->sp_cursor,
+    e = launch_spread<float>(int(npts), lat->geom, lat->sg, pts, lat->sp_counts, lat->sp_off, lat->sp_cursor,
                              lat->sp_list, lat->sp_sorted, static_cast<float*>(lat->d_force), stream());
   t.done();
   HIP_TRY(e);

@@ -63,23 +63,25 @@ struct SpreadGeom {
   int ntx, nty, ntz;
   int ntiles;
 };
-// Per-point stencil data by point id (capacity `cap`): wrapped site
-// coordinate wc[(d*4+k)*cap + id], Peskin weight del[(d*4+k)*cap + id], force
-// F[d*cap + id].  Point id = 2*(array base + user index) + (0 tail, 1 head).
-struct PointBuf {
-  int* wc;
-  double* del;
-  double* F;
-  size_t cap;
+// Stencil record of one spreading point (tail or head of a swimmer), stored
+// by point id = 2*(array base + user index) + (0 tail, 1 head): Peskin weights
+// del[d*4+k] and wrapped site coordinates wc[d*4+k] of the four stencil
+// positions k along each axis d, and the point force F.
+struct SpreadPoint {
+  double del[12];
+  double F[3];
+  int wc[12];
+  int pad[2];
 };
+static_assert(sizeof(SpreadPoint) == 176, "SpreadPoint layout");
 SpreadGeom make_spread_geom(const Geom& g);
 size_t spread_list_capacity(const SpreadGeom& sg, size_t npts);
 // counts[ntiles] must be zero before the first launch_points of a step.
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
-                         const Geom& g, const SpreadGeom& sg, const PointBuf& pb, uint32_t* counts, hipStream_t s);
+                         const Geom& g, const SpreadGeom& sg, SpreadPoint* pts, uint32_t* counts, hipStream_t s);
 // off[ntiles+1], cursor[ntiles], list/sorted[spread_list_capacity]
 template <typename T>
-hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const PointBuf& pb, uint32_t* counts,
+hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const SpreadPoint* pts, uint32_t* counts,
                          uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, T* force, hipStream_t s);
 
 // ---- spatial ordering of particle storage (particle_order.hip)

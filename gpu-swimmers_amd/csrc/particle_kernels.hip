@@ -16,7 +16,7 @@
 namespace lbs {
 namespace {
 
-constexpr int BLOCK = 256;
+constexpr int PBLOCK = 256;
 
 // Peskin (2002) four-point regularised delta, delta.cu:6-25.
 __device__ __forceinline__ double peskin(double x) {
@@ -80,6 +80,54 @@ __device__ __forceinline__ void interp(const T* __restrict__ u, const Geom& g, c
       }
     }
   }
+}
+
+// ---- stencil gathers by lane groups -----------------------------------------
+// A group of 16 lanes serves one particle: lane c = 4j + k owns stencil column
+// (j, k) and loops over i, so in each load instruction the group reads four
+// runs of four consecutive z sites instead of 16 scattered words.  The 64
+// products delta3d * u_d go to LDS at term t = 16 i + 4 j + k and one lane per
+// component then adds them in t order -- the i, j, k order of interp.cu:32-45 --
+// so the result is bit-identical to the one-lane loop above.
+constexpr int GROUP = 16;
+constexpr int GROUPS = PBLOCK / GROUP;
+constexpr int TERMS = 64 * 3;  // (t, d) pairs of one interpolation, stored t-major
+
+template <typename T>
+__device__ __forceinline__ void interp_terms(const T* __restrict__ u, const Geom& g, const double (&r)[3], int c,
+                                             double* __restrict__ terms) {
+  const int j = c >> 2, k = c & 3;
+  const double x0 = ceil(r[0] - 2.);
+  uint32_t ix[4];
+  double dx[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double x = x0 + i;
+    ix[i] = wrap_index(x, g.nx);
+    dx[i] = peskin(r[0] - x);
+  }
+  const double y = ceil(r[1] - 2.) + j;
+  const double z = ceil(r[2] - 2.) + k;
+  const uint32_t iyz = wrap_index(y, g.ny) * uint32_t(g.nz) + wrap_index(z, g.nz);
+  const double dy = peskin(r[1] - y), dz = peskin(r[2] - z);
+  const size_t n = g.n;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double delta3d = dx[i] * dy * dz;
+    const uint32_t ijk = ix[i] * g.nyz + iyz;
+    double* out = terms + 3 * (16 * i + c);
+    out[0] = delta3d * double(u[ijk]);
+    out[1] = delta3d * double(u[n + ijk]);
+    out[2] = delta3d * double(u[2 * n + ijk]);
+  }
+}
+
+// Sum of component d of the 64 terms in t order, starting from 0.0.
+__device__ __forceinline__ double sum_terms(const double* __restrict__ terms, int d) {
+  double acc = 0.0;
+#pragma unroll 16
+  for (int t = 0; t < 64; ++t) acc += terms[3 * t + d];
+  return acc;
 }
 
 // ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ----
@@ -151,12 +199,12 @@ __device__ __forceinline__ void spread_atomic(T* addr, double val) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_swimmers_add_forces(int ns, double P, double l,
+__global__ __launch_bounds__(PBLOCK) void k_swimmers_add_forces(int ns, double P, double l,
                                                                const double* __restrict__ r,
                                                                const double* __restrict__ nv,
                                                                T* __restrict__ force, Geom g) {
   const int lane = threadIdx.x & 63;
-  const int is = int((blockIdx.x * BLOCK + threadIdx.x) >> 6);
+  const int is = int((blockIdx.x * PBLOCK + threadIdx.x) >> 6);
   if (is >= ns) return;
   const size_t n = g.n;
   const int size[3] = {g.nx, g.ny, g.nz};
@@ -212,37 +260,50 @@ __global__ __launch_bounds__(BLOCK) void k_swimmers_add_forces(int ns, double P,
 }
 
 // DoSwimmerArrayMove, SwimmerArray.cu:123-210 (norm starts at 0; with
-// rotation off and no tumble, n is kept).
+// rotation off and no tumble, n is kept).  A lane group per swimmer gathers
+// the stencils at r and at r - l n; lane 0 of the group finishes the move.
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_swimmers_move(int ns, SwimParams cp, double* __restrict__ r,
-                                                         double* __restrict__ v, double* __restrict__ nv,
-                                                         uint64_t* __restrict__ prng, const int* __restrict__ perm,
-                                                         const T* __restrict__ u, Geom g,
-                                                         unsigned long long* __restrict__ tumbles) {
-  const int is = int(blockIdx.x * BLOCK + threadIdx.x);
-  if (is >= ns) return;
-  const int user = perm[is];  // the random stream belongs to the swimmer, not its storage slot
-  const int size[3] = {g.nx, g.ny, g.nz};
-  double R[3], N[3];
+__global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp, double* __restrict__ r,
+                                                          double* __restrict__ v, double* __restrict__ nv,
+                                                          uint64_t* __restrict__ prng, const int* __restrict__ perm,
+                                                          const T* __restrict__ u, Geom g,
+                                                          unsigned long long* __restrict__ tumbles) {
+  constexpr int STRIDE = 2 * TERMS + 7;  // odd in doubles: groups start on different banks
+  __shared__ double lds[GROUPS * STRIDE];
+  const int grp = int(threadIdx.x) / GROUP, c = int(threadIdx.x) % GROUP;
+  const int is = int(blockIdx.x) * GROUPS + grp;
+  const bool active = is < ns;
+  double* terms = lds + grp * STRIDE;
+  double R[3], N[3], rMinus[3];
+  if (active) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    R[d] = r[size_t(ns) * d + is];
-    N[d] = nv[size_t(ns) * d + is];
+    for (int d = 0; d < 3; ++d) {
+      R[d] = r[size_t(ns) * d + is];
+      N[d] = nv[size_t(ns) * d + is];
+      rMinus[d] = R[d] - N[d] * cp.l;
+    }
+    interp_terms(u, g, R, c, terms);
+    interp_terms(u, g, rMinus, c, terms + TERMS);
   }
-  double vel[3];
-  interp(u, g, R, vel);
-  double rDot[3], rMinus[3];
+  __syncthreads();
+  if (active && c < 6) terms[2 * TERMS + c] = sum_terms(terms + (c < 3 ? 0 : TERMS), c % 3);
+  __syncthreads();
+  if (!active || c != 0) return;
+  const int size[3] = {g.nx, g.ny, g.nz};
+  const int user = perm[is];  // the random stream belongs to the swimmer, not its storage slot
+  const double* vel = terms + 2 * TERMS;
+  const double* vMinus = terms + 2 * TERMS + 3;
+  double rDot[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     rDot[d] = cp.mobility * cp.P * N[d];
     if (!cp.translational_advection_off) rDot[d] += vel[d];
-    rMinus[d] = R[d] - N[d] * cp.l;
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     v[size_t(ns) * d + is] = rDot[d];
-    R[d] += rDot[d];
-    r[size_t(ns) * d + is] = fmod(R[d] + size[d], double(size[d]));
+    const double Rn = R[d] + rDot[d];
+    r[size_t(ns) * d + is] = fmod(Rn + size[d], double(size[d]));
   }
   const uint64_t ctr = prng[is];
   const float U = u01f(draw(cp.seed, user, ctr, 0u).x[0]);
@@ -255,8 +316,6 @@ __global__ __launch_bounds__(BLOCK) void k_swimmers_move(int ns, SwimParams cp, 
     for (int d = 0; d < 3; ++d) norm += newn[d] * newn[d];
     atomicAdd(tumbles, 1ull);
   } else if (!cp.rotational_advection_off) {
-    double vMinus[3];
-    interp(u, g, rMinus, vMinus);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       double nDot = vel[d] - vMinus[d];
@@ -275,25 +334,32 @@ __global__ __launch_bounds__(BLOCK) void k_swimmers_move(int ns, SwimParams cp, 
   prng[is] = ctr + 1;
 }
 
-// DoTracerArrayMove, TracerArray.cu:12-43 (s updated per particle).
+// DoTracerArrayMove, TracerArray.cu:12-43 (s updated per particle).  A lane
+// group per tracer gathers the stencil; lane d of the group updates component d.
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
-                                                        double* __restrict__ v, const T* __restrict__ u, Geom g) {
-  const int it = int(blockIdx.x * BLOCK + threadIdx.x);
-  if (it >= nt) return;
-  const int size[3] = {g.nx, g.ny, g.nz};
+__global__ __launch_bounds__(PBLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
+                                                         double* __restrict__ v, const T* __restrict__ u, Geom g) {
+  constexpr int STRIDE = TERMS + 1;
+  __shared__ double lds[GROUPS * STRIDE];
+  const int grp = int(threadIdx.x) / GROUP, c = int(threadIdx.x) % GROUP;
+  const int it = int(blockIdx.x) * GROUPS + grp;
+  const bool active = it < nt;
+  double* terms = lds + grp * STRIDE;
   double R[3];
+  if (active) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) R[d] = r[size_t(nt) * d + it];
-  double vel[3];
-  interp(u, g, R, vel);
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    v[size_t(nt) * d + it] = vel[d];
-    R[d] += vel[d];
-    r[size_t(nt) * d + it] = fmod(R[d] + size[d], double(size[d]));
-    sd[size_t(nt) * d + it] += vel[d];
+    for (int d = 0; d < 3; ++d) R[d] = r[size_t(nt) * d + it];
+    interp_terms(u, g, R, c, terms);
   }
+  __syncthreads();
+  if (!active || c >= 3) return;
+  const int d = c;
+  const int size = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+  const double vel = sum_terms(terms, d);
+  v[size_t(nt) * d + it] = vel;
+  const double Rn = R[d] + vel;
+  r[size_t(nt) * d + it] = fmod(Rn + size, double(size));
+  sd[size_t(nt) * d + it] += vel;
 }
 
 // ---- deterministic force spreading -----------------------------------------
@@ -325,10 +391,11 @@ __device__ __forceinline__ int tiles_of(const int (&wc)[4], int tdim, int ntd, i
   return m;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_points(int ns, int base, double P, double l, const double* __restrict__ r,
+__global__ __launch_bounds__(PBLOCK) void k_points(int ns, int base, double P, double l, const double* __restrict__ r,
                                                   const double* __restrict__ nv, const int* __restrict__ perm,
-                                                  Geom g, SpreadGeom sg, PointBuf pb, uint32_t* __restrict__ counts) {
-  const int slot = int(blockIdx.x * BLOCK + threadIdx.x);
+                                                  Geom g, SpreadGeom sg, SpreadPoint* __restrict__ pts,
+                                                  uint32_t* __restrict__ counts) {
+  const int slot = int(blockIdx.x * PBLOCK + threadIdx.x);
   if (slot >= ns) return;
   const int size[3] = {g.nx, g.ny, g.nz};
   const int tdim[3] = {sg.tx, sg.ty, sg.tz};
@@ -344,28 +411,25 @@ __global__ __launch_bounds__(BLOCK) void k_points(int ns, int base, double P, do
 #pragma unroll
   for (int a = 0; a < 3; ++a) Ft[a] = -P * na[a];
   for (int end = 0; end < 2; ++end) {
-    const size_t id = 2 * (size_t(base) + size_t(user)) + end;
-    double rp[3], Fp[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      rp[a] = end ? ra[a] : ra[a] - na[a] * l;
-      Fp[a] = end ? Ft[a] * -1.0 : Ft[a];
-    }
+    SpreadPoint sp;
     int tl[3][4], nt[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double x0 = ceil(rp[d] - 2.);
+      const double rp = end ? ra[d] : ra[d] - na[d] * l;
+      const double x0 = ceil(rp - 2.);
       int wc[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double x = x0 + k;
         wc[k] = int(wrap_index(x, size[d]));
-        pb.wc[size_t(d * 4 + k) * pb.cap + id] = wc[k];
-        pb.del[size_t(d * 4 + k) * pb.cap + id] = peskin(rp[d] - x);
+        sp.wc[d * 4 + k] = wc[k];
+        sp.del[d * 4 + k] = peskin(rp - x);
       }
       nt[d] = tiles_of(wc, tdim[d], ntd[d], tl[d]);
-      pb.F[size_t(d) * pb.cap + id] = Fp[d];
+      sp.F[d] = end ? Ft[d] * -1.0 : Ft[d];
     }
+    sp.pad[0] = sp.pad[1] = 0;
+    pts[2 * (size_t(base) + size_t(user)) + end] = sp;
     for (int a = 0; a < nt[0]; ++a)
       for (int b = 0; b < nt[1]; ++b)
         for (int c = 0; c < nt[2]; ++c)
@@ -373,36 +437,50 @@ __global__ __launch_bounds__(BLOCK) void k_points(int ns, int base, double P, do
   }
 }
 
-// Exclusive scan of counts[0..nt) into off[0..nt], one workgroup; also clears cursor.
+// Exclusive scan of counts[0..nt) into off[0..nt] by one workgroup, 32
+// consecutive tiles per thread per pass; also clears cursor.
 __global__ __launch_bounds__(1024) void k_scan_tiles(int nt, const uint32_t* __restrict__ counts,
                                                      uint32_t* __restrict__ off, uint32_t* __restrict__ cursor) {
+  constexpr int PER = 32, SPAN = 1024 * PER;
   __shared__ uint32_t part[1024];
   const int t = int(threadIdx.x);
-  const int chunk = (nt + 1023) / 1024;
-  const int lo = t * chunk, hi = min(nt, lo + chunk);
-  uint32_t sum = 0;
-  for (int i = lo; i < hi; ++i) sum += counts[i];
-  part[t] = sum;
-  __syncthreads();
-  for (int w = 1; w < 1024; w <<= 1) {
-    const uint32_t v = t >= w ? part[t - w] : 0u;
+  uint32_t carry = 0;
+  for (int base = 0; base < nt; base += SPAN) {
+    const int lo = base + t * PER;
+    uint32_t v[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      v[i] = lo + i < nt ? counts[lo + i] : 0u;
+      sum += v[i];
+    }
+    part[t] = sum;
     __syncthreads();
-    part[t] += v;
+    for (int w = 1; w < 1024; w <<= 1) {
+      const uint32_t add = t >= w ? part[t - w] : 0u;
+      __syncthreads();
+      part[t] += add;
+      __syncthreads();
+    }
+    uint32_t run = carry + part[t] - sum;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      if (lo + i < nt) {
+        off[lo + i] = run;
+        cursor[lo + i] = 0u;
+      }
+      run += v[i];
+    }
+    carry += part[1023];
     __syncthreads();
   }
-  uint32_t run = part[t] - sum;
-  for (int i = lo; i < hi; ++i) {
-    off[i] = run;
-    run += counts[i];
-    cursor[i] = 0u;
-  }
-  if (t == 1023) off[nt] = part[1023];
+  if (t == 0) off[nt] = carry;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_fill_lists(int npts, SpreadGeom sg, PointBuf pb,
+__global__ __launch_bounds__(PBLOCK) void k_fill_lists(int npts, SpreadGeom sg, const SpreadPoint* __restrict__ pts,
                                                       const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
                                                       uint32_t* __restrict__ list) {
-  const int id = int(blockIdx.x * BLOCK + threadIdx.x);
+  const int id = int(blockIdx.x * PBLOCK + threadIdx.x);
   if (id >= npts) return;
   const int tdim[3] = {sg.tx, sg.ty, sg.tz};
   const int ntd[3] = {sg.ntx, sg.nty, sg.ntz};
@@ -411,7 +489,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill_lists(int npts, SpreadGeom sg, P
   for (int d = 0; d < 3; ++d) {
     int wc[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) wc[k] = pb.wc[size_t(d * 4 + k) * pb.cap + id];
+    for (int k = 0; k < 4; ++k) wc[k] = pts[id].wc[d * 4 + k];
     nt[d] = tiles_of(wc, tdim[d], ntd[d], tl[d]);
   }
   for (int a = 0; a < nt[0]; ++a)
@@ -423,11 +501,11 @@ __global__ __launch_bounds__(BLOCK) void k_fill_lists(int npts, SpreadGeom sg, P
       }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_sort_lists(const uint32_t* __restrict__ off,
+__global__ __launch_bounds__(PBLOCK) void k_sort_lists(const uint32_t* __restrict__ off,
                                                       const uint32_t* __restrict__ list,
                                                       uint32_t* __restrict__ sorted) {
   const uint32_t o = off[blockIdx.x], cnt = off[blockIdx.x + 1] - o;
-  for (uint32_t e = threadIdx.x; e < cnt; e += BLOCK) {
+  for (uint32_t e = threadIdx.x; e < cnt; e += PBLOCK) {
     const uint32_t id = list[o + e];
     uint32_t rank = 0;
     for (uint32_t k = 0; k < cnt; ++k) rank += list[o + k] < id ? 1u : 0u;
@@ -435,67 +513,83 @@ __global__ __launch_bounds__(BLOCK) void k_sort_lists(const uint32_t* __restrict
   }
 }
 
-constexpr int SPREAD_CHUNK = 128;
+constexpr int SPREAD_CHUNK = 64;
+constexpr int POINT_WORDS = int(sizeof(SpreadPoint) / 8);
+
+// Adds the contributions of one point to one site, in the (i, j, k) loop order
+// of AccumulateDeltaForce.  `simple`: every lattice side is >= 4, so each axis
+// has at most one matching stencil position, found by a wrapped difference.
+__device__ __forceinline__ void add_point(const SpreadPoint& p, int X, int Y, int Z, const int (&size)[3],
+                                          bool simple, double (&acc)[3]) {
+  if (simple) {
+    int di = X - p.wc[0];
+    di += di < 0 ? size[0] : 0;
+    if (di >= 4) return;
+    int dj = Y - p.wc[4];
+    dj += dj < 0 ? size[1] : 0;
+    if (dj >= 4) return;
+    int dk = Z - p.wc[8];
+    dk += dk < 0 ? size[2] : 0;
+    if (dk >= 4) return;
+    const double delta3d = p.del[di] * p.del[4 + dj] * p.del[8 + dk];
+    acc[0] += delta3d * p.F[0];
+    acc[1] += delta3d * p.F[1];
+    acc[2] += delta3d * p.F[2];
+    return;
+  }
+  for (int i = 0; i < 4; ++i) {
+    if (p.wc[i] != X) continue;
+    for (int j = 0; j < 4; ++j) {
+      if (p.wc[4 + j] != Y) continue;
+      const double dxy = p.del[i] * p.del[4 + j];
+      for (int k = 0; k < 4; ++k) {
+        if (p.wc[8 + k] != Z) continue;
+        const double delta3d = dxy * p.del[8 + k];
+        acc[0] += delta3d * p.F[0];
+        acc[1] += delta3d * p.F[1];
+        acc[2] += delta3d * p.F[2];
+      }
+    }
+  }
+}
 
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_spread_tiles(Geom g, SpreadGeom sg, PointBuf pb,
+__global__ __launch_bounds__(PBLOCK) void k_spread_tiles(Geom g, SpreadGeom sg, const SpreadPoint* __restrict__ pts,
                                                         const uint32_t* __restrict__ off,
                                                         const uint32_t* __restrict__ sorted, T* __restrict__ force) {
-  __shared__ int swc[SPREAD_CHUNK][12];
-  __shared__ double sdel[SPREAD_CHUNK][12];
-  __shared__ double sF[SPREAD_CHUNK][3];
+  __shared__ SpreadPoint sp[SPREAD_CHUNK];
   const int tile = int(blockIdx.x);
   const int tk = tile % sg.ntz, tj = (tile / sg.ntz) % sg.nty, ti = tile / (sg.ntz * sg.nty);
   const uint32_t o = off[tile], cnt = off[tile + 1] - o;
-  const int tsites = sg.tx * sg.ty * sg.tz;
-  constexpr int PER = 2;  // sites per thread; tiles hold at most 512 sites
+  const int size[3] = {g.nx, g.ny, g.nz};
+  const bool simple = g.nx >= 4 && g.ny >= 4 && g.nz >= 4;
+  constexpr int PER = 2;  // 512 sites per tile, 256 threads
   int X[PER], Y[PER], Z[PER];
   bool ok[PER];
   double acc[PER][3];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    const int L = int(threadIdx.x) + q * BLOCK;
+    const int L = int(threadIdx.x) + q * PBLOCK;
     const int lz = L % sg.tz, ly = (L / sg.tz) % sg.ty, lx = L / (sg.tz * sg.ty);
     X[q] = ti * sg.tx + lx;
     Y[q] = tj * sg.ty + ly;
     Z[q] = tk * sg.tz + lz;
-    ok[q] = L < tsites && X[q] < g.nx && Y[q] < g.ny && Z[q] < g.nz;
+    ok[q] = X[q] < g.nx && Y[q] < g.ny && Z[q] < g.nz;
     acc[q][0] = acc[q][1] = acc[q][2] = 0.0;
   }
   for (uint32_t base = 0; base < cnt; base += SPREAD_CHUNK) {
     const int m = int(min(uint32_t(SPREAD_CHUNK), cnt - base));
     __syncthreads();
-    for (int e = int(threadIdx.x); e < m; e += BLOCK) {
-      const size_t id = sorted[o + base + e];
-#pragma unroll
-      for (int c = 0; c < 12; ++c) {
-        swc[e][c] = pb.wc[size_t(c) * pb.cap + id];
-        sdel[e][c] = pb.del[size_t(c) * pb.cap + id];
-      }
-#pragma unroll
-      for (int d = 0; d < 3; ++d) sF[e][d] = pb.F[size_t(d) * pb.cap + id];
+    uint64_t* dst = reinterpret_cast<uint64_t*>(sp);
+    for (int w = int(threadIdx.x); w < m * POINT_WORDS; w += PBLOCK) {
+      const int e = w / POINT_WORDS, c = w - e * POINT_WORDS;
+      dst[w] = reinterpret_cast<const uint64_t*>(pts + sorted[o + base + e])[c];
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       if (!ok[q]) continue;
-      for (int e = 0; e < m; ++e) {
-        // stencil points (i, j, k) that land on this site, in loop order
-        for (int i = 0; i < 4; ++i) {
-          if (swc[e][i] != X[q]) continue;
-          for (int j = 0; j < 4; ++j) {
-            if (swc[e][4 + j] != Y[q]) continue;
-            const double dxy = sdel[e][i] * sdel[e][4 + j];
-            for (int k = 0; k < 4; ++k) {
-              if (swc[e][8 + k] != Z[q]) continue;
-              const double delta3d = dxy * sdel[e][8 + k];
-              acc[q][0] += delta3d * sF[e][0];
-              acc[q][1] += delta3d * sF[e][1];
-              acc[q][2] += delta3d * sF[e][2];
-            }
-          }
-        }
-      }
+      for (int e = 0; e < m; ++e) add_point(sp[e], X[q], Y[q], Z[q], size, simple, acc[q]);
     }
   }
   const size_t n = g.n;
@@ -527,7 +621,7 @@ __global__ void k_interp_probe(const double* __restrict__ u, const double* __res
   v[2 * size_t(count) + i] = out[2];
 }
 
-inline unsigned blocks_for(size_t threads) { return unsigned((threads + BLOCK - 1) / BLOCK); }
+inline unsigned blocks_for(size_t threads) { return unsigned((threads + PBLOCK - 1) / PBLOCK); }
 
 }  // namespace
 
@@ -535,7 +629,7 @@ template <typename T>
 hipError_t launch_swimmers_add_forces(int ns, const SwimParams& p, const double* r, const double* n, T* force,
                                       const Geom& g, hipStream_t s) {
   if (ns <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_swimmers_add_forces<T>, dim3(blocks_for(size_t(ns) * 64)), dim3(BLOCK), 0, s, ns, p.P,
+  hipLaunchKernelGGL(k_swimmers_add_forces<T>, dim3(blocks_for(size_t(ns) * 64)), dim3(PBLOCK), 0, s, ns, p.P,
                      p.l, r, n, force, g);
   return hipGetLastError();
 }
@@ -545,7 +639,8 @@ hipError_t launch_swimmers_move(int ns, const SwimParams& p, double* r, double* 
                                 const int* perm, const T* u, const Geom& g, unsigned long long* tumbles,
                                 hipStream_t s) {
   if (ns <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_swimmers_move<T>, dim3(blocks_for(size_t(ns))), dim3(BLOCK), 0, s, ns, p, r, v, n, prng,
+  hipLaunchKernelGGL(k_swimmers_move<T>, dim3(unsigned((ns + GROUPS - 1) / GROUPS)), dim3(PBLOCK), 0, s, ns, p, r, v,
+                     n, prng,
                      perm, u, g, tumbles);
   return hipGetLastError();
 }
@@ -554,7 +649,8 @@ template <typename T>
 hipError_t launch_tracers_move(int nt, double* r, double* sd, double* v, const T* u, const Geom& g,
                                hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tracers_move<T>, dim3(blocks_for(size_t(nt))), dim3(BLOCK), 0, s, nt, r, sd, v, u, g);
+  hipLaunchKernelGGL(k_tracers_move<T>, dim3(unsigned((nt + GROUPS - 1) / GROUPS)), dim3(PBLOCK), 0, s, nt, r, sd,
+                     v, u, g);
   return hipGetLastError();
 }
 
@@ -594,39 +690,39 @@ size_t spread_list_capacity(const SpreadGeom& sg, size_t npts) {
 }
 
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
-                         const Geom& g, const SpreadGeom& sg, const PointBuf& pb, uint32_t* counts, hipStream_t s) {
+                         const Geom& g, const SpreadGeom& sg, SpreadPoint* pts, uint32_t* counts, hipStream_t s) {
   if (ns <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_points, dim3(blocks_for(size_t(ns))), dim3(BLOCK), 0, s, ns, base, p.P, p.l, r, n, perm, g,
-                     sg, pb, counts);
+  hipLaunchKernelGGL(k_points, dim3(blocks_for(size_t(ns))), dim3(PBLOCK), 0, s, ns, base, p.P, p.l, r, n, perm, g,
+                     sg, pts, counts);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const PointBuf& pb, uint32_t* counts,
+hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const SpreadPoint* pts, uint32_t* counts,
                          uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, T* force, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, sg.ntiles, counts, off, cursor);
   if (npts > 0)
-    hipLaunchKernelGGL(k_fill_lists, dim3(blocks_for(size_t(npts))), dim3(BLOCK), 0, s, npts, sg, pb, off, cursor,
+    hipLaunchKernelGGL(k_fill_lists, dim3(blocks_for(size_t(npts))), dim3(PBLOCK), 0, s, npts, sg, pts, off, cursor,
                        list);
-  hipLaunchKernelGGL(k_sort_lists, dim3(sg.ntiles), dim3(BLOCK), 0, s, off, list, sorted);
-  hipLaunchKernelGGL(k_spread_tiles<T>, dim3(sg.ntiles), dim3(BLOCK), 0, s, g, sg, pb, off, sorted, force);
+  hipLaunchKernelGGL(k_sort_lists, dim3(sg.ntiles), dim3(PBLOCK), 0, s, off, list, sorted);
+  hipLaunchKernelGGL(k_spread_tiles<T>, dim3(sg.ntiles), dim3(PBLOCK), 0, s, g, sg, pts, off, sorted, force);
   return hipGetLastError();
 }
-template hipError_t launch_spread<double>(int, const Geom&, const SpreadGeom&, const PointBuf&, uint32_t*, uint32_t*,
+template hipError_t launch_spread<double>(int, const Geom&, const SpreadGeom&, const SpreadPoint*, uint32_t*, uint32_t*,
                                           uint32_t*, uint32_t*, uint32_t*, double*, hipStream_t);
-template hipError_t launch_spread<float>(int, const Geom&, const SpreadGeom&, const PointBuf&, uint32_t*, uint32_t*,
+template hipError_t launch_spread<float>(int, const Geom&, const SpreadGeom&, const SpreadPoint*, uint32_t*, uint32_t*,
                                          uint32_t*, uint32_t*, uint32_t*, float*, hipStream_t);
 
 hipError_t launch_peskin_probe(const double* x, double* out, int count, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_peskin_probe, dim3(blocks_for(size_t(count))), dim3(BLOCK), 0, s, x, out, count);
+  hipLaunchKernelGGL(k_peskin_probe, dim3(blocks_for(size_t(count))), dim3(PBLOCK), 0, s, x, out, count);
   return hipGetLastError();
 }
 
 hipError_t launch_interp_probe(const double* u, const double* r, double* v, int count, const Geom& g,
                                hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_interp_probe, dim3(blocks_for(size_t(count))), dim3(BLOCK), 0, s, u, r, v, count, g);
+  hipLaunchKernelGGL(k_interp_probe, dim3(blocks_for(size_t(count))), dim3(PBLOCK), 0, s, u, r, v, count, g);
   return hipGetLastError();
 }
 

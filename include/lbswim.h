@@ -120,12 +120,12 @@ int lbswim_synchronize(void);
 /* ---- Lattice (lbswim/liblbswim/Lattice.h:53-67; lbswim/_lb.pxd:22-30) ---- */
 
 /* Lattice::Lattice(nx, ny, nz, tau_s, tau_b) (lb.cu:40-63) plus the storage form.
- * Sizes >= 1 with 15*nx*ny*nz < 2^31; tau > 0. Device fields start at zero. */
+ * Sizes >= 1 with nx*ny*nz < 2^31; tau > 0. Device fields start at zero. */
 int lbswim_lattice_create(int nx, int ny, int nz, double tau_s, double tau_b, int storage,
                           lbswim_lattice** out);
 /* Lattice::~Lattice (lb.cu:66-67). NULL is ignored. */
 void lbswim_lattice_destroy(lbswim_lattice* lat);
-/* Lattice::Step: collide + push-stream, swap fOld/fNew, time_step++ (lb.cu:220-247). */
+/* Lattice::Step: collide + stream, swap fOld/fNew, time_step++ (lb.cu:220-247). */
 int lbswim_lattice_step(lbswim_lattice* lat);
 /* Lattice::CalcHydro: rho, u from fOld and force (lb.cu:297-312). */
 int lbswim_lattice_calc_hydro(lbswim_lattice* lat);
@@ -145,7 +145,10 @@ int lbswim_lattice_storage(lbswim_lattice* lat, int* out);
 /* LatticeData::{rho,u,force,fOld,fNew} as SharedArray<double>{host,device,size}
  * (Lattice.h:45-49; Shared.h:4-10). *host is the FP64 mirror (stable for the
  * lattice's life), *device the current device buffer in the storage type,
- * *size the element count (n, 3n or 15n). Any output pointer may be NULL. */
+ * *size the element count (n, 3n or 15n). Any output pointer may be NULL.
+ * The device distribution buffers hold post-collision values g (streaming is
+ * done by pulling); the post-stream fOld the host sees is f_p(x) = g_p(x - c_p),
+ * and *_h2d/*_d2h apply that shift. */
 int lbswim_lattice_field(lbswim_lattice* lat, int field, double** host, void** device, size_t* size);
 /* SharedArray::H2D / D2H of one field (Shared.cu:15-26). For fOld/fNew the
  * device side is the buffer currently holding that role, as after the
@@ -171,7 +174,9 @@ int lbswim_swimmers_add_forces(lbswim_swimmers* sw, lbswim_lattice* lat);
 /* SwimmerArray::Move(Lattice*) (SwimmerArray.cu:212-220). */
 int lbswim_swimmers_move(lbswim_swimmers* sw, lbswim_lattice* lat);
 /* SwimmerArray::{r,v,n,prng} as SharedArray{host,device,size}; *elem_size is
- * 8 for all four (double, double, double, uint64 counter). */
+ * 8 for all four (double, double, double, uint64 counter).  The host mirror is
+ * in the user's order; the device copy is kept in a spatially sorted order
+ * that the library changes from time to time (h2d/d2h translate). */
 int lbswim_swimmers_field(lbswim_swimmers* sw, int field, void** host, void** device, size_t* count,
                           size_t* elem_size);
 int lbswim_swimmers_h2d(lbswim_swimmers* sw, int field);
@@ -187,7 +192,7 @@ void lbswim_tracers_destroy(lbswim_tracers* tr);
 int lbswim_tracers_num(const lbswim_tracers* tr, int* out);
 /* TracerArray::Move(Lattice*) (TracerArray.cu:45-52). */
 int lbswim_tracers_move(lbswim_tracers* tr, lbswim_lattice* lat);
-/* TracerArray::{r,s,v} (TracerArray.h:10-13). */
+/* TracerArray::{r,s,v} (TracerArray.h:10-13); device order as for swimmers. */
 int lbswim_tracers_field(lbswim_tracers* tr, int field, void** host, void** device, size_t* count);
 int lbswim_tracers_h2d(lbswim_tracers* tr, int field);
 int lbswim_tracers_d2h(lbswim_tracers* tr, int field);

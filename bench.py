@@ -72,6 +72,60 @@ def time_gpu(case, storage, steps, warmup):
     return res
 
 
+def ensemble_cases(nsims=64, seed=1234):
+    """64 independent sims: cubic sizes from {16,24,32,48,64}, tau ~ U[0.2,1],
+    1-256 swimmers with alpha ~ U[0,0.05], per-sim seeds 1234+i."""
+    from cases import make_case
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nsims):
+        L = int(rng.choice([16, 24, 32, 48, 64]))
+        out.append(make_case((L, L, L), 0, float(rng.uniform(0.2, 1.0)), ns=int(rng.integers(1, 257)), nt=0,
+                             alpha=float(rng.uniform(0.0, 0.05)), seed=seed + i))
+    return out
+
+
+def bench_ensemble(storage, steps, nsims=64, tune_steps=5):
+    """Batched (one launch per phase, graph replay, autotuned order) against
+    the same sims run one after another with the plain per-system step."""
+    from cases import build_lbswim
+    import lbswim
+    from lbswim.ensemble import Ensemble, ORDERS
+
+    cases = ensemble_cases(nsims)
+    sites = sum(c.size[0] * c.size[1] * c.size[2] for c in cases)
+    # sequential plain runs
+    t_seq = 0.0
+    for c in cases:
+        lat, system, sw, tr = build_lbswim(c, storage)
+        system.Step(2)
+        lbswim.synchronize()
+        t0 = time.perf_counter()
+        system.Step(steps)
+        lbswim.synchronize()
+        t_seq += time.perf_counter() - t0
+        del lat, system, sw, tr
+    # batched
+    ens = Ensemble(storage)
+    keep = []
+    for c in cases:
+        lat, system, sw, tr = build_lbswim(c, storage)
+        ens.add(lat, [sw])
+        keep.append((lat, sw))
+    choice = ens.autotune(tune_steps)
+    ens.step(2)
+    lbswim.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    lbswim.synchronize()
+    t_ens = time.perf_counter() - t0
+    return {"sims": nsims, "steps": steps, "sites": sites, "ms_batched": round(t_ens * 1e3, 2),
+            "ms_sequential": round(t_seq * 1e3, 2), "sims_per_s": round(nsims / t_ens, 2),
+            "sims_per_s_sequential": round(nsims / t_seq, 2), "speedup_vs_sequential": round(t_seq / t_ens, 2),
+            "mlups": round(sites * steps / t_ens / 1e6, 1), "order": ORDERS[choice],
+            "graph": True}
+
+
 def cpu_baseline(case, steps=3):
     """The FP64 oracle (OpenMP over the host cores) on the same workload."""
     import lbswim_oracle as O
@@ -115,6 +169,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="f64,f32")
     ap.add_argument("--probe", choices=["copy"], help="run the streaming-copy bandwidth probe instead")
+    ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-sim ensemble leg (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -182,6 +237,12 @@ def main():
                              "collide_frac": round(f["collide_gbs"] / HBM_PEAK_GBS, 4),
                              "step_hbm_frac": round(f["step_gbs"] / HBM_PEAK_GBS, 4),
                              "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in f["kernels"].items()}}
+    if a.ensemble_steps > 0:
+        out["ensemble"] = {}
+        for form in forms:
+            log("ensemble %s: 64 sims x %d steps, batched vs sequential" % (form, a.ensemble_steps))
+            out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps)
+            log("ensemble %s: %s" % (form, out["ensemble"][form]))
     if not a.no_cpu:
         log("CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(case)

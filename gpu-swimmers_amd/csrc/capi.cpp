@@ -4,6 +4,7 @@
 // entry point replaces are in include/lbswim.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -151,7 +152,7 @@ struct lbswim_lattice {
   // workspace of the deterministic spreading in lbswim_system_step
   SpreadGeom sg{};
   uint32_t* sp_counts = nullptr;  // ntiles
-  uint32_t* sp_off = nullptr;     // ntiles + 1
+  uint32_t* sp_off = nullptr;     // ntiles + 1 offsets + scan scratch
   uint32_t* sp_cursor = nullptr;  // ntiles
   size_t sp_pts_cap = 0;
   void* sp_pts = nullptr;  // SpreadPoint records for sp_pts_cap points
@@ -921,7 +922,7 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     const size_t nt = size_t(lat->sg.ntiles);
     int rc;
     if ((rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_counts), 4 * nt)) ||
-        (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_off), 4 * (nt + 1))) ||
+        (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_off), 4 * (nt + 1 + (nt + 1023) / 1024))) ||
         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_cursor), 4 * nt)))
       return rc;
   }
@@ -1101,6 +1102,462 @@ int lbswim_diag_interp(int nx, int ny, int nz, const double* u, const double* r,
   dev_free(dr);
   dev_free(dv);
   return rc;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Ensembles: many independent systems advanced by batched launches.
+// ============================================================================
+
+namespace {
+
+struct EnsMember {
+  lbswim_lattice* lat = nullptr;  // nullptr: free slot
+  std::vector<lbswim_swimmers*> sw;
+  std::vector<lbswim_tracers*> tr;
+  // state captured in the device tables: lattice buffer role and particle
+  // storage pointers (a member used outside the ensemble may change them)
+  int cur_at_build = 0;
+  std::vector<const void*> storage_at_build;
+};
+
+std::vector<const void*> storage_signature(const EnsMember& m) {
+  std::vector<const void*> sig;
+  for (auto* s : m.sw) {
+    sig.push_back(s->st.d[LBSWIM_PART_R]);
+    sig.push_back(s->st.perm);
+  }
+  for (auto* t : m.tr) {
+    sig.push_back(t->st.d[LBSWIM_PART_R]);
+    sig.push_back(t->st.perm);
+  }
+  return sig;
+}
+
+// Workgroup orders the autotuner chooses between (results do not depend on it).
+enum { ORDER_MEMBER_MAJOR = 0, ORDER_ROUND_ROBIN = 1, ORDER_LARGEST_FIRST = 2, ORDER_COUNT = 3 };
+
+template <typename T>
+int upload(T** dev, size_t* cap, const std::vector<T>& host) {
+  if (host.empty()) return LBSWIM_OK;
+  if (*cap < host.size()) {
+    dev_free(*dev);
+    *dev = nullptr;
+    *cap = 0;
+    if (int rc = dev_alloc_zero(reinterpret_cast<void**>(dev), sizeof(T) * host.size())) return rc;
+    *cap = host.size();
+  }
+  HIP_TRY(hipMemcpyAsync(*dev, host.data(), sizeof(T) * host.size(), hipMemcpyHostToDevice, stream()));
+  return LBSWIM_OK;
+}
+
+// Reorders per-member blocks: member-major keeps each member contiguous,
+// round-robin interleaves members block by block, largest-first is
+// member-major with the biggest lattices first.
+std::vector<WorkItem> order_work(const std::vector<std::vector<WorkItem>>& per, int order,
+                                 const std::vector<size_t>& weight) {
+  std::vector<WorkItem> out;
+  std::vector<size_t> idx(per.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+  if (order == ORDER_LARGEST_FIRST)
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return weight[a] > weight[b]; });
+  if (order == ORDER_ROUND_ROBIN) {
+    size_t most = 0;
+    for (auto& v : per) most = std::max(most, v.size());
+    for (size_t k = 0; k < most; ++k)
+      for (size_t i : idx)
+        if (k < per[i].size()) out.push_back(per[i][k]);
+  } else {
+    for (size_t i : idx) out.insert(out.end(), per[i].begin(), per[i].end());
+  }
+  return out;
+}
+
+}  // namespace
+
+struct lbswim_ensemble {
+  int storage = LBSWIM_STORE_F64;
+  std::vector<EnsMember> members;
+  bool dirty = true;  // device tables out of date
+  int parity = 0;
+  int order = ORDER_MEMBER_MAJOR;
+  bool use_graph = true;
+  // device tables
+  LatDesc* d_lat = nullptr;
+  size_t cap_lat = 0;
+  WorkItem* d_latwork = nullptr;
+  size_t cap_latwork = 0;
+  int n_latwork = 0;
+  SwimDesc* d_sw = nullptr;
+  size_t cap_sw = 0;
+  WorkItem* d_swwork = nullptr;
+  size_t cap_swwork = 0;
+  int n_swwork = 0;
+  WorkItem* d_ptwork = nullptr;
+  size_t cap_ptwork = 0;
+  int n_ptwork = 0;
+  TracerDesc* d_tr = nullptr;
+  size_t cap_tr = 0;
+  WorkItem* d_trwork = nullptr;
+  size_t cap_trwork = 0;
+  int n_trwork = 0;
+  WorkItem* d_tilework = nullptr;
+  size_t cap_tilework = 0;
+  int ntiles = 0;
+  // spreading workspace
+  int npts = 0;
+  uint32_t* counts = nullptr;
+  size_t cap_counts = 0;
+  uint32_t* off = nullptr;
+  size_t cap_off = 0;
+  uint32_t* cursor = nullptr;
+  size_t cap_cursor = 0;
+  uint32_t* list = nullptr;
+  uint32_t* sorted = nullptr;
+  size_t cap_list = 0;
+  SpreadPoint* pts = nullptr;
+  size_t cap_pts = 0;
+  // two captured steps (parities 0,1 from `graph_parity`)
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool graph_valid = false;
+  int graph_parity = 0;
+};
+
+namespace {
+
+void ens_drop_graph(lbswim_ensemble* e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  e->graph_valid = false;
+}
+
+int ens_build(lbswim_ensemble* e) {
+  HIP_TRY(hipStreamSynchronize(stream()));
+  ens_drop_graph(e);
+  std::vector<LatDesc> lat;
+  std::vector<std::vector<WorkItem>> latwork, tilework;
+  std::vector<size_t> weight;
+  std::vector<SwimDesc> sw;
+  std::vector<WorkItem> swwork, ptwork, trwork;
+  std::vector<TracerDesc> tr;
+  uint32_t tile_base = 0, point_base = 0;
+  for (auto& m : e->members) {
+    if (!m.lat) continue;
+    lbswim_lattice* L = m.lat;
+    const uint32_t mi = uint32_t(lat.size());
+    LatDesc d{};
+    d.g[0] = L->d_f[0];
+    d.g[1] = L->d_f[1];
+    d.force = L->d_force;
+    d.rho = L->d_rho;
+    d.u = L->d_u;
+    d.geom = L->geom;
+    d.tau_s = L->params.tau_s;
+    d.tau_b = L->params.tau_b;
+    d.omega_s = 1.0 / (d.tau_s + 0.5);
+    d.omega_b = 1.0 / (d.tau_b + 0.5);
+    d.cur = L->cur;
+    d.sg = make_spread_geom(L->geom);
+    d.tile_base = tile_base;
+    lat.push_back(d);
+    std::vector<WorkItem> lw, tw;
+    for (uint32_t b = 0; b < lattice_blocks(L->geom); ++b) lw.push_back(WorkItem{mi, b});
+    for (uint32_t t = 0; t < uint32_t(d.sg.ntiles); ++t) tw.push_back(WorkItem{mi, t});
+    latwork.push_back(lw);
+    tilework.push_back(tw);
+    weight.push_back(L->n);
+    tile_base += uint32_t(d.sg.ntiles);
+    for (lbswim_swimmers* s : m.sw) {
+      if (int rc = pstore_maybe_reorder(s->st, L->geom)) return rc;
+      const uint32_t ai = uint32_t(sw.size());
+      SwimDesc a{};
+      a.r = fld(s->st, LBSWIM_PART_R);
+      a.v = fld(s->st, LBSWIM_PART_V);
+      a.n = fld(s->st, LBSWIM_PART_N);
+      a.prng = s->st.d[LBSWIM_PART_PRNG];
+      a.perm = s->st.perm;
+      a.tumbles = s->d_tumbles;
+      a.p = to_kernel(s->common_dev);
+      a.num = s->st.num;
+      a.member = int(mi);
+      a.point_base = point_base;
+      point_base += 2 * uint32_t(a.num);
+      sw.push_back(a);
+      for (uint32_t g = 0; g < uint32_t((a.num + PARTICLE_GROUPS - 1) / PARTICLE_GROUPS); ++g)
+        swwork.push_back(WorkItem{ai, g});
+      for (uint32_t b = 0; b < uint32_t((a.num + 255) / 256); ++b) ptwork.push_back(WorkItem{ai, b});
+    }
+    for (lbswim_tracers* t : m.tr) {
+      if (int rc = pstore_maybe_reorder(t->st, L->geom)) return rc;
+      const uint32_t ai = uint32_t(tr.size());
+      TracerDesc a{};
+      a.r = fld(t->st, LBSWIM_PART_R);
+      a.s = fld(t->st, LBSWIM_PART_S);
+      a.v = fld(t->st, LBSWIM_PART_V);
+      a.num = t->st.num;
+      a.member = int(mi);
+      tr.push_back(a);
+      for (uint32_t g = 0; g < uint32_t((a.num + PARTICLE_GROUPS - 1) / PARTICLE_GROUPS); ++g)
+        trwork.push_back(WorkItem{ai, g});
+    }
+  }
+  for (auto& m : e->members) {
+    if (!m.lat) continue;
+    m.cur_at_build = m.lat->cur;
+    m.storage_at_build = storage_signature(m);
+  }
+  const std::vector<WorkItem> lw = order_work(latwork, e->order, weight);
+  const std::vector<WorkItem> tw = order_work(tilework, e->order, weight);
+  int rc;
+  if ((rc = upload(&e->d_lat, &e->cap_lat, lat)) || (rc = upload(&e->d_latwork, &e->cap_latwork, lw)) ||
+      (rc = upload(&e->d_tilework, &e->cap_tilework, tw)) || (rc = upload(&e->d_sw, &e->cap_sw, sw)) ||
+      (rc = upload(&e->d_swwork, &e->cap_swwork, swwork)) || (rc = upload(&e->d_ptwork, &e->cap_ptwork, ptwork)) ||
+      (rc = upload(&e->d_tr, &e->cap_tr, tr)) || (rc = upload(&e->d_trwork, &e->cap_trwork, trwork)))
+    return rc;
+  e->n_latwork = int(lw.size());
+  e->n_swwork = int(swwork.size());
+  e->n_ptwork = int(ptwork.size());
+  e->n_trwork = int(trwork.size());
+  e->ntiles = int(tile_base);
+  e->npts = int(point_base);
+  // spreading workspace
+  size_t lcap = 0;
+  for (const auto& a : sw) lcap += spread_list_capacity(lat[a.member].sg, 2 * size_t(a.num));
+  auto grow_to = [&](auto** p, size_t* cap, size_t need, size_t elem) -> int {
+    if (*cap >= need) return LBSWIM_OK;
+    dev_free(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (int r = dev_alloc_zero(reinterpret_cast<void**>(p), elem * (need ? need : 1))) return r;
+    *cap = need;
+    return LBSWIM_OK;
+  };
+  const size_t nt = size_t(e->ntiles);
+  size_t cap_sorted = e->cap_list;
+  if ((rc = grow_to(&e->counts, &e->cap_counts, nt, 4)) ||
+      (rc = grow_to(&e->off, &e->cap_off, nt + 1 + (nt + 1023) / 1024, 4)) ||
+      (rc = grow_to(&e->cursor, &e->cap_cursor, nt, 4)) || (rc = grow_to(&e->pts, &e->cap_pts, size_t(e->npts), 176)) ||
+      (rc = grow_to(&e->sorted, &cap_sorted, lcap, 4)) || (rc = grow_to(&e->list, &e->cap_list, lcap, 4)))
+    return rc;
+  HIP_TRY(hipStreamSynchronize(stream()));
+  e->parity = 0;
+  e->dirty = false;
+  return LBSWIM_OK;
+}
+
+template <typename T>
+int ens_enqueue_step(lbswim_ensemble* e, int parity) {
+  hipStream_t s = stream();
+  HIP_TRY(launch_fill_zero(e->counts, 4 * size_t(e->ntiles), s));
+  HIP_TRY(launch_batch_points(e->d_lat, e->d_sw, e->d_ptwork, e->n_ptwork, e->pts, e->counts, s));
+  HIP_TRY(launch_batch_spread<T>(e->d_lat, e->npts, e->ntiles, e->d_tilework, e->pts, e->counts, e->off, e->cursor,
+                                 e->list, e->sorted, s));
+  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, s));
+  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, s));
+  HIP_TRY(launch_batch_swimmers_move<T>(e->d_lat, e->d_sw, e->d_swwork, e->n_swwork, s));
+  HIP_TRY(launch_batch_tracers_move<T>(e->d_lat, e->d_tr, e->d_trwork, e->n_trwork, s));
+  return LBSWIM_OK;
+}
+
+int ens_enqueue(lbswim_ensemble* e, int parity) {
+  return e->storage == LBSWIM_STORE_F64 ? ens_enqueue_step<double>(e, parity) : ens_enqueue_step<float>(e, parity);
+}
+
+int ens_capture(lbswim_ensemble* e) {
+  ens_drop_graph(e);
+  HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+  int rc = ens_enqueue(e, e->parity);
+  if (!rc) rc = ens_enqueue(e, e->parity ^ 1);
+  hipGraph_t g = nullptr;
+  hipError_t he = hipStreamEndCapture(stream(), &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (he != hipSuccess) return fail(LBSWIM_EHIP, "graph capture failed: %s", hipGetErrorString(he));
+  e->graph = g;
+  he = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+  if (he != hipSuccess) {
+    ens_drop_graph(e);
+    return fail(LBSWIM_EHIP, "graph instantiation failed: %s", hipGetErrorString(he));
+  }
+  e->graph_valid = true;
+  e->graph_parity = e->parity;
+  return LBSWIM_OK;
+}
+
+// Host bookkeeping after `k` steps: the reference's pointer swap and counter.
+void ens_advance(lbswim_ensemble* e, int k) {
+  for (auto& m : e->members) {
+    if (!m.lat) continue;
+    if (k & 1) m.lat->cur ^= 1;
+    m.lat->time_step += k;
+    for (auto* s : m.sw) s->st.moves_since_order += k;
+    for (auto* t : m.tr) t->st.moves_since_order += k;
+  }
+  if (k & 1) e->parity ^= 1;
+}
+
+int ens_check_member(lbswim_ensemble* e, int member) {
+  CHECK_ARG(member >= 0 && member < int(e->members.size()) && e->members[member].lat, "no ensemble member %d",
+            member);
+  return LBSWIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbswim_ensemble_create(int storage, lbswim_ensemble** out) {
+  CHECK_ARG(out != nullptr, "out is NULL");
+  *out = nullptr;
+  CHECK_ARG(storage == LBSWIM_STORE_F64 || storage == LBSWIM_STORE_F32DEV, "unknown storage form %d", storage);
+  if (int rc = ensure_runtime()) return rc;
+  lbswim_ensemble* e = new (std::nothrow) lbswim_ensemble();
+  if (!e) return fail(LBSWIM_ENOMEM, "ensemble allocation failed");
+  e->storage = storage;
+  *out = e;
+  return LBSWIM_OK;
+}
+
+void lbswim_ensemble_destroy(lbswim_ensemble* e) {
+  if (!e) return;
+  if (g_rt.ready) (void)hipStreamSynchronize(stream());
+  ens_drop_graph(e);
+  dev_free(e->d_lat);
+  dev_free(e->d_latwork);
+  dev_free(e->d_sw);
+  dev_free(e->d_swwork);
+  dev_free(e->d_ptwork);
+  dev_free(e->d_tr);
+  dev_free(e->d_trwork);
+  dev_free(e->d_tilework);
+  dev_free(e->counts);
+  dev_free(e->off);
+  dev_free(e->cursor);
+  dev_free(e->list);
+  dev_free(e->sorted);
+  dev_free(e->pts);
+  delete e;
+}
+
+int lbswim_ensemble_add(lbswim_ensemble* e, lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
+                        lbswim_tracers* const* tr, int ntr, int* member) {
+  CHECK_ARG(e && lat, "NULL argument");
+  CHECK_ARG(lat->storage == e->storage, "lattice storage form differs from the ensemble's");
+  CHECK_ARG(nsw >= 0 && ntr >= 0 && (nsw == 0 || sw) && (ntr == 0 || tr), "invalid particle lists");
+  for (auto& m : e->members) CHECK_ARG(m.lat != lat, "lattice is already a member");
+  EnsMember m;
+  m.lat = lat;
+  for (int i = 0; i < nsw; ++i) {
+    CHECK_ARG(sw[i] != nullptr, "swimmer handle %d is NULL", i);
+    m.sw.push_back(sw[i]);
+  }
+  for (int i = 0; i < ntr; ++i) {
+    CHECK_ARG(tr[i] != nullptr, "tracer handle %d is NULL", i);
+    m.tr.push_back(tr[i]);
+  }
+  size_t slot = 0;
+  while (slot < e->members.size() && e->members[slot].lat) ++slot;
+  if (slot == e->members.size())
+    e->members.push_back(m);
+  else
+    e->members[slot] = m;
+  e->dirty = true;
+  if (member) *member = int(slot);
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_remove(lbswim_ensemble* e, int member) {
+  CHECK_ARG(e, "NULL argument");
+  if (int rc = ens_check_member(e, member)) return rc;
+  HIP_TRY(hipStreamSynchronize(stream()));
+  e->members[member] = EnsMember();
+  e->dirty = true;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_size(lbswim_ensemble* e, int* members) {
+  CHECK_ARG(e && members, "NULL argument");
+  int k = 0;
+  for (auto& m : e->members) k += m.lat ? 1 : 0;
+  *members = k;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_set_graph(lbswim_ensemble* e, int on) {
+  CHECK_ARG(e, "NULL argument");
+  e->use_graph = on != 0;
+  if (!e->use_graph) ens_drop_graph(e);
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_set_order(lbswim_ensemble* e, int order) {
+  CHECK_ARG(e && order >= 0 && order < ORDER_COUNT, "invalid argument");
+  if (e->order != order) e->dirty = true;
+  e->order = order;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps) {
+  CHECK_ARG(e && nsteps >= 0, "invalid argument");
+  if (nsteps == 0) return LBSWIM_OK;
+  for (auto& m : e->members) {
+    if (!m.lat || e->dirty) continue;
+    if (m.lat->cur != (m.cur_at_build ^ e->parity) || storage_signature(m) != m.storage_at_build) e->dirty = true;
+  }
+  if (e->dirty)
+    if (int rc = ens_build(e)) return rc;
+  int done = 0;
+  if (e->use_graph && nsteps >= 2) {
+    if (!e->graph_valid || e->graph_parity != e->parity)
+      if (int rc = ens_capture(e)) return rc;
+    for (; done + 2 <= nsteps; done += 2) HIP_TRY(hipGraphLaunch(e->exec, stream()));
+    ens_advance(e, done);
+  }
+  for (; done < nsteps; ++done) {
+    if (int rc = ens_enqueue(e, e->parity)) return rc;
+    ens_advance(e, 1);
+  }
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice) {
+  CHECK_ARG(e && steps_per_trial > 0, "invalid argument");
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  int best = e->order;
+  float best_ms = 0.f;
+  int rc = LBSWIM_OK;
+  for (int o = 0; o < ORDER_COUNT && !rc; ++o) {
+    e->order = o;
+    e->dirty = true;
+    if ((rc = lbswim_ensemble_step(e, 2))) break;  // build + capture + warm-up
+    (void)hipEventRecord(a, stream());
+    if ((rc = lbswim_ensemble_step(e, steps_per_trial))) break;
+    (void)hipEventRecord(b, stream());
+    (void)hipEventSynchronize(b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (o == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = o;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc) return rc;
+  if (e->order != best) {
+    e->order = best;
+    e->dirty = true;
+  }
+  if (choice) *choice = best;
+  return LBSWIM_OK;
 }
 
 }  // extern "C"

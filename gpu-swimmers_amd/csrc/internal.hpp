@@ -79,10 +79,72 @@ size_t spread_list_capacity(const SpreadGeom& sg, size_t npts);
 // counts[ntiles] must be zero before the first launch_points of a step.
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
                          const Geom& g, const SpreadGeom& sg, SpreadPoint* pts, uint32_t* counts, hipStream_t s);
-// off[ntiles+1], cursor[ntiles], list/sorted[spread_list_capacity]
+// off[ntiles + 1 + ceil(ntiles/1024)] (offsets, then scan scratch), cursor[ntiles],
+// list/sorted[spread_list_capacity]
 template <typename T>
 hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const SpreadPoint* pts, uint32_t* counts,
                          uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, T* force, hipStream_t s);
+
+// ---- batched launches over the members of an ensemble
+// One lattice of a batch.  g[cur ^ parity] holds the state at the start of a
+// step launched with `parity`.
+struct LatDesc {
+  void* g[2];
+  void* force;
+  void* rho;
+  void* u;
+  Geom geom;
+  double tau_s, tau_b, omega_s, omega_b;
+  int cur;
+  uint32_t tile_base;  // first spreading tile of this member in the batch
+  SpreadGeom sg;
+};
+// One workgroup of a batched launch: member index and the block (or tile,
+// or particle group) of that member it processes.
+struct WorkItem {
+  uint32_t member, unit;
+};
+// One swimmer array of a batch.
+struct SwimDesc {
+  double* r;
+  double* v;
+  double* n;
+  uint64_t* prng;
+  const int* perm;
+  unsigned long long* tumbles;
+  SwimParams p;
+  int num;
+  int member;
+  uint32_t point_base;  // first point id of this array in the batch
+};
+// One tracer array of a batch.
+struct TracerDesc {
+  double* r;
+  double* s;
+  double* v;
+  int num;
+  int member;
+};
+uint32_t lattice_blocks(const Geom& g);  // workgroups of a one-lattice launch
+constexpr int PARTICLE_GROUPS = 16;      // particles per workgroup of the move kernels
+template <typename T>
+hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, hipStream_t s);
+// Force spreading of a batch: zero the tile counts first; points of array a go
+// to ids point_base_a + 2*user + end; work over swimmer blocks of 256.
+hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
+                               SpreadPoint* pts, uint32_t* counts, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_spread(const LatDesc* ldesc, int npts, int ntiles, const WorkItem* tile_work,
+                               const SpreadPoint* pts, uint32_t* counts, uint32_t* off, uint32_t* cursor,
+                               uint32_t* list, uint32_t* sorted, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_swimmers_move(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
+                                      hipStream_t s);
+template <typename T>
+hipError_t launch_batch_tracers_move(const LatDesc* ldesc, const TracerDesc* tdesc, const WorkItem* work, int nwork,
+                                     hipStream_t s);
 
 // ---- spatial ordering of particle storage (particle_order.hip)
 hipError_t launch_iota(int np, int* p, hipStream_t s);

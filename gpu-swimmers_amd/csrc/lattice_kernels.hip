@@ -221,12 +221,12 @@ __device__ __forceinline__ void store_all(double* __restrict__ gNew, uint32_t n,
 }
 
 // DoStep (lb.cu:72-218): pull-stream, then collide into the site's own slots.
-__global__ __launch_bounds__(BLOCK) void k_collide_stream_f64(const double* __restrict__ gOld,
+__device__ __forceinline__ void collide_f64_body(const double* __restrict__ gOld,
                                                               const double* __restrict__ force,
                                                               double* __restrict__ gNew, Geom g,
                                                               double tau_s, double tau_b, double omega_s,
-                                                              double omega_b) {
-  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+                                                              double omega_b, uint32_t blk) {
+  const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
   const Nbr d = neighbours(decompose(s, g), g);
@@ -240,11 +240,11 @@ __global__ __launch_bounds__(BLOCK) void k_collide_stream_f64(const double* __re
 }
 
 // lb.cu:250-295: only the density and momentum modes are needed.
-__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f64(const double* __restrict__ gCur,
+__device__ __forceinline__ void hydro_f64_body(const double* __restrict__ gCur,
                                                           double* __restrict__ force,
                                                           double* __restrict__ rho,
-                                                          double* __restrict__ u, Geom g, int zero_force) {
-  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+                                                          double* __restrict__ u, Geom g, int zero_force, uint32_t blk) {
+  const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
   double fl[Q];
@@ -394,12 +394,12 @@ __device__ __forceinline__ void inverse_dev32(const Moments32& m, float (&g)[Q])
   g[11] = W2 * (s10 - t10);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_collide_stream_f32(const float* __restrict__ fOld,
+__device__ __forceinline__ void collide_f32_body(const float* __restrict__ fOld,
                                                               const float* __restrict__ force,
                                                               float* __restrict__ fNew, Geom g,
                                                               float tau_s, float tau_b, float omega_s,
-                                                              float omega_b) {
-  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+                                                              float omega_b, uint32_t blk) {
+  const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
   float gl[Q];
@@ -413,11 +413,11 @@ __global__ __launch_bounds__(BLOCK) void k_collide_stream_f32(const float* __res
   for (int p = 0; p < Q; ++p) store_stream(&fNew[size_t(p) * n + s], out[p]);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f32(const float* __restrict__ f,
+__device__ __forceinline__ void hydro_f32_body(const float* __restrict__ f,
                                                           float* __restrict__ force,
                                                           float* __restrict__ drho_out,
-                                                          float* __restrict__ u, Geom g, int zero_force) {
-  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+                                                          float* __restrict__ u, Geom g, int zero_force, uint32_t blk) {
+  const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
   float gl[Q];
@@ -461,6 +461,65 @@ __global__ __launch_bounds__(BLOCK) void k_init_from_hydro_f32(const float* __re
   [&]<int... P>(std::integer_sequence<int, P...>) {
     ((f[size_t(P) * n + upstream<P>(d)] = out[P]), ...);
   }(std::make_integer_sequence<int, Q>{});
+}
+
+// ------------------------------------------------------------------------
+// Launch wrappers: one lattice, or a batch of lattices described by LatDesc
+// entries with one WorkItem (member, block of that member) per workgroup.
+// ------------------------------------------------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_collide_stream_f64(const double* __restrict__ gOld,
+                                                              const double* __restrict__ force,
+                                                              double* __restrict__ gNew, Geom g, double tau_s,
+                                                              double tau_b, double omega_s, double omega_b) {
+  collide_f64_body(gOld, force, gNew, g, tau_s, tau_b, omega_s, omega_b, blockIdx.x);
+}
+__global__ __launch_bounds__(BLOCK) void k_collide_stream_f32(const float* __restrict__ gOld,
+                                                              const float* __restrict__ force,
+                                                              float* __restrict__ gNew, Geom g, float tau_s,
+                                                              float tau_b, float omega_s, float omega_b) {
+  collide_f32_body(gOld, force, gNew, g, tau_s, tau_b, omega_s, omega_b, blockIdx.x);
+}
+__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f64(const double* __restrict__ gCur, double* __restrict__ force,
+                                                          double* __restrict__ rho, double* __restrict__ u, Geom g,
+                                                          int zero_force) {
+  hydro_f64_body(gCur, force, rho, u, g, zero_force, blockIdx.x);
+}
+__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f32(const float* __restrict__ gCur, float* __restrict__ force,
+                                                          float* __restrict__ rho, float* __restrict__ u, Geom g,
+                                                          int zero_force) {
+  hydro_f32_body(gCur, force, rho, u, g, zero_force, blockIdx.x);
+}
+
+// At the start of a batched step with `parity` p the state of member m is in
+// g[cur_m ^ p]; the collision writes g[cur_m ^ p ^ 1], which CalcHydro reads.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kb_collide(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
+                                                    int parity) {
+  const WorkItem w = work[blockIdx.x];
+  const LatDesc& m = desc[w.member];
+  const int c = m.cur ^ parity;
+  if constexpr (sizeof(T) == 8)
+    collide_f64_body(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
+                     static_cast<double*>(m.g[c ^ 1]), m.geom, m.tau_s, m.tau_b, m.omega_s, m.omega_b, w.unit);
+  else
+    collide_f32_body(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
+                     static_cast<float*>(m.g[c ^ 1]), m.geom, float(m.tau_s), float(m.tau_b), float(m.omega_s),
+                     float(m.omega_b), w.unit);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kb_hydro(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
+                                                  int parity) {
+  const WorkItem w = work[blockIdx.x];
+  const LatDesc& m = desc[w.member];
+  const int c = m.cur ^ parity ^ 1;
+  if constexpr (sizeof(T) == 8)
+    hydro_f64_body(static_cast<const double*>(m.g[c]), static_cast<double*>(m.force), static_cast<double*>(m.rho),
+                   static_cast<double*>(m.u), m.geom, 0, w.unit);
+  else
+    hydro_f32_body(static_cast<const float*>(m.g[c]), static_cast<float*>(m.force), static_cast<float*>(m.rho),
+                   static_cast<float*>(m.u), m.geom, 0, w.unit);
 }
 
 // ------------------------------------------------------------------------
@@ -585,6 +644,25 @@ hipError_t launch_init_from_hydro<float>(const float* rho, const float* u, float
   hipLaunchKernelGGL(k_init_from_hydro_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, rho, u, f, g);
   return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_collide<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity);
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_hydro<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity);
+  return hipGetLastError();
+}
+template hipError_t launch_batch_collide<double>(const LatDesc*, const WorkItem*, int, int, hipStream_t);
+template hipError_t launch_batch_collide<float>(const LatDesc*, const WorkItem*, int, int, hipStream_t);
+template hipError_t launch_batch_hydro<double>(const LatDesc*, const WorkItem*, int, int, hipStream_t);
+template hipError_t launch_batch_hydro<float>(const LatDesc*, const WorkItem*, int, int, hipStream_t);
+
+uint32_t lattice_blocks(const Geom& g) { return grid_for(g.n); }
 
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
   if (bytes == 0) return hipSuccess;

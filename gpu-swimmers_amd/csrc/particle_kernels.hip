@@ -262,16 +262,18 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_add_forces(int ns, double P
 // DoSwimmerArrayMove, SwimmerArray.cu:123-210 (norm starts at 0; with
 // rotation off and no tumble, n is kept).  A lane group per swimmer gathers
 // the stencils at r and at r - l n; lane 0 of the group finishes the move.
+constexpr int SWIM_STRIDE = 2 * TERMS + 7;  // odd in doubles: groups start on different banks
+
 template <typename T>
-__global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp, double* __restrict__ r,
-                                                          double* __restrict__ v, double* __restrict__ nv,
-                                                          uint64_t* __restrict__ prng, const int* __restrict__ perm,
-                                                          const T* __restrict__ u, Geom g,
-                                                          unsigned long long* __restrict__ tumbles) {
-  constexpr int STRIDE = 2 * TERMS + 7;  // odd in doubles: groups start on different banks
-  __shared__ double lds[GROUPS * STRIDE];
+__device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp, double* __restrict__ r,
+                                                   double* __restrict__ v, double* __restrict__ nv,
+                                                   uint64_t* __restrict__ prng, const int* __restrict__ perm,
+                                                   const T* __restrict__ u, const Geom& g,
+                                                   unsigned long long* __restrict__ tumbles, uint32_t blk,
+                                                   double* __restrict__ lds) {
+  constexpr int STRIDE = SWIM_STRIDE;
   const int grp = int(threadIdx.x) / GROUP, c = int(threadIdx.x) % GROUP;
-  const int is = int(blockIdx.x) * GROUPS + grp;
+  const int is = int(blk) * GROUPS + grp;
   const bool active = is < ns;
   double* terms = lds + grp * STRIDE;
   double R[3], N[3], rMinus[3];
@@ -336,13 +338,15 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp,
 
 // DoTracerArrayMove, TracerArray.cu:12-43 (s updated per particle).  A lane
 // group per tracer gathers the stencil; lane d of the group updates component d.
+constexpr int TRACER_STRIDE = TERMS + 1;
+
 template <typename T>
-__global__ __launch_bounds__(PBLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
-                                                         double* __restrict__ v, const T* __restrict__ u, Geom g) {
-  constexpr int STRIDE = TERMS + 1;
-  __shared__ double lds[GROUPS * STRIDE];
+__device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r, double* __restrict__ sd,
+                                                  double* __restrict__ v, const T* __restrict__ u, const Geom& g,
+                                                  uint32_t blk, double* __restrict__ lds) {
+  constexpr int STRIDE = TRACER_STRIDE;
   const int grp = int(threadIdx.x) / GROUP, c = int(threadIdx.x) % GROUP;
-  const int it = int(blockIdx.x) * GROUPS + grp;
+  const int it = int(blk) * GROUPS + grp;
   const bool active = it < nt;
   double* terms = lds + grp * STRIDE;
   double R[3];
@@ -360,6 +364,48 @@ __global__ __launch_bounds__(PBLOCK) void k_tracers_move(int nt, double* __restr
   const double Rn = R[d] + vel;
   r[size_t(nt) * d + it] = fmod(Rn + size, double(size));
   sd[size_t(nt) * d + it] += vel;
+}
+
+template <typename T>
+__global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp, double* __restrict__ r,
+                                                          double* __restrict__ v, double* __restrict__ nv,
+                                                          uint64_t* __restrict__ prng, const int* __restrict__ perm,
+                                                          const T* __restrict__ u, Geom g,
+                                                          unsigned long long* __restrict__ tumbles) {
+  __shared__ double lds[GROUPS * SWIM_STRIDE];
+  swimmers_move_body(ns, cp, r, v, nv, prng, perm, u, g, tumbles, blockIdx.x, lds);
+}
+
+template <typename T>
+__global__ __launch_bounds__(PBLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
+                                                         double* __restrict__ v, const T* __restrict__ u, Geom g) {
+  __shared__ double lds[GROUPS * TRACER_STRIDE];
+  tracers_move_body(nt, r, sd, v, u, g, blockIdx.x, lds);
+}
+
+// Batched moves: WorkItem.member indexes the particle-array table, unit is
+// the group block of that array; u and the geometry come from its lattice.
+template <typename T>
+__global__ __launch_bounds__(PBLOCK) void kb_swimmers_move(const LatDesc* __restrict__ ldesc,
+                                                           const SwimDesc* __restrict__ sdesc,
+                                                           const WorkItem* __restrict__ work) {
+  __shared__ double lds[GROUPS * SWIM_STRIDE];
+  const WorkItem w = work[blockIdx.x];
+  const SwimDesc& a = sdesc[w.member];
+  const LatDesc& m = ldesc[a.member];
+  swimmers_move_body(a.num, a.p, a.r, a.v, a.n, a.prng, a.perm, static_cast<const T*>(m.u), m.geom, a.tumbles,
+                     w.unit, lds);
+}
+
+template <typename T>
+__global__ __launch_bounds__(PBLOCK) void kb_tracers_move(const LatDesc* __restrict__ ldesc,
+                                                          const TracerDesc* __restrict__ tdesc,
+                                                          const WorkItem* __restrict__ work) {
+  __shared__ double lds[GROUPS * TRACER_STRIDE];
+  const WorkItem w = work[blockIdx.x];
+  const TracerDesc& a = tdesc[w.member];
+  const LatDesc& m = ldesc[a.member];
+  tracers_move_body(a.num, a.r, a.s, a.v, static_cast<const T*>(m.u), m.geom, w.unit, lds);
 }
 
 // ---- deterministic force spreading -----------------------------------------
@@ -391,11 +437,12 @@ __device__ __forceinline__ int tiles_of(const int (&wc)[4], int tdim, int ntd, i
   return m;
 }
 
-__global__ __launch_bounds__(PBLOCK) void k_points(int ns, int base, double P, double l, const double* __restrict__ r,
-                                                  const double* __restrict__ nv, const int* __restrict__ perm,
-                                                  Geom g, SpreadGeom sg, SpreadPoint* __restrict__ pts,
-                                                  uint32_t* __restrict__ counts) {
-  const int slot = int(blockIdx.x * PBLOCK + threadIdx.x);
+__device__ __forceinline__ void points_body(int ns, uint32_t point_base, double P, double l,
+                                            const double* __restrict__ r, const double* __restrict__ nv,
+                                            const int* __restrict__ perm, const Geom& g, const SpreadGeom& sg,
+                                            uint32_t tile_base, int member, SpreadPoint* __restrict__ pts,
+                                            uint32_t* __restrict__ counts, uint32_t blk) {
+  const int slot = int(blk * PBLOCK + threadIdx.x);
   if (slot >= ns) return;
   const int size[3] = {g.nx, g.ny, g.nz};
   const int tdim[3] = {sg.tx, sg.ty, sg.tz};
@@ -428,60 +475,89 @@ __global__ __launch_bounds__(PBLOCK) void k_points(int ns, int base, double P, d
       nt[d] = tiles_of(wc, tdim[d], ntd[d], tl[d]);
       sp.F[d] = end ? Ft[d] * -1.0 : Ft[d];
     }
-    sp.pad[0] = sp.pad[1] = 0;
-    pts[2 * (size_t(base) + size_t(user)) + end] = sp;
+    sp.pad[0] = member;
+    sp.pad[1] = 0;
+    pts[size_t(point_base) + 2 * size_t(user) + end] = sp;
     for (int a = 0; a < nt[0]; ++a)
       for (int b = 0; b < nt[1]; ++b)
         for (int c = 0; c < nt[2]; ++c)
-          atomicAdd(&counts[(size_t(tl[0][a]) * sg.nty + tl[1][b]) * sg.ntz + tl[2][c]], 1u);
+          atomicAdd(&counts[tile_base + (size_t(tl[0][a]) * sg.nty + tl[1][b]) * sg.ntz + tl[2][c]], 1u);
   }
 }
 
-// Exclusive scan of counts[0..nt) into off[0..nt] by one workgroup, 32
-// consecutive tiles per thread per pass; also clears cursor.
-__global__ __launch_bounds__(1024) void k_scan_tiles(int nt, const uint32_t* __restrict__ counts,
-                                                     uint32_t* __restrict__ off, uint32_t* __restrict__ cursor) {
-  constexpr int PER = 32, SPAN = 1024 * PER;
-  __shared__ uint32_t part[1024];
+__global__ __launch_bounds__(PBLOCK) void k_points(int ns, int base, double P, double l, const double* __restrict__ r,
+                                                  const double* __restrict__ nv, const int* __restrict__ perm,
+                                                  Geom g, SpreadGeom sg, SpreadPoint* __restrict__ pts,
+                                                  uint32_t* __restrict__ counts) {
+  points_body(ns, 2 * uint32_t(base), P, l, r, nv, perm, g, sg, 0, 0, pts, counts, blockIdx.x);
+}
+
+__global__ __launch_bounds__(PBLOCK) void kb_points(const LatDesc* __restrict__ ldesc,
+                                                   const SwimDesc* __restrict__ sdesc,
+                                                   const WorkItem* __restrict__ work, SpreadPoint* __restrict__ pts,
+                                                   uint32_t* __restrict__ counts) {
+  const WorkItem w = work[blockIdx.x];
+  const SwimDesc& a = sdesc[w.member];
+  const LatDesc& m = ldesc[a.member];
+  points_body(a.num, a.point_base, a.p.P, a.p.l, a.r, a.n, a.perm, m.geom, m.sg, m.tile_base, a.member, pts, counts,
+              w.unit);
+}
+
+// Exclusive scan of counts[0..nt) into off[0..nt] in two coalesced passes
+// over SCAN_BLOCK tiles per workgroup; the second pass also clears cursor.
+constexpr int SCAN_BLOCK = 1024;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* part, uint32_t* total) {
   const int t = int(threadIdx.x);
-  uint32_t carry = 0;
-  for (int base = 0; base < nt; base += SPAN) {
-    const int lo = base + t * PER;
-    uint32_t v[PER];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      v[i] = lo + i < nt ? counts[lo + i] : 0u;
-      sum += v[i];
-    }
-    part[t] = sum;
+  part[t] = v;
+  __syncthreads();
+  for (int w = 1; w < SCAN_BLOCK; w <<= 1) {
+    const uint32_t add = t >= w ? part[t - w] : 0u;
     __syncthreads();
-    for (int w = 1; w < 1024; w <<= 1) {
-      const uint32_t add = t >= w ? part[t - w] : 0u;
-      __syncthreads();
-      part[t] += add;
-      __syncthreads();
-    }
-    uint32_t run = carry + part[t] - sum;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      if (lo + i < nt) {
-        off[lo + i] = run;
-        cursor[lo + i] = 0u;
-      }
-      run += v[i];
-    }
-    carry += part[1023];
+    part[t] += add;
     __syncthreads();
   }
-  if (t == 0) off[nt] = carry;
+  const uint32_t incl = part[t];
+  *total = part[SCAN_BLOCK - 1];
+  __syncthreads();
+  return incl - v;
 }
 
-__global__ __launch_bounds__(PBLOCK) void k_fill_lists(int npts, SpreadGeom sg, const SpreadPoint* __restrict__ pts,
-                                                      const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
-                                                      uint32_t* __restrict__ list) {
-  const int id = int(blockIdx.x * PBLOCK + threadIdx.x);
-  if (id >= npts) return;
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(int nt, const uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ sums) {
+  __shared__ uint32_t part[SCAN_BLOCK];
+  const int i = int(blockIdx.x) * SCAN_BLOCK + int(threadIdx.x);
+  uint32_t total = 0;
+  block_exclusive_scan(i < nt ? counts[i] : 0u, part, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(int nt, const uint32_t* __restrict__ counts,
+                                                           const uint32_t* __restrict__ sums,
+                                                           uint32_t* __restrict__ off, uint32_t* __restrict__ cursor) {
+  __shared__ uint32_t part[SCAN_BLOCK];
+  __shared__ uint32_t base;
+  // offset of this workgroup: sum of the earlier workgroups' totals
+  uint32_t mine = 0;
+  for (int b = int(threadIdx.x); b < int(blockIdx.x); b += SCAN_BLOCK) mine += sums[b];
+  uint32_t all = 0;
+  block_exclusive_scan(mine, part, &all);
+  if (threadIdx.x == 0) base = all;
+  __syncthreads();
+  const int i = int(blockIdx.x) * SCAN_BLOCK + int(threadIdx.x);
+  const uint32_t v = i < nt ? counts[i] : 0u;
+  uint32_t total = 0;
+  const uint32_t ex = block_exclusive_scan(v, part, &total);
+  if (i < nt) {
+    off[i] = base + ex;
+    cursor[i] = 0u;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[nt] = base + total;
+}
+
+__device__ __forceinline__ void fill_point(const SpreadPoint& p, uint32_t id, const SpreadGeom& sg,
+                                           uint32_t tile_base, const uint32_t* __restrict__ off,
+                                           uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
   const int tdim[3] = {sg.tx, sg.ty, sg.tz};
   const int ntd[3] = {sg.ntx, sg.nty, sg.ntz};
   int tl[3][4], nt[3];
@@ -489,16 +565,36 @@ __global__ __launch_bounds__(PBLOCK) void k_fill_lists(int npts, SpreadGeom sg, 
   for (int d = 0; d < 3; ++d) {
     int wc[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) wc[k] = pts[id].wc[d * 4 + k];
+    for (int k = 0; k < 4; ++k) wc[k] = p.wc[d * 4 + k];
     nt[d] = tiles_of(wc, tdim[d], ntd[d], tl[d]);
   }
   for (int a = 0; a < nt[0]; ++a)
     for (int b = 0; b < nt[1]; ++b)
       for (int c = 0; c < nt[2]; ++c) {
-        const size_t tile = (size_t(tl[0][a]) * sg.nty + tl[1][b]) * sg.ntz + tl[2][c];
+        const size_t tile = tile_base + (size_t(tl[0][a]) * sg.nty + tl[1][b]) * sg.ntz + tl[2][c];
         const uint32_t pos = atomicAdd(&cursor[tile], 1u);
-        list[off[tile] + pos] = uint32_t(id);
+        list[off[tile] + pos] = id;
       }
+}
+
+__global__ __launch_bounds__(PBLOCK) void k_fill_lists(int npts, SpreadGeom sg, const SpreadPoint* __restrict__ pts,
+                                                      const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
+                                                      uint32_t* __restrict__ list) {
+  const int id = int(blockIdx.x * PBLOCK + threadIdx.x);
+  if (id >= npts) return;
+  fill_point(pts[id], uint32_t(id), sg, 0, off, cursor, list);
+}
+
+// Batched: the member of each point is in SpreadPoint::pad[0].
+__global__ __launch_bounds__(PBLOCK) void kb_fill_lists(const LatDesc* __restrict__ ldesc, int npts,
+                                                       const SpreadPoint* __restrict__ pts,
+                                                       const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
+                                                       uint32_t* __restrict__ list) {
+  const int id = int(blockIdx.x * PBLOCK + threadIdx.x);
+  if (id >= npts) return;
+  const SpreadPoint& p = pts[id];
+  const LatDesc& m = ldesc[p.pad[0]];
+  fill_point(p, uint32_t(id), m.sg, m.tile_base, off, cursor, list);
 }
 
 __global__ __launch_bounds__(PBLOCK) void k_sort_lists(const uint32_t* __restrict__ off,
@@ -554,13 +650,12 @@ __device__ __forceinline__ void add_point(const SpreadPoint& p, int X, int Y, in
 }
 
 template <typename T>
-__global__ __launch_bounds__(PBLOCK) void k_spread_tiles(Geom g, SpreadGeom sg, const SpreadPoint* __restrict__ pts,
-                                                        const uint32_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ sorted, T* __restrict__ force) {
-  __shared__ SpreadPoint sp[SPREAD_CHUNK];
-  const int tile = int(blockIdx.x);
+__device__ __forceinline__ void spread_tile_body(const Geom& g, const SpreadGeom& sg,
+                                                 const SpreadPoint* __restrict__ pts, const uint32_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ sorted, T* __restrict__ force, int tile,
+                                                 uint32_t gtile, SpreadPoint* __restrict__ sp) {
   const int tk = tile % sg.ntz, tj = (tile / sg.ntz) % sg.nty, ti = tile / (sg.ntz * sg.nty);
-  const uint32_t o = off[tile], cnt = off[tile + 1] - o;
+  const uint32_t o = off[gtile], cnt = off[gtile + 1] - o;
   const int size[3] = {g.nx, g.ny, g.nz};
   const bool simple = g.nx >= 4 && g.ny >= 4 && g.nz >= 4;
   constexpr int PER = 2;  // 512 sites per tile, 256 threads
@@ -601,6 +696,26 @@ __global__ __launch_bounds__(PBLOCK) void k_spread_tiles(Geom g, SpreadGeom sg, 
     force[n + site] = T(acc[q][1]);
     force[2 * n + site] = T(acc[q][2]);
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(PBLOCK) void k_spread_tiles(Geom g, SpreadGeom sg, const SpreadPoint* __restrict__ pts,
+                                                        const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ sorted, T* __restrict__ force) {
+  __shared__ SpreadPoint sp[SPREAD_CHUNK];
+  spread_tile_body(g, sg, pts, off, sorted, force, int(blockIdx.x), blockIdx.x, sp);
+}
+
+template <typename T>
+__global__ __launch_bounds__(PBLOCK) void kb_spread_tiles(const LatDesc* __restrict__ ldesc,
+                                                         const WorkItem* __restrict__ tile_work,
+                                                         const SpreadPoint* __restrict__ pts,
+                                                         const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ sorted) {
+  __shared__ SpreadPoint sp[SPREAD_CHUNK];
+  const WorkItem w = tile_work[blockIdx.x];
+  const LatDesc& m = ldesc[w.member];
+  spread_tile_body(m.geom, m.sg, pts, off, sorted, static_cast<T*>(m.force), int(w.unit), blockIdx.x, sp);
 }
 
 // ---- device-math probes used by the parity tests ----
@@ -700,7 +815,11 @@ hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r,
 template <typename T>
 hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const SpreadPoint* pts, uint32_t* counts,
                          uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, T* force, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, sg.ntiles, counts, off, cursor);
+  // the per-workgroup totals of the scan live after the nt+1 offsets
+  const unsigned nsb = unsigned((sg.ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  uint32_t* sums = off + sg.ntiles + 1;
+  hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, sg.ntiles, counts, sums);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, sg.ntiles, counts, sums, off, cursor);
   if (npts > 0)
     hipLaunchKernelGGL(k_fill_lists, dim3(blocks_for(size_t(npts))), dim3(PBLOCK), 0, s, npts, sg, pts, off, cursor,
                        list);
@@ -712,6 +831,58 @@ template hipError_t launch_spread<double>(int, const Geom&, const SpreadGeom&, c
                                           uint32_t*, uint32_t*, uint32_t*, double*, hipStream_t);
 template hipError_t launch_spread<float>(int, const Geom&, const SpreadGeom&, const SpreadPoint*, uint32_t*, uint32_t*,
                                          uint32_t*, uint32_t*, uint32_t*, float*, hipStream_t);
+
+hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
+                               SpreadPoint* pts, uint32_t* counts, hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_points, dim3(nwork), dim3(PBLOCK), 0, s, ldesc, sdesc, work, pts, counts);
+  return hipGetLastError();
+}
+
+// Tiles of all members are numbered consecutively (member m from its
+// tile_base); tile_work lists (member, local tile) for each of them.
+template <typename T>
+hipError_t launch_batch_spread(const LatDesc* ldesc, int npts, int ntiles, const WorkItem* tile_work,
+                               const SpreadPoint* pts, uint32_t* counts, uint32_t* off, uint32_t* cursor,
+                               uint32_t* list, uint32_t* sorted, hipStream_t s) {
+  const unsigned nsb = unsigned((ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  uint32_t* sums = off + ntiles + 1;
+  hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, ntiles, counts, sums);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, ntiles, counts, sums, off, cursor);
+  if (npts > 0)
+    hipLaunchKernelGGL(kb_fill_lists, dim3(blocks_for(size_t(npts))), dim3(PBLOCK), 0, s, ldesc, npts, pts, off,
+                       cursor, list);
+  hipLaunchKernelGGL(k_sort_lists, dim3(ntiles), dim3(PBLOCK), 0, s, off, list, sorted);
+  hipLaunchKernelGGL(kb_spread_tiles<T>, dim3(ntiles), dim3(PBLOCK), 0, s, ldesc, tile_work, pts, off, sorted);
+  return hipGetLastError();
+}
+template hipError_t launch_batch_spread<double>(const LatDesc*, int, int, const WorkItem*, const SpreadPoint*,
+                                                uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, hipStream_t);
+template hipError_t launch_batch_spread<float>(const LatDesc*, int, int, const WorkItem*, const SpreadPoint*,
+                                               uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, hipStream_t);
+
+template <typename T>
+hipError_t launch_batch_swimmers_move(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
+                                      hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_swimmers_move<T>, dim3(nwork), dim3(PBLOCK), 0, s, ldesc, sdesc, work);
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_batch_tracers_move(const LatDesc* ldesc, const TracerDesc* tdesc, const WorkItem* work, int nwork,
+                                     hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_tracers_move<T>, dim3(nwork), dim3(PBLOCK), 0, s, ldesc, tdesc, work);
+  return hipGetLastError();
+}
+template hipError_t launch_batch_swimmers_move<double>(const LatDesc*, const SwimDesc*, const WorkItem*, int,
+                                                       hipStream_t);
+template hipError_t launch_batch_swimmers_move<float>(const LatDesc*, const SwimDesc*, const WorkItem*, int,
+                                                      hipStream_t);
+template hipError_t launch_batch_tracers_move<double>(const LatDesc*, const TracerDesc*, const WorkItem*, int,
+                                                      hipStream_t);
+template hipError_t launch_batch_tracers_move<float>(const LatDesc*, const TracerDesc*, const WorkItem*, int,
+                                                     hipStream_t);
 
 hipError_t launch_peskin_probe(const double* x, double* out, int count, hipStream_t s) {
   if (count <= 0) return hipSuccess;

@@ -94,6 +94,18 @@ cdef extern from "lbswim.h":
     int lbswim_tracers_d2h(lbswim_tracers* tr, int field)
     int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers** sw, int nsw, lbswim_tracers** tr, int ntr,
                            int nsteps)
+    ctypedef struct lbswim_ensemble:
+        pass
+    int lbswim_ensemble_create(int storage, lbswim_ensemble** out)
+    void lbswim_ensemble_destroy(lbswim_ensemble* e)
+    int lbswim_ensemble_add(lbswim_ensemble* e, lbswim_lattice* lat, lbswim_swimmers** sw, int nsw,
+                            lbswim_tracers** tr, int ntr, int* member)
+    int lbswim_ensemble_remove(lbswim_ensemble* e, int member)
+    int lbswim_ensemble_size(lbswim_ensemble* e, int* members)
+    int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps)
+    int lbswim_ensemble_set_graph(lbswim_ensemble* e, int on)
+    int lbswim_ensemble_set_order(lbswim_ensemble* e, int order)
+    int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice)
     int lbswim_lattice_totals(lbswim_lattice* lat, double* out)
     int lbswim_lattice_set_timing(lbswim_lattice* lat, int on)
     int lbswim_lattice_kernel_time(lbswim_lattice* lat, int kernel, double* total_ms, long* count)
@@ -839,3 +851,86 @@ def diag_interp(size, u, r):
     if count:
         check(lbswim_diag_interp(size[0], size[1], size[2], &ua[0], &ra[0], &v[0], count))
     return v.reshape(3, count)
+
+
+# ---------------------------------------------------------------------------
+# ensembles (no reference counterpart)
+# ---------------------------------------------------------------------------
+
+cdef class Ensemble:
+    """Independent systems (lattice + swimmer and tracer arrays) advanced by
+    batched launches; each member evolves exactly as it would alone.
+
+    Ensemble(storage="f64"); add(lat, things) -> member id; step(n);
+    remove(member); autotune(k); set_graph(bool).
+    """
+    cdef lbswim_ensemble* impl
+    cdef dict _members
+    cdef str _storage
+
+    def __cinit__(self, storage="f64"):
+        self.impl = NULL
+        if storage not in STORAGE:
+            raise ValueError("storage must be 'f64' or 'f32', not %r" % (storage,))
+        check(lbswim_ensemble_create(STORAGE[storage], &self.impl))
+        self._members = {}
+        self._storage = storage
+
+    def __dealloc__(self):
+        if self.impl is not NULL:
+            lbswim_ensemble_destroy(self.impl)
+            self.impl = NULL
+
+    def add(self, Lattice lat, things=()):
+        cdef lbswim_swimmers* sw[64]
+        cdef lbswim_tracers* tr[64]
+        cdef int nsw = 0, ntr = 0, member = -1
+        for th in things:
+            if isinstance(th, SwimmerArray):
+                if nsw == 64:
+                    raise ValueError("at most 64 swimmer arrays per member")
+                sw[nsw] = (<SwimmerArray>th).handle()
+                nsw += 1
+            elif isinstance(th, TracerArray):
+                if ntr == 64:
+                    raise ValueError("at most 64 tracer arrays per member")
+                tr[ntr] = (<TracerArray>th).handle()
+                ntr += 1
+            else:
+                raise TypeError("ensemble members take native swimmer and tracer arrays only")
+        kinds = [isinstance(th, SwimmerArray) for th in things]
+        if kinds != sorted(kinds, reverse=True):
+            raise ValueError("list swimmer arrays before tracer arrays (the step moves them in that order)")
+        check(lbswim_ensemble_add(self.impl, lat.handle(), sw, nsw, tr, ntr, &member))
+        self._members[member] = (lat, tuple(things))  # keep the objects alive
+        return member
+
+    def remove(self, int member):
+        check(lbswim_ensemble_remove(self.impl, member))
+        del self._members[member]
+
+    def member(self, int member):
+        return self._members[member]
+
+    @property
+    def members(self):
+        return dict(self._members)
+
+    def __len__(self):
+        cdef int k = 0
+        check(lbswim_ensemble_size(self.impl, &k))
+        return k
+
+    def step(self, int nsteps=1):
+        check(lbswim_ensemble_step(self.impl, nsteps))
+
+    def set_graph(self, bint on):
+        check(lbswim_ensemble_set_graph(self.impl, on))
+
+    def set_order(self, int order):
+        check(lbswim_ensemble_set_order(self.impl, order))
+
+    def autotune(self, int steps_per_trial=10):
+        cdef int choice = 0
+        check(lbswim_ensemble_autotune(self.impl, steps_per_trial, &choice))
+        return choice

@@ -205,6 +205,34 @@ int lbswim_tracers_d2h(lbswim_tracers* tr, int field);
 int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
                        int ntr, int nsteps);
 
+/* ---- ensembles (no reference counterpart) ------------------------------
+ * Many independent systems -- each a lattice with its swimmer and tracer
+ * arrays, as in lbswim_system_step -- advanced together: every phase of the
+ * coupled step is one launch over all members (descriptor tables on the
+ * device), and pairs of steps are replayed from a captured hipGraph.  Each
+ * member's result is bit-identical to stepping it alone.  Members stay owned
+ * by the caller and must outlive their membership; all members share the
+ * ensemble's storage form.  Members may also be used directly between
+ * ensemble steps (the ensemble notices and rebuilds its tables). */
+typedef struct lbswim_ensemble lbswim_ensemble;
+int lbswim_ensemble_create(int storage, lbswim_ensemble** out);
+void lbswim_ensemble_destroy(lbswim_ensemble* e);
+/* Adds a system; *member receives its slot id (slots of removed members are reused). */
+int lbswim_ensemble_add(lbswim_ensemble* e, lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
+                        lbswim_tracers* const* tr, int ntr, int* member);
+int lbswim_ensemble_remove(lbswim_ensemble* e, int member);
+int lbswim_ensemble_size(lbswim_ensemble* e, int* members);
+/* nsteps coupled steps of every member. */
+int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps);
+/* 1 (default): replay captured graphs of two steps; 0: launch eagerly. */
+int lbswim_ensemble_set_graph(lbswim_ensemble* e, int on);
+/* Workgroup order of the batched launches: 0 member-major, 1 round-robin over
+ * members, 2 largest member first. Results do not depend on it. */
+int lbswim_ensemble_set_order(lbswim_ensemble* e, int order);
+/* Times each order over steps_per_trial (+2 warm-up) real steps of the
+ * ensemble -- the members advance by those steps -- and keeps the fastest. */
+int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice);
+
 /* ---- diagnostics (no reference counterpart) ---------------------------- */
 
 /* FP64 totals of the current distributions: out[0] = total mass, out[1..3] = total momentum. */

@@ -1,0 +1,113 @@
+"""GPU tests of batched ensembles: every member must evolve exactly as it
+does alone (lbswim_system_step), whatever the batch around it, the launch
+order chosen by the autotuner, graph replay or eager launches."""
+import sys
+
+import numpy as np
+import pytest
+
+from cases import build_lbswim, collect_lbswim, make_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def uneven_cases(k, seed=0, sizes=(16, 24, 32, 48), steps=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        L = int(rng.choice(sizes))
+        dims = (L, L, L) if i % 3 else (L, L // 2 + 3, L + 5)  # some non-cubic, odd-sized
+        ns = int(rng.integers(1, 65))
+        nt = int(rng.integers(0, 300))
+        tau = float(rng.uniform(0.2, 1.0))
+        alpha = float(rng.uniform(0.0, 0.05))
+        out.append(make_case(dims, steps, tau, ns=ns, nt=nt, alpha=alpha, seed=1234 + i + 100 * seed))
+    return out
+
+
+def build_member(case, storage):
+    lat, system, sw, tr = build_lbswim(case, storage)
+    things = [t for t in (sw, tr) if t is not None]
+    return lat, system, sw, tr, things
+
+
+def run_alone(case, storage, steps):
+    lat, system, sw, tr, _ = build_member(case, storage)
+    system.Step(steps)
+    return collect_lbswim(lat, sw, tr)
+
+
+def assert_same(a, b, label):
+    for k in a:
+        assert np.array_equal(a[k], b[k]), (label, k)
+
+
+@pytest.mark.parametrize("storage,graph,steps", [("f64", True, 9), ("f64", False, 4), ("f32", True, 8)])
+def test_members_match_alone(storage, graph, steps):
+    from lbswim.ensemble import Ensemble
+    cases = uneven_cases(8, seed=1)
+    say("ensemble vs alone: %s graph=%s steps=%d sizes=%s" % (storage, graph, steps, [c.size for c in cases]))
+    ens = Ensemble(storage)
+    ens.set_graph(graph)
+    built = []
+    for c in cases:
+        lat, system, sw, tr, things = build_member(c, storage)
+        ens.add(lat, things)
+        built.append((lat, sw, tr))
+    ens.step(steps)
+    for i, (c, (lat, sw, tr)) in enumerate(zip(cases, built)):
+        assert lat.time_step == steps
+        assert_same(collect_lbswim(lat, sw, tr), run_alone(c, storage, steps), "member %d" % i)
+
+
+def test_membership_changes_and_direct_use():
+    from lbswim.ensemble import Ensemble
+    cases = uneven_cases(5, seed=2)
+    say("ensemble membership changes")
+    ens = Ensemble("f64")
+    built = []
+    for c in cases[:4]:
+        lat, system, sw, tr, things = build_member(c, "f64")
+        built.append((ens.add(lat, things), lat, system, sw, tr))
+    ens.step(3)
+    ens.remove(built[1][0])           # member 1 leaves after 3 steps
+    lat, system, sw, tr, things = build_member(cases[4], "f64")
+    mid = ens.add(lat, things)        # a newcomer takes the free slot
+    assert mid == built[1][0]
+    ens.step(4)
+    built[0][2].Step(2)               # member 0 also stepped on its own
+    ens.step(1)
+    expect = {0: 3 + 4 + 2 + 1, 2: 8, 3: 8}
+    for j, steps in expect.items():
+        _, L, _, S, T = built[j]
+        assert L.time_step == steps
+        assert_same(collect_lbswim(L, S, T), run_alone(cases[j], "f64", steps), "member %d" % j)
+    assert lat.time_step == 5
+    assert_same(collect_lbswim(lat, sw, tr), run_alone(cases[4], "f64", 5), "newcomer")
+    _, L1, _, S1, T1 = built[1]
+    assert L1.time_step == 3
+    assert_same(collect_lbswim(L1, S1, T1), run_alone(cases[1], "f64", 3), "removed member")
+
+
+def test_autotune_keeps_results():
+    from lbswim.ensemble import Ensemble
+    cases = uneven_cases(6, seed=3)
+    say("ensemble autotune")
+    ens = Ensemble("f64")
+    built = []
+    for c in cases:
+        lat, system, sw, tr, things = build_member(c, "f64")
+        ens.add(lat, things)
+        built.append((lat, sw, tr))
+    choice = ens.autotune(3)          # 3 orders x (2 + 3) real steps
+    assert choice in (0, 1, 2)
+    ens.step(5)
+    total = 3 * (2 + 3) + 5
+    for c, (lat, sw, tr) in zip(cases, built):
+        assert lat.time_step == total
+        assert_same(collect_lbswim(lat, sw, tr), run_alone(c, "f64", total), "after autotune")

@@ -715,7 +715,8 @@ __global__ __launch_bounds__(PBLOCK) void kb_spread_tiles(const LatDesc* __restr
   __shared__ SpreadPoint sp[SPREAD_CHUNK];
   const WorkItem w = tile_work[blockIdx.x];
   const LatDesc& m = ldesc[w.member];
-  spread_tile_body(m.geom, m.sg, pts, off, sorted, static_cast<T*>(m.force), int(w.unit), blockIdx.x, sp);
+  spread_tile_body(m.geom, m.sg, pts, off, sorted, static_cast<T*>(m.force), int(w.unit), m.tile_base + w.unit,
+                   sp);
 }
 
 // ---- device-math probes used by the parity tests ----

@@ -126,6 +126,22 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5):
             "graph": True}
 
 
+def bench_serving(nreq, rate, seed=1234):
+    """Poisson stream of requests (cube size U[24,96], steps log-uniform in
+    [200,5000], 16-4096 swimmers, 0-1e5 tracers, FP32/FP64 50/50) served with
+    continuous batching under an HBM budget, then the same stream served one
+    request at a time with the plain step."""
+    from lbswim.serving import Server, poisson_requests, run_sequential
+    reqs = poisson_requests(nreq, rate, seed)
+    srv = Server(hbm_budget=64 << 30, max_members=64, chunk=100, keep_results=False)
+    batched = srv.run(reqs, realtime=True)
+    seq = run_sequential(poisson_requests(nreq, rate, seed), realtime=True)
+    r = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
+    return {"rate_per_s": rate, "batched": r(batched), "sequential": r(seq),
+            "throughput_ratio": round(batched["mlups"] / seq["mlups"], 2),
+            "p99_ratio": round(seq["p99_s"] / batched["p99_s"], 2)}
+
+
 def cpu_baseline(case, steps=3):
     """The FP64 oracle (OpenMP over the host cores) on the same workload."""
     import lbswim_oracle as O
@@ -170,6 +186,8 @@ def main():
     ap.add_argument("--forms", default="f64,f32")
     ap.add_argument("--probe", choices=["copy"], help="run the streaming-copy bandwidth probe instead")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-sim ensemble leg (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate of the serving leg (1/s)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -243,6 +261,10 @@ def main():
             log("ensemble %s: 64 sims x %d steps, batched vs sequential" % (form, a.ensemble_steps))
             out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps)
             log("ensemble %s: %s" % (form, out["ensemble"][form]))
+    if a.serve_requests > 0:
+        log("serving: %d requests at %.1f/s, continuous batching vs sequential" % (a.serve_requests, a.serve_rate))
+        out["serving"] = bench_serving(a.serve_requests, a.serve_rate)
+        log("serving: %s" % out["serving"])
     if not a.no_cpu:
         log("CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(case)

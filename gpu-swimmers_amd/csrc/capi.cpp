@@ -1035,6 +1035,13 @@ int lbswim_lattice_kernel_time(lbswim_lattice* lat, int kernel, double* total_ms
   return LBSWIM_OK;
 }
 
+int lbswim_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  CHECK_ARG(free_bytes && total_bytes, "NULL argument");
+  if (int rc = ensure_runtime()) return rc;
+  HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+  return LBSWIM_OK;
+}
+
 int lbswim_diag_copy(size_t count, int reps, double* ms) {
   CHECK_ARG(count > 0 && reps > 0 && ms, "invalid argument");
   if (int rc = ensure_runtime()) return rc;

@@ -109,6 +109,7 @@ cdef extern from "lbswim.h":
     int lbswim_lattice_totals(lbswim_lattice* lat, double* out)
     int lbswim_lattice_set_timing(lbswim_lattice* lat, int on)
     int lbswim_lattice_kernel_time(lbswim_lattice* lat, int kernel, double* total_ms, long* count)
+    int lbswim_device_memory(size_t* free_bytes, size_t* total_bytes)
     int lbswim_diag_copy(size_t count, int reps, double* ms)
     int lbswim_diag_peskin(const double* x, double* out, int count)
     int lbswim_diag_interp(int nx, int ny, int nz, const double* u, const double* r, double* v, int count)
@@ -825,6 +826,13 @@ def system_step(Lattice lat, list swimmers, list tracers, int nsteps=1):
     for i in range(ntr):
         tr[i] = (<TracerArray?>tracers[i]).handle()
     check(lbswim_system_step(lat.handle(), sw, nsw, tr, ntr, nsteps))
+
+
+def device_memory():
+    """(free, total) device memory in bytes."""
+    cdef size_t f = 0, t = 0
+    check(lbswim_device_memory(&f, &t))
+    return f, t
 
 
 def diag_copy(size_t count, int reps):

@@ -242,6 +242,8 @@ int lbswim_lattice_totals(lbswim_lattice* lat, double out[4]);
 int lbswim_lattice_set_timing(lbswim_lattice* lat, int on);
 /* Sum of recorded durations (ms) and launch count for one LBSWIM_K_* kernel. */
 int lbswim_lattice_kernel_time(lbswim_lattice* lat, int kernel, double* total_ms, long* count);
+/* Free and total device memory in bytes (hipMemGetInfo). */
+int lbswim_device_memory(size_t* free_bytes, size_t* total_bytes);
 /* Copy `count` doubles device->device with a plain streaming kernel `reps` times;
  * *ms is the average per copy (bandwidth probe). */
 int lbswim_diag_copy(size_t count, int reps, double* ms);

@@ -68,8 +68,33 @@ def time_gpu(case, storage, steps, warmup):
     res["collide_gbs"] = COLLIDE_BYTES[storage] * n / (coll * 1e-3) / 1e9
     res["step_gbs"] = STEP_BYTES[storage] * n / (ms_step * 1e-3) / 1e9
     res["tumbles"] = sw.tumbles()
+    # final state for the FP32-vs-FP64 error report (same inputs, same steps)
+    for name in ("rho", "u"):
+        getattr(lat, name).D2H()
+    sw.D2H()
+    tr.D2H()
+    res["state"] = {"rho": np.array(lat.rho.data).ravel(), "u": np.array(lat.u.data).ravel(),
+                    "sw_r": np.array(sw.r.data), "sw_n": np.array(sw.n.data), "tr_r": np.array(tr.r.data)}
     del lat, system, sw, tr
     return res
+
+
+def precision_report(f64, f32, L):
+    """Norm-wise relative errors ||a-b||_inf/||b||_inf of the FP32-storage run
+    against the FP64 run of the same inputs and step count.  Trajectory
+    errors use the minimum-image displacement (positions are periodic)."""
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    def traj(a, b, L):
+        d = np.abs(a - b)
+        d = np.minimum(d, L - d)
+        return float(np.max(d))
+
+    a, b = f32["state"], f64["state"]
+    return {"u_rel": rel(a["u"], b["u"]), "drho_rel": rel(a["rho"] - 1.0, b["rho"] - 1.0),
+            "swimmer_r_abs": traj(a["sw_r"], b["sw_r"], L), "swimmer_n_abs": float(np.max(np.abs(a["sw_n"] - b["sw_n"]))),
+            "tracer_r_abs": traj(a["tr_r"], b["tr_r"], L), "note": "after warm-up + timed steps on identical inputs"}
 
 
 def ensemble_cases(nsims=64, seed=1234):
@@ -247,6 +272,8 @@ def main():
         "step_hbm_frac": round(m["step_gbs"] / HBM_PEAK_GBS, 4),
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in m["kernels"].items()},
     }
+    if "f32" in res and "f64" in res:
+        out["fp32_error_vs_fp64"] = precision_report(res["f64"], res["f32"], float(a.size))
     if "f32" in res and main_form != "f32":
         f = res["f32"]
         out["fp32_store"] = {"mlups": round(f["mlups"], 2), "ms_per_step": round(f["ms_per_step"], 4),

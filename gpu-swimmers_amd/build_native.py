@@ -81,9 +81,22 @@ def build_cython():
     return out
 
 
+EXAMPLE = os.path.join(LIBDIR, "lbswim_readme_coupled")
+
+
+def build_example():
+    """The C++ API example used by the GPU tests (tests/cpp/readme_coupled.cpp)."""
+    src = os.path.join(ROOT, "tests", "cpp", "readme_coupled.cpp")
+    tmp = EXAMPLE + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", tmp, src, "-L" + LIBDIR, "-llbswim_amd",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, EXAMPLE)
+    return EXAMPLE
+
+
 def build_all(cython=True, jobs=4):
     lib = build_hip(jobs)
-    outs = [lib]
+    outs = [lib, build_example()]
     if cython:
         outs.append(build_cython())
     return outs

@@ -112,7 +112,8 @@ def build(req):
     if req.swimmers:
         cp = swimmers.CommonParams()
         cp.P, cp.l, cp.alpha, cp.seed = 1e-3, 0.1, req.alpha, req.seed
-        cp.mobility = (1.0 / 0.1 - 1.0 / 1.5) / (6.0 * math.pi * req.tau / 3.0)
+        eta = req.tau / 3
+        cp.mobility = (1.0 / 0.1 - 1.0 / 1.5) / (6.0 * math.pi * eta)
         sw = swimmers.Array(req.swimmers, cp)
         sw.r[:] = sw_r
         sw.n[:] = sw_n

@@ -28,5 +28,6 @@ def isotropic(rng, num):
 
 
 def readme_mobility(tau, a=0.1, hydro=1.5):
-    """(1/a - 1/hydro)/(6 pi eta), eta = tau/3 (README.md:93-106)."""
-    return (1.0 / a - 1.0 / hydro) / (6.0 * np.pi * tau / 3.0)
+    """(1/a - 1/hydro)/(6 pi eta) with eta = tau/3, evaluated as README.md:88-106 writes it."""
+    eta = tau / 3
+    return (1.0 / a - 1.0 / hydro) / (6.0 * np.pi * eta)

@@ -25,12 +25,21 @@ sys.path[:0] = [os.path.join(ROOT, "gpu-swimmers_amd"), os.path.join(ROOT, "test
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
-# algorithmic bytes per site of one collide+stream launch: 15 f read + 15 f
-# written + 3 force read, at the storage width of each form
-COLLIDE_BYTES = {"f64": 30 * 8 + 3 * 8, "f32": 30 * 4 + 3 * 4}
-# whole coupled step per site: Step + CalcHydro (15 f + 3 F read, rho + 3 u
-# written) + ZeroForce (3 F written)
-STEP_BYTES = {"f64": 464, "f32": 232}
+ELEM = {"f64": 8, "f32": 4}
+
+
+def collide_bytes(form, virtual):
+    """Algorithmic bytes per site of one collide+stream launch: 15 f read and
+    15 f written, plus 3 force components read unless the force is virtual
+    (evaluated from the per-block point lists, which stay in cache)."""
+    return (30 + (0 if virtual else 3)) * ELEM[form]
+
+
+def step_bytes(form, virtual):
+    """Lattice bytes per site of the whole coupled step: collide+stream,
+    CalcHydro (15 f read, rho and 3 u written, 3 force read) and the force
+    array written once -- the last two only with a stored force."""
+    return (30 + 15 + 4 + (0 if virtual else 3 + 3 + 3)) * ELEM[form]
 
 
 def log(msg):
@@ -42,11 +51,11 @@ def make_workload(size, seed=1234):
     return make_case((size, size, size), 0, 0.25, ns=32768, nt=1_000_000, alpha=0.01, seed=seed)
 
 
-def time_gpu(case, storage, steps, warmup):
+def time_gpu(case, storage, steps, warmup, force_mode="auto"):
     from cases import build_lbswim
     import lbswim
 
-    lat, system, sw, tr = build_lbswim(case, storage)
+    lat, system, sw, tr = build_lbswim(case, storage, force_mode)
     system.Step(warmup)
     lbswim.synchronize()
     lat.set_timing(True)
@@ -57,16 +66,19 @@ def time_gpu(case, storage, steps, warmup):
     t1 = time.perf_counter()
     times = lat.kernel_times()
     lat.set_timing(False)
+    virtual = lat.force_is_virtual
     n = lat.addr.n
     ms_step = (t1 - t0) * 1e3 / steps
     mlups = n * steps / (t1 - t0) / 1e6
-    res = {"ms_per_step": ms_step, "mlups": mlups, "kernels": {}}
+    res = {"ms_per_step": ms_step, "mlups": mlups, "kernels": {}, "virtual_force": virtual}
     for name, (ms, cnt) in times.items():
         if cnt:
             res["kernels"][name] = {"avg_ms": ms / cnt, "launches": cnt}
     coll = res["kernels"]["collide_stream"]["avg_ms"]
-    res["collide_gbs"] = COLLIDE_BYTES[storage] * n / (coll * 1e-3) / 1e9
-    res["step_gbs"] = STEP_BYTES[storage] * n / (ms_step * 1e-3) / 1e9
+    res["collide_bytes"] = collide_bytes(storage, virtual)
+    res["step_bytes"] = step_bytes(storage, virtual)
+    res["collide_gbs"] = res["collide_bytes"] * n / (coll * 1e-3) / 1e9
+    res["step_gbs"] = res["step_bytes"] * n / (ms_step * 1e-3) / 1e9
     res["tumbles"] = sw.tumbles()
     # final state for the FP32-vs-FP64 error report (same inputs, same steps)
     for name in ("rho", "u"):
@@ -191,14 +203,17 @@ def cpu_baseline(case, steps=3):
                       "OpenMP over %d threads, %.1f s" % (steps, cores, dt)}
 
 
-def load_traffic(storage):
-    """HBM bytes per collide launch from the committed rocprofv3 --pmc summary."""
+def load_traffic(storage, virtual):
+    """HBM bytes per collide launch from the committed rocprofv3 --pmc summary
+    of the same kernel (null if the summary covers another one)."""
     path = os.path.join(ROOT, "profiles", "collide_stream_%s_256cubed_pmc.json" % storage)
+    kernel = "k_collide_stream_%s%s" % ("vf_" if virtual else "", storage)
     try:
         with open(path) as fh:
-            return json.load(fh).get("hbm_bytes_per_launch")
+            d = json.load(fh)
     except (OSError, ValueError):
         return None
+    return d.get("hbm_bytes_per_launch") if d.get("kernel") == kernel else None
 
 
 def main():
@@ -209,6 +224,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="f64,f32")
+    ap.add_argument("--force-mode", default="auto", choices=["auto", "stored", "virtual"],
+                    help="swimmer force of the coupled step: virtual where sparse (auto), stored, virtual")
     ap.add_argument("--probe", choices=["copy"], help="run the streaming-copy bandwidth probe instead")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-sim ensemble leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving leg (0: skip)")
@@ -233,14 +250,14 @@ def main():
     res = {}
     for form in forms:
         log("timing %s: %d warm-up + %d steps" % (form, a.warmup, a.steps))
-        res[form] = time_gpu(case, form, a.steps, a.warmup)
+        res[form] = time_gpu(case, form, a.steps, a.warmup, a.force_mode)
         log("%s: %.3f ms/step, %.0f MLUPS, collide %.0f GB/s" % (form, res[form]["ms_per_step"],
                                                                  res[form]["mlups"], res[form]["collide_gbs"]))
     main_form = "f64" if "f64" in res else forms[0]
     m = res[main_form]
     n = a.size ** 3
     coll = m["kernels"]["collide_stream"]
-    traffic = load_traffic(main_form)
+    traffic = load_traffic(main_form, m["virtual_force"])
     out = {
         "metric": "MLUPS of the coupled step (ZeroForce+AddForces+collide/stream+CalcHydro+Move) at %d^3, "
                   "FP64 drop-in" % a.size,
@@ -259,17 +276,19 @@ def main():
                    if a.size == 256 else "%d^3 reduced" % a.size,
                    "lattice": [a.size] * 3, "swimmers": 32768, "tracers": 1_000_000, "parallelism": "single GPU"},
         "roofline": {
-            "kernel": "collide_stream (%s)" % main_form,
+            "kernel": "k_collide_stream_%s%s" % ("vf_" if m["virtual_force"] else "", main_form),
             "bound": "hbm",
             "achieved": round(m["collide_gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(m["collide_gbs"] / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": COLLIDE_BYTES[main_form] * n,
+            "algorithmic_bytes_per_launch": m["collide_bytes"] * n,
+            "virtual_force": m["virtual_force"],
             "avg_launch_ms": round(coll["avg_ms"], 4),
         },
         "step_hbm_frac": round(m["step_gbs"] / HBM_PEAK_GBS, 4),
+        "step_bytes_per_site": m["step_bytes"],
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in m["kernels"].items()},
     }
     if "f32" in res and "f64" in res:

@@ -150,15 +150,19 @@ struct lbswim_lattice {
   std::vector<TimeRec> recs;
   std::vector<hipEvent_t> pool;
   // workspace of the deterministic spreading in lbswim_system_step
-  SpreadGeom sg{};
-  uint32_t* sp_counts = nullptr;  // ntiles
-  uint32_t* sp_off = nullptr;     // ntiles + 1 offsets + scan scratch
-  uint32_t* sp_cursor = nullptr;  // ntiles
+  uint32_t sp_lists = 0;          // one per 64 sites
+  uint32_t* sp_counts = nullptr;  // sp_lists
+  uint32_t* sp_off = nullptr;     // sp_lists + 1 offsets + scan scratch
+  uint32_t* sp_cursor = nullptr;  // sp_lists
   size_t sp_pts_cap = 0;
   void* sp_pts = nullptr;  // SpreadPoint records for sp_pts_cap points
   size_t sp_list_cap = 0;
   uint32_t* sp_list = nullptr;
   uint32_t* sp_sorted = nullptr;
+  // Virtual force: after a fused step the force lives in the sorted point
+  // lists above and d_force is stale until materialize_force writes it.
+  int force_mode = LBSWIM_FORCE_AUTO;
+  bool force_virtual = false;
 };
 
 // Device storage of one particle array in slot order (particle_order.hip):
@@ -397,18 +401,43 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   delete lat;
 }
 
+// The force of a virtual-force lattice as the reference holds it: written
+// from the point lists of the last fused step (ZeroForce + AddForces).
+static int materialize_force(lbswim_lattice* lat) {
+  if (!lat->force_virtual) return LBSWIM_OK;
+  KernelTimer t(lat, LBSWIM_K_ADDF);
+  const SpreadPoint* pts = static_cast<const SpreadPoint*>(lat->sp_pts);
+  hipError_t e;
+  if (lat->storage == LBSWIM_STORE_F64)
+    e = launch_spread_force<double>(lat->geom, pts, lat->sp_off, lat->sp_sorted, static_cast<double*>(lat->d_force),
+                                    stream());
+  else
+    e = launch_spread_force<float>(lat->geom, pts, lat->sp_off, lat->sp_sorted, static_cast<float*>(lat->d_force),
+                                   stream());
+  t.done();
+  HIP_TRY(e);
+  lat->force_virtual = false;
+  return LBSWIM_OK;
+}
+
+static VForce virtual_force(const lbswim_lattice* lat) {
+  return VForce{static_cast<const SpreadPoint*>(lat->sp_pts), lat->sp_off, lat->sp_sorted};
+}
+
 static int lattice_step_impl(lbswim_lattice* lat) {
   const double ts = lat->params.tau_s, tb = lat->params.tau_b;
+  const VForce vf = virtual_force(lat);
+  const VForce* pvf = lat->force_virtual ? &vf : nullptr;
   KernelTimer t(lat, LBSWIM_K_COLLIDE);
   hipError_t e;
   if (lat->storage == LBSWIM_STORE_F64)
     e = launch_collide_stream<double>(static_cast<const double*>(lat->d_f[lat->cur]),
                                       static_cast<const double*>(lat->d_force),
-                                      static_cast<double*>(lat->d_f[1 - lat->cur]), lat->geom, ts, tb, stream());
+                                      static_cast<double*>(lat->d_f[1 - lat->cur]), lat->geom, ts, tb, pvf, stream());
   else
     e = launch_collide_stream<float>(static_cast<const float*>(lat->d_f[lat->cur]),
                                      static_cast<const float*>(lat->d_force),
-                                     static_cast<float*>(lat->d_f[1 - lat->cur]), lat->geom, ts, tb, stream());
+                                     static_cast<float*>(lat->d_f[1 - lat->cur]), lat->geom, ts, tb, pvf, stream());
   t.done();
   HIP_TRY(e);
   lat->cur = 1 - lat->cur;  // lb.cu:242-244
@@ -416,23 +445,26 @@ static int lattice_step_impl(lbswim_lattice* lat) {
   return LBSWIM_OK;
 }
 
-static int lattice_hydro_impl(lbswim_lattice* lat, bool zero_force) {
+static int lattice_hydro_impl(lbswim_lattice* lat) {
+  const VForce vf = virtual_force(lat);
+  const VForce* pvf = lat->force_virtual ? &vf : nullptr;
   KernelTimer t(lat, LBSWIM_K_HYDRO);
   hipError_t e;
   if (lat->storage == LBSWIM_STORE_F64)
-    e = launch_calc_hydro<double>(static_cast<const double*>(lat->d_f[lat->cur]), static_cast<double*>(lat->d_force),
-                                  static_cast<double*>(lat->d_rho), static_cast<double*>(lat->d_u), lat->geom,
-                                  zero_force, stream());
+    e = launch_calc_hydro<double>(static_cast<const double*>(lat->d_f[lat->cur]),
+                                  static_cast<const double*>(lat->d_force), static_cast<double*>(lat->d_rho),
+                                  static_cast<double*>(lat->d_u), lat->geom, pvf, stream());
   else
-    e = launch_calc_hydro<float>(static_cast<const float*>(lat->d_f[lat->cur]), static_cast<float*>(lat->d_force),
-                                 static_cast<float*>(lat->d_rho), static_cast<float*>(lat->d_u), lat->geom,
-                                 zero_force, stream());
+    e = launch_calc_hydro<float>(static_cast<const float*>(lat->d_f[lat->cur]), static_cast<const float*>(lat->d_force),
+                                 static_cast<float*>(lat->d_rho), static_cast<float*>(lat->d_u), lat->geom, pvf,
+                                 stream());
   t.done();
   HIP_TRY(e);
   return LBSWIM_OK;
 }
 
 static int lattice_zero_force_impl(lbswim_lattice* lat) {
+  lat->force_virtual = false;
   KernelTimer t(lat, LBSWIM_K_ZERO);
   hipError_t e = launch_fill_zero(lat->d_force, 3 * lat->elem * lat->n, stream());
   t.done();
@@ -442,12 +474,36 @@ static int lattice_zero_force_impl(lbswim_lattice* lat) {
 
 int lbswim_lattice_step(lbswim_lattice* lat) {
   if (int rc = check_lattice(lat)) return rc;
+  if (int rc = materialize_force(lat)) return rc;
   return lattice_step_impl(lat);
 }
 
 int lbswim_lattice_calc_hydro(lbswim_lattice* lat) {
   if (int rc = check_lattice(lat)) return rc;
-  return lattice_hydro_impl(lat, false);
+  if (int rc = materialize_force(lat)) return rc;
+  return lattice_hydro_impl(lat);
+}
+
+int lbswim_lattice_set_force_mode(lbswim_lattice* lat, int mode) {
+  if (int rc = check_lattice(lat)) return rc;
+  CHECK_ARG(mode == LBSWIM_FORCE_STORED || mode == LBSWIM_FORCE_AUTO || mode == LBSWIM_FORCE_VIRTUAL,
+            "unknown force mode %d", mode);
+  if (mode == LBSWIM_FORCE_STORED)
+    if (int rc = materialize_force(lat)) return rc;
+  lat->force_mode = mode;
+  return LBSWIM_OK;
+}
+
+int lbswim_lattice_sync_force(lbswim_lattice* lat) {
+  if (int rc = check_lattice(lat)) return rc;
+  return materialize_force(lat);
+}
+
+int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual) {
+  if (int rc = check_lattice(lat)) return rc;
+  CHECK_ARG(is_virtual != nullptr, "is_virtual is NULL");
+  *is_virtual = lat->force_virtual ? 1 : 0;
+  return LBSWIM_OK;
 }
 
 int lbswim_lattice_init_from_hydro(lbswim_lattice* lat) {
@@ -510,7 +566,11 @@ int lbswim_lattice_field(lbswim_lattice* lat, int field, double** host, void** d
   if (host) {
     if (int rc = lattice_host(lat, field, host)) return rc;
   }
-  if (device) *device = field_device(lat, field);
+  if (device) {
+    if (field == LBSWIM_FORCE)
+      if (int rc = materialize_force(lat)) return rc;
+    *device = field_device(lat, field);
+  }
   if (size) *size = field_count(lat, field);
   return LBSWIM_OK;
 }
@@ -523,6 +583,7 @@ int lbswim_lattice_h2d(lbswim_lattice* lat, int field) {
   const size_t cnt = field_count(lat, field);
   void* dev = field_device(lat, field);
   const size_t n = lat->n;
+  if (field == LBSWIM_FORCE) lat->force_virtual = false;  // the host array replaces the force
   if (lat->storage == LBSWIM_STORE_F64) {
     if (!is_dist(field)) return copy_h2d(dev, h, cnt * sizeof(double));
     std::vector<double> stage(cnt);
@@ -547,6 +608,8 @@ int lbswim_lattice_d2h(lbswim_lattice* lat, int field) {
   CHECK_ARG(field >= LBSWIM_RHO && field <= LBSWIM_FNEW, "unknown lattice field %d", field);
   double* h = nullptr;
   if (int rc = lattice_host(lat, field, &h)) return rc;
+  if (field == LBSWIM_FORCE)
+    if (int rc = materialize_force(lat)) return rc;
   const size_t cnt = field_count(lat, field);
   const void* dev = field_device(lat, field);
   const size_t n = lat->n;
@@ -731,6 +794,7 @@ int lbswim_swimmers_common_d2h(lbswim_swimmers* sw) {
 int lbswim_swimmers_add_forces(lbswim_swimmers* sw, lbswim_lattice* lat) {
   CHECK_ARG(sw, "swimmer handle is NULL");
   if (int rc = check_lattice(lat)) return rc;
+  if (int rc = materialize_force(lat)) return rc;
   KernelTimer t(lat, LBSWIM_K_ADDF);
   const SwimParams p = to_kernel(sw->common_dev);
   PStore& st = sw->st;
@@ -911,15 +975,26 @@ int grow(void** p, size_t* cap, size_t need, size_t elem) {
   return LBSWIM_OK;
 }
 
+// Mean entries per list up to which the collide and CalcHydro kernels
+// evaluate the force themselves (each wavefront walks its list twice per
+// step); denser systems write the force array once instead.
+constexpr size_t VF_MAX_MEAN_ENTRIES = 32;
+
+bool use_virtual_force(int mode, size_t npts, size_t lists) {
+  // a stencil reaches about 17 lists of 64 sites
+  return mode == LBSWIM_FORCE_VIRTUAL || (mode == LBSWIM_FORCE_AUTO && npts * 17 <= VF_MAX_MEAN_ENTRIES * lists);
+}
+
 // ZeroForce + AddForces of every swimmer array, as one deterministic gather.
+// With the virtual force the lists are left for the next collide and CalcHydro.
 int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
   size_t npts = 0;
   for (int i = 0; i < nsw; ++i) npts += 2 * size_t(sw[i]->st.num);
   if (npts == 0) return lattice_zero_force_impl(lat);
-  CHECK_ARG(npts <= size_t(INT_MAX), "too many swimmers in one system");
+  CHECK_ARG(npts <= size_t(INT_MAX) / 64, "too many swimmers in one system");
   if (!lat->sp_counts) {
-    lat->sg = make_spread_geom(lat->geom);
-    const size_t nt = size_t(lat->sg.ntiles);
+    lat->sp_lists = spread_lists(lat->geom);
+    const size_t nt = size_t(lat->sp_lists);
     int rc;
     if ((rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_counts), 4 * nt)) ||
         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_off), 4 * (nt + 1 + (nt + 1023) / 1024))) ||
@@ -931,7 +1006,7 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     if (int rc = grow(&lat->sp_pts, &cap_bytes, npts * sizeof(SpreadPoint), 1)) return rc;
     lat->sp_pts_cap = npts;
   }
-  const size_t lcap = spread_list_capacity(lat->sg, npts);
+  const size_t lcap = spread_list_capacity(npts);
   if (lat->sp_list_cap < lcap) {
     size_t c1 = lat->sp_list_cap, c2 = lat->sp_list_cap;
     if (int rc = grow(reinterpret_cast<void**>(&lat->sp_list), &c1, lcap, 4)) return rc;
@@ -939,25 +1014,29 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     lat->sp_list_cap = lcap;
   }
   SpreadPoint* pts = static_cast<SpreadPoint*>(lat->sp_pts);
+  const bool virt = use_virtual_force(lat->force_mode, npts, lat->sp_lists);
 
   KernelTimer t(lat, LBSWIM_K_ADDF);
-  HIP_TRY(launch_fill_zero(lat->sp_counts, 4 * size_t(lat->sg.ntiles), stream()));
   int offset = 0;
   for (int i = 0; i < nsw; ++i) {
     PStore& st = sw[i]->st;
     HIP_TRY(launch_points(st.num, offset, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R),
-                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, lat->sg, pts, lat->sp_counts, stream()));
+                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, pts, stream()));
     offset += st.num;
   }
-  hipError_t e;
-  if (lat->storage == LBSWIM_STORE_F64)
-    e = launch_spread<double>(int(npts), lat->geom, lat->sg, pts, lat->sp_counts, lat->sp_off, lat->sp_cursor,
-                              lat->sp_list, lat->sp_sorted, static_cast<double*>(lat->d_force), stream());
-  else
-    e = launch_spread<float>(int(npts), lat->geom, lat->sg, pts, lat->sp_counts, lat->sp_off, lat->sp_cursor,
-                             lat->sp_list, lat->sp_sorted, static_cast<float*>(lat->d_force), stream());
+  hipError_t e = launch_spread_lists(int(npts), lat->geom, pts, lat->sp_counts, lat->sp_off, lat->sp_cursor,
+                                     lat->sp_list, lat->sp_sorted, stream());
+  if (e == hipSuccess && !virt) {
+    if (lat->storage == LBSWIM_STORE_F64)
+      e = launch_spread_force<double>(lat->geom, pts, lat->sp_off, lat->sp_sorted, static_cast<double*>(lat->d_force),
+                                      stream());
+    else
+      e = launch_spread_force<float>(lat->geom, pts, lat->sp_off, lat->sp_sorted, static_cast<float*>(lat->d_force),
+                                     stream());
+  }
   t.done();
   HIP_TRY(e);
+  lat->force_virtual = virt;
   return LBSWIM_OK;
 }
 
@@ -975,7 +1054,7 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
     int rc;
     if ((rc = spread_forces(lat, sw, nsw))) return rc;
     if ((rc = lattice_step_impl(lat))) return rc;
-    if ((rc = lattice_hydro_impl(lat, false))) return rc;
+    if ((rc = lattice_hydro_impl(lat))) return rc;
     for (int i = 0; i < nsw; ++i)
       if ((rc = swimmers_move_impl(sw[i], lat))) return rc;
     for (int i = 0; i < ntr; ++i)
@@ -1209,11 +1288,11 @@ struct lbswim_ensemble {
   WorkItem* d_trwork = nullptr;
   size_t cap_trwork = 0;
   int n_trwork = 0;
-  WorkItem* d_tilework = nullptr;
-  size_t cap_tilework = 0;
-  int ntiles = 0;
+  int nlists = 0;  // spreading lists of all members
   // spreading workspace
   int npts = 0;
+  int force_mode = LBSWIM_FORCE_AUTO;
+  bool vf = false;  // collide/CalcHydro evaluate the force from the lists
   uint32_t* counts = nullptr;
   size_t cap_counts = 0;
   uint32_t* off = nullptr;
@@ -1246,7 +1325,7 @@ int ens_build(lbswim_ensemble* e) {
   HIP_TRY(hipStreamSynchronize(stream()));
   ens_drop_graph(e);
   std::vector<LatDesc> lat;
-  std::vector<std::vector<WorkItem>> latwork, tilework;
+  std::vector<std::vector<WorkItem>> latwork;
   std::vector<size_t> weight;
   std::vector<SwimDesc> sw;
   std::vector<WorkItem> swwork, ptwork, trwork;
@@ -1268,16 +1347,13 @@ int ens_build(lbswim_ensemble* e) {
     d.omega_s = 1.0 / (d.tau_s + 0.5);
     d.omega_b = 1.0 / (d.tau_b + 0.5);
     d.cur = L->cur;
-    d.sg = make_spread_geom(L->geom);
     d.tile_base = tile_base;
     lat.push_back(d);
-    std::vector<WorkItem> lw, tw;
+    std::vector<WorkItem> lw;
     for (uint32_t b = 0; b < lattice_blocks(L->geom); ++b) lw.push_back(WorkItem{mi, b});
-    for (uint32_t t = 0; t < uint32_t(d.sg.ntiles); ++t) tw.push_back(WorkItem{mi, t});
     latwork.push_back(lw);
-    tilework.push_back(tw);
     weight.push_back(L->n);
-    tile_base += uint32_t(d.sg.ntiles);
+    tile_base += spread_lists(L->geom);
     for (lbswim_swimmers* s : m.sw) {
       if (int rc = pstore_maybe_reorder(s->st, L->geom)) return rc;
       const uint32_t ai = uint32_t(sw.size());
@@ -1296,7 +1372,7 @@ int ens_build(lbswim_ensemble* e) {
       sw.push_back(a);
       for (uint32_t g = 0; g < uint32_t((a.num + PARTICLE_GROUPS - 1) / PARTICLE_GROUPS); ++g)
         swwork.push_back(WorkItem{ai, g});
-      for (uint32_t b = 0; b < uint32_t((a.num + 255) / 256); ++b) ptwork.push_back(WorkItem{ai, b});
+      for (uint32_t b = 0; b < uint32_t((2 * size_t(a.num) + 255) / 256); ++b) ptwork.push_back(WorkItem{ai, b});
     }
     for (lbswim_tracers* t : m.tr) {
       if (int rc = pstore_maybe_reorder(t->st, L->geom)) return rc;
@@ -1318,10 +1394,9 @@ int ens_build(lbswim_ensemble* e) {
     m.storage_at_build = storage_signature(m);
   }
   const std::vector<WorkItem> lw = order_work(latwork, e->order, weight);
-  const std::vector<WorkItem> tw = order_work(tilework, e->order, weight);
   int rc;
   if ((rc = upload(&e->d_lat, &e->cap_lat, lat)) || (rc = upload(&e->d_latwork, &e->cap_latwork, lw)) ||
-      (rc = upload(&e->d_tilework, &e->cap_tilework, tw)) || (rc = upload(&e->d_sw, &e->cap_sw, sw)) ||
+      (rc = upload(&e->d_sw, &e->cap_sw, sw)) ||
       (rc = upload(&e->d_swwork, &e->cap_swwork, swwork)) || (rc = upload(&e->d_ptwork, &e->cap_ptwork, ptwork)) ||
       (rc = upload(&e->d_tr, &e->cap_tr, tr)) || (rc = upload(&e->d_trwork, &e->cap_trwork, trwork)))
     return rc;
@@ -1329,11 +1404,12 @@ int ens_build(lbswim_ensemble* e) {
   e->n_swwork = int(swwork.size());
   e->n_ptwork = int(ptwork.size());
   e->n_trwork = int(trwork.size());
-  e->ntiles = int(tile_base);
+  e->nlists = int(tile_base);
   e->npts = int(point_base);
+  e->vf = use_virtual_force(e->force_mode, size_t(e->npts), size_t(e->nlists));
   // spreading workspace
   size_t lcap = 0;
-  for (const auto& a : sw) lcap += spread_list_capacity(lat[a.member].sg, 2 * size_t(a.num));
+  for (const auto& a : sw) lcap += spread_list_capacity(2 * size_t(a.num));
   auto grow_to = [&](auto** p, size_t* cap, size_t need, size_t elem) -> int {
     if (*cap >= need) return LBSWIM_OK;
     dev_free(*p);
@@ -1343,7 +1419,7 @@ int ens_build(lbswim_ensemble* e) {
     *cap = need;
     return LBSWIM_OK;
   };
-  const size_t nt = size_t(e->ntiles);
+  const size_t nt = size_t(e->nlists);
   size_t cap_sorted = e->cap_list;
   if ((rc = grow_to(&e->counts, &e->cap_counts, nt, 4)) ||
       (rc = grow_to(&e->off, &e->cap_off, nt + 1 + (nt + 1023) / 1024, 4)) ||
@@ -1359,12 +1435,14 @@ int ens_build(lbswim_ensemble* e) {
 template <typename T>
 int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   hipStream_t s = stream();
-  HIP_TRY(launch_fill_zero(e->counts, 4 * size_t(e->ntiles), s));
-  HIP_TRY(launch_batch_points(e->d_lat, e->d_sw, e->d_ptwork, e->n_ptwork, e->pts, e->counts, s));
-  HIP_TRY(launch_batch_spread<T>(e->d_lat, e->npts, e->ntiles, e->d_tilework, e->pts, e->counts, e->off, e->cursor,
-                                 e->list, e->sorted, s));
-  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, s));
-  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, s));
+  HIP_TRY(launch_batch_points(e->d_lat, e->d_sw, e->d_ptwork, e->n_ptwork, e->pts, s));
+  HIP_TRY(launch_batch_lists(e->d_lat, e->npts, e->nlists, e->pts, e->counts, e->off, e->cursor, e->list, e->sorted,
+                             s));
+  const VForce vf{e->pts, e->off, e->sorted};
+  if (!e->vf)
+    HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_latwork, e->d_latwork, e->pts, e->off, e->sorted, s));
+  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, e->vf ? &vf : nullptr, s));
+  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, e->vf ? &vf : nullptr, s));
   HIP_TRY(launch_batch_swimmers_move<T>(e->d_lat, e->d_sw, e->d_swwork, e->n_swwork, s));
   HIP_TRY(launch_batch_tracers_move<T>(e->d_lat, e->d_tr, e->d_trwork, e->n_trwork, s));
   return LBSWIM_OK;
@@ -1372,6 +1450,19 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
 
 int ens_enqueue(lbswim_ensemble* e, int parity) {
   return e->storage == LBSWIM_STORE_F64 ? ens_enqueue_step<double>(e, parity) : ens_enqueue_step<float>(e, parity);
+}
+
+// After batched steps with the virtual force: write every member's force
+// array from the last step's lists, as the reference leaves it.
+int ens_materialize_force(lbswim_ensemble* e) {
+  if (!e->vf) return LBSWIM_OK;
+  hipError_t r = e->storage == LBSWIM_STORE_F64
+                     ? launch_batch_spread_force<double>(e->d_lat, e->n_latwork, e->d_latwork, e->pts, e->off,
+                                                         e->sorted, stream())
+                     : launch_batch_spread_force<float>(e->d_lat, e->n_latwork, e->d_latwork, e->pts, e->off,
+                                                        e->sorted, stream());
+  HIP_TRY(r);
+  return LBSWIM_OK;
 }
 
 int ens_capture(lbswim_ensemble* e) {
@@ -1403,6 +1494,7 @@ void ens_advance(lbswim_ensemble* e, int k) {
     if (!m.lat) continue;
     if (k & 1) m.lat->cur ^= 1;
     m.lat->time_step += k;
+    m.lat->force_virtual = false;  // the batched step spreads into the force array
     for (auto* s : m.sw) s->st.moves_since_order += k;
     for (auto* t : m.tr) t->st.moves_since_order += k;
   }
@@ -1442,7 +1534,6 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   dev_free(e->d_ptwork);
   dev_free(e->d_tr);
   dev_free(e->d_trwork);
-  dev_free(e->d_tilework);
   dev_free(e->counts);
   dev_free(e->off);
   dev_free(e->cursor);
@@ -1510,6 +1601,15 @@ int lbswim_ensemble_set_order(lbswim_ensemble* e, int order) {
   return LBSWIM_OK;
 }
 
+int lbswim_ensemble_set_force_mode(lbswim_ensemble* e, int mode) {
+  CHECK_ARG(e, "ensemble is NULL");
+  CHECK_ARG(mode == LBSWIM_FORCE_STORED || mode == LBSWIM_FORCE_AUTO || mode == LBSWIM_FORCE_VIRTUAL,
+            "unknown force mode %d", mode);
+  if (e->force_mode != mode) e->dirty = true;
+  e->force_mode = mode;
+  return LBSWIM_OK;
+}
+
 int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps) {
   CHECK_ARG(e && nsteps >= 0, "invalid argument");
   if (nsteps == 0) return LBSWIM_OK;
@@ -1530,7 +1630,7 @@ int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps) {
     if (int rc = ens_enqueue(e, e->parity)) return rc;
     ens_advance(e, 1);
   }
-  return LBSWIM_OK;
+  return ens_materialize_force(e);
 }
 
 int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice) {

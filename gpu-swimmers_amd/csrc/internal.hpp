@@ -26,11 +26,21 @@ struct SwimParams {
 Geom make_geom(int nx, int ny, int nz);
 
 // ---- lattice (storage T = double: reference arithmetic; T = float: deviation form)
+struct SpreadPoint;
+// Sorted point lists of the lattice's 64-site runs (see launch_spread_lists).
+// Passed to the collide and CalcHydro launchers instead of a stored force
+// field, they evaluate every site's force from its block's list.
+struct VForce {
+  const SpreadPoint* pts;
+  const uint32_t* off;
+  const uint32_t* sorted;
+};
+// `vf` null: the force is read from `force`.
 template <typename T>
 hipError_t launch_collide_stream(const T* fOld, const T* force, T* fNew, const Geom& g, double tau_s,
-                                 double tau_b, hipStream_t s);
+                                 double tau_b, const VForce* vf, hipStream_t s);
 template <typename T>
-hipError_t launch_calc_hydro(const T* f, T* force, T* rho, T* u, const Geom& g, bool zero_force,
+hipError_t launch_calc_hydro(const T* f, const T* force, T* rho, T* u, const Geom& g, const VForce* vf,
                              hipStream_t s);
 template <typename T>
 hipError_t launch_init_from_hydro(const T* rho, const T* u, T* f, const Geom& g, hipStream_t s);
@@ -56,13 +66,6 @@ hipError_t launch_tracers_move(int nt, double* r, double* sdisp, double* v, cons
                                hipStream_t s);
 
 // ---- deterministic spreading (ZeroForce + AddForces of all swimmer arrays)
-// Lattice tiles of tx*ty*tz <= 512 sites; each stencil touches at most
-// min(4, tiles) tiles per dimension.
-struct SpreadGeom {
-  int tx, ty, tz;
-  int ntx, nty, ntz;
-  int ntiles;
-};
 // Stencil record of one spreading point (tail or head of a swimmer), stored
 // by point id = 2*(array base + user index) + (0 tail, 1 head): Peskin weights
 // del[d*4+k] and wrapped site coordinates wc[d*4+k] of the four stencil
@@ -74,16 +77,23 @@ struct SpreadPoint {
   int pad[2];
 };
 static_assert(sizeof(SpreadPoint) == 176, "SpreadPoint layout");
-SpreadGeom make_spread_geom(const Geom& g);
-size_t spread_list_capacity(const SpreadGeom& sg, size_t npts);
-// counts[ntiles] must be zero before the first launch_points of a step.
+// One list per run of 64 consecutive sites -- one wavefront of the lattice
+// kernels' 256-site workgroups: spread_lists(g) = 4 ceil(n / 256) of them.
+uint32_t spread_lists(const Geom& g);
+size_t spread_list_capacity(size_t npts);
+// Point records of the swimmers of one array, ids from 2*base.
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
-                         const Geom& g, const SpreadGeom& sg, SpreadPoint* pts, uint32_t* counts, hipStream_t s);
-// off[ntiles + 1 + ceil(ntiles/1024)] (offsets, then scan scratch), cursor[ntiles],
-// list/sorted[spread_list_capacity]
+                         const Geom& g, SpreadPoint* pts, hipStream_t s);
+// counts[nl], off[nl + 1 + ceil(nl/1024)] (offsets, then scan scratch),
+// cursor[nl], list/sorted[spread_list_capacity]; afterwards
+// sorted[off[l]..off[l+1]) holds the ids of the points reaching sites
+// 64 l .. 64 l + 63 in increasing order.
+hipError_t launch_spread_lists(int npts, const Geom& g, const SpreadPoint* pts, uint32_t* counts, uint32_t* off,
+                               uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s);
+// Writes the force of every site from the lists (ZeroForce + AddForces).
 template <typename T>
-hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const SpreadPoint* pts, uint32_t* counts,
-                         uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, T* force, hipStream_t s);
+hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
+                               T* force, hipStream_t s);
 
 // ---- batched launches over the members of an ensemble
 // One lattice of a batch.  g[cur ^ parity] holds the state at the start of a
@@ -96,8 +106,7 @@ struct LatDesc {
   Geom geom;
   double tau_s, tau_b, omega_s, omega_b;
   int cur;
-  uint32_t tile_base;  // first spreading tile of this member in the batch
-  SpreadGeom sg;
+  uint32_t tile_base;  // first spreading list of this member in the batch
 };
 // One workgroup of a batched launch: member index and the block (or tile,
 // or particle group) of that member it processes.
@@ -127,18 +136,25 @@ struct TracerDesc {
 };
 uint32_t lattice_blocks(const Geom& g);  // workgroups of a one-lattice launch
 constexpr int PARTICLE_GROUPS = 16;      // particles per workgroup of the move kernels
+// vf: the force of member m's block b is evaluated from lists tile_base_m + 4 b ...
 template <typename T>
-hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, hipStream_t s);
+hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
+                                hipStream_t s);
 template <typename T>
-hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, hipStream_t s);
-// Force spreading of a batch: zero the tile counts first; points of array a go
-// to ids point_base_a + 2*user + end; work over swimmer blocks of 256.
+hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
+                              hipStream_t s);
+// Point records of a batch: points of array a get ids point_base_a + 2*user +
+// end; work items are blocks of 128 swimmers (256 points) of an array.
 hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
-                               SpreadPoint* pts, uint32_t* counts, hipStream_t s);
+                               SpreadPoint* pts, hipStream_t s);
+// Sorted lists of all nlists lists of the batch (see launch_spread_lists).
+hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const SpreadPoint* pts, uint32_t* counts,
+                              uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s);
+// Writes the force arrays of the lattice blocks in block_work from the lists.
 template <typename T>
-hipError_t launch_batch_spread(const LatDesc* ldesc, int npts, int ntiles, const WorkItem* tile_work,
-                               const SpreadPoint* pts, uint32_t* counts, uint32_t* off, uint32_t* cursor,
-                               uint32_t* list, uint32_t* sorted, hipStream_t s);
+hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nblocks, const WorkItem* block_work,
+                                     const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
+                                     hipStream_t s);
 template <typename T>
 hipError_t launch_batch_swimmers_move(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
                                       hipStream_t s);

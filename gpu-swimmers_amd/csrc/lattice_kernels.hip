@@ -33,6 +33,7 @@
 
 #include "d3q15.hpp"
 #include "internal.hpp"
+#include "spread.hpp"
 
 namespace lbs {
 
@@ -51,6 +52,7 @@ Geom make_geom(int nx, int ny, int nz) {
 namespace {
 
 constexpr int BLOCK = 256;
+static_assert(BLOCK == SITE_BLOCK, "force lists are numbered by lattice workgroup");
 
 struct Site {
   uint32_t i, j, k;
@@ -221,18 +223,30 @@ __device__ __forceinline__ void store_all(double* __restrict__ gNew, uint32_t n,
 }
 
 // DoStep (lb.cu:72-218): pull-stream, then collide into the site's own slots.
-__device__ __forceinline__ void collide_f64_body(const double* __restrict__ gOld,
-                                                              const double* __restrict__ force,
-                                                              double* __restrict__ gNew, Geom g,
-                                                              double tau_s, double tau_b, double omega_s,
-                                                              double omega_b, uint32_t blk) {
+// VF: the force comes from the wavefront's point list (wave_force; lists
+// vlist.. of the block) instead of the force array.  No barrier is involved,
+// so lanes past the lattice end leave at once.
+template <bool VF>
+__device__ __forceinline__ void collide_f64_body(const double* __restrict__ gOld, const double* __restrict__ force,
+                                                 double* __restrict__ gNew, Geom g, double tau_s, double tau_b,
+                                                 double omega_s, double omega_b, uint32_t blk, const VForce& vf,
+                                                 uint32_t vlist) {
   const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
-  const Nbr d = neighbours(decompose(s, g), g);
+  const Site c = decompose(s, g);
+  const Nbr d = neighbours(c, g);
   double f[Q];
   pull_all(gOld, n, d, f, std::make_integer_sequence<int, Q>{});
-  const double F[3] = {force[s], force[size_t(n) + s], force[2 * size_t(n) + s]};
+  double F[3];
+  // the list walk uses scalar loads only, so it overlaps the loads of f
+  if constexpr (VF)
+    wave_force(vf.pts, vf.off, vf.sorted, vlist + threadIdx.x / WAVE_SITES, true, int(c.i), int(c.j), int(c.k), g, F);
+  else {
+    F[0] = force[s];
+    F[1] = force[size_t(n) + s];
+    F[2] = force[2 * size_t(n) + s];
+  }
   double mode[NKEEP];
   fwd_all(f, mode, std::make_integer_sequence<int, NKEEP>{});
   relax_f64(mode, F, tau_s, tau_b, omega_s, omega_b);
@@ -240,29 +254,32 @@ __device__ __forceinline__ void collide_f64_body(const double* __restrict__ gOld
 }
 
 // lb.cu:250-295: only the density and momentum modes are needed.
-__device__ __forceinline__ void hydro_f64_body(const double* __restrict__ gCur,
-                                                          double* __restrict__ force,
-                                                          double* __restrict__ rho,
-                                                          double* __restrict__ u, Geom g, int zero_force, uint32_t blk) {
+template <bool VF>
+__device__ __forceinline__ void hydro_f64_body(const double* __restrict__ gCur, const double* __restrict__ force,
+                                               double* __restrict__ rho, double* __restrict__ u, Geom g, uint32_t blk,
+                                               const VForce& vf, uint32_t vlist) {
   const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
+  const Site c = decompose(s, g);
   double fl[Q];
-  pull_all(gCur, n, neighbours(decompose(s, g), g), fl, std::make_integer_sequence<int, Q>{});
+  pull_all(gCur, n, neighbours(c, g), fl, std::make_integer_sequence<int, Q>{});
+  double F[3];
+  if constexpr (VF)
+    wave_force(vf.pts, vf.off, vf.sorted, vlist + threadIdx.x / WAVE_SITES, true, int(c.i), int(c.j), int(c.k), g, F);
   const double m0 = fwd<0>(fl);
   const double m1 = fwd<1>(fl);
   const double m2 = fwd<2>(fl);
   const double m3 = fwd<3>(fl);
-  const double F0 = force[s], F1 = force[size_t(n) + s], F2 = force[2 * size_t(n) + s];
-  rho[s] = m0;
-  u[s] = (m1 + 0.5 * F0) / m0;
-  u[size_t(n) + s] = (m2 + 0.5 * F1) / m0;
-  u[2 * size_t(n) + s] = (m3 + 0.5 * F2) / m0;
-  if (zero_force) {
-    force[s] = 0.0;
-    force[size_t(n) + s] = 0.0;
-    force[2 * size_t(n) + s] = 0.0;
+  if constexpr (!VF) {
+    F[0] = force[s];
+    F[1] = force[size_t(n) + s];
+    F[2] = force[2 * size_t(n) + s];
   }
+  rho[s] = m0;
+  u[s] = (m1 + 0.5 * F[0]) / m0;
+  u[size_t(n) + s] = (m2 + 0.5 * F[1]) / m0;
+  u[2 * size_t(n) + s] = (m3 + 0.5 * F[2]) / m0;
 }
 
 // lb.cu:314-371, including the reference's SXZ = rho*ux*uy (lb.cu:356).
@@ -394,17 +411,29 @@ __device__ __forceinline__ void inverse_dev32(const Moments32& m, float (&g)[Q])
   g[11] = W2 * (s10 - t10);
 }
 
-__device__ __forceinline__ void collide_f32_body(const float* __restrict__ fOld,
-                                                              const float* __restrict__ force,
-                                                              float* __restrict__ fNew, Geom g,
-                                                              float tau_s, float tau_b, float omega_s,
-                                                              float omega_b, uint32_t blk) {
+// The force is rounded to FP32 as the stored force array holds it.
+template <bool VF>
+__device__ __forceinline__ void collide_f32_body(const float* __restrict__ fOld, const float* __restrict__ force,
+                                                 float* __restrict__ fNew, Geom g, float tau_s, float tau_b,
+                                                 float omega_s, float omega_b, uint32_t blk, const VForce& vf,
+                                                 uint32_t vlist) {
   const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
+  const Site c = decompose(s, g);
   float gl[Q];
-  pull_all(fOld, n, neighbours(decompose(s, g), g), gl, std::make_integer_sequence<int, Q>{});
-  const float F[3] = {force[s], force[size_t(n) + s], force[2 * size_t(n) + s]};
+  pull_all(fOld, n, neighbours(c, g), gl, std::make_integer_sequence<int, Q>{});
+  float F[3];
+  if constexpr (VF) {
+    double acc[3];
+    wave_force(vf.pts, vf.off, vf.sorted, vlist + threadIdx.x / WAVE_SITES, true, int(c.i), int(c.j), int(c.k), g,
+               acc);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) F[a] = float(acc[a]);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) F[a] = force[size_t(a) * n + s];
+  }
   Moments32 m = moments_dev32(gl);
   relax_dev32(m, F, tau_s, tau_b, omega_s, omega_b);
   float out[Q];
@@ -413,24 +442,32 @@ __device__ __forceinline__ void collide_f32_body(const float* __restrict__ fOld,
   for (int p = 0; p < Q; ++p) store_stream(&fNew[size_t(p) * n + s], out[p]);
 }
 
-__device__ __forceinline__ void hydro_f32_body(const float* __restrict__ f,
-                                                          float* __restrict__ force,
-                                                          float* __restrict__ drho_out,
-                                                          float* __restrict__ u, Geom g, int zero_force, uint32_t blk) {
+template <bool VF>
+__device__ __forceinline__ void hydro_f32_body(const float* __restrict__ f, const float* __restrict__ force,
+                                               float* __restrict__ drho_out, float* __restrict__ u, Geom g,
+                                               uint32_t blk, const VForce& vf, uint32_t vlist) {
   const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
+  const Site c = decompose(s, g);
   float gl[Q];
-  pull_all(f, n, neighbours(decompose(s, g), g), gl, std::make_integer_sequence<int, Q>{});
+  pull_all(f, n, neighbours(c, g), gl, std::make_integer_sequence<int, Q>{});
+  float F[3];
+  if constexpr (VF) {
+    double acc[3];
+    wave_force(vf.pts, vf.off, vf.sorted, vlist + threadIdx.x / WAVE_SITES, true, int(c.i), int(c.j), int(c.k), g,
+               acc);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) F[a] = float(acc[a]);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) F[a] = force[size_t(a) * n + s];
+  }
   const Moments32 m = moments_dev32(gl);
   const float inv_rho = 1.0f / (1.0f + m.drho);
   drho_out[s] = m.drho;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) u[size_t(a) * n + s] = __builtin_fmaf(0.5f, force[size_t(a) * n + s], m.j[a]) * inv_rho;
-  if (zero_force) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) force[size_t(a) * n + s] = 0.0f;
-  }
+  for (int a = 0; a < 3; ++a) u[size_t(a) * n + s] = __builtin_fmaf(0.5f, F[a], m.j[a]) * inv_rho;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_init_from_hydro_f32(const float* __restrict__ drho,
@@ -472,54 +509,98 @@ __global__ __launch_bounds__(BLOCK) void k_collide_stream_f64(const double* __re
                                                               const double* __restrict__ force,
                                                               double* __restrict__ gNew, Geom g, double tau_s,
                                                               double tau_b, double omega_s, double omega_b) {
-  collide_f64_body(gOld, force, gNew, g, tau_s, tau_b, omega_s, omega_b, blockIdx.x);
+  collide_f64_body<false>(gOld, force, gNew, g, tau_s, tau_b, omega_s, omega_b, blockIdx.x, VForce{}, 0);
+}
+__global__ __launch_bounds__(BLOCK) void k_collide_stream_vf_f64(const double* __restrict__ gOld, VForce vf,
+                                                                 double* __restrict__ gNew, Geom g, double tau_s,
+                                                                 double tau_b, double omega_s, double omega_b) {
+  collide_f64_body<true>(gOld, nullptr, gNew, g, tau_s, tau_b, omega_s, omega_b, blockIdx.x, vf, blockIdx.x * LISTS_PER_BLOCK);
 }
 __global__ __launch_bounds__(BLOCK) void k_collide_stream_f32(const float* __restrict__ gOld,
                                                               const float* __restrict__ force,
                                                               float* __restrict__ gNew, Geom g, float tau_s,
                                                               float tau_b, float omega_s, float omega_b) {
-  collide_f32_body(gOld, force, gNew, g, tau_s, tau_b, omega_s, omega_b, blockIdx.x);
+  collide_f32_body<false>(gOld, force, gNew, g, tau_s, tau_b, omega_s, omega_b, blockIdx.x, VForce{}, 0);
 }
-__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f64(const double* __restrict__ gCur, double* __restrict__ force,
-                                                          double* __restrict__ rho, double* __restrict__ u, Geom g,
-                                                          int zero_force) {
-  hydro_f64_body(gCur, force, rho, u, g, zero_force, blockIdx.x);
+__global__ __launch_bounds__(BLOCK) void k_collide_stream_vf_f32(const float* __restrict__ gOld, VForce vf,
+                                                                 float* __restrict__ gNew, Geom g, float tau_s,
+                                                                 float tau_b, float omega_s, float omega_b) {
+  collide_f32_body<true>(gOld, nullptr, gNew, g, tau_s, tau_b, omega_s, omega_b, blockIdx.x, vf, blockIdx.x * LISTS_PER_BLOCK);
 }
-__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f32(const float* __restrict__ gCur, float* __restrict__ force,
-                                                          float* __restrict__ rho, float* __restrict__ u, Geom g,
-                                                          int zero_force) {
-  hydro_f32_body(gCur, force, rho, u, g, zero_force, blockIdx.x);
+__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f64(const double* __restrict__ gCur,
+                                                          const double* __restrict__ force, double* __restrict__ rho,
+                                                          double* __restrict__ u, Geom g) {
+  hydro_f64_body<false>(gCur, force, rho, u, g, blockIdx.x, VForce{}, 0);
+}
+__global__ __launch_bounds__(BLOCK) void k_calc_hydro_vf_f64(const double* __restrict__ gCur, VForce vf,
+                                                             double* __restrict__ rho, double* __restrict__ u,
+                                                             Geom g) {
+  hydro_f64_body<true>(gCur, nullptr, rho, u, g, blockIdx.x, vf, blockIdx.x * LISTS_PER_BLOCK);
+}
+__global__ __launch_bounds__(BLOCK) void k_calc_hydro_f32(const float* __restrict__ gCur,
+                                                          const float* __restrict__ force, float* __restrict__ rho,
+                                                          float* __restrict__ u, Geom g) {
+  hydro_f32_body<false>(gCur, force, rho, u, g, blockIdx.x, VForce{}, 0);
+}
+__global__ __launch_bounds__(BLOCK) void k_calc_hydro_vf_f32(const float* __restrict__ gCur, VForce vf,
+                                                             float* __restrict__ rho, float* __restrict__ u, Geom g) {
+  hydro_f32_body<true>(gCur, nullptr, rho, u, g, blockIdx.x, vf, blockIdx.x * LISTS_PER_BLOCK);
 }
 
 // At the start of a batched step with `parity` p the state of member m is in
 // g[cur_m ^ p]; the collision writes g[cur_m ^ p ^ 1], which CalcHydro reads.
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void kb_collide(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
-                                                    int parity) {
+// VF: the force of block `unit` of member m comes from lists tile_base_m + 4 unit ...
+template <typename T, bool VF>
+__device__ __forceinline__ void kb_collide_body(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
+                                                int parity, const VForce& vf) {
   const WorkItem w = work[blockIdx.x];
   const LatDesc& m = desc[w.member];
   const int c = m.cur ^ parity;
   if constexpr (sizeof(T) == 8)
-    collide_f64_body(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
-                     static_cast<double*>(m.g[c ^ 1]), m.geom, m.tau_s, m.tau_b, m.omega_s, m.omega_b, w.unit);
+    collide_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
+                         static_cast<double*>(m.g[c ^ 1]), m.geom, m.tau_s, m.tau_b, m.omega_s, m.omega_b, w.unit, vf,
+                         m.tile_base + w.unit * LISTS_PER_BLOCK);
   else
-    collide_f32_body(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
-                     static_cast<float*>(m.g[c ^ 1]), m.geom, float(m.tau_s), float(m.tau_b), float(m.omega_s),
-                     float(m.omega_b), w.unit);
+    collide_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
+                         static_cast<float*>(m.g[c ^ 1]), m.geom, float(m.tau_s), float(m.tau_b), float(m.omega_s),
+                         float(m.omega_b), w.unit, vf, m.tile_base + w.unit * LISTS_PER_BLOCK);
 }
 
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void kb_hydro(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
-                                                  int parity) {
+template <typename T, bool VF>
+__device__ __forceinline__ void kb_hydro_body(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
+                                              int parity, const VForce& vf) {
   const WorkItem w = work[blockIdx.x];
   const LatDesc& m = desc[w.member];
   const int c = m.cur ^ parity ^ 1;
   if constexpr (sizeof(T) == 8)
-    hydro_f64_body(static_cast<const double*>(m.g[c]), static_cast<double*>(m.force), static_cast<double*>(m.rho),
-                   static_cast<double*>(m.u), m.geom, 0, w.unit);
+    hydro_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
+                       static_cast<double*>(m.rho), static_cast<double*>(m.u), m.geom, w.unit, vf,
+                       m.tile_base + w.unit * LISTS_PER_BLOCK);
   else
-    hydro_f32_body(static_cast<const float*>(m.g[c]), static_cast<float*>(m.force), static_cast<float*>(m.rho),
-                   static_cast<float*>(m.u), m.geom, 0, w.unit);
+    hydro_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
+                       static_cast<float*>(m.rho), static_cast<float*>(m.u), m.geom, w.unit, vf,
+                       m.tile_base + w.unit * LISTS_PER_BLOCK);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kb_collide(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
+                                                    int parity) {
+  kb_collide_body<T, false>(desc, work, parity, VForce{});
+}
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kb_collide_vf(const LatDesc* __restrict__ desc,
+                                                       const WorkItem* __restrict__ work, int parity, VForce vf) {
+  kb_collide_body<T, true>(desc, work, parity, vf);
+}
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kb_hydro(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
+                                                  int parity) {
+  kb_hydro_body<T, false>(desc, work, parity, VForce{});
+}
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kb_hydro_vf(const LatDesc* __restrict__ desc,
+                                                     const WorkItem* __restrict__ work, int parity, VForce vf) {
+  kb_hydro_body<T, true>(desc, work, parity, vf);
 }
 
 // ------------------------------------------------------------------------
@@ -591,41 +672,53 @@ inline unsigned grid_for(uint32_t n) { return (n + BLOCK - 1) / BLOCK; }
 // ---- launchers -------------------------------------------------------------
 
 template <>
-hipError_t launch_collide_stream<double>(const double* fOld, const double* force, double* fNew,
-                                         const Geom& g, double tau_s, double tau_b, hipStream_t s) {
+hipError_t launch_collide_stream<double>(const double* fOld, const double* force, double* fNew, const Geom& g,
+                                         double tau_s, double tau_b, const VForce* vf, hipStream_t s) {
   if (g.n == 0) return hipSuccess;
   // omega = 1/(tau + 1/2) as lb.cu:107-108 computes it, once per launch.
   const double omega_s = 1.0 / (tau_s + 0.5), omega_b = 1.0 / (tau_b + 0.5);
-  hipLaunchKernelGGL(k_collide_stream_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, fOld, force, fNew, g,
-                     tau_s, tau_b, omega_s, omega_b);
+  if (vf)
+    hipLaunchKernelGGL(k_collide_stream_vf_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, fOld, *vf, fNew, g, tau_s,
+                       tau_b, omega_s, omega_b);
+  else
+    hipLaunchKernelGGL(k_collide_stream_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, fOld, force, fNew, g, tau_s,
+                       tau_b, omega_s, omega_b);
   return hipGetLastError();
 }
 
 template <>
 hipError_t launch_collide_stream<float>(const float* fOld, const float* force, float* fNew, const Geom& g,
-                                        double tau_s, double tau_b, hipStream_t s) {
+                                        double tau_s, double tau_b, const VForce* vf, hipStream_t s) {
   if (g.n == 0) return hipSuccess;
   const float os = float(1.0 / (tau_s + 0.5)), ob = float(1.0 / (tau_b + 0.5));
-  hipLaunchKernelGGL(k_collide_stream_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, fOld, force, fNew, g,
-                     float(tau_s), float(tau_b), os, ob);
+  if (vf)
+    hipLaunchKernelGGL(k_collide_stream_vf_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, fOld, *vf, fNew, g,
+                       float(tau_s), float(tau_b), os, ob);
+  else
+    hipLaunchKernelGGL(k_collide_stream_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, fOld, force, fNew, g,
+                       float(tau_s), float(tau_b), os, ob);
   return hipGetLastError();
 }
 
 template <>
-hipError_t launch_calc_hydro<double>(const double* f, double* force, double* rho, double* u, const Geom& g,
-                                     bool zero_force, hipStream_t s) {
+hipError_t launch_calc_hydro<double>(const double* f, const double* force, double* rho, double* u, const Geom& g,
+                                     const VForce* vf, hipStream_t s) {
   if (g.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_calc_hydro_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g,
-                     int(zero_force));
+  if (vf)
+    hipLaunchKernelGGL(k_calc_hydro_vf_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, *vf, rho, u, g);
+  else
+    hipLaunchKernelGGL(k_calc_hydro_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g);
   return hipGetLastError();
 }
 
 template <>
-hipError_t launch_calc_hydro<float>(const float* f, float* force, float* rho, float* u, const Geom& g,
-                                    bool zero_force, hipStream_t s) {
+hipError_t launch_calc_hydro<float>(const float* f, const float* force, float* rho, float* u, const Geom& g,
+                                    const VForce* vf, hipStream_t s) {
   if (g.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_calc_hydro_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g,
-                     int(zero_force));
+  if (vf)
+    hipLaunchKernelGGL(k_calc_hydro_vf_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, *vf, rho, u, g);
+  else
+    hipLaunchKernelGGL(k_calc_hydro_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g);
   return hipGetLastError();
 }
 
@@ -646,21 +739,29 @@ hipError_t launch_init_from_hydro<float>(const float* rho, const float* u, float
 }
 
 template <typename T>
-hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, hipStream_t s) {
+hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
+                                hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_collide<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity);
+  if (vf)
+    hipLaunchKernelGGL(kb_collide_vf<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf);
+  else
+    hipLaunchKernelGGL(kb_collide<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity);
   return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, hipStream_t s) {
+hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
+                              hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_hydro<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity);
+  if (vf)
+    hipLaunchKernelGGL(kb_hydro_vf<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf);
+  else
+    hipLaunchKernelGGL(kb_hydro<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity);
   return hipGetLastError();
 }
-template hipError_t launch_batch_collide<double>(const LatDesc*, const WorkItem*, int, int, hipStream_t);
-template hipError_t launch_batch_collide<float>(const LatDesc*, const WorkItem*, int, int, hipStream_t);
-template hipError_t launch_batch_hydro<double>(const LatDesc*, const WorkItem*, int, int, hipStream_t);
-template hipError_t launch_batch_hydro<float>(const LatDesc*, const WorkItem*, int, int, hipStream_t);
+template hipError_t launch_batch_collide<double>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
+template hipError_t launch_batch_collide<float>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
+template hipError_t launch_batch_hydro<double>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
+template hipError_t launch_batch_hydro<float>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
 
 uint32_t lattice_blocks(const Geom& g) { return grid_for(g.n); }
 

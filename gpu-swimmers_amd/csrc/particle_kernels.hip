@@ -12,6 +12,7 @@
 
 #include "d3q15.hpp"
 #include "internal.hpp"
+#include "spread.hpp"
 
 namespace lbs {
 namespace {
@@ -410,101 +411,86 @@ __global__ __launch_bounds__(PBLOCK) void kb_tracers_move(const LatDesc* __restr
 
 // ---- deterministic force spreading -----------------------------------------
 // ZeroForce followed by AddForces of every swimmer array, computed as a gather
-// per lattice tile so that each site adds its contributions in swimmer order,
+// per run of 64 consecutive lattice sites (one list per wavefront of the
+// lattice kernels) so that each site adds its contributions in swimmer order,
 // tail before head -- the order a sequential restatement of
-// SwimmerArray.cu:42-112 uses -- without atomics on the force field.
-//   1. k_points: per swimmer, the tail and head stencils (wrapped site
+// SwimmerArray.cu:42-112 uses -- without atomics on the force.
+//   1. k_points: the tail and head stencils of every swimmer (wrapped site
 //      coordinates and Peskin weights, exactly as AccumulateDeltaForce forms
-//      them) are stored by point id = 2*(array base + user index) + end, and
-//      every tile a stencil touches is counted;
-//   2. k_scan_tiles: exclusive scan of the counts (one workgroup);
-//   3. k_fill_lists: point ids appended to their tiles' lists;
-//   4. k_sort_lists: each list ordered by point id (rank sort);
-//   5. k_spread_tiles: one workgroup per tile sums, for each site, the
-//      contributions of its tile's points in id order and writes all three
-//      force components of every site once (zeros where no point reaches).
+//      them) are stored by point id = 2*(array base + user index) + end;
+//   2. k_count_rows: every list a stencil row touches is counted;
+//   3. k_scan_sums / k_scan_apply: exclusive scan of the counts;
+//   4. k_fill_rows: point ids appended to their lists;
+//   5. k_sort_lists: each list ordered by point id (stable rank sort);
+//   6. either k_spread_blocks writes the force of every site (zeros where no
+//      point reaches), or the collide and CalcHydro kernels evaluate the force
+//      of their own sites from the same lists (wave_force).
 
-__device__ __forceinline__ int tiles_of(const int (&wc)[4], int tdim, int ntd, int (&out)[4]) {
-  int m = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int t = wc[k] / tdim;
-    bool seen = false;
-    for (int q = 0; q < m; ++q) seen = seen || (out[q] == t);
-    if (!seen) out[m++] = t;
-  }
-  (void)ntd;
-  return m;
-}
-
+// One thread per point: thread t is end t & 1 of the swimmer in slot t >> 1.
 __device__ __forceinline__ void points_body(int ns, uint32_t point_base, double P, double l,
                                             const double* __restrict__ r, const double* __restrict__ nv,
-                                            const int* __restrict__ perm, const Geom& g, const SpreadGeom& sg,
-                                            uint32_t tile_base, int member, SpreadPoint* __restrict__ pts,
-                                            uint32_t* __restrict__ counts, uint32_t blk) {
-  const int slot = int(blk * PBLOCK + threadIdx.x);
-  if (slot >= ns) return;
+                                            const int* __restrict__ perm, const Geom& g, int member,
+                                            SpreadPoint* __restrict__ pts, uint32_t blk) {
+  const int t = int(blk * PBLOCK + threadIdx.x);
+  if (t >= 2 * ns) return;
+  const int slot = t >> 1, end = t & 1;
   const int size[3] = {g.nx, g.ny, g.nz};
-  const int tdim[3] = {sg.tx, sg.ty, sg.tz};
-  const int ntd[3] = {sg.ntx, sg.nty, sg.ntz};
-  const int user = perm[slot];
-  double ra[3], na[3];
+  SpreadPoint sp;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    ra[a] = r[size_t(ns) * a + slot];
-    na[a] = nv[size_t(ns) * a + slot];
-  }
-  double Ft[3];
+  for (int d = 0; d < 3; ++d) {
+    const double ra = r[size_t(ns) * d + slot], na = nv[size_t(ns) * d + slot];
+    const double Ft = -P * na;
+    const double rp = end ? ra : ra - na * l;
+    const double x0 = ceil(rp - 2.);
 #pragma unroll
-  for (int a = 0; a < 3; ++a) Ft[a] = -P * na[a];
-  for (int end = 0; end < 2; ++end) {
-    SpreadPoint sp;
-    int tl[3][4], nt[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double rp = end ? ra[d] : ra[d] - na[d] * l;
-      const double x0 = ceil(rp - 2.);
-      int wc[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double x = x0 + k;
-        wc[k] = int(wrap_index(x, size[d]));
-        sp.wc[d * 4 + k] = wc[k];
-        sp.del[d * 4 + k] = peskin(rp - x);
-      }
-      nt[d] = tiles_of(wc, tdim[d], ntd[d], tl[d]);
-      sp.F[d] = end ? Ft[d] * -1.0 : Ft[d];
+    for (int k = 0; k < 4; ++k) {
+      const double x = x0 + k;
+      sp.wc[d * 4 + k] = int(wrap_index(x, size[d]));
+      sp.del[d * 4 + k] = peskin(rp - x);
     }
-    sp.pad[0] = member;
-    sp.pad[1] = 0;
-    pts[size_t(point_base) + 2 * size_t(user) + end] = sp;
-    for (int a = 0; a < nt[0]; ++a)
-      for (int b = 0; b < nt[1]; ++b)
-        for (int c = 0; c < nt[2]; ++c)
-          atomicAdd(&counts[tile_base + (size_t(tl[0][a]) * sg.nty + tl[1][b]) * sg.ntz + tl[2][c]], 1u);
+    sp.F[d] = end ? Ft * -1.0 : Ft;
   }
+  sp.pad[0] = member;
+  sp.pad[1] = 0;
+  pts[size_t(point_base) + 2 * size_t(perm[slot]) + end] = sp;
 }
 
 __global__ __launch_bounds__(PBLOCK) void k_points(int ns, int base, double P, double l, const double* __restrict__ r,
                                                   const double* __restrict__ nv, const int* __restrict__ perm,
-                                                  Geom g, SpreadGeom sg, SpreadPoint* __restrict__ pts,
-                                                  uint32_t* __restrict__ counts) {
-  points_body(ns, 2 * uint32_t(base), P, l, r, nv, perm, g, sg, 0, 0, pts, counts, blockIdx.x);
+                                                  Geom g, SpreadPoint* __restrict__ pts) {
+  points_body(ns, 2 * uint32_t(base), P, l, r, nv, perm, g, 0, pts, blockIdx.x);
 }
 
+// Batched: work items are blocks of 128 swimmers (256 points) of an array.
 __global__ __launch_bounds__(PBLOCK) void kb_points(const LatDesc* __restrict__ ldesc,
                                                    const SwimDesc* __restrict__ sdesc,
-                                                   const WorkItem* __restrict__ work, SpreadPoint* __restrict__ pts,
-                                                   uint32_t* __restrict__ counts) {
+                                                   const WorkItem* __restrict__ work, SpreadPoint* __restrict__ pts) {
   const WorkItem w = work[blockIdx.x];
   const SwimDesc& a = sdesc[w.member];
   const LatDesc& m = ldesc[a.member];
-  points_body(a.num, a.point_base, a.p.P, a.p.l, a.r, a.n, a.perm, m.geom, m.sg, m.tile_base, a.member, pts, counts,
-              w.unit);
+  points_body(a.num, a.point_base, a.p.P, a.p.l, a.r, a.n, a.perm, m.geom, a.member, pts, w.unit);
+}
+
+// One thread per (point, stencil row): counts the lists the row reaches.
+// `Batched`: geometry and first list from the point's member.
+template <bool Batched>
+__global__ __launch_bounds__(PBLOCK) void k_count_rows(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
+                                                      const SpreadPoint* __restrict__ pts,
+                                                      uint32_t* __restrict__ counts) {
+  const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
+  if (t >= 16 * npts) return;
+  const SpreadPoint& p = pts[t >> 4];
+  uint32_t base = 0;
+  if constexpr (Batched) {
+    const LatDesc& m = ldesc[p.pad[0]];
+    g = m.geom;
+    base = m.tile_base;
+  }
+  for_each_row_list(p, int(t & 15), g, [&](uint32_t b) { atomicAdd(&counts[base + b], 1u); });
 }
 
 // Exclusive scan of counts[0..nt) into off[0..nt] in two coalesced passes
-// over SCAN_BLOCK tiles per workgroup; the second pass also clears cursor.
+// over SCAN_BLOCK lists per workgroup; the second pass also clears cursor.
 constexpr int SCAN_BLOCK = 1024;
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* part, uint32_t* total) {
@@ -555,168 +541,106 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(int nt, const uint32_
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[nt] = base + total;
 }
 
-__device__ __forceinline__ void fill_point(const SpreadPoint& p, uint32_t id, const SpreadGeom& sg,
-                                           uint32_t tile_base, const uint32_t* __restrict__ off,
-                                           uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
-  const int tdim[3] = {sg.tx, sg.ty, sg.tz};
-  const int ntd[3] = {sg.ntx, sg.nty, sg.ntz};
-  int tl[3][4], nt[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    int wc[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) wc[k] = p.wc[d * 4 + k];
-    nt[d] = tiles_of(wc, tdim[d], ntd[d], tl[d]);
-  }
-  for (int a = 0; a < nt[0]; ++a)
-    for (int b = 0; b < nt[1]; ++b)
-      for (int c = 0; c < nt[2]; ++c) {
-        const size_t tile = tile_base + (size_t(tl[0][a]) * sg.nty + tl[1][b]) * sg.ntz + tl[2][c];
-        const uint32_t pos = atomicAdd(&cursor[tile], 1u);
-        list[off[tile] + pos] = id;
-      }
-}
-
-__global__ __launch_bounds__(PBLOCK) void k_fill_lists(int npts, SpreadGeom sg, const SpreadPoint* __restrict__ pts,
-                                                      const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
-                                                      uint32_t* __restrict__ list) {
-  const int id = int(blockIdx.x * PBLOCK + threadIdx.x);
-  if (id >= npts) return;
-  fill_point(pts[id], uint32_t(id), sg, 0, off, cursor, list);
-}
-
-// Batched: the member of each point is in SpreadPoint::pad[0].
-__global__ __launch_bounds__(PBLOCK) void kb_fill_lists(const LatDesc* __restrict__ ldesc, int npts,
-                                                       const SpreadPoint* __restrict__ pts,
-                                                       const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
-                                                       uint32_t* __restrict__ list) {
-  const int id = int(blockIdx.x * PBLOCK + threadIdx.x);
-  if (id >= npts) return;
+// One thread per (point, stencil row): appends the point id to the lists the
+// row reaches.
+template <bool Batched>
+__global__ __launch_bounds__(PBLOCK) void k_fill_rows(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
+                                                     const SpreadPoint* __restrict__ pts,
+                                                     const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
+                                                     uint32_t* __restrict__ list) {
+  const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
+  if (t >= 16 * npts) return;
+  const uint32_t id = t >> 4;
   const SpreadPoint& p = pts[id];
-  const LatDesc& m = ldesc[p.pad[0]];
-  fill_point(p, uint32_t(id), m.sg, m.tile_base, off, cursor, list);
+  uint32_t base = 0;
+  if constexpr (Batched) {
+    const LatDesc& m = ldesc[p.pad[0]];
+    g = m.geom;
+    base = m.tile_base;
+  }
+  for_each_row_list(p, int(t & 15), g, [&](uint32_t b) {
+    const uint32_t tile = base + b;
+    const uint32_t pos = atomicAdd(&cursor[tile], 1u);
+    list[off[tile] + pos] = id;
+  });
 }
 
-__global__ __launch_bounds__(PBLOCK) void k_sort_lists(const uint32_t* __restrict__ off,
+// Stable rank sort of every list: equal ids (a point listed twice) keep
+// their list order, so every position of `sorted` is written once.  Each lane
+// sorts one list of up to SHORT_LIST entries alone; longer lists are then
+// ranked by the whole wavefront, one at a time.
+constexpr uint32_t SHORT_LIST = 32;
+
+__device__ __forceinline__ void rank_entries(const uint32_t* __restrict__ src, uint32_t cnt, uint32_t first,
+                                             uint32_t step, uint32_t* __restrict__ dst) {
+  for (uint32_t e = first; e < cnt; e += step) {
+    const uint32_t id = src[e];
+    uint32_t rank = 0;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const uint32_t v = src[k];
+      rank += (v < id || (v == id && k < e)) ? 1u : 0u;
+    }
+    dst[rank] = id;
+  }
+}
+
+__global__ __launch_bounds__(PBLOCK) void k_sort_lists(uint32_t nlists, const uint32_t* __restrict__ off,
                                                       const uint32_t* __restrict__ list,
                                                       uint32_t* __restrict__ sorted) {
-  const uint32_t o = off[blockIdx.x], cnt = off[blockIdx.x + 1] - o;
-  for (uint32_t e = threadIdx.x; e < cnt; e += PBLOCK) {
-    const uint32_t id = list[o + e];
-    uint32_t rank = 0;
-    for (uint32_t k = 0; k < cnt; ++k) rank += list[o + k] < id ? 1u : 0u;
-    sorted[o + rank] = id;
+  const uint32_t wl = blockIdx.x * PBLOCK + threadIdx.x;
+  const uint32_t lane = threadIdx.x % WAVE_SITES;
+  uint32_t o = 0, cnt = 0;
+  if (wl < nlists) {
+    o = off[wl];
+    cnt = off[wl + 1] - o;
+    if (cnt <= SHORT_LIST) rank_entries(list + o, cnt, 0, 1, sorted + o);
+  }
+  uint64_t longs = __ballot(cnt > SHORT_LIST);
+  while (longs) {
+    const int src_lane = __builtin_ctzll(longs);
+    longs &= longs - 1;
+    const uint32_t lo = __shfl(o, src_lane), lc = __shfl(cnt, src_lane);
+    rank_entries(list + lo, lc, lane, WAVE_SITES, sorted + lo);
   }
 }
 
-constexpr int SPREAD_CHUNK = 64;
-constexpr int POINT_WORDS = int(sizeof(SpreadPoint) / 8);
-
-// Adds the contributions of one point to one site, in the (i, j, k) loop order
-// of AccumulateDeltaForce.  `simple`: every lattice side is >= 4, so each axis
-// has at most one matching stencil position, found by a wrapped difference.
-__device__ __forceinline__ void add_point(const SpreadPoint& p, int X, int Y, int Z, const int (&size)[3],
-                                          bool simple, double (&acc)[3]) {
-  if (simple) {
-    int di = X - p.wc[0];
-    di += di < 0 ? size[0] : 0;
-    if (di >= 4) return;
-    int dj = Y - p.wc[4];
-    dj += dj < 0 ? size[1] : 0;
-    if (dj >= 4) return;
-    int dk = Z - p.wc[8];
-    dk += dk < 0 ? size[2] : 0;
-    if (dk >= 4) return;
-    const double delta3d = p.del[di] * p.del[4 + dj] * p.del[8 + dk];
-    acc[0] += delta3d * p.F[0];
-    acc[1] += delta3d * p.F[1];
-    acc[2] += delta3d * p.F[2];
-    return;
-  }
-  for (int i = 0; i < 4; ++i) {
-    if (p.wc[i] != X) continue;
-    for (int j = 0; j < 4; ++j) {
-      if (p.wc[4 + j] != Y) continue;
-      const double dxy = p.del[i] * p.del[4 + j];
-      for (int k = 0; k < 4; ++k) {
-        if (p.wc[8 + k] != Z) continue;
-        const double delta3d = dxy * p.del[8 + k];
-        acc[0] += delta3d * p.F[0];
-        acc[1] += delta3d * p.F[1];
-        acc[2] += delta3d * p.F[2];
-      }
-    }
-  }
-}
-
+// Force of the sites of lattice block `blk` (lists first_list.. of its
+// wavefronts) written to the force array.
 template <typename T>
-__device__ __forceinline__ void spread_tile_body(const Geom& g, const SpreadGeom& sg,
-                                                 const SpreadPoint* __restrict__ pts, const uint32_t* __restrict__ off,
-                                                 const uint32_t* __restrict__ sorted, T* __restrict__ force, int tile,
-                                                 uint32_t gtile, SpreadPoint* __restrict__ sp) {
-  const int tk = tile % sg.ntz, tj = (tile / sg.ntz) % sg.nty, ti = tile / (sg.ntz * sg.nty);
-  const uint32_t o = off[gtile], cnt = off[gtile + 1] - o;
-  const int size[3] = {g.nx, g.ny, g.nz};
-  const bool simple = g.nx >= 4 && g.ny >= 4 && g.nz >= 4;
-  constexpr int PER = 2;  // 512 sites per tile, 256 threads
-  int X[PER], Y[PER], Z[PER];
-  bool ok[PER];
-  double acc[PER][3];
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int L = int(threadIdx.x) + q * PBLOCK;
-    const int lz = L % sg.tz, ly = (L / sg.tz) % sg.ty, lx = L / (sg.tz * sg.ty);
-    X[q] = ti * sg.tx + lx;
-    Y[q] = tj * sg.ty + ly;
-    Z[q] = tk * sg.tz + lz;
-    ok[q] = X[q] < g.nx && Y[q] < g.ny && Z[q] < g.nz;
-    acc[q][0] = acc[q][1] = acc[q][2] = 0.0;
-  }
-  for (uint32_t base = 0; base < cnt; base += SPREAD_CHUNK) {
-    const int m = int(min(uint32_t(SPREAD_CHUNK), cnt - base));
-    __syncthreads();
-    uint64_t* dst = reinterpret_cast<uint64_t*>(sp);
-    for (int w = int(threadIdx.x); w < m * POINT_WORDS; w += PBLOCK) {
-      const int e = w / POINT_WORDS, c = w - e * POINT_WORDS;
-      dst[w] = reinterpret_cast<const uint64_t*>(pts + sorted[o + base + e])[c];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      if (!ok[q]) continue;
-      for (int e = 0; e < m; ++e) add_point(sp[e], X[q], Y[q], Z[q], size, simple, acc[q]);
-    }
-  }
+__device__ __forceinline__ void spread_block_body(const Geom& g, const SpreadPoint* __restrict__ pts,
+                                                  const uint32_t* __restrict__ off,
+                                                  const uint32_t* __restrict__ sorted, T* __restrict__ force,
+                                                  uint32_t blk, uint32_t first_list) {
+  const uint32_t s = blk * SITE_BLOCK + threadIdx.x;
+  const bool ok = s < g.n;
+  const SiteXYZ c = site_xyz(ok ? s : 0u, g);
+  double acc[3];
+  wave_force(pts, off, sorted, first_list + threadIdx.x / WAVE_SITES, ok, c.x, c.y, c.z, g, acc);
+  if (!ok) return;
   const size_t n = g.n;
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    if (!ok[q]) continue;
-    const size_t site = (size_t(X[q]) * g.ny + Y[q]) * g.nz + Z[q];
-    force[site] = T(acc[q][0]);
-    force[n + site] = T(acc[q][1]);
-    force[2 * n + site] = T(acc[q][2]);
-  }
+  force[s] = T(acc[0]);
+  force[n + s] = T(acc[1]);
+  force[2 * n + s] = T(acc[2]);
 }
 
 template <typename T>
-__global__ __launch_bounds__(PBLOCK) void k_spread_tiles(Geom g, SpreadGeom sg, const SpreadPoint* __restrict__ pts,
-                                                        const uint32_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ sorted, T* __restrict__ force) {
-  __shared__ SpreadPoint sp[SPREAD_CHUNK];
-  spread_tile_body(g, sg, pts, off, sorted, force, int(blockIdx.x), blockIdx.x, sp);
+__global__ __launch_bounds__(SITE_BLOCK) void k_spread_blocks(Geom g, const SpreadPoint* __restrict__ pts,
+                                                             const uint32_t* __restrict__ off,
+                                                             const uint32_t* __restrict__ sorted,
+                                                             T* __restrict__ force) {
+  spread_block_body(g, pts, off, sorted, force, blockIdx.x, blockIdx.x * LISTS_PER_BLOCK);
 }
 
 template <typename T>
-__global__ __launch_bounds__(PBLOCK) void kb_spread_tiles(const LatDesc* __restrict__ ldesc,
-                                                         const WorkItem* __restrict__ tile_work,
-                                                         const SpreadPoint* __restrict__ pts,
-                                                         const uint32_t* __restrict__ off,
-                                                         const uint32_t* __restrict__ sorted) {
-  __shared__ SpreadPoint sp[SPREAD_CHUNK];
-  const WorkItem w = tile_work[blockIdx.x];
+__global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __restrict__ ldesc,
+                                                              const WorkItem* __restrict__ block_work,
+                                                              const SpreadPoint* __restrict__ pts,
+                                                              const uint32_t* __restrict__ off,
+                                                              const uint32_t* __restrict__ sorted) {
+  const WorkItem w = block_work[blockIdx.x];
   const LatDesc& m = ldesc[w.member];
-  spread_tile_body(m.geom, m.sg, pts, off, sorted, static_cast<T*>(m.force), int(w.unit), m.tile_base + w.unit,
-                   sp);
+  spread_block_body(m.geom, pts, off, sorted, static_cast<T*>(m.force), w.unit,
+                    m.tile_base + w.unit * LISTS_PER_BLOCK);
 }
 
 // ---- device-math probes used by the parity tests ----
@@ -786,81 +710,90 @@ template hipError_t launch_tracers_move<float>(int, double*, double*, double*, c
                                                hipStream_t);
 
 
-SpreadGeom make_spread_geom(const Geom& g) {
-  SpreadGeom sg;
-  int tz = 1;
-  while (tz < g.nz && tz < 32) tz <<= 1;
-  sg.tz = tz;
-  sg.ty = 4;
-  sg.tx = 512 / (sg.ty * sg.tz);
-  sg.ntx = (g.nx + sg.tx - 1) / sg.tx;
-  sg.nty = (g.ny + sg.ty - 1) / sg.ty;
-  sg.ntz = (g.nz + sg.tz - 1) / sg.tz;
-  sg.ntiles = sg.ntx * sg.nty * sg.ntz;
-  return sg;
+size_t spread_list_capacity(size_t npts) {
+  // A stencil has 16 (x, y) rows of 4 sites; the z-run of a row (one wrap at
+  // most) lies in at most 4 lists, so a point is listed at most 64 times.
+  return npts * 64;
 }
 
-size_t spread_list_capacity(const SpreadGeom& sg, size_t npts) {
-  const size_t px = sg.ntx < 4 ? sg.ntx : 4, py = sg.nty < 4 ? sg.nty : 4, pz = sg.ntz < 4 ? sg.ntz : 4;
-  return npts * px * py * pz;
-}
+uint32_t spread_lists(const Geom& g) { return (g.n + SITE_BLOCK - 1) / SITE_BLOCK * LISTS_PER_BLOCK; }
 
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
-                         const Geom& g, const SpreadGeom& sg, SpreadPoint* pts, uint32_t* counts, hipStream_t s) {
+                         const Geom& g, SpreadPoint* pts, hipStream_t s) {
   if (ns <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_points, dim3(blocks_for(size_t(ns))), dim3(PBLOCK), 0, s, ns, base, p.P, p.l, r, n, perm, g,
-                     sg, pts, counts);
+  hipLaunchKernelGGL(k_points, dim3(blocks_for(2 * size_t(ns))), dim3(PBLOCK), 0, s, ns, base, p.P, p.l, r, n, perm, g,
+                     pts);
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_spread(int npts, const Geom& g, const SpreadGeom& sg, const SpreadPoint* pts, uint32_t* counts,
-                         uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, T* force, hipStream_t s) {
-  // the per-workgroup totals of the scan live after the nt+1 offsets
-  const unsigned nsb = unsigned((sg.ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK);
-  uint32_t* sums = off + sg.ntiles + 1;
-  hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, sg.ntiles, counts, sums);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, sg.ntiles, counts, sums, off, cursor);
+// Counts, offsets, lists and sorted lists of nt lists from the npts point
+// records (counts zeroed here).
+template <bool Batched>
+static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ldesc, const SpreadPoint* pts,
+                              uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted,
+                              hipStream_t s) {
+  if (hipError_t e = launch_fill_zero(counts, 4 * size_t(nt), s)) return e;
+  const unsigned rows = blocks_for(16 * size_t(npts));
   if (npts > 0)
-    hipLaunchKernelGGL(k_fill_lists, dim3(blocks_for(size_t(npts))), dim3(PBLOCK), 0, s, npts, sg, pts, off, cursor,
-                       list);
-  hipLaunchKernelGGL(k_sort_lists, dim3(sg.ntiles), dim3(PBLOCK), 0, s, off, list, sorted);
-  hipLaunchKernelGGL(k_spread_tiles<T>, dim3(sg.ntiles), dim3(PBLOCK), 0, s, g, sg, pts, off, sorted, force);
+    hipLaunchKernelGGL(k_count_rows<Batched>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc, pts, counts);
+  // the per-workgroup totals of the scan live after the nt+1 offsets
+  const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  uint32_t* sums = off + nt + 1;
+  hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, nt, counts, sums);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, nt, counts, sums, off, cursor);
+  if (npts > 0)
+    hipLaunchKernelGGL(k_fill_rows<Batched>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc, pts, off,
+                       cursor, list);
+  hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(size_t(nt))), dim3(PBLOCK), 0, s, uint32_t(nt), off, list, sorted);
   return hipGetLastError();
 }
-template hipError_t launch_spread<double>(int, const Geom&, const SpreadGeom&, const SpreadPoint*, uint32_t*, uint32_t*,
-                                          uint32_t*, uint32_t*, uint32_t*, double*, hipStream_t);
-template hipError_t launch_spread<float>(int, const Geom&, const SpreadGeom&, const SpreadPoint*, uint32_t*, uint32_t*,
-                                         uint32_t*, uint32_t*, uint32_t*, float*, hipStream_t);
+
+hipError_t launch_spread_lists(int npts, const Geom& g, const SpreadPoint* pts, uint32_t* counts, uint32_t* off,
+                               uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s) {
+  return build_lists<false>(npts, int(spread_lists(g)), g, nullptr, pts, counts, off, cursor, list, sorted, s);
+}
+
+template <typename T>
+hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
+                               T* force, hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_spread_blocks<T>, dim3((g.n + SITE_BLOCK - 1) / SITE_BLOCK), dim3(SITE_BLOCK), 0, s, g, pts, off,
+                     sorted, force);
+  return hipGetLastError();
+}
+template hipError_t launch_spread_force<double>(const Geom&, const SpreadPoint*, const uint32_t*, const uint32_t*,
+                                                double*, hipStream_t);
+template hipError_t launch_spread_force<float>(const Geom&, const SpreadPoint*, const uint32_t*, const uint32_t*,
+                                               float*, hipStream_t);
 
 hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
-                               SpreadPoint* pts, uint32_t* counts, hipStream_t s) {
+                               SpreadPoint* pts, hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_points, dim3(nwork), dim3(PBLOCK), 0, s, ldesc, sdesc, work, pts, counts);
+  hipLaunchKernelGGL(kb_points, dim3(nwork), dim3(PBLOCK), 0, s, ldesc, sdesc, work, pts);
   return hipGetLastError();
 }
 
-// Tiles of all members are numbered consecutively (member m from its
-// tile_base); tile_work lists (member, local tile) for each of them.
+// Lists of all members are numbered consecutively (member m from its
+// tile_base); block_work lists (member, lattice block) for every block.
+hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const SpreadPoint* pts, uint32_t* counts,
+                              uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s) {
+  if (nlists <= 0) return hipSuccess;
+  return build_lists<true>(npts, nlists, Geom{}, ldesc, pts, counts, off, cursor, list, sorted, s);
+}
+
 template <typename T>
-hipError_t launch_batch_spread(const LatDesc* ldesc, int npts, int ntiles, const WorkItem* tile_work,
-                               const SpreadPoint* pts, uint32_t* counts, uint32_t* off, uint32_t* cursor,
-                               uint32_t* list, uint32_t* sorted, hipStream_t s) {
-  const unsigned nsb = unsigned((ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK);
-  uint32_t* sums = off + ntiles + 1;
-  hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, ntiles, counts, sums);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, ntiles, counts, sums, off, cursor);
-  if (npts > 0)
-    hipLaunchKernelGGL(kb_fill_lists, dim3(blocks_for(size_t(npts))), dim3(PBLOCK), 0, s, ldesc, npts, pts, off,
-                       cursor, list);
-  hipLaunchKernelGGL(k_sort_lists, dim3(ntiles), dim3(PBLOCK), 0, s, off, list, sorted);
-  hipLaunchKernelGGL(kb_spread_tiles<T>, dim3(ntiles), dim3(PBLOCK), 0, s, ldesc, tile_work, pts, off, sorted);
+hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nblocks, const WorkItem* block_work,
+                                     const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
+                                     hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_spread_blocks<T>, dim3(unsigned(nblocks)), dim3(SITE_BLOCK), 0, s, ldesc, block_work, pts,
+                     off, sorted);
   return hipGetLastError();
 }
-template hipError_t launch_batch_spread<double>(const LatDesc*, int, int, const WorkItem*, const SpreadPoint*,
-                                                uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, hipStream_t);
-template hipError_t launch_batch_spread<float>(const LatDesc*, int, int, const WorkItem*, const SpreadPoint*,
-                                               uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, hipStream_t);
+template hipError_t launch_batch_spread_force<double>(const LatDesc*, int, const WorkItem*, const SpreadPoint*,
+                                                      const uint32_t*, const uint32_t*, hipStream_t);
+template hipError_t launch_batch_spread_force<float>(const LatDesc*, int, const WorkItem*, const SpreadPoint*,
+                                                     const uint32_t*, const uint32_t*, hipStream_t);
 
 template <typename T>
 hipError_t launch_batch_swimmers_move(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,

@@ -1,0 +1,115 @@
+// Force at a lattice site from the sorted point list of its 64-site run.
+//
+// The coupled step's ZeroForce + AddForces (SwimmerArray.cu:42-112) is
+// evaluated as a gather: every point (the tail or head of a swimmer) is listed
+// under each run of 64 consecutive sites (one wavefront of the lattice
+// kernels) its 4x4x4 stencil reaches, the lists are sorted by point id =
+// 2*(array base + user index) + end, and a site adds the contributions of its
+// list's points in list order -- the swimmer order of a sequential
+// restatement -- starting from zero.  A point may be listed twice in one list;
+// the second entry is skipped.  Shared by the spreading kernel (which writes
+// the force field) and the collide/CalcHydro kernels (which use the force
+// without storing it).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "d3q15.hpp"
+#include "internal.hpp"
+
+namespace lbs {
+
+constexpr int SITE_BLOCK = 256;  // sites per workgroup of the lattice kernels
+constexpr int WAVE_SITES = 64;   // sites per list: one wavefront's sites
+constexpr int LISTS_PER_BLOCK = SITE_BLOCK / WAVE_SITES;
+
+struct SiteXYZ {
+  int x, y, z;
+};
+
+// Coordinates of linear site s = (x*ny + y)*nz + z.
+__device__ __forceinline__ SiteXYZ site_xyz(uint32_t s, const Geom& g) {
+  const uint32_t t = fastdiv(s, g.div_nz);
+  const uint32_t x = fastdiv(t, g.div_ny);
+  return SiteXYZ{int(x), int(t - x * uint32_t(g.ny)), int(s - t * uint32_t(g.nz))};
+}
+
+// Calls f(list) for the list (site / 64) of each of the 4 stencil sites of
+// row `row` = 4 i + j of a point, skipping a list equal to the previous one.
+template <typename F>
+__device__ __forceinline__ void for_each_row_list(const SpreadPoint& p, int row, const Geom& g, F&& f) {
+  const uint32_t base =
+      (uint32_t(p.wc[row >> 2]) * uint32_t(g.ny) + uint32_t(p.wc[4 + (row & 3)])) * uint32_t(g.nz);
+  uint32_t last = 0xFFFFFFFFu;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t b = (base + uint32_t(p.wc[8 + k])) / WAVE_SITES;
+    if (b != last) f(b);
+    last = b;
+  }
+}
+
+__device__ __forceinline__ double pick4(double a0, double a1, double a2, double a3, int i) {
+  return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
+}
+
+// Adds one point's contributions to site (X, Y, Z) in the (i, j, k) loop order
+// of AccumulateDeltaForce.  `simple`: every lattice side is >= 4, so each axis
+// has at most one matching stencil position, found by a wrapped difference.
+__device__ __forceinline__ void add_point(const SpreadPoint* __restrict__ p, int X, int Y, int Z, const Geom& g,
+                                          bool simple, double (&acc)[3]) {
+  if (simple) {
+    int di = X - p->wc[0];
+    di += di < 0 ? g.nx : 0;
+    int dj = Y - p->wc[4];
+    dj += dj < 0 ? g.ny : 0;
+    int dk = Z - p->wc[8];
+    dk += dk < 0 ? g.nz : 0;
+    if ((di | dj | dk) >= 4) return;
+    // p is wavefront-uniform: the weights come through the scalar cache and
+    // each lane selects its own, so no vector memory access is made here
+    const double* d = p->del;
+    const double delta3d =
+        pick4(d[0], d[1], d[2], d[3], di) * pick4(d[4], d[5], d[6], d[7], dj) * pick4(d[8], d[9], d[10], d[11], dk);
+    acc[0] += delta3d * p->F[0];
+    acc[1] += delta3d * p->F[1];
+    acc[2] += delta3d * p->F[2];
+    return;
+  }
+  for (int i = 0; i < 4; ++i) {
+    if (p->wc[i] != X) continue;
+    for (int j = 0; j < 4; ++j) {
+      if (p->wc[4 + j] != Y) continue;
+      const double dxy = p->del[i] * p->del[4 + j];
+      for (int k = 0; k < 4; ++k) {
+        if (p->wc[8 + k] != Z) continue;
+        const double delta3d = dxy * p->del[8 + k];
+        acc[0] += delta3d * p->F[0];
+        acc[1] += delta3d * p->F[1];
+        acc[2] += delta3d * p->F[2];
+      }
+    }
+  }
+}
+
+// Force at this lane's site (X, Y, Z) from list `wl`, the list of the
+// wavefront's 64 sites (ok = false for lanes past the lattice end).  The list
+// index is wavefront-uniform, so ids and point records are read through the
+// scalar cache; no LDS and no workgroup barrier.
+__device__ __forceinline__ void wave_force(const SpreadPoint* __restrict__ pts, const uint32_t* __restrict__ off,
+                                           const uint32_t* __restrict__ sorted, uint32_t wl, bool ok, int X, int Y,
+                                           int Z, const Geom& g, double (&acc)[3]) {
+  acc[0] = acc[1] = acc[2] = 0.0;
+  wl = __builtin_amdgcn_readfirstlane(wl);
+  const uint32_t o = off[wl], end = off[wl + 1];
+  const bool simple = g.nx >= 4 && g.ny >= 4 && g.nz >= 4;
+  uint32_t prev = 0xFFFFFFFFu;
+  for (uint32_t e = o; e < end; ++e) {
+    const uint32_t id = sorted[e];
+    if (id == prev) continue;
+    prev = id;
+    if (ok) add_point(pts + id, X, Y, Z, g, simple, acc);
+  }
+}
+
+}  // namespace lbs

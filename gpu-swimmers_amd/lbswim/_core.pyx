@@ -20,6 +20,7 @@ cnp.import_array()
 cdef extern from "lbswim.h":
     int LBSWIM_OK, LBSWIM_EINVAL, LBSWIM_ENOMEM, LBSWIM_EHIP, LBSWIM_ENODEV
     int LBSWIM_STORE_F64, LBSWIM_STORE_F32DEV
+    int LBSWIM_FORCE_STORED, LBSWIM_FORCE_AUTO, LBSWIM_FORCE_VIRTUAL
     int LBSWIM_RHO, LBSWIM_U, LBSWIM_FORCE, LBSWIM_FOLD, LBSWIM_FNEW
     int LBSWIM_PART_R, LBSWIM_PART_V, LBSWIM_PART_N, LBSWIM_PART_S, LBSWIM_PART_PRNG
     int LBSWIM_K_COUNT
@@ -67,6 +68,9 @@ cdef extern from "lbswim.h":
     int lbswim_lattice_calc_hydro(lbswim_lattice* lat)
     int lbswim_lattice_init_from_hydro(lbswim_lattice* lat)
     int lbswim_lattice_zero_force(lbswim_lattice* lat)
+    int lbswim_lattice_set_force_mode(lbswim_lattice* lat, int mode)
+    int lbswim_lattice_sync_force(lbswim_lattice* lat)
+    int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual)
     int lbswim_lattice_params(lbswim_lattice* lat, const lbswim_params** host)
     int lbswim_lattice_addressing(lbswim_lattice* lat, const lbswim_addressing** host)
     int lbswim_lattice_get_time_step(lbswim_lattice* lat, int* out)
@@ -105,6 +109,7 @@ cdef extern from "lbswim.h":
     int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps)
     int lbswim_ensemble_set_graph(lbswim_ensemble* e, int on)
     int lbswim_ensemble_set_order(lbswim_ensemble* e, int order)
+    int lbswim_ensemble_set_force_mode(lbswim_ensemble* e, int mode)
     int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice)
     int lbswim_lattice_totals(lbswim_lattice* lat, double* out)
     int lbswim_lattice_set_timing(lbswim_lattice* lat, int on)
@@ -145,6 +150,16 @@ cdef object wrap_view(object owner, void* data, int ndim, cnp.npy_intp* shape, i
 def synchronize():
     """Wait until all enqueued device work has finished."""
     check(lbswim_synchronize())
+
+
+FORCE_MODES = ("stored", "auto", "virtual")
+
+
+cdef int force_mode_id(mode) except -1:
+    ids = {"stored": LBSWIM_FORCE_STORED, "auto": LBSWIM_FORCE_AUTO, "virtual": LBSWIM_FORCE_VIRTUAL}
+    if mode not in ids:
+        raise ValueError("force mode must be one of %s" % (FORCE_MODES,))
+    return ids[mode]
 
 
 # ---------------------------------------------------------------------------
@@ -376,6 +391,23 @@ cdef class Lattice:
 
     def ZeroForce(self):
         check(lbswim_lattice_zero_force(self.handle()))
+
+    def set_force_mode(self, mode):
+        """"auto" (default): the coupled step keeps the swimmer force virtual
+        where swimmers are sparse (the force array is written on demand);
+        "stored": always write it; "virtual": never write it in the step."""
+        check(lbswim_lattice_set_force_mode(self.handle(), force_mode_id(mode)))
+
+    def sync_force(self):
+        """Write the force array if the last coupled step left it virtual."""
+        check(lbswim_lattice_sync_force(self.handle()))
+
+    @property
+    def force_is_virtual(self):
+        """True while the last coupled step's force is not yet written to the force array."""
+        cdef int v = 0
+        check(lbswim_lattice_force_is_virtual(self.handle(), &v))
+        return bool(v)
 
     @property
     def time_step(self):
@@ -937,6 +969,10 @@ cdef class Ensemble:
 
     def set_order(self, int order):
         check(lbswim_ensemble_set_order(self.impl, order))
+
+    def set_force_mode(self, mode):
+        """"auto" (default), "stored" or "virtual" swimmer force in the batched steps."""
+        check(lbswim_ensemble_set_force_mode(self.impl, force_mode_id(mode)))
 
     def autotune(self, int steps_per_trial=10):
         cdef int choice = 0

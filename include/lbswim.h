@@ -49,6 +49,11 @@ extern "C" {
 #define LBSWIM_STORE_F64 0    /* FP64 everywhere: the reference's arithmetic */
 #define LBSWIM_STORE_F32DEV 1 /* FP32 f as deviation from the rest state, FP32 rho/u/force */
 
+/* how the coupled step hands the swimmer force to the lattice kernels */
+#define LBSWIM_FORCE_STORED 0 /* written to the force array, read back by Step and CalcHydro */
+#define LBSWIM_FORCE_AUTO 1   /* virtual where the swimmers are sparse (the default) */
+#define LBSWIM_FORCE_VIRTUAL 2 /* always virtual */
+
 /* lattice fields (LatticeData members, lbswim/liblbswim/Lattice.h:45-49) */
 #define LBSWIM_RHO 0
 #define LBSWIM_U 1
@@ -201,9 +206,25 @@ int lbswim_tracers_d2h(lbswim_tracers* tr, int field);
 
 /* `nsteps` repetitions of System.Step (lbswim/System.py:9-16): ZeroForce;
  * AddForces of every swimmer array; Step; CalcHydro; Move of every swimmer
- * array, then of every tracer array. */
+ * array, then of every tracer array.
+ * Virtual force: with LBSWIM_FORCE_AUTO and at most ~4 swimmer ends per 256
+ * sites, ZeroForce + AddForces only build sorted per-block point lists and
+ * Step and CalcHydro evaluate each site's force from them, bit-identical to
+ * the stored force, without writing or reading the force array.  The force
+ * array is then written on demand: by lbswim_lattice_d2h(LBSWIM_FORCE),
+ * lbswim_lattice_field(LBSWIM_FORCE) with a device pointer requested,
+ * lbswim_lattice_sync_force, and before any standalone Step, CalcHydro or
+ * AddForces.  A caller that reads the force array on the device by a pointer
+ * obtained earlier calls lbswim_lattice_sync_force first. */
 int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
                        int ntr, int nsteps);
+/* LBSWIM_FORCE_STORED, _AUTO (default) or _VIRTUAL. No reference counterpart. */
+int lbswim_lattice_set_force_mode(lbswim_lattice* lat, int mode);
+/* Writes the force array if the last coupled step left it virtual. No reference counterpart. */
+int lbswim_lattice_sync_force(lbswim_lattice* lat);
+/* *is_virtual = 1 if the last coupled step left the force virtual (not yet
+ * written to the force array). No reference counterpart. */
+int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual);
 
 /* ---- ensembles (no reference counterpart) ------------------------------
  * Many independent systems -- each a lattice with its swimmer and tracer
@@ -222,13 +243,17 @@ int lbswim_ensemble_add(lbswim_ensemble* e, lbswim_lattice* lat, lbswim_swimmers
                         lbswim_tracers* const* tr, int ntr, int* member);
 int lbswim_ensemble_remove(lbswim_ensemble* e, int member);
 int lbswim_ensemble_size(lbswim_ensemble* e, int* members);
-/* nsteps coupled steps of every member. */
+/* nsteps coupled steps of every member.  The batched steps use the virtual
+ * force where the swimmers are sparse; every member's force array holds the
+ * last step's force when the call returns. */
 int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps);
 /* 1 (default): replay captured graphs of two steps; 0: launch eagerly. */
 int lbswim_ensemble_set_graph(lbswim_ensemble* e, int on);
 /* Workgroup order of the batched launches: 0 member-major, 1 round-robin over
  * members, 2 largest member first. Results do not depend on it. */
 int lbswim_ensemble_set_order(lbswim_ensemble* e, int order);
+/* LBSWIM_FORCE_STORED, _AUTO (default) or _VIRTUAL for the batched steps. */
+int lbswim_ensemble_set_force_mode(lbswim_ensemble* e, int mode);
 /* Times each order over steps_per_trial (+2 warm-up) real steps of the
  * ensemble -- the members advance by those steps -- and keeps the fastest. */
 int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice);

@@ -90,12 +90,14 @@ def _collect(f, rho, u, force, sw, tr, ns, nt):
                 tr_v=np.array(tr.v) if nt else e3)
 
 
-def build_lbswim(case, storage="f64"):
+def build_lbswim(case, storage="f64", force_mode=None):
     """Lattice + arrays in the lbswim package, initialised from `case`."""
     from lbswim import lb, swimmers, tracers
     from lbswim.System import System
 
     lat = lb.Lattice(*case.size, case.tau_s, case.tau_b, storage=storage)
+    if force_mode is not None:
+        lat.set_force_mode(force_mode)
     lat.rho.data.ravel()[:] = case.rho
     lat.rho.H2D()
     lat.u.data.ravel()[:] = case.u
@@ -137,7 +139,7 @@ def collect_lbswim(lat, sw, tr):
                     sw, tr, ns, nt)
 
 
-def run_lbswim(case, storage="f64", steps=None):
-    lat, system, sw, tr = build_lbswim(case, storage)
+def run_lbswim(case, storage="f64", steps=None, force_mode=None):
+    lat, system, sw, tr = build_lbswim(case, storage, force_mode)
     system.Step(case.steps if steps is None else steps)
     return collect_lbswim(lat, sw, tr), (sw.tumbles() if sw is not None else 0)

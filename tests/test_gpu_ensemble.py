@@ -47,13 +47,17 @@ def assert_same(a, b, label):
         assert np.array_equal(a[k], b[k]), (label, k)
 
 
-@pytest.mark.parametrize("storage,graph,steps", [("f64", True, 9), ("f64", False, 4), ("f32", True, 8)])
-def test_members_match_alone(storage, graph, steps):
+@pytest.mark.parametrize("storage,graph,steps,force_mode", [("f64", True, 9, "auto"), ("f64", False, 4, "auto"),
+                                                            ("f32", True, 8, "auto"), ("f64", True, 7, "virtual"),
+                                                            ("f64", False, 3, "stored"), ("f32", True, 5, "virtual")])
+def test_members_match_alone(storage, graph, steps, force_mode):
     from lbswim.ensemble import Ensemble
     cases = uneven_cases(8, seed=1)
-    say("ensemble vs alone: %s graph=%s steps=%d sizes=%s" % (storage, graph, steps, [c.size for c in cases]))
+    say("ensemble vs alone: %s graph=%s steps=%d force %s sizes=%s" % (storage, graph, steps, force_mode,
+                                                                        [c.size for c in cases]))
     ens = Ensemble(storage)
     ens.set_graph(graph)
+    ens.set_force_mode(force_mode)
     built = []
     for c in cases:
         lat, system, sw, tr, things = build_member(c, storage)

@@ -2,10 +2,13 @@
 the CPU oracle on the same seeded inputs.
 
 Bar: the FP64 storage form keeps the reference's arithmetic, so lattice
-fields must agree bit for bit on single operations and within 1e-12
-norm-wise relative (||a-b||_inf/||b||_inf) after long coupled runs, where
-only the order of concurrent force atomics may differ.  The FP32 storage form
-must stay within 1e-5 relative of the FP64 run in u.
+fields must agree bit for bit on single operations and coupled steps, and
+within 1e-12 norm-wise relative (||a-b||_inf/||b||_inf) after long coupled
+runs (bit-identical in practice: the fused step spreads forces without
+atomics).  The FP32 storage form must stay within 1e-5 relative of the FP64
+run in u.  The swimmer force of the fused step is checked both stored (the
+force array written and read back) and virtual (evaluated by the collide and
+CalcHydro kernels from per-block point lists, written only on demand).
 """
 import os
 import pickle
@@ -116,41 +119,45 @@ def test_lattice_ops_bitwise(size):
     assert np.array_equal(gl.fNew.data.ravel(), ref.fNew)
 
 
-def test_system_step_single_ops_bitwise():
+@pytest.mark.parametrize("force_mode", ["stored", "virtual"])
+def test_system_step_single_ops_bitwise(force_mode):
     """One coupled step with swimmers whose stencils cross the periodic faces."""
     case = make_case((12, 9, 10), 1, 0.3, 0.6, ns=40, nt=50, alpha=0.0, seed=5, drho=1e-2, du=1e-2, P=1e-2, l=0.7)
     case.sw_r[:, :4] = np.array([[0.1, 11.9, 0.0, 5.0], [8.95, 0.2, 0.0, 4.0], [9.99, 0.01, 0.0, 3.0]])
-    say("system step bitwise: %s ns=40 nt=50" % (case.size,))
+    say("system step bitwise: %s ns=40 nt=50 force %s" % (case.size, force_mode))
     ref, _ = run_oracle(O, case)
-    got, _ = run_lbswim(case)
+    got, _ = run_lbswim(case, force_mode=force_mode)
     # the fused step spreads forces as a per-site gather in swimmer order, so
     # even sites reached by several swimmers match the sequential oracle exactly
     for k in ref:
         assert np.array_equal(got[k], ref[k]), k
 
 
+@pytest.mark.parametrize("force_mode", ["stored", "virtual"])
 @pytest.mark.parametrize("size,ns,l", [((3, 2, 5), 7, 0.4), ((1, 6, 2), 3, 1.5), ((9, 33, 40), 300, 0.1),
-                                       ((40, 20, 70), 2000, 2.5)])
-def test_spreading_exact_on_awkward_shapes(size, ns, l):
+                                       ((40, 20, 70), 2000, 2.5), ((5, 3, 300), 40, 0.8), ((6, 4, 257), 60, 1.2)])
+def test_spreading_exact_on_awkward_shapes(size, ns, l, force_mode):
     """Stencils wrapping several times onto lattices thinner than 4, partial
-    tiles, dense overlapping swimmers, long swimmers spanning tiles."""
+    blocks, z-runs crossing block boundaries and the periodic face, dense
+    overlapping swimmers, long swimmers spanning blocks."""
     case = make_case(size, 3, 0.3, ns=ns, nt=0, alpha=0.0, seed=ns, drho=1e-2, du=1e-2, P=1e-2, l=l)
-    say("spreading exact: %s ns=%d l=%g" % (size, ns, l))
+    say("spreading exact: %s ns=%d l=%g force %s" % (size, ns, l, force_mode))
     ref, _ = run_oracle(O, case)
-    got, _ = run_lbswim(case)
+    got, _ = run_lbswim(case, force_mode=force_mode)
     for k in ("force", "fOld", "sw_r", "sw_n"):
         assert np.array_equal(got[k], ref[k]), k
 
 
-def test_two_swimmer_arrays_in_order():
+@pytest.mark.parametrize("force_mode", ["stored", "virtual"])
+def test_two_swimmer_arrays_in_order(force_mode):
     """Two swimmer arrays: contributions are added array by array, as System.Step does."""
     from lbswim import swimmers
     from lbswim.System import System
     size = (10, 9, 11)
     c1 = make_case(size, 0, 0.3, ns=20, nt=0, seed=1, drho=1e-2, du=1e-2, P=1e-2, l=0.5)
     c2 = make_case(size, 0, 0.3, ns=15, nt=0, seed=2, drho=1e-2, du=1e-2, P=2e-2, l=0.3)
-    say("two swimmer arrays")
-    lat, system, sw1, _ = build_lbswim(c1)
+    say("two swimmer arrays, force %s" % force_mode)
+    lat, system, sw1, _ = build_lbswim(c1, force_mode=force_mode)
     cp = swimmers.CommonParams()
     for k, v in c2.common.items():
         setattr(cp, k, v)
@@ -175,6 +182,39 @@ def test_two_swimmer_arrays_in_order():
     assert np.array_equal(lat.force.data.ravel(), ol.force)
     assert np.array_equal(lat.fOld.data.ravel(), ol.fOld)
     assert np.array_equal(sw2.r.data, s2.r)
+
+
+def test_virtual_force_written_on_demand():
+    """After fused steps with a virtual force, every path that reads or
+    replaces the force array sees what the stored form holds: D2H, a
+    standalone Step/CalcHydro, a standalone AddForces on top, and an H2D."""
+    case = make_case((14, 12, 20), 0, 0.3, ns=30, nt=20, seed=8, drho=1e-2, du=1e-2, P=1e-2, l=0.6)
+    say("virtual force on demand")
+    runs = {}
+    for mode in ("stored", "virtual"):
+        lat, system, sw, tr = build_lbswim(case, force_mode=mode)
+        system.Step(3)
+        out = {"after_fused": collect_lbswim(lat, sw, tr)}
+        system.Step(2)
+        lat.Step()                    # standalone Step reads the force of the last fused step
+        lat.CalcHydro()
+        out["standalone"] = collect_lbswim(lat, sw, tr)
+        system.Step(1)
+        sw.AddForces(lat)             # adds on top of the fused step's force
+        lat.force.D2H()
+        out["addforces"] = np.array(lat.force.data).ravel()
+        system.Step(1)
+        lat.force.data.ravel()[:] = 1e-4
+        lat.force.H2D()               # replaces the virtual force
+        lat.Step()
+        out["h2d"] = collect_lbswim(lat, sw, tr)
+        runs[mode] = out
+    for stage in ("after_fused", "standalone", "h2d"):
+        for k, v in runs["stored"][stage].items():
+            assert np.array_equal(runs["virtual"][stage][k], v), (stage, k)
+    # the standalone AddForces adds with hardware atomics: equal to rounding
+    assert rel_inf(runs["virtual"]["addforces"], runs["stored"]["addforces"]) < 1e-14
+    assert np.any(runs["stored"]["after_fused"]["force"] != 0)
 
 
 def test_standalone_add_forces_atomic_path():

@@ -135,7 +135,8 @@ struct TracerDesc {
   int member;
 };
 uint32_t lattice_blocks(const Geom& g);  // workgroups of a one-lattice launch
-constexpr int PARTICLE_GROUPS = 16;      // particles per workgroup of the move kernels
+constexpr int PARTICLE_GROUPS = 16;      // swimmers per workgroup of the swimmer move kernel
+constexpr int TRACER_GROUPS = 64;        // tracers per workgroup of the tracer move kernel
 // vf: the force of member m's block b is evaluated from lists tile_base_m + 4 b ...
 template <typename T>
 hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,

@@ -39,10 +39,22 @@ __device__ __forceinline__ double peskin(double x) {
 
 // ((int)x + n) % n of interp.cu:26 for x >= -n, extended to a true periodic
 // wrap (and NaN mapped to 0) so that no input can index outside the lattice.
+// Coordinates already inside [0, n) skip the division.
 __device__ __forceinline__ uint32_t wrap_index(double x, int n) {
   if (!(x > -2147483648.0 && x < 2147483647.0)) x = 0.0;
-  int v = int(x) % n;
+  const int v0 = int(x);
+  if (uint32_t(v0) < uint32_t(n)) return uint32_t(v0);
+  const int v = v0 % n;
   return uint32_t(v < 0 ? v + n : v);
+}
+
+// fmod(x, y) of SwimmerArray.cu:161 / TracerArray.cu:33 for y > 0: when
+// 0 <= x < 2y the result is x or x - y, both exact (Sterbenz), so the
+// division is needed only outside that range.
+__device__ __forceinline__ double wrap_position(double x, double y) {
+  if (x >= 0.0 && x < y) return x;
+  if (x >= y && x < 2.0 * y) return x - y;
+  return fmod(x, y);
 }
 
 // InterpVelocity, interp.cu:8-47: v_d = sum_{i,j,k} dx_i dy_j dz_k u_d(site),
@@ -97,20 +109,26 @@ constexpr int TERMS = 64 * 3;  // (t, d) pairs of one interpolation, stored t-ma
 template <typename T>
 __device__ __forceinline__ void interp_terms(const T* __restrict__ u, const Geom& g, const double (&r)[3], int c,
                                              double* __restrict__ terms) {
+  // Lane q < 12 of the group forms stencil coordinate q (axis q / 4, offset
+  // q % 4) once; the lanes then read the ones they need from each other.
+  const int q = c < 12 ? c : 0;
+  const int axis = q >> 2;
+  const double ra = axis == 0 ? r[0] : (axis == 1 ? r[1] : r[2]);
+  const int na = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+  const double x = ceil(ra - 2.) + (q & 3);
+  const int mine_idx = int(wrap_index(x, na));
+  const double mine_del = peskin(ra - x);
   const int j = c >> 2, k = c & 3;
-  const double x0 = ceil(r[0] - 2.);
   uint32_t ix[4];
   double dx[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const double x = x0 + i;
-    ix[i] = wrap_index(x, g.nx);
-    dx[i] = peskin(r[0] - x);
+    ix[i] = uint32_t(__shfl(mine_idx, i, GROUP));
+    dx[i] = __shfl(mine_del, i, GROUP);
   }
-  const double y = ceil(r[1] - 2.) + j;
-  const double z = ceil(r[2] - 2.) + k;
-  const uint32_t iyz = wrap_index(y, g.ny) * uint32_t(g.nz) + wrap_index(z, g.nz);
-  const double dy = peskin(r[1] - y), dz = peskin(r[2] - z);
+  const uint32_t iy = uint32_t(__shfl(mine_idx, 4 + j, GROUP)), iz = uint32_t(__shfl(mine_idx, 8 + k, GROUP));
+  const double dy = __shfl(mine_del, 4 + j, GROUP), dz = __shfl(mine_del, 8 + k, GROUP);
+  const uint32_t iyz = iy * uint32_t(g.nz) + iz;
   const size_t n = g.n;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -306,7 +324,7 @@ __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp,
   for (int d = 0; d < 3; ++d) {
     v[size_t(ns) * d + is] = rDot[d];
     const double Rn = R[d] + rDot[d];
-    r[size_t(ns) * d + is] = fmod(Rn + size[d], double(size[d]));
+    r[size_t(ns) * d + is] = wrap_position(Rn + size[d], double(size[d]));
   }
   const uint64_t ctr = prng[is];
   const float U = u01f(draw(cp.seed, user, ctr, 0u).x[0]);
@@ -337,34 +355,93 @@ __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp,
   prng[is] = ctr + 1;
 }
 
-// DoTracerArrayMove, TracerArray.cu:12-43 (s updated per particle).  A lane
-// group per tracer gathers the stencil; lane d of the group updates component d.
-constexpr int TRACER_STRIDE = TERMS + 1;
+// DoTracerArrayMove, TracerArray.cu:12-43 (s updated per particle), with a
+// lane group of 16 per tracer and the stencil streamed plane by plane: for each x
+// offset i the 16 lanes of the group write their 16 (j, k) products to a
+// 48-double LDS slice of the group and lanes 0..2 add them to their
+// component in t order, then the next plane reuses the slice.  A group's
+// lanes are in one wavefront and LDS operations of a wavefront complete in
+// order, so no workgroup barrier is needed and each workgroup holds only
+// GROUPS * 49 doubles of LDS.
+constexpr int PLANE_STRIDE = 49;  // 48 products + 1: groups start on different banks
+constexpr int TGROUPS = TRACER_GROUPS;  // tracers per workgroup: 1024 threads, contiguous in storage order
+constexpr int TBLOCK = TGROUPS * GROUP;
 
 template <typename T>
 __device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r, double* __restrict__ sd,
                                                   double* __restrict__ v, const T* __restrict__ u, const Geom& g,
                                                   uint32_t blk, double* __restrict__ lds) {
-  constexpr int STRIDE = TRACER_STRIDE;
   const int grp = int(threadIdx.x) / GROUP, c = int(threadIdx.x) % GROUP;
-  const int it = int(blk) * GROUPS + grp;
-  const bool active = it < nt;
-  double* terms = lds + grp * STRIDE;
-  double R[3];
-  if (active) {
+  const int it = int(blk) * TGROUPS + grp;
+  if (it >= nt) return;  // whole group: the lanes of a group share `it`
+  double* terms = lds + grp * PLANE_STRIDE;
+  const double R[3] = {r[it], r[size_t(nt) + it], r[2 * size_t(nt) + it]};
+  const int d = c < 3 ? c : 0;
+  const double s_old = c < 3 ? sd[size_t(nt) * d + it] : 0.0;
+  // stencil coordinates, one per lane, shared in the group
+  const int q = c < 12 ? c : 0;
+  const int axis = q >> 2;
+  const double ra = axis == 0 ? R[0] : (axis == 1 ? R[1] : R[2]);
+  const int na = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+  const double x = ceil(ra - 2.) + (q & 3);
+  const int mine_idx = int(wrap_index(x, na));
+  const double mine_del = peskin(ra - x);
+  const int j = c >> 2, k = c & 3;
+  uint32_t ix[4];
+  double dx[4];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) R[d] = r[size_t(nt) * d + it];
-    interp_terms(u, g, R, c, terms);
+  for (int i = 0; i < 4; ++i) {
+    ix[i] = uint32_t(__shfl(mine_idx, i, GROUP));
+    dx[i] = __shfl(mine_del, i, GROUP);
   }
-  __syncthreads();
-  if (!active || c >= 3) return;
-  const int d = c;
+  const uint32_t iyz = uint32_t(__shfl(mine_idx, 4 + j, GROUP)) * uint32_t(g.nz) + uint32_t(__shfl(mine_idx, 8 + k, GROUP));
+  const double dy = __shfl(mine_del, 4 + j, GROUP), dz = __shfl(mine_del, 8 + k, GROUP);
+  const size_t n = g.n;
+  double uv[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t ijk = ix[i] * g.nyz + iyz;
+    uv[i][0] = double(u[ijk]);
+    uv[i][1] = double(u[n + ijk]);
+    uv[i][2] = double(u[2 * n + ijk]);
+  }
+  double vel = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double delta3d = dx[i] * dy * dz;
+    terms[3 * c + 0] = delta3d * uv[i][0];
+    terms[3 * c + 1] = delta3d * uv[i][1];
+    terms[3 * c + 2] = delta3d * uv[i][2];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (c < 3) {
+#pragma unroll
+      for (int cc = 0; cc < 16; ++cc) vel += terms[3 * cc + d];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (c >= 3) return;
   const int size = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
-  const double vel = sum_terms(terms, d);
+  const double Rd = d == 0 ? R[0] : (d == 1 ? R[1] : R[2]);
   v[size_t(nt) * d + it] = vel;
-  const double Rn = R[d] + vel;
-  r[size_t(nt) * d + it] = fmod(Rn + size, double(size));
-  sd[size_t(nt) * d + it] += vel;
+  const double Rn = Rd + vel;
+  r[size_t(nt) * d + it] = wrap_position(Rn + size, double(size));
+  sd[size_t(nt) * d + it] = s_old + vel;
+}
+
+// Workgroups are handed to the 8 XCDs (each with its own L2) round-robin by
+// workgroup id.  Particles are stored in spatial order, so each XCD is given
+// one contiguous range of the `nb` particle blocks instead: launched with
+// round_xcd(nb) workgroups, workgroup b takes block (b % 8) * round_xcd(nb)/8
+// + b / 8 (a block >= nb means nothing to do), and neighbouring particles
+// share their stencils' lines in one L2.
+constexpr uint32_t XCDS = 8;
+__host__ __device__ inline uint32_t round_xcd(uint32_t nb) { return (nb + XCDS - 1) / XCDS * XCDS; }
+__device__ __forceinline__ uint32_t xcd_block(uint32_t nb8) {
+  return (blockIdx.x % XCDS) * (nb8 / XCDS) + blockIdx.x / XCDS;
 }
 
 template <typename T>
@@ -372,26 +449,34 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp,
                                                           double* __restrict__ v, double* __restrict__ nv,
                                                           uint64_t* __restrict__ prng, const int* __restrict__ perm,
                                                           const T* __restrict__ u, Geom g,
-                                                          unsigned long long* __restrict__ tumbles) {
+                                                          unsigned long long* __restrict__ tumbles, uint32_t nb) {
   __shared__ double lds[GROUPS * SWIM_STRIDE];
-  swimmers_move_body(ns, cp, r, v, nv, prng, perm, u, g, tumbles, blockIdx.x, lds);
+  const uint32_t blk = xcd_block(round_xcd(nb));
+  if (blk >= nb) return;
+  swimmers_move_body(ns, cp, r, v, nv, prng, perm, u, g, tumbles, blk, lds);
 }
 
 template <typename T>
-__global__ __launch_bounds__(PBLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
-                                                         double* __restrict__ v, const T* __restrict__ u, Geom g) {
-  __shared__ double lds[GROUPS * TRACER_STRIDE];
-  tracers_move_body(nt, r, sd, v, u, g, blockIdx.x, lds);
+__global__ __launch_bounds__(TBLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
+                                                         double* __restrict__ v, const T* __restrict__ u, Geom g,
+                                                         uint32_t nb) {
+  __shared__ double lds[TGROUPS * PLANE_STRIDE];
+  const uint32_t blk = xcd_block(round_xcd(nb));
+  if (blk >= nb) return;
+  tracers_move_body(nt, r, sd, v, u, g, blk, lds);
 }
 
 // Batched moves: WorkItem.member indexes the particle-array table, unit is
 // the group block of that array; u and the geometry come from its lattice.
+// The work list is split into XCD ranges the same way.
 template <typename T>
 __global__ __launch_bounds__(PBLOCK) void kb_swimmers_move(const LatDesc* __restrict__ ldesc,
                                                            const SwimDesc* __restrict__ sdesc,
-                                                           const WorkItem* __restrict__ work) {
+                                                           const WorkItem* __restrict__ work, uint32_t nwork) {
   __shared__ double lds[GROUPS * SWIM_STRIDE];
-  const WorkItem w = work[blockIdx.x];
+  const uint32_t wi = xcd_block(round_xcd(nwork));
+  if (wi >= nwork) return;
+  const WorkItem w = work[wi];
   const SwimDesc& a = sdesc[w.member];
   const LatDesc& m = ldesc[a.member];
   swimmers_move_body(a.num, a.p, a.r, a.v, a.n, a.prng, a.perm, static_cast<const T*>(m.u), m.geom, a.tumbles,
@@ -399,11 +484,13 @@ __global__ __launch_bounds__(PBLOCK) void kb_swimmers_move(const LatDesc* __rest
 }
 
 template <typename T>
-__global__ __launch_bounds__(PBLOCK) void kb_tracers_move(const LatDesc* __restrict__ ldesc,
+__global__ __launch_bounds__(TBLOCK) void kb_tracers_move(const LatDesc* __restrict__ ldesc,
                                                           const TracerDesc* __restrict__ tdesc,
-                                                          const WorkItem* __restrict__ work) {
-  __shared__ double lds[GROUPS * TRACER_STRIDE];
-  const WorkItem w = work[blockIdx.x];
+                                                          const WorkItem* __restrict__ work, uint32_t nwork) {
+  __shared__ double lds[TGROUPS * PLANE_STRIDE];
+  const uint32_t wi = xcd_block(round_xcd(nwork));
+  if (wi >= nwork) return;
+  const WorkItem w = work[wi];
   const TracerDesc& a = tdesc[w.member];
   const LatDesc& m = ldesc[a.member];
   tracers_move_body(a.num, a.r, a.s, a.v, static_cast<const T*>(m.u), m.geom, w.unit, lds);
@@ -565,11 +652,42 @@ __global__ __launch_bounds__(PBLOCK) void k_fill_rows(uint32_t npts, Geom g, con
   });
 }
 
-// Stable rank sort of every list: equal ids (a point listed twice) keep
-// their list order, so every position of `sorted` is written once.  Each lane
-// sorts one list of up to SHORT_LIST entries alone; longer lists are then
-// ranked by the whole wavefront, one at a time.
-constexpr uint32_t SHORT_LIST = 32;
+// Sort of every list by point id.  Each lane sorts one list of up to 16
+// entries in registers with Batcher's odd-even merge networks (19
+// comparators for 8 inputs, 63 for 16; equal ids -- a point listed twice --
+// are interchangeable); longer lists are then ranked by the whole wavefront,
+// one at a time, equal ids keeping their order so that every position of
+// `sorted` is written once.
+constexpr uint32_t SHORT_LIST = 16;
+
+__device__ __forceinline__ void cmp_swap(uint32_t& a, uint32_t& b) {
+  const uint32_t lo = min(a, b), hi = max(a, b);
+  a = lo;
+  b = hi;
+}
+
+constexpr uint8_t NET8[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                 {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+constexpr uint8_t NET16[63][2] = {
+    {0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}, {4, 5}, {6, 7}, {4, 6}, {5, 7}, {5, 6}, {0, 4}, {2, 6},
+    {2, 4}, {1, 5}, {3, 7}, {3, 5}, {1, 2}, {3, 4}, {5, 6}, {8, 9}, {10, 11}, {8, 10}, {9, 11}, {9, 10},
+    {12, 13}, {14, 15}, {12, 14}, {13, 15}, {13, 14}, {8, 12}, {10, 14}, {10, 12}, {9, 13}, {11, 15}, {11, 13}, {9, 10},
+    {11, 12}, {13, 14}, {0, 8}, {4, 12}, {4, 8}, {2, 10}, {6, 14}, {6, 10}, {2, 4}, {6, 8}, {10, 12}, {1, 9},
+    {5, 13}, {5, 9}, {3, 11}, {7, 15}, {7, 11}, {3, 5}, {7, 9}, {11, 13}, {1, 2}, {3, 4}, {5, 6}, {7, 8},
+    {9, 10}, {11, 12}, {13, 14}};
+
+template <int N, int C>
+__device__ __forceinline__ void sort_net(const uint32_t* __restrict__ src, uint32_t cnt, uint32_t* __restrict__ dst,
+                                         const uint8_t (&net)[C][2]) {
+  uint32_t v[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = uint32_t(k) < cnt ? src[k] : 0xFFFFFFFFu;  // ids stay below 2^31
+#pragma unroll
+  for (int c = 0; c < C; ++c) cmp_swap(v[net[c][0]], v[net[c][1]]);
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+    if (uint32_t(k) < cnt) dst[k] = v[k];
+}
 
 __device__ __forceinline__ void rank_entries(const uint32_t* __restrict__ src, uint32_t cnt, uint32_t first,
                                              uint32_t step, uint32_t* __restrict__ dst) {
@@ -593,7 +711,10 @@ __global__ __launch_bounds__(PBLOCK) void k_sort_lists(uint32_t nlists, const ui
   if (wl < nlists) {
     o = off[wl];
     cnt = off[wl + 1] - o;
-    if (cnt <= SHORT_LIST) rank_entries(list + o, cnt, 0, 1, sorted + o);
+    if (cnt <= 8)
+      sort_net<8>(list + o, cnt, sorted + o, NET8);
+    else if (cnt <= SHORT_LIST)
+      sort_net<16>(list + o, cnt, sorted + o, NET16);
   }
   uint64_t longs = __ballot(cnt > SHORT_LIST);
   while (longs) {
@@ -679,9 +800,9 @@ hipError_t launch_swimmers_move(int ns, const SwimParams& p, double* r, double* 
                                 const int* perm, const T* u, const Geom& g, unsigned long long* tumbles,
                                 hipStream_t s) {
   if (ns <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_swimmers_move<T>, dim3(unsigned((ns + GROUPS - 1) / GROUPS)), dim3(PBLOCK), 0, s, ns, p, r, v,
-                     n, prng,
-                     perm, u, g, tumbles);
+  const uint32_t nb = uint32_t((ns + GROUPS - 1) / GROUPS);
+  hipLaunchKernelGGL(k_swimmers_move<T>, dim3(round_xcd(nb)), dim3(PBLOCK), 0, s, ns, p, r, v, n, prng, perm, u, g,
+                     tumbles, nb);
   return hipGetLastError();
 }
 
@@ -689,8 +810,8 @@ template <typename T>
 hipError_t launch_tracers_move(int nt, double* r, double* sd, double* v, const T* u, const Geom& g,
                                hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tracers_move<T>, dim3(unsigned((nt + GROUPS - 1) / GROUPS)), dim3(PBLOCK), 0, s, nt, r, sd,
-                     v, u, g);
+  const uint32_t nb = uint32_t((nt + TGROUPS - 1) / TGROUPS);
+  hipLaunchKernelGGL(k_tracers_move<T>, dim3(round_xcd(nb)), dim3(TBLOCK), 0, s, nt, r, sd, v, u, g, nb);
   return hipGetLastError();
 }
 
@@ -799,14 +920,16 @@ template <typename T>
 hipError_t launch_batch_swimmers_move(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
                                       hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_swimmers_move<T>, dim3(nwork), dim3(PBLOCK), 0, s, ldesc, sdesc, work);
+  hipLaunchKernelGGL(kb_swimmers_move<T>, dim3(round_xcd(uint32_t(nwork))), dim3(PBLOCK), 0, s, ldesc, sdesc, work,
+                     uint32_t(nwork));
   return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_batch_tracers_move(const LatDesc* ldesc, const TracerDesc* tdesc, const WorkItem* work, int nwork,
                                      hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_tracers_move<T>, dim3(nwork), dim3(PBLOCK), 0, s, ldesc, tdesc, work);
+  hipLaunchKernelGGL(kb_tracers_move<T>, dim3(round_xcd(uint32_t(nwork))), dim3(TBLOCK), 0, s, ldesc, tdesc, work,
+                     uint32_t(nwork));
   return hipGetLastError();
 }
 template hipError_t launch_batch_swimmers_move<double>(const LatDesc*, const SwimDesc*, const WorkItem*, int,

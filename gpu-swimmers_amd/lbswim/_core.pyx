@@ -113,6 +113,7 @@ cdef extern from "lbswim.h":
     int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice)
     int lbswim_lattice_totals(lbswim_lattice* lat, double* out)
     int lbswim_lattice_set_timing(lbswim_lattice* lat, int on)
+    int lbswim_lattice_set_timing_mask(lbswim_lattice* lat, unsigned mask)
     int lbswim_lattice_kernel_time(lbswim_lattice* lat, int kernel, double* total_ms, long* count)
     int lbswim_device_memory(size_t* free_bytes, size_t* total_bytes)
     int lbswim_diag_copy(size_t count, int reps, double* ms)
@@ -487,8 +488,16 @@ cdef class Lattice:
         check(lbswim_lattice_totals(self.handle(), out))
         return np.array([out[0], out[1], out[2], out[3]])
 
-    def set_timing(self, bint on):
-        check(lbswim_lattice_set_timing(self.handle(), on))
+    def set_timing(self, on):
+        """True/False: record every kernel or none; or an iterable of kernel
+        names (from KERNELS) to record only those."""
+        if on is True or on is False or on in (0, 1):
+            check(lbswim_lattice_set_timing(self.handle(), 1 if on else 0))
+            return
+        mask = 0
+        for name in on:
+            mask |= 1 << KERNELS.index(name)
+        check(lbswim_lattice_set_timing_mask(self.handle(), mask))
 
     def kernel_times(self):
         """{kernel: (total_ms, launches)} recorded since set_timing(True)."""

@@ -183,16 +183,27 @@ def bench_serving(nreq, rate, seed=1234):
     from lbswim.serving import Server, poisson_requests, run_sequential
     reqs = poisson_requests(nreq, rate, seed)
     srv = Server(hbm_budget=64 << 30, max_members=64, chunk=100, keep_results=False)
-    batched = srv.run(reqs, realtime=True)
-    seq = run_sequential(poisson_requests(nreq, rate, seed), realtime=True)
+    count = {"n": 0}
+
+    def progress(label):
+        def on_done(req):
+            count["n"] += 1
+            if count["n"] % 8 == 0 or count["n"] == nreq:
+                log("serving %s: %d of %d requests done" % (label, count["n"], nreq))
+        return on_done
+
+    batched = srv.run(reqs, realtime=True, on_done=progress("batched"))
+    count["n"] = 0
+    seq = run_sequential(poisson_requests(nreq, rate, seed), realtime=True, on_done=progress("sequential"))
     r = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
     return {"rate_per_s": rate, "batched": r(batched), "sequential": r(seq),
             "throughput_ratio": round(batched["mlups"] / seq["mlups"], 2),
             "p99_ratio": round(seq["p99_s"] / batched["p99_s"], 2)}
 
 
-def cpu_baseline(case, steps=3):
-    """The FP64 oracle (OpenMP over the host cores) on the same workload."""
+def cpu_baseline(case, min_seconds=10.0, max_steps=64):
+    """The FP64 oracle (OpenMP over the host cores) on the same workload: coupled
+    steps in chunks of 4 until about `min_seconds` of CPU time have passed."""
     import lbswim_oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     cores = min(cores, os.cpu_count() or cores)
@@ -207,8 +218,11 @@ def cpu_baseline(case, steps=3):
     tr = O.Tracers(case.tr_r.shape[1])
     tr.r[:] = case.tr_r
     O.run(lat, 1, sw, tr)  # warm-up
+    steps = 0
     t0 = time.perf_counter()
-    O.run(lat, steps, sw, tr)
+    while steps < max_steps and time.perf_counter() - t0 < min_seconds:
+        O.run(lat, 4, sw, tr)
+        steps += 4
     dt = time.perf_counter() - t0
     return {"value": lat.n * steps / dt / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
             "sample": "%d coupled steps (after 1 warm-up) of the same 256^3 workload, FP64 oracle, "

@@ -170,7 +170,7 @@ struct lbswim_lattice {
   // workspace of the deterministic spreading in lbswim_system_step
   uint32_t sp_lists = 0;          // one per 64 sites
   uint32_t* sp_counts = nullptr;  // sp_lists
-  uint32_t* sp_off = nullptr;     // sp_lists + 1 offsets + scan scratch
+  uint32_t* sp_off = nullptr;     // sp_lists + 1 offsets + scan and sort scratch
   uint32_t* sp_cursor = nullptr;  // sp_lists
   size_t sp_pts_cap = 0;
   void* sp_pts = nullptr;  // SpreadPoint records for sp_pts_cap points
@@ -1015,7 +1015,7 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     const size_t nt = size_t(lat->sp_lists);
     int rc;
     if ((rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_counts), 4 * nt)) ||
-        (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_off), 4 * (nt + 1 + (nt + 1023) / 1024))) ||
+        (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_off), 4 * (nt + 2 + (nt + 1023) / 1024))) ||
         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_cursor), 4 * nt)))
       return rc;
   }
@@ -1340,9 +1340,13 @@ struct lbswim_ensemble {
   // device tables
   LatDesc* d_lat = nullptr;
   size_t cap_lat = 0;
+  // lattice blocks of the members whose force is stored / virtual
   WorkItem* d_latwork = nullptr;
   size_t cap_latwork = 0;
   int n_latwork = 0;
+  WorkItem* d_latwork_vf = nullptr;
+  size_t cap_latwork_vf = 0;
+  int n_latwork_vf = 0;
   SwimDesc* d_sw = nullptr;
   size_t cap_sw = 0;
   WorkItem* d_swwork = nullptr;
@@ -1360,7 +1364,6 @@ struct lbswim_ensemble {
   // spreading workspace
   int npts = 0;
   int force_mode = LBSWIM_FORCE_AUTO;
-  bool vf = false;  // collide/CalcHydro evaluate the force from the lists
   uint32_t* counts = nullptr;
   size_t cap_counts = 0;
   uint32_t* off = nullptr;
@@ -1393,8 +1396,9 @@ int ens_build(lbswim_ensemble* e) {
   HIP_TRY(hipStreamSynchronize(stream()));
   ens_drop_graph(e);
   std::vector<LatDesc> lat;
-  std::vector<std::vector<WorkItem>> latwork;
-  std::vector<size_t> weight;
+  // per member: its lattice blocks, in the stored- or the virtual-force group
+  std::vector<std::vector<WorkItem>> latwork[2];
+  std::vector<size_t> weight[2];
   std::vector<SwimDesc> sw;
   std::vector<WorkItem> swwork, ptwork, trwork;
   std::vector<TracerDesc> tr;
@@ -1417,10 +1421,14 @@ int ens_build(lbswim_ensemble* e) {
     d.cur = L->cur;
     d.tile_base = tile_base;
     lat.push_back(d);
+    size_t member_pts = 0;
+    for (lbswim_swimmers* s : m.sw) member_pts += 2 * size_t(s->st.num);
+    // the same density rule as a single lattice, member by member
+    const int group = use_virtual_force(e->force_mode, member_pts, spread_lists(L->geom)) ? 1 : 0;
     std::vector<WorkItem> lw;
     for (uint32_t b = 0; b < lattice_blocks(L->geom); ++b) lw.push_back(WorkItem{mi, b});
-    latwork.push_back(lw);
-    weight.push_back(L->n);
+    latwork[group].push_back(lw);
+    weight[group].push_back(L->n);
     tile_base += spread_lists(L->geom);
     for (lbswim_swimmers* s : m.sw) {
       if (int rc = pstore_maybe_reorder(s->st, L->geom)) return rc;
@@ -1461,20 +1469,22 @@ int ens_build(lbswim_ensemble* e) {
     m.cur_at_build = m.lat->cur;
     m.storage_at_build = storage_signature(m);
   }
-  const std::vector<WorkItem> lw = order_work(latwork, e->order, weight);
+  const std::vector<WorkItem> lw = order_work(latwork[0], e->order, weight[0]);
+  const std::vector<WorkItem> lwv = order_work(latwork[1], e->order, weight[1]);
   int rc;
   if ((rc = upload(&e->d_lat, &e->cap_lat, lat)) || (rc = upload(&e->d_latwork, &e->cap_latwork, lw)) ||
+      (rc = upload(&e->d_latwork_vf, &e->cap_latwork_vf, lwv)) ||
       (rc = upload(&e->d_sw, &e->cap_sw, sw)) ||
       (rc = upload(&e->d_swwork, &e->cap_swwork, swwork)) || (rc = upload(&e->d_ptwork, &e->cap_ptwork, ptwork)) ||
       (rc = upload(&e->d_tr, &e->cap_tr, tr)) || (rc = upload(&e->d_trwork, &e->cap_trwork, trwork)))
     return rc;
   e->n_latwork = int(lw.size());
+  e->n_latwork_vf = int(lwv.size());
   e->n_swwork = int(swwork.size());
   e->n_ptwork = int(ptwork.size());
   e->n_trwork = int(trwork.size());
   e->nlists = int(tile_base);
   e->npts = int(point_base);
-  e->vf = use_virtual_force(e->force_mode, size_t(e->npts), size_t(e->nlists));
   // spreading workspace
   size_t lcap = 0;
   for (const auto& a : sw) lcap += spread_list_capacity(2 * size_t(a.num));
@@ -1490,7 +1500,7 @@ int ens_build(lbswim_ensemble* e) {
   const size_t nt = size_t(e->nlists);
   size_t cap_sorted = e->cap_list;
   if ((rc = grow_to(&e->counts, &e->cap_counts, nt, 4)) ||
-      (rc = grow_to(&e->off, &e->cap_off, nt + 1 + (nt + 1023) / 1024, 4)) ||
+      (rc = grow_to(&e->off, &e->cap_off, nt + 2 + (nt + 1023) / 1024, 4)) ||
       (rc = grow_to(&e->cursor, &e->cap_cursor, nt, 4)) || (rc = grow_to(&e->pts, &e->cap_pts, size_t(e->npts), 176)) ||
       (rc = grow_to(&e->sorted, &cap_sorted, lcap, 4)) || (rc = grow_to(&e->list, &e->cap_list, lcap, 4)))
     return rc;
@@ -1507,10 +1517,12 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   HIP_TRY(launch_batch_lists(e->d_lat, e->npts, e->nlists, e->pts, e->counts, e->off, e->cursor, e->list, e->sorted,
                              s));
   const VForce vf{e->pts, e->off, e->sorted};
-  if (!e->vf)
-    HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_latwork, e->d_latwork, e->pts, e->off, e->sorted, s));
-  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, e->vf ? &vf : nullptr, s));
-  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, e->vf ? &vf : nullptr, s));
+  // members with a stored force: write it, then read it; the others evaluate it from the lists
+  HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_latwork, e->d_latwork, e->pts, e->off, e->sorted, s));
+  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, nullptr, s));
+  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork_vf, e->n_latwork_vf, parity, &vf, s));
+  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, nullptr, s));
+  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork_vf, e->n_latwork_vf, parity, &vf, s));
   HIP_TRY(launch_batch_swimmers_move<T>(e->d_lat, e->d_sw, e->d_swwork, e->n_swwork, s));
   HIP_TRY(launch_batch_tracers_move<T>(e->d_lat, e->d_tr, e->d_trwork, e->n_trwork, s));
   return LBSWIM_OK;
@@ -1523,11 +1535,10 @@ int ens_enqueue(lbswim_ensemble* e, int parity) {
 // After batched steps with the virtual force: write every member's force
 // array from the last step's lists, as the reference leaves it.
 int ens_materialize_force(lbswim_ensemble* e) {
-  if (!e->vf) return LBSWIM_OK;
   hipError_t r = e->storage == LBSWIM_STORE_F64
-                     ? launch_batch_spread_force<double>(e->d_lat, e->n_latwork, e->d_latwork, e->pts, e->off,
+                     ? launch_batch_spread_force<double>(e->d_lat, e->n_latwork_vf, e->d_latwork_vf, e->pts, e->off,
                                                          e->sorted, stream())
-                     : launch_batch_spread_force<float>(e->d_lat, e->n_latwork, e->d_latwork, e->pts, e->off,
+                     : launch_batch_spread_force<float>(e->d_lat, e->n_latwork_vf, e->d_latwork_vf, e->pts, e->off,
                                                         e->sorted, stream());
   HIP_TRY(r);
   return LBSWIM_OK;
@@ -1597,6 +1608,7 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   ens_drop_graph(e);
   dev_free(e->d_lat);
   dev_free(e->d_latwork);
+  dev_free(e->d_latwork_vf);
   dev_free(e->d_sw);
   dev_free(e->d_swwork);
   dev_free(e->d_ptwork);

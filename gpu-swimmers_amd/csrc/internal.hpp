@@ -84,7 +84,7 @@ size_t spread_list_capacity(size_t npts);
 // Point records of the swimmers of one array, ids from 2*base.
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
                          const Geom& g, SpreadPoint* pts, hipStream_t s);
-// counts[nl], off[nl + 1 + ceil(nl/1024)] (offsets, then scan scratch),
+// counts[nl], off[nl + 2 + ceil(nl/1024)] (offsets, then scan and sort scratch),
 // cursor[nl], list/sorted[spread_list_capacity]; afterwards
 // sorted[off[l]..off[l+1]) holds the ids of the points reaching sites
 // 64 l .. 64 l + 63 in increasing order.

@@ -276,10 +276,11 @@ __device__ __forceinline__ void hydro_f64_body(const double* __restrict__ gCur, 
     F[1] = force[size_t(n) + s];
     F[2] = force[2 * size_t(n) + s];
   }
-  rho[s] = m0;
-  u[s] = (m1 + 0.5 * F[0]) / m0;
-  u[size_t(n) + s] = (m2 + 0.5 * F[1]) / m0;
-  u[2 * size_t(n) + s] = (m3 + 0.5 * F[2]) / m0;
+  // streaming stores: the fields are far larger than the caches
+  store_stream(&rho[s], m0);
+  store_stream(&u[s], (m1 + 0.5 * F[0]) / m0);
+  store_stream(&u[size_t(n) + s], (m2 + 0.5 * F[1]) / m0);
+  store_stream(&u[2 * size_t(n) + s], (m3 + 0.5 * F[2]) / m0);
 }
 
 // lb.cu:314-371, including the reference's SXZ = rho*ux*uy (lb.cu:356).
@@ -465,9 +466,9 @@ __device__ __forceinline__ void hydro_f32_body(const float* __restrict__ f, cons
   }
   const Moments32 m = moments_dev32(gl);
   const float inv_rho = 1.0f / (1.0f + m.drho);
-  drho_out[s] = m.drho;
+  store_stream(&drho_out[s], m.drho);
 #pragma unroll
-  for (int a = 0; a < 3; ++a) u[size_t(a) * n + s] = __builtin_fmaf(0.5f, F[a], m.j[a]) * inv_rho;
+  for (int a = 0; a < 3; ++a) store_stream(&u[size_t(a) * n + s], __builtin_fmaf(0.5f, F[a], m.j[a]) * inv_rho);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_init_from_hydro_f32(const float* __restrict__ drho,

@@ -8,6 +8,7 @@
 // CPU restatement does.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "d3q15.hpp"
@@ -625,7 +626,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(int nt, const uint32_
     off[i] = base + ex;
     cursor[i] = 0u;
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[nt] = base + total;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    off[nt] = base + total;
+    off[nt + 1 + gridDim.x] = 0u;  // queue counter of the sort, after the scan's partial sums
+  }
 }
 
 // One thread per (point, stencil row): appends the point id to the lists the
@@ -652,13 +656,16 @@ __global__ __launch_bounds__(PBLOCK) void k_fill_rows(uint32_t npts, Geom g, con
   });
 }
 
-// Sort of every list by point id.  Each lane sorts one list of up to 16
-// entries in registers with Batcher's odd-even merge networks (19
-// comparators for 8 inputs, 63 for 16; equal ids -- a point listed twice --
-// are interchangeable); longer lists are then ranked by the whole wavefront,
-// one at a time, equal ids keeping their order so that every position of
-// `sorted` is written once.
+// Sort of every list by point id, in two passes.  k_sort_lists: one lane per
+// list sorts lists of up to 16 entries in registers with Batcher's odd-even
+// merge networks (19 comparators for 8 inputs, 63 for 16; equal ids -- a point
+// listed twice -- are interchangeable) and queues the longer ones.
+// k_sort_long: one wavefront per queued list ranks its entries (stable: equal
+// ids keep their order, so every position of `sorted` is written once), from
+// a copy in LDS when it fits.
 constexpr uint32_t SHORT_LIST = 16;
+constexpr uint32_t LONG_LDS = 2048;  // entries per wavefront staged in LDS
+constexpr int LONG_BLOCK = 256;
 
 __device__ __forceinline__ void cmp_swap(uint32_t& a, uint32_t& b) {
   const uint32_t lo = min(a, b), hi = max(a, b);
@@ -689,9 +696,27 @@ __device__ __forceinline__ void sort_net(const uint32_t* __restrict__ src, uint3
     if (uint32_t(k) < cnt) dst[k] = v[k];
 }
 
-__device__ __forceinline__ void rank_entries(const uint32_t* __restrict__ src, uint32_t cnt, uint32_t first,
-                                             uint32_t step, uint32_t* __restrict__ dst) {
-  for (uint32_t e = first; e < cnt; e += step) {
+// `queue` (nlists entries) and `nqueued` receive the lists longer than 16;
+// nqueued is zeroed by the scan.
+__global__ __launch_bounds__(PBLOCK) void k_sort_lists(uint32_t nlists, const uint32_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ list,
+                                                      uint32_t* __restrict__ sorted, uint32_t* __restrict__ queue,
+                                                      uint32_t* __restrict__ nqueued) {
+  const uint32_t wl = blockIdx.x * PBLOCK + threadIdx.x;
+  if (wl >= nlists) return;
+  const uint32_t o = off[wl], cnt = off[wl + 1] - o;
+  if (cnt <= 8)
+    sort_net<8>(list + o, cnt, sorted + o, NET8);
+  else if (cnt <= SHORT_LIST)
+    sort_net<16>(list + o, cnt, sorted + o, NET16);
+  else
+    queue[atomicAdd(nqueued, 1u)] = wl;
+}
+
+// Stable ranks of list src[0..cnt) by the lanes of one wavefront.
+__device__ __forceinline__ void rank_list(const uint32_t* __restrict__ src, uint32_t cnt, uint32_t lane,
+                                          uint32_t* __restrict__ dst) {
+  for (uint32_t e = lane; e < cnt; e += WAVE_SITES) {
     const uint32_t id = src[e];
     uint32_t rank = 0;
     for (uint32_t k = 0; k < cnt; ++k) {
@@ -702,26 +727,33 @@ __device__ __forceinline__ void rank_entries(const uint32_t* __restrict__ src, u
   }
 }
 
-__global__ __launch_bounds__(PBLOCK) void k_sort_lists(uint32_t nlists, const uint32_t* __restrict__ off,
-                                                      const uint32_t* __restrict__ list,
-                                                      uint32_t* __restrict__ sorted) {
-  const uint32_t wl = blockIdx.x * PBLOCK + threadIdx.x;
-  const uint32_t lane = threadIdx.x % WAVE_SITES;
-  uint32_t o = 0, cnt = 0;
-  if (wl < nlists) {
-    o = off[wl];
-    cnt = off[wl + 1] - o;
-    if (cnt <= 8)
-      sort_net<8>(list + o, cnt, sorted + o, NET8);
-    else if (cnt <= SHORT_LIST)
-      sort_net<16>(list + o, cnt, sorted + o, NET16);
-  }
-  uint64_t longs = __ballot(cnt > SHORT_LIST);
-  while (longs) {
-    const int src_lane = __builtin_ctzll(longs);
-    longs &= longs - 1;
-    const uint32_t lo = __shfl(o, src_lane), lc = __shfl(cnt, src_lane);
-    rank_entries(list + lo, lc, lane, WAVE_SITES, sorted + lo);
+// Fixed grid; wavefront w takes queued lists w, w + waves, ...
+__global__ __launch_bounds__(LONG_BLOCK) void k_sort_long(const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ list,
+                                                         uint32_t* __restrict__ sorted,
+                                                         const uint32_t* __restrict__ queue,
+                                                         const uint32_t* __restrict__ nqueued) {
+  __shared__ uint32_t stage[LONG_BLOCK / WAVE_SITES][LONG_LDS];
+  const uint32_t wave = threadIdx.x / WAVE_SITES, lane = threadIdx.x % WAVE_SITES;
+  const uint32_t waves = gridDim.x * (LONG_BLOCK / WAVE_SITES);
+  const uint32_t n = *nqueued;
+  uint32_t* mine = stage[wave];
+  for (uint32_t q = blockIdx.x * (LONG_BLOCK / WAVE_SITES) + wave; q < n; q += waves) {
+    const uint32_t wl = queue[q];
+    const uint32_t o = off[wl], cnt = off[wl + 1] - o;
+    if (cnt <= LONG_LDS) {
+      for (uint32_t e = lane; e < cnt; e += WAVE_SITES) mine[e] = list[o + e];
+      // the wavefront's LDS accesses complete in order
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      rank_list(mine, cnt, lane, sorted + o);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      rank_list(list + o, cnt, lane, sorted + o);
+    }
   }
 }
 
@@ -865,7 +897,12 @@ static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ld
   if (npts > 0)
     hipLaunchKernelGGL(k_fill_rows<Batched>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc, pts, off,
                        cursor, list);
-  hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(size_t(nt))), dim3(PBLOCK), 0, s, uint32_t(nt), off, list, sorted);
+  // lists longer than 16 are queued in `cursor` (free once the lists are filled)
+  uint32_t* nqueued = off + nt + 1 + nsb;
+  hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(size_t(nt))), dim3(PBLOCK), 0, s, uint32_t(nt), off, list, sorted,
+                     cursor, nqueued);
+  const unsigned long_blocks = unsigned(std::min<size_t>(1024, (size_t(nt) + 3) / 4));
+  hipLaunchKernelGGL(k_sort_long, dim3(long_blocks), dim3(LONG_BLOCK), 0, s, off, list, sorted, cursor, nqueued);
   return hipGetLastError();
 }
 

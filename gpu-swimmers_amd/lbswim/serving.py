@@ -208,10 +208,11 @@ class Server:
                 if a[3] == key:
                     a[5] -= k
 
-    def run(self, requests, realtime=True, on_chunk=None):
+    def run(self, requests, realtime=True, on_chunk=None, on_done=None):
         """Serve `requests` (arrival times in seconds from the start).  With
         realtime=False arrivals are ignored and everything is admitted as
-        budget allows.  Returns summary statistics."""
+        budget allows.  on_done(request) is called as each result arrives.
+        Returns summary statistics."""
         queue = [(r.arrival if realtime else 0.0, r.rid, r) for r in requests]
         heapq.heapify(queue)
         t0 = time.perf_counter()
@@ -243,6 +244,8 @@ class Server:
                 for rid in finished:
                     done.append(self.active[rid][0])
                     self.release(rid, now, keep=True)
+                    if on_done is not None:
+                        on_done(done[-1])
         wall = clock()
         lat = np.array([r.latency() for r in done]) if done else np.zeros(0)
         updates = sum(r.sites * r.steps for r in done)
@@ -256,9 +259,9 @@ class Server:
         }
 
 
-def run_sequential(requests, realtime=True):
+def run_sequential(requests, realtime=True, on_done=None):
     """Baseline: one request at a time in arrival order, each alone with the
-    plain per-system step (no batching)."""
+    plain per-system step (no batching); on_done(request) after each."""
     from .System import System
 
     t0 = time.perf_counter()
@@ -278,6 +281,8 @@ def run_sequential(requests, realtime=True):
         done = time.perf_counter() - t0
         lat_s.append(done - (req.arrival if realtime else 0.0))
         updates += req.sites * req.steps
+        if on_done is not None:
+            on_done(req)
         del lat, things, system
     wall = time.perf_counter() - t0
     lat_s = np.array(lat_s)

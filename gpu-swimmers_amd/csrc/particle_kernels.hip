@@ -97,58 +97,11 @@ __device__ __forceinline__ void interp(const T* __restrict__ u, const Geom& g, c
 }
 
 // ---- stencil gathers by lane groups -----------------------------------------
-// A group of 16 lanes serves one particle: lane c = 4j + k owns stencil column
-// (j, k) and loops over i, so in each load instruction the group reads four
-// runs of four consecutive z sites instead of 16 scattered words.  The 64
-// products delta3d * u_d go to LDS at term t = 16 i + 4 j + k and one lane per
-// component then adds them in t order -- the i, j, k order of interp.cu:32-45 --
-// so the result is bit-identical to the one-lane loop above.
+// A group of 16 lanes serves one particle (group_interp below): lane c = 4j + k
+// owns stencil column (j, k), so in each load instruction the group reads four
+// runs of four consecutive z sites instead of 16 scattered words.
 constexpr int GROUP = 16;
 constexpr int GROUPS = PBLOCK / GROUP;
-constexpr int TERMS = 64 * 3;  // (t, d) pairs of one interpolation, stored t-major
-
-template <typename T>
-__device__ __forceinline__ void interp_terms(const T* __restrict__ u, const Geom& g, const double (&r)[3], int c,
-                                             double* __restrict__ terms) {
-  // Lane q < 12 of the group forms stencil coordinate q (axis q / 4, offset
-  // q % 4) once; the lanes then read the ones they need from each other.
-  const int q = c < 12 ? c : 0;
-  const int axis = q >> 2;
-  const double ra = axis == 0 ? r[0] : (axis == 1 ? r[1] : r[2]);
-  const int na = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
-  const double x = ceil(ra - 2.) + (q & 3);
-  const int mine_idx = int(wrap_index(x, na));
-  const double mine_del = peskin(ra - x);
-  const int j = c >> 2, k = c & 3;
-  uint32_t ix[4];
-  double dx[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    ix[i] = uint32_t(__shfl(mine_idx, i, GROUP));
-    dx[i] = __shfl(mine_del, i, GROUP);
-  }
-  const uint32_t iy = uint32_t(__shfl(mine_idx, 4 + j, GROUP)), iz = uint32_t(__shfl(mine_idx, 8 + k, GROUP));
-  const double dy = __shfl(mine_del, 4 + j, GROUP), dz = __shfl(mine_del, 8 + k, GROUP);
-  const uint32_t iyz = iy * uint32_t(g.nz) + iz;
-  const size_t n = g.n;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const double delta3d = dx[i] * dy * dz;
-    const uint32_t ijk = ix[i] * g.nyz + iyz;
-    double* out = terms + 3 * (16 * i + c);
-    out[0] = delta3d * double(u[ijk]);
-    out[1] = delta3d * double(u[n + ijk]);
-    out[2] = delta3d * double(u[2 * n + ijk]);
-  }
-}
-
-// Sum of component d of the 64 terms in t order, starting from 0.0.
-__device__ __forceinline__ double sum_terms(const double* __restrict__ terms, int d) {
-  double acc = 0.0;
-#pragma unroll 16
-  for (int t = 0; t < 64; ++t) acc += terms[3 * t + d];
-  return acc;
-}
 
 // ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ----
 struct U4 {
@@ -279,11 +232,73 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_add_forces(int ns, double P
   }
 }
 
-// DoSwimmerArrayMove, SwimmerArray.cu:123-210 (norm starts at 0; with
-// rotation off and no tumble, n is kept).  A lane group per swimmer gathers
-// the stencils at r and at r - l n; lane 0 of the group finishes the move.
-constexpr int SWIM_STRIDE = 2 * TERMS + 7;  // odd in doubles: groups start on different banks
+// InterpVelocity (interp.cu:8-47) by a group of 16 lanes, the stencil
+// streamed plane by plane: lane q < 12 forms stencil coordinate q (axis q / 4,
+// offset q % 4) once and the lanes read the ones they need from each other;
+// lane c = 4j + k gathers column (j, k) of every x plane i; for each plane the
+// 16 lanes write their products to the group's 48-double LDS slice and lanes
+// 0..2 add them to their component in t = 16i + 4j + k order -- the i, j, k
+// order of the reference -- then the next plane reuses the slice.  A group's
+// lanes are in one wavefront and LDS operations of a wavefront complete in
+// order, so no workgroup barrier is needed.  Returns component c on lanes
+// c < 3 (0 on the others).  Every lane of the group must call it.
+constexpr int PLANE_STRIDE = 49;  // 48 products + 1: groups start on different banks
 
+template <typename T>
+__device__ __forceinline__ double group_interp(const T* __restrict__ u, const Geom& g, const double (&r)[3], int c,
+                                               double* __restrict__ terms) {
+  const int q = c < 12 ? c : 0;
+  const int axis = q >> 2;
+  const double ra = axis == 0 ? r[0] : (axis == 1 ? r[1] : r[2]);
+  const int na = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+  const double x = ceil(ra - 2.) + (q & 3);
+  const int mine_idx = int(wrap_index(x, na));
+  const double mine_del = peskin(ra - x);
+  const int j = c >> 2, k = c & 3;
+  uint32_t ix[4];
+  double dx[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    ix[i] = uint32_t(__shfl(mine_idx, i, GROUP));
+    dx[i] = __shfl(mine_del, i, GROUP);
+  }
+  const uint32_t iyz =
+      uint32_t(__shfl(mine_idx, 4 + j, GROUP)) * uint32_t(g.nz) + uint32_t(__shfl(mine_idx, 8 + k, GROUP));
+  const double dy = __shfl(mine_del, 4 + j, GROUP), dz = __shfl(mine_del, 8 + k, GROUP);
+  const size_t n = g.n;
+  double uv[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t ijk = ix[i] * g.nyz + iyz;
+    uv[i][0] = double(u[ijk]);
+    uv[i][1] = double(u[n + ijk]);
+    uv[i][2] = double(u[2 * n + ijk]);
+  }
+  const int d = c < 3 ? c : 0;
+  double vel = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double delta3d = dx[i] * dy * dz;
+    terms[3 * c + 0] = delta3d * uv[i][0];
+    terms[3 * c + 1] = delta3d * uv[i][1];
+    terms[3 * c + 2] = delta3d * uv[i][2];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (c < 3) {
+#pragma unroll
+      for (int cc = 0; cc < 16; ++cc) vel += terms[3 * cc + d];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  return vel;
+}
+
+// DoSwimmerArrayMove, SwimmerArray.cu:123-210 (norm starts at 0; with
+// rotation off and no tumble, n is kept).  A lane group per swimmer
+// interpolates at r and at r - l n; lane 0 of the group finishes the move.
 template <typename T>
 __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp, double* __restrict__ r,
                                                    double* __restrict__ v, double* __restrict__ nv,
@@ -291,30 +306,28 @@ __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp,
                                                    const T* __restrict__ u, const Geom& g,
                                                    unsigned long long* __restrict__ tumbles, uint32_t blk,
                                                    double* __restrict__ lds) {
-  constexpr int STRIDE = SWIM_STRIDE;
   const int grp = int(threadIdx.x) / GROUP, c = int(threadIdx.x) % GROUP;
   const int is = int(blk) * GROUPS + grp;
-  const bool active = is < ns;
-  double* terms = lds + grp * STRIDE;
+  if (is >= ns) return;  // whole group: the lanes of a group share `is`
+  double* terms = lds + grp * PLANE_STRIDE;
   double R[3], N[3], rMinus[3];
-  if (active) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      R[d] = r[size_t(ns) * d + is];
-      N[d] = nv[size_t(ns) * d + is];
-      rMinus[d] = R[d] - N[d] * cp.l;
-    }
-    interp_terms(u, g, R, c, terms);
-    interp_terms(u, g, rMinus, c, terms + TERMS);
+  for (int d = 0; d < 3; ++d) {
+    R[d] = r[size_t(ns) * d + is];
+    N[d] = nv[size_t(ns) * d + is];
+    rMinus[d] = R[d] - N[d] * cp.l;
   }
-  __syncthreads();
-  if (active && c < 6) terms[2 * TERMS + c] = sum_terms(terms + (c < 3 ? 0 : TERMS), c % 3);
-  __syncthreads();
-  if (!active || c != 0) return;
+  const double a = group_interp(u, g, R, c, terms);
+  const double b = group_interp(u, g, rMinus, c, terms);
+  double vel[3], vMinus[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vel[d] = __shfl(a, d, GROUP);
+    vMinus[d] = __shfl(b, d, GROUP);
+  }
+  if (c != 0) return;
   const int size[3] = {g.nx, g.ny, g.nz};
   const int user = perm[is];  // the random stream belongs to the swimmer, not its storage slot
-  const double* vel = terms + 2 * TERMS;
-  const double* vMinus = terms + 2 * TERMS + 3;
   double rDot[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -356,15 +369,8 @@ __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp,
   prng[is] = ctr + 1;
 }
 
-// DoTracerArrayMove, TracerArray.cu:12-43 (s updated per particle), with a
-// lane group of 16 per tracer and the stencil streamed plane by plane: for each x
-// offset i the 16 lanes of the group write their 16 (j, k) products to a
-// 48-double LDS slice of the group and lanes 0..2 add them to their
-// component in t order, then the next plane reuses the slice.  A group's
-// lanes are in one wavefront and LDS operations of a wavefront complete in
-// order, so no workgroup barrier is needed and each workgroup holds only
-// GROUPS * 49 doubles of LDS.
-constexpr int PLANE_STRIDE = 49;  // 48 products + 1: groups start on different banks
+// DoTracerArrayMove, TracerArray.cu:12-43 (s updated per particle), a lane
+// group of 16 per tracer (group_interp).
 constexpr int TGROUPS = TRACER_GROUPS;  // tracers per workgroup: 1024 threads, contiguous in storage order
 constexpr int TBLOCK = TGROUPS * GROUP;
 
@@ -379,51 +385,7 @@ __device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r
   const double R[3] = {r[it], r[size_t(nt) + it], r[2 * size_t(nt) + it]};
   const int d = c < 3 ? c : 0;
   const double s_old = c < 3 ? sd[size_t(nt) * d + it] : 0.0;
-  // stencil coordinates, one per lane, shared in the group
-  const int q = c < 12 ? c : 0;
-  const int axis = q >> 2;
-  const double ra = axis == 0 ? R[0] : (axis == 1 ? R[1] : R[2]);
-  const int na = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
-  const double x = ceil(ra - 2.) + (q & 3);
-  const int mine_idx = int(wrap_index(x, na));
-  const double mine_del = peskin(ra - x);
-  const int j = c >> 2, k = c & 3;
-  uint32_t ix[4];
-  double dx[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    ix[i] = uint32_t(__shfl(mine_idx, i, GROUP));
-    dx[i] = __shfl(mine_del, i, GROUP);
-  }
-  const uint32_t iyz = uint32_t(__shfl(mine_idx, 4 + j, GROUP)) * uint32_t(g.nz) + uint32_t(__shfl(mine_idx, 8 + k, GROUP));
-  const double dy = __shfl(mine_del, 4 + j, GROUP), dz = __shfl(mine_del, 8 + k, GROUP);
-  const size_t n = g.n;
-  double uv[4][3];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const uint32_t ijk = ix[i] * g.nyz + iyz;
-    uv[i][0] = double(u[ijk]);
-    uv[i][1] = double(u[n + ijk]);
-    uv[i][2] = double(u[2 * n + ijk]);
-  }
-  double vel = 0.0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const double delta3d = dx[i] * dy * dz;
-    terms[3 * c + 0] = delta3d * uv[i][0];
-    terms[3 * c + 1] = delta3d * uv[i][1];
-    terms[3 * c + 2] = delta3d * uv[i][2];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (c < 3) {
-#pragma unroll
-      for (int cc = 0; cc < 16; ++cc) vel += terms[3 * cc + d];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
+  const double vel = group_interp(u, g, R, c, terms);
   if (c >= 3) return;
   const int size = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
   const double Rd = d == 0 ? R[0] : (d == 1 ? R[1] : R[2]);
@@ -451,7 +413,7 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp,
                                                           uint64_t* __restrict__ prng, const int* __restrict__ perm,
                                                           const T* __restrict__ u, Geom g,
                                                           unsigned long long* __restrict__ tumbles, uint32_t nb) {
-  __shared__ double lds[GROUPS * SWIM_STRIDE];
+  __shared__ double lds[GROUPS * PLANE_STRIDE];
   const uint32_t blk = xcd_block(round_xcd(nb));
   if (blk >= nb) return;
   swimmers_move_body(ns, cp, r, v, nv, prng, perm, u, g, tumbles, blk, lds);
@@ -474,7 +436,7 @@ template <typename T>
 __global__ __launch_bounds__(PBLOCK) void kb_swimmers_move(const LatDesc* __restrict__ ldesc,
                                                            const SwimDesc* __restrict__ sdesc,
                                                            const WorkItem* __restrict__ work, uint32_t nwork) {
-  __shared__ double lds[GROUPS * SWIM_STRIDE];
+  __shared__ double lds[GROUPS * PLANE_STRIDE];
   const uint32_t wi = xcd_block(round_xcd(nwork));
   if (wi >= nwork) return;
   const WorkItem w = work[wi];

@@ -35,6 +35,20 @@ def collide_bytes(form, virtual):
     return (30 + (0 if virtual else 3)) * ELEM[form]
 
 
+def particle_bytes(ns, nt):
+    """Particle-array bytes of one coupled step: swimmers read r, n for the
+    spreading points (48 B) and for the move (48 B), write r, v, n (72 B) and
+    read and write the draw counter (16 B), plus 2 point records of 176 B
+    written and read once; tracers read r, s (48 B) and write r, s, v (72 B)."""
+    return ns * (48 + 48 + 72 + 16 + 2 * 2 * 176) + nt * (48 + 72)
+
+
+def gathered_bytes(ns, nt, elem):
+    """Stencil gathers of u (64 sites x 3 components) in the moves: two
+    stencils per swimmer, one per tracer; mostly L2 hits."""
+    return (2 * ns + nt) * 64 * 3 * elem
+
+
 def step_bytes(form, virtual):
     """Lattice bytes per site of the whole coupled step: collide+stream,
     CalcHydro (15 f read, rho and 3 u written, 3 force read) and the force
@@ -51,7 +65,7 @@ def make_workload(size, seed=1234):
     return make_case((size, size, size), 0, 0.25, ns=32768, nt=1_000_000, alpha=0.01, seed=seed)
 
 
-def time_gpu(case, storage, steps, warmup, force_mode="auto"):
+def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200):
     from cases import build_lbswim
     import lbswim
 
@@ -92,7 +106,12 @@ def time_gpu(case, storage, steps, warmup, force_mode="auto"):
     res["collide_gbs"] = res["collide_bytes"] * n / (coll * 1e-3) / 1e9
     res["step_gbs"] = res["step_bytes"] * n / (ms_step * 1e-3) / 1e9
     res["tumbles"] = sw.tumbles()
-    # final state for the FP32-vs-FP64 error report (same inputs, same steps)
+    # final state for the FP32-vs-FP64 error report (same inputs, same step
+    # count: untimed steps up to total_steps)
+    done = warmup + steps + breakdown_steps
+    if total_steps > done:
+        system.Step(total_steps - done)
+    res["state_steps"] = max(total_steps, done)
     for name in ("rho", "u"):
         getattr(lat, name).D2H()
     sw.D2H()
@@ -118,7 +137,8 @@ def precision_report(f64, f32, L):
     a, b = f32["state"], f64["state"]
     return {"u_rel": rel(a["u"], b["u"]), "drho_rel": rel(a["rho"] - 1.0, b["rho"] - 1.0),
             "swimmer_r_abs": traj(a["sw_r"], b["sw_r"], L), "swimmer_n_abs": float(np.max(np.abs(a["sw_n"] - b["sw_n"]))),
-            "tracer_r_abs": traj(a["tr_r"], b["tr_r"], L), "note": "after warm-up + timed steps on identical inputs"}
+            "tracer_r_abs": traj(a["tr_r"], b["tr_r"], L),
+            "note": "after %d coupled steps of both forms from identical inputs" % f64["state_steps"]}
 
 
 def ensemble_cases(nsims=64, seed=1234):
@@ -315,6 +335,8 @@ def main():
         },
         "step_hbm_frac": round(m["step_gbs"] / HBM_PEAK_GBS, 4),
         "step_bytes_per_site": m["step_bytes"],
+        "particle_bytes_per_step": particle_bytes(32768, 1_000_000),
+        "gathered_bytes_per_step": gathered_bytes(32768, 1_000_000, ELEM[main_form]),
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in m["kernels"].items()},
         "kernels_note": "per-kernel averages from a second pass of min(steps, 20) steps with every kernel "
                         "bracketed by HIP events (%.4f ms/step with those events); tracer moves run on a second "

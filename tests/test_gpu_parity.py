@@ -325,6 +325,33 @@ def test_f32_storage_error_against_f64(size, steps):
     assert errs["sw_r"] <= 1e-9 and errs["tr_r"] <= 1e-9
 
 
+def test_f32_storage_error_at_256_cubed_200_steps():
+    """The full-size precision configuration: 256^3, 32768 swimmers (alpha
+    0.01), 1e6 tracers, 200 steps of both forms from identical inputs."""
+    case = make_case((256, 256, 256), 200, 0.25, ns=32768, nt=1_000_000, alpha=0.01, seed=1234)
+    say("f32 vs f64 at 256^3, 200 steps")
+    out = {}
+    for storage in ("f64", "f32"):
+        lat, system, sw, tr = build_lbswim(case, storage)
+        system.Step(200)
+        lat.u.D2H()
+        lat.rho.D2H()
+        sw.D2H()
+        tr.D2H()
+        out[storage] = {"u": np.array(lat.u.data).ravel(), "drho": np.array(lat.rho.data).ravel() - 1.0,
+                        "sw_r": np.array(sw.r.data), "tr_r": np.array(tr.r.data), "tumbles": sw.tumbles()}
+        del lat, system, sw, tr
+    a, b = out["f64"], out["f32"]
+    errs = {k: rel_inf(b[k], a[k]) for k in ("u", "drho")}
+    for k in ("sw_r", "tr_r"):  # minimum-image displacement
+        d = np.abs(b[k] - a[k])
+        errs[k] = float(np.max(np.minimum(d, 256.0 - d)))
+    say("f32 errors at 256^3: %s" % {k: "%.2e" % v for k, v in errs.items()})
+    assert errs["u"] <= F32_U_TOL and errs["drho"] <= F32_U_TOL
+    assert errs["sw_r"] <= 1e-7 and errs["tr_r"] <= 1e-7
+    assert a["tumbles"] == b["tumbles"]  # the draws do not depend on the storage form
+
+
 # ------------------------------------------------------------- properties
 
 def test_conservation_at_256_cubed():

@@ -154,7 +154,7 @@ def ensemble_cases(nsims=64, seed=1234):
     return out
 
 
-def bench_ensemble(storage, steps, nsims=64, tune_steps=5):
+def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True):
     """Batched (one launch per phase, graph replay, autotuned order) against
     the same sims run one after another with the plain per-system step."""
     from cases import build_lbswim
@@ -176,6 +176,7 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5):
         del lat, system, sw, tr
     # batched
     ens = Ensemble(storage)
+    ens.set_graph(graph)
     keep = []
     for c in cases:
         lat, system, sw, tr = build_lbswim(c, storage)
@@ -192,7 +193,7 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5):
             "ms_sequential": round(t_seq * 1e3, 2), "sims_per_s": round(nsims / t_ens, 2),
             "sims_per_s_sequential": round(nsims / t_seq, 2), "speedup_vs_sequential": round(t_seq / t_ens, 2),
             "mlups": round(sites * steps / t_ens / 1e6, 1), "order": ORDERS[choice],
-            "graph": True}
+            "graph": graph}
 
 
 def bench_serving(nreq, rate, seed=1234):
@@ -275,6 +276,8 @@ def main():
     ap.add_argument("--probe", choices=["copy"], help="run the streaming-copy bandwidth probe instead")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-sim ensemble leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--ensemble-eager", action="store_true",
+                    help="launch the ensemble's steps eagerly instead of replaying captured graphs")
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate of the serving leg (1/s)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -356,7 +359,7 @@ def main():
         out["ensemble"] = {}
         for form in forms:
             log("ensemble %s: 64 sims x %d steps, batched vs sequential" % (form, a.ensemble_steps))
-            out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps)
+            out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps, graph=not a.ensemble_eager)
             log("ensemble %s: %s" % (form, out["ensemble"][form]))
     if a.serve_requests > 0:
         log("serving: %d requests at %.1f/s, continuous batching vs sequential" % (a.serve_requests, a.serve_rate))

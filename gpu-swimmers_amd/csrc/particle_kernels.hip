@@ -725,12 +725,13 @@ template <typename T>
 __device__ __forceinline__ void spread_block_body(const Geom& g, const SpreadPoint* __restrict__ pts,
                                                   const uint32_t* __restrict__ off,
                                                   const uint32_t* __restrict__ sorted, T* __restrict__ force,
-                                                  uint32_t blk, uint32_t first_list) {
+                                                  uint32_t blk, uint32_t first_list, WaveStage* __restrict__ stage) {
   const uint32_t s = blk * SITE_BLOCK + threadIdx.x;
   const bool ok = s < g.n;
   const SiteXYZ c = site_xyz(ok ? s : 0u, g);
   double acc[3];
-  wave_force(pts, off, sorted, first_list + threadIdx.x / WAVE_SITES, ok, c.x, c.y, c.z, g, acc);
+  wave_force_staged(pts, off, sorted, first_list + threadIdx.x / WAVE_SITES, ok, c.x, c.y, c.z, g,
+                    stage + threadIdx.x / WAVE_SITES, acc);
   if (!ok) return;
   const size_t n = g.n;
   force[s] = T(acc[0]);
@@ -743,7 +744,8 @@ __global__ __launch_bounds__(SITE_BLOCK) void k_spread_blocks(Geom g, const Spre
                                                              const uint32_t* __restrict__ off,
                                                              const uint32_t* __restrict__ sorted,
                                                              T* __restrict__ force) {
-  spread_block_body(g, pts, off, sorted, force, blockIdx.x, blockIdx.x * LISTS_PER_BLOCK);
+  __shared__ WaveStage stage[LISTS_PER_BLOCK];
+  spread_block_body(g, pts, off, sorted, force, blockIdx.x, blockIdx.x * LISTS_PER_BLOCK, stage);
 }
 
 template <typename T>
@@ -752,10 +754,11 @@ __global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __
                                                               const SpreadPoint* __restrict__ pts,
                                                               const uint32_t* __restrict__ off,
                                                               const uint32_t* __restrict__ sorted) {
+  __shared__ WaveStage stage[LISTS_PER_BLOCK];
   const WorkItem w = block_work[blockIdx.x];
   const LatDesc& m = ldesc[w.member];
   spread_block_body(m.geom, pts, off, sorted, static_cast<T*>(m.force), w.unit,
-                    m.tile_base + w.unit * LISTS_PER_BLOCK);
+                    m.tile_base + w.unit * LISTS_PER_BLOCK, stage);
 }
 
 // ---- device-math probes used by the parity tests ----
@@ -863,7 +866,8 @@ static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ld
   uint32_t* nqueued = off + nt + 1 + nsb;
   hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(size_t(nt))), dim3(PBLOCK), 0, s, uint32_t(nt), off, list, sorted,
                      cursor, nqueued);
-  const unsigned long_blocks = unsigned(std::min<size_t>(1024, (size_t(nt) + 3) / 4));
+  // one wavefront per list, at most 512 workgroups
+  const unsigned long_blocks = unsigned(std::min<size_t>(512, (size_t(nt) + 3) / 4));
   hipLaunchKernelGGL(k_sort_long, dim3(long_blocks), dim3(LONG_BLOCK), 0, s, off, list, sorted, cursor, nqueued);
   return hipGetLastError();
 }

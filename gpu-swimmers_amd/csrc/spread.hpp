@@ -96,6 +96,12 @@ __device__ __forceinline__ void add_point(const SpreadPoint* __restrict__ p, int
 // wavefront's 64 sites (ok = false for lanes past the lattice end).  The list
 // index is wavefront-uniform, so ids and point records are read through the
 // scalar cache; no LDS and no workgroup barrier.
+//
+// When nz is a multiple of 64 every list lies inside one z-row, so X and Y are
+// the same on all lanes: each listed point then has one stencil row (i, j) on
+// this list, found and weighted (dx_i * dy_j) once for the wavefront, and a
+// lane only selects its z weight -- the same products in the same order as
+// add_point, with a quarter of the vector instructions.
 __device__ __forceinline__ void wave_force(const SpreadPoint* __restrict__ pts, const uint32_t* __restrict__ off,
                                            const uint32_t* __restrict__ sorted, uint32_t wl, bool ok, int X, int Y,
                                            int Z, const Geom& g, double (&acc)[3]) {
@@ -104,11 +110,94 @@ __device__ __forceinline__ void wave_force(const SpreadPoint* __restrict__ pts, 
   const uint32_t o = off[wl], end = off[wl + 1];
   const bool simple = g.nx >= 4 && g.ny >= 4 && g.nz >= 4;
   uint32_t prev = 0xFFFFFFFFu;
+  if (simple && g.nz % WAVE_SITES == 0) {
+    // all 64 sites valid and in one row: X, Y uniform
+    const int Xu = __builtin_amdgcn_readfirstlane(X), Yu = __builtin_amdgcn_readfirstlane(Y);
+    for (uint32_t e = o; e < end; ++e) {
+      const uint32_t id = sorted[e];
+      if (id == prev) continue;
+      prev = id;
+      const SpreadPoint* __restrict__ p = pts + id;
+      int di = Xu - p->wc[0];
+      di += di < 0 ? g.nx : 0;
+      int dj = Yu - p->wc[4];
+      dj += dj < 0 ? g.ny : 0;
+      if ((di | dj) >= 4) continue;  // not on this row (cannot happen for a listed point)
+      const double dxy = p->del[di] * p->del[4 + dj];
+      const double F0 = p->F[0], F1 = p->F[1], F2 = p->F[2];
+      int dk = Z - p->wc[8];
+      dk += dk < 0 ? g.nz : 0;
+      // per-lane select of the z weight from the four uniform ones, with bit
+      // masks so that the compiler keeps it branch-free
+      uint64_t zbits = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        zbits |= __builtin_bit_cast(uint64_t, p->del[8 + q]) & (uint64_t(0) - uint64_t(dk == q));
+      const double delta3d = dxy * __builtin_bit_cast(double, zbits);
+      if (dk < 4) {
+        acc[0] += delta3d * F0;
+        acc[1] += delta3d * F1;
+        acc[2] += delta3d * F2;
+      }
+    }
+    return;
+  }
   for (uint32_t e = o; e < end; ++e) {
     const uint32_t id = sorted[e];
     if (id == prev) continue;
     prev = id;
     if (ok) add_point(pts + id, X, Y, Z, g, simple, acc);
+  }
+}
+
+// LDS staging of one chunk of a list, one per wavefront.
+constexpr int STAGE_POINTS = 32;
+constexpr int POINT_WORDS = int(sizeof(SpreadPoint) / 8);
+struct WaveStage {
+  SpreadPoint pt[STAGE_POINTS];
+  uint32_t id[STAGE_POINTS];
+};
+
+// wave_force for long lists: the wavefront copies chunks of 32 point records
+// into its LDS slice with all lanes, then every lane walks the chunk from LDS,
+// so an entry costs a few LDS reads instead of a dependent round trip to the
+// caches.  Same order, same result.
+__device__ __forceinline__ void wave_force_staged(const SpreadPoint* __restrict__ pts,
+                                                  const uint32_t* __restrict__ off,
+                                                  const uint32_t* __restrict__ sorted, uint32_t wl, bool ok, int X,
+                                                  int Y, int Z, const Geom& g, WaveStage* __restrict__ st,
+                                                  double (&acc)[3]) {
+  acc[0] = acc[1] = acc[2] = 0.0;
+  wl = __builtin_amdgcn_readfirstlane(wl);
+  const uint32_t o = off[wl], end = off[wl + 1];
+  const bool simple = g.nx >= 4 && g.ny >= 4 && g.nz >= 4;
+  const int lane = int(threadIdx.x % WAVE_SITES);
+  uint32_t prev = 0xFFFFFFFFu;
+  for (uint32_t base = o; base < end; base += STAGE_POINTS) {
+    const int m = int(min(uint32_t(STAGE_POINTS), end - base));
+    if (lane < m) st->id[lane] = sorted[base + lane];
+    // the wavefront's LDS accesses complete in order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // ids from LDS: the record loads of a lane are independent of each other
+    uint64_t* dst = reinterpret_cast<uint64_t*>(st->pt);
+    for (int w = lane; w < m * POINT_WORDS; w += WAVE_SITES) {
+      const int e = w / POINT_WORDS, c = w - e * POINT_WORDS;
+      dst[w] = reinterpret_cast<const uint64_t*>(pts + st->id[e])[c];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int e = 0; e < m; ++e) {
+      const uint32_t id = st->id[e];
+      if (id == prev) continue;
+      prev = id;
+      if (ok) add_point(&st->pt[e], X, Y, Z, g, simple, acc);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 

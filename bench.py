@@ -342,8 +342,7 @@ def main():
         "gathered_bytes_per_step": gathered_bytes(32768, 1_000_000, ELEM[main_form]),
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in m["kernels"].items()},
         "kernels_note": "per-kernel averages from a second pass of min(steps, 20) steps with every kernel "
-                        "bracketed by HIP events (%.4f ms/step with those events); tracer moves run on a second "
-                        "stream alongside the next step's collide" % m["breakdown_ms_per_step"],
+                        "bracketed by HIP events (%.4f ms/step with those events)" % m["breakdown_ms_per_step"],
     }
     if "f32" in res and "f64" in res:
         out["fp32_error_vs_fp64"] = precision_report(res["f64"], res["f32"], float(a.size))

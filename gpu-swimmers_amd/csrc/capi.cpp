@@ -49,19 +49,12 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 // One stream orders the device work of every object, like the reference's
-// default stream.  Created on first use; default priority.  The coupled step
-// also runs tracer moves on a second stream (default priority too), joined
-// back into the first by events before anything else can see their data.
+// default stream.  Created on first use; default priority.
 struct Runtime {
   bool ready = false;
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;
-  hipEvent_t u_ready = nullptr;      // CalcHydro of the current step done
-  hipEvent_t tracers_done = nullptr; // tracer moves of the current step done
 };
 Runtime g_rt;
-// Work issued while set goes to this stream instead (see OnStream).
-thread_local hipStream_t t_stream = nullptr;
 
 int ensure_runtime() {
   if (g_rt.ready) return LBSWIM_OK;
@@ -69,22 +62,11 @@ int ensure_runtime() {
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev < 1) return fail(LBSWIM_ENODEV, "no HIP device available (%s)", hipGetErrorString(e));
   HIP_TRY(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
-  HIP_TRY(hipStreamCreateWithFlags(&g_rt.side, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreateWithFlags(&g_rt.u_ready, hipEventDisableTiming));
-  HIP_TRY(hipEventCreateWithFlags(&g_rt.tracers_done, hipEventDisableTiming));
   g_rt.ready = true;
   return LBSWIM_OK;
 }
 
-hipStream_t stream() { return t_stream ? t_stream : g_rt.stream; }
-
-// Routes the library's launches to `s` for the guard's lifetime.
-struct OnStream {
-  explicit OnStream(hipStream_t s) { t_stream = s; }
-  ~OnStream() { t_stream = nullptr; }
-  OnStream(const OnStream&) = delete;
-  OnStream& operator=(const OnStream&) = delete;
-};
+hipStream_t stream() { return g_rt.stream; }
 
 int dev_alloc_zero(void** p, size_t bytes) {
   *p = nullptr;
@@ -1060,50 +1042,6 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
 
 }  // namespace
 
-// Tracer moves only read u and their own arrays, and nothing in the lattice
-// step reads tracers.  So the tracer moves of step t are issued on the side
-// stream once the point lists of step t+1 are built, and run alongside the
-// collide of step t+1 (bandwidth-bound, while the moves are latency-bound);
-// CalcHydro of step t+1, which overwrites u, waits for them.  Every launch
-// and its data are the same as in the sequential order, so results are
-// unchanged.
-struct SideTracers {
-  lbswim_lattice* lat;
-  lbswim_tracers* const* tr;
-  int ntr;
-  bool deferred = false;  // moves of the last step not issued yet
-  bool pending = false;   // issued on the side stream, not yet joined
-
-  // Issue the deferred moves on the side stream after the main stream's
-  // current point.
-  int issue() {
-    if (!deferred) return LBSWIM_OK;
-    deferred = false;
-    HIP_TRY(hipEventRecord(g_rt.u_ready, g_rt.stream));
-    HIP_TRY(hipStreamWaitEvent(g_rt.side, g_rt.u_ready, 0));
-    pending = true;  // joined even if an issue below fails
-    {
-      OnStream on(g_rt.side);
-      for (int i = 0; i < ntr; ++i)
-        if (int rc = tracers_move_impl(tr[i], lat)) return rc;
-    }
-    HIP_TRY(hipEventRecord(g_rt.tracers_done, g_rt.side));
-    return LBSWIM_OK;
-  }
-  // Main stream waits for the issued moves.
-  int join() {
-    if (!pending) return LBSWIM_OK;
-    pending = false;
-    HIP_TRY(hipStreamWaitEvent(g_rt.stream, g_rt.tracers_done, 0));
-    return LBSWIM_OK;
-  }
-  int finish() {
-    if (int rc = issue()) return rc;
-    return join();
-  }
-  ~SideTracers() { (void)finish(); }
-};
-
 int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr, int ntr,
                        int nsteps) {
   if (int rc = check_lattice(lat)) return rc;
@@ -1112,19 +1050,17 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
   CHECK_ARG(ntr == 0 || tr != nullptr, "tracer list is NULL");
   for (int i = 0; i < nsw; ++i) CHECK_ARG(sw[i] != nullptr, "swimmer handle %d is NULL", i);
   for (int i = 0; i < ntr; ++i) CHECK_ARG(tr[i] != nullptr, "tracer handle %d is NULL", i);
-  SideTracers side{lat, tr, ntr};
   for (int s = 0; s < nsteps; ++s) {
     int rc;
     if ((rc = spread_forces(lat, sw, nsw))) return rc;
-    if ((rc = side.issue())) return rc;  // tracer moves of step s-1
     if ((rc = lattice_step_impl(lat))) return rc;
-    if ((rc = side.join())) return rc;  // u of step s-1 no longer needed
     if ((rc = lattice_hydro_impl(lat))) return rc;
     for (int i = 0; i < nsw; ++i)
       if ((rc = swimmers_move_impl(sw[i], lat))) return rc;
-    side.deferred = ntr > 0;
+    for (int i = 0; i < ntr; ++i)
+      if ((rc = tracers_move_impl(tr[i], lat))) return rc;
   }
-  return side.finish();
+  return LBSWIM_OK;
 }
 
 // ---- diagnostics ----------------------------------------------------------------

@@ -262,8 +262,8 @@ int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choic
 
 /* FP64 totals of the current distributions: out[0] = total mass, out[1..3] = total momentum. */
 int lbswim_lattice_totals(lbswim_lattice* lat, double out[4]);
-/* Per-kernel timing with HIP events on the stream each kernel runs on; on = 1
- * starts recording every kernel (and clears earlier records), 0 stops. */
+/* Per-kernel timing with HIP events on the library stream; on = 1 starts
+ * recording every kernel (and clears earlier records), 0 stops. */
 int lbswim_lattice_set_timing(lbswim_lattice* lat, int on);
 /* As set_timing, recording only the kernels whose bit (1 << LBSWIM_K_*) is
  * set: each recorded launch adds an event pair, which costs a few

@@ -35,6 +35,13 @@ def collide_bytes(form, virtual):
     return (30 + (0 if virtual else 3)) * ELEM[form]
 
 
+def fused_bytes(form, virtual):
+    """Algorithmic bytes per site of one fused collide+stream+CalcHydro
+    launch: 15 f read, 15 f written, rho and 3 u written, plus 3 force
+    components read unless the force is virtual."""
+    return (34 + (0 if virtual else 3)) * ELEM[form]
+
+
 def particle_bytes(ns, nt):
     """Particle-array bytes of one coupled step: swimmers read r, n for the
     spreading points (48 B) and for the move (48 B), write r, v, n (72 B) and
@@ -49,11 +56,13 @@ def gathered_bytes(ns, nt, elem):
     return (2 * ns + nt) * 64 * 3 * elem
 
 
-def step_bytes(form, virtual):
+def step_bytes(form, virtual, fused):
     """Lattice bytes per site of the whole coupled step: collide+stream,
-    CalcHydro (15 f read, rho and 3 u written, 3 force read) and the force
-    array written once -- the last two only with a stored force."""
-    return (30 + 15 + 4 + (0 if virtual else 3 + 3 + 3)) * ELEM[form]
+    CalcHydro (15 f read -- not with the fused launch, which pulls from the
+    values it has just collided -- rho and 3 u written, 3 force read) and the
+    force array written once; the force terms only with a stored force."""
+    force = 0 if virtual else (3 if fused else 3 + 3) + 3
+    return (30 + (0 if fused else 15) + 4 + force) * ELEM[form]
 
 
 def log(msg):
@@ -70,17 +79,19 @@ def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200):
     import lbswim
 
     lat, system, sw, tr = build_lbswim(case, storage, force_mode)
+    fused = lat.fusion[0]
+    main = "step_hydro" if fused else "collide_stream"  # the dominant lattice kernel
     system.Step(warmup)
     lbswim.synchronize()
-    # timed region: HIP events around the collide launches only (each event
-    # pair costs a few microseconds of device time)
-    lat.set_timing(["collide_stream"])
+    # timed region: HIP events around the dominant kernel's launches only (each
+    # event pair costs a few microseconds of device time)
+    lat.set_timing([main])
     lbswim.synchronize()
     t0 = time.perf_counter()
     system.Step(steps)
     lbswim.synchronize()
     t1 = time.perf_counter()
-    coll_ms, coll_cnt = lat.kernel_times()["collide_stream"]
+    main_ms, main_cnt = lat.kernel_times()[main]
     virtual = lat.force_is_virtual
     # per-kernel breakdown: a second pass with every kernel bracketed by events
     breakdown_steps = min(steps, 20)
@@ -94,16 +105,17 @@ def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200):
     n = lat.addr.n
     ms_step = (t1 - t0) * 1e3 / steps
     mlups = n * steps / (t1 - t0) / 1e6
-    res = {"ms_per_step": ms_step, "mlups": mlups, "kernels": {}, "virtual_force": virtual,
-           "breakdown_ms_per_step": (tb1 - tb0) * 1e3 / breakdown_steps}
+    res = {"ms_per_step": ms_step, "mlups": mlups, "kernels": {}, "virtual_force": virtual, "fused": fused,
+           "fusion": list(lat.fusion), "breakdown_ms_per_step": (tb1 - tb0) * 1e3 / breakdown_steps}
     for name, (ms, cnt) in times.items():
         if cnt:
             res["kernels"][name] = {"avg_ms": ms / cnt, "launches": cnt}
-    coll = coll_ms / coll_cnt
-    res["collide_avg_ms"] = coll
-    res["collide_bytes"] = collide_bytes(storage, virtual)
-    res["step_bytes"] = step_bytes(storage, virtual)
-    res["collide_gbs"] = res["collide_bytes"] * n / (coll * 1e-3) / 1e9
+    avg = main_ms / main_cnt
+    res["main_kernel"] = "k_%s_%s%s" % (main, "vf_" if virtual else "", storage)
+    res["main_avg_ms"] = avg
+    res["main_bytes"] = (fused_bytes if fused else collide_bytes)(storage, virtual)
+    res["step_bytes"] = step_bytes(storage, virtual, fused)
+    res["main_gbs"] = res["main_bytes"] * n / (avg * 1e-3) / 1e9
     res["step_gbs"] = res["step_bytes"] * n / (ms_step * 1e-3) / 1e9
     res["tumbles"] = sw.tumbles()
     # final state for the FP32-vs-FP64 error report (same inputs, same step
@@ -250,11 +262,11 @@ def cpu_baseline(case, min_seconds=10.0, max_steps=64):
                       "OpenMP over %d threads, %.1f s" % (steps, cores, dt)}
 
 
-def load_traffic(storage, virtual):
-    """HBM bytes per collide launch from the committed rocprofv3 --pmc summary
-    of the same kernel (null if the summary covers another one)."""
-    path = os.path.join(ROOT, "profiles", "collide_stream_%s_256cubed_pmc.json" % storage)
-    kernel = "k_collide_stream_%s%s" % ("vf_" if virtual else "", storage)
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` (k_<name>_[vf_]<form>) from the
+    committed rocprofv3 --pmc summary of it (null if there is none)."""
+    name = kernel[2:].replace("vf_", "")
+    path = os.path.join(ROOT, "profiles", "%s_256cubed_pmc.json" % name)
     try:
         with open(path) as fh:
             d = json.load(fh)
@@ -300,13 +312,12 @@ def main():
     for form in forms:
         log("timing %s: %d warm-up + %d steps" % (form, a.warmup, a.steps))
         res[form] = time_gpu(case, form, a.steps, a.warmup, a.force_mode)
-        log("%s: %.3f ms/step, %.0f MLUPS, collide %.0f GB/s" % (form, res[form]["ms_per_step"],
-                                                                 res[form]["mlups"], res[form]["collide_gbs"]))
+        log("%s: %.3f ms/step, %.0f MLUPS, %s %.0f GB/s" % (form, res[form]["ms_per_step"], res[form]["mlups"],
+                                                            res[form]["main_kernel"], res[form]["main_gbs"]))
     main_form = "f64" if "f64" in res else forms[0]
     m = res[main_form]
     n = a.size ** 3
-    coll_avg = m["collide_avg_ms"]
-    traffic = load_traffic(main_form, m["virtual_force"])
+    traffic = load_traffic(m["main_kernel"])
     out = {
         "metric": "MLUPS of the coupled step (ZeroForce+AddForces+collide/stream+CalcHydro+Move) at %d^3, "
                   "FP64 drop-in" % a.size,
@@ -325,16 +336,17 @@ def main():
                    if a.size == 256 else "%d^3 reduced" % a.size,
                    "lattice": [a.size] * 3, "swimmers": 32768, "tracers": 1_000_000, "parallelism": "single GPU"},
         "roofline": {
-            "kernel": "k_collide_stream_%s%s" % ("vf_" if m["virtual_force"] else "", main_form),
+            "kernel": m["main_kernel"],
             "bound": "hbm",
-            "achieved": round(m["collide_gbs"], 1),
+            "achieved": round(m["main_gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(m["collide_gbs"] / HBM_PEAK_GBS, 4),
+            "frac": round(m["main_gbs"] / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": m["collide_bytes"] * n,
+            "algorithmic_bytes_per_launch": m["main_bytes"] * n,
             "virtual_force": m["virtual_force"],
-            "avg_launch_ms": round(coll_avg, 4),
+            "fusion": m["fusion"],
+            "avg_launch_ms": round(m["main_avg_ms"], 4),
         },
         "step_hbm_frac": round(m["step_gbs"] / HBM_PEAK_GBS, 4),
         "step_bytes_per_site": m["step_bytes"],
@@ -350,8 +362,8 @@ def main():
         f = res["f32"]
         out["fp32_store"] = {"mlups": round(f["mlups"], 2), "ms_per_step": round(f["ms_per_step"], 4),
                              "speedup_vs_fp64": round(f["mlups"] / m["mlups"], 3),
-                             "collide_gbs": round(f["collide_gbs"], 1),
-                             "collide_frac": round(f["collide_gbs"] / HBM_PEAK_GBS, 4),
+                             "kernel": f["main_kernel"], "kernel_gbs": round(f["main_gbs"], 1),
+                             "kernel_frac": round(f["main_gbs"] / HBM_PEAK_GBS, 4),
                              "step_hbm_frac": round(f["step_gbs"] / HBM_PEAK_GBS, 4),
                              "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in f["kernels"].items()}}
     if a.ensemble_steps > 0:

@@ -159,10 +159,16 @@ struct lbswim_lattice {
   size_t sp_list_cap = 0;
   uint32_t* sp_list = nullptr;
   uint32_t* sp_sorted = nullptr;
+  double* sp_edge = nullptr;  // 6 sp_lists: force at both ends of each list's run (fused step)
   // Virtual force: after a fused step the force lives in the sorted point
   // lists above and d_force is stale until materialize_force writes it.
   int force_mode = LBSWIM_FORCE_AUTO;
   bool force_virtual = false;
+  // Step + CalcHydro fused into one launch in the coupled step (where the
+  // lattice qualifies); rows/strips 0 = chosen from the lattice size
+  bool fuse = true;
+  int fuse_rows = 0;
+  int fuse_strips = 0;
 };
 
 // Device storage of one particle array in slot order (particle_order.hip):
@@ -345,6 +351,8 @@ int lbswim_lattice_create(int nx, int ny, int nz, double tau_s, double tau_b, in
   if (!lat) return fail(LBSWIM_ENOMEM, "lattice allocation failed");
   lat->storage = storage;
   lat->elem = storage == LBSWIM_STORE_F64 ? 8 : 4;
+  // fusion pays off in FP64; in FP32 the two launches are as fast at 256^3
+  lat->fuse = storage == LBSWIM_STORE_F64;
   fill_params(&lat->params, tau_s, tau_b);
   lat->addr.size[0] = nx;
   lat->addr.size[1] = ny;
@@ -389,6 +397,7 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->sp_counts);
   dev_free(lat->sp_off);
   dev_free(lat->sp_cursor);
+  dev_free(lat->sp_edge);
   dev_free(lat->sp_pts);
   dev_free(lat->sp_list);
   dev_free(lat->sp_sorted);
@@ -421,7 +430,7 @@ static int materialize_force(lbswim_lattice* lat) {
 }
 
 static VForce virtual_force(const lbswim_lattice* lat) {
-  return VForce{static_cast<const SpreadPoint*>(lat->sp_pts), lat->sp_off, lat->sp_sorted};
+  return VForce{static_cast<const SpreadPoint*>(lat->sp_pts), lat->sp_off, lat->sp_sorted, lat->sp_edge};
 }
 
 static int lattice_step_impl(lbswim_lattice* lat) {
@@ -460,6 +469,40 @@ static int lattice_hydro_impl(lbswim_lattice* lat) {
                                  stream());
   t.done();
   HIP_TRY(e);
+  return LBSWIM_OK;
+}
+
+// Tile rows of the fused step: the requested value, else the largest of 8, 4,
+// 2 dividing ny (8 runs fastest at 256^3); 0 if the lattice does not qualify.
+static int fusion_rows(const lbswim_lattice* lat) {
+  if (!lat->fuse) return 0;
+  if (lat->fuse_rows) return fused_step_supported(lat->geom, lat->fuse_rows, lat->elem) ? lat->fuse_rows : 0;
+  for (int ty : {8, 4, 2})
+    if (fused_step_supported(lat->geom, ty, lat->elem)) return ty;
+  return 0;
+}
+
+// Step + CalcHydro of the coupled step in one launch (fusion_rows(lat) > 0).
+static int lattice_step_hydro_impl(lbswim_lattice* lat, int rows) {
+  const double ts = lat->params.tau_s, tb = lat->params.tau_b;
+  const VForce vf = virtual_force(lat);
+  const VForce* pvf = lat->force_virtual ? &vf : nullptr;
+  KernelTimer t(lat, LBSWIM_K_FUSED);
+  hipError_t e;
+  if (lat->storage == LBSWIM_STORE_F64)
+    e = launch_step_hydro<double>(static_cast<const double*>(lat->d_f[lat->cur]),
+                                  static_cast<const double*>(lat->d_force), static_cast<double*>(lat->d_f[1 - lat->cur]),
+                                  static_cast<double*>(lat->d_rho), static_cast<double*>(lat->d_u), lat->geom, ts, tb,
+                                  pvf, rows, lat->fuse_strips, stream());
+  else
+    e = launch_step_hydro<float>(static_cast<const float*>(lat->d_f[lat->cur]), static_cast<const float*>(lat->d_force),
+                                 static_cast<float*>(lat->d_f[1 - lat->cur]), static_cast<float*>(lat->d_rho),
+                                 static_cast<float*>(lat->d_u), lat->geom, ts, tb, pvf, rows, lat->fuse_strips,
+                                 stream());
+  t.done();
+  HIP_TRY(e);
+  lat->cur = 1 - lat->cur;
+  lat->time_step++;
   return LBSWIM_OK;
 }
 
@@ -503,6 +546,26 @@ int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual) {
   if (int rc = check_lattice(lat)) return rc;
   CHECK_ARG(is_virtual != nullptr, "is_virtual is NULL");
   *is_virtual = lat->force_virtual ? 1 : 0;
+  return LBSWIM_OK;
+}
+
+int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int strips) {
+  if (int rc = check_lattice(lat)) return rc;
+  CHECK_ARG(rows == 0 || rows == 2 || rows == 4 || rows == 8, "rows must be 0, 2, 4 or 8 (got %d)", rows);
+  CHECK_ARG(strips >= 0, "negative strips");
+  lat->fuse = enable != 0;
+  lat->fuse_rows = rows;
+  lat->fuse_strips = strips;
+  return LBSWIM_OK;
+}
+
+int lbswim_lattice_fusion(lbswim_lattice* lat, int* active, int* rows, int* strips) {
+  if (int rc = check_lattice(lat)) return rc;
+  CHECK_ARG(active != nullptr, "active is NULL");
+  const int ty = fusion_rows(lat);
+  *active = ty > 0 ? 1 : 0;
+  if (rows) *rows = ty;
+  if (strips) *strips = lat->fuse_strips;
   return LBSWIM_OK;
 }
 
@@ -1024,8 +1087,14 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
                           fld(st, LBSWIM_PART_N), st.perm, lat->geom, pts, stream()));
     offset += st.num;
   }
+  // the fused step's z-halo sites read the force at the ends of the runs
+  const bool edges = virt && fusion_rows(lat) > 0;
+  if (edges && !lat->sp_edge)
+    if (int rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_edge), 6 * sizeof(double) * lat->sp_lists))
+      return rc;
   hipError_t e = launch_spread_lists(int(npts), lat->geom, pts, lat->sp_counts, lat->sp_off, lat->sp_cursor,
                                      lat->sp_list, lat->sp_sorted, stream());
+  if (e == hipSuccess && edges) e = launch_edge_force(lat->geom, pts, lat->sp_off, lat->sp_sorted, lat->sp_edge, stream());
   if (e == hipSuccess && !virt) {
     if (lat->storage == LBSWIM_STORE_F64)
       e = launch_spread_force<double>(lat->geom, pts, lat->sp_off, lat->sp_sorted, static_cast<double*>(lat->d_force),
@@ -1053,8 +1122,12 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
   for (int s = 0; s < nsteps; ++s) {
     int rc;
     if ((rc = spread_forces(lat, sw, nsw))) return rc;
-    if ((rc = lattice_step_impl(lat))) return rc;
-    if ((rc = lattice_hydro_impl(lat))) return rc;
+    if (const int rows = fusion_rows(lat)) {
+      if ((rc = lattice_step_hydro_impl(lat, rows))) return rc;
+    } else {
+      if ((rc = lattice_step_impl(lat))) return rc;
+      if ((rc = lattice_hydro_impl(lat))) return rc;
+    }
     for (int i = 0; i < nsw; ++i)
       if ((rc = swimmers_move_impl(sw[i], lat))) return rc;
     for (int i = 0; i < ntr; ++i)

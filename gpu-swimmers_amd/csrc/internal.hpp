@@ -34,6 +34,7 @@ struct VForce {
   const SpreadPoint* pts;
   const uint32_t* off;
   const uint32_t* sorted;
+  const double* edge;  // launch_edge_force output (fused step only)
 };
 // `vf` null: the force is read from `force`.
 template <typename T>
@@ -42,6 +43,15 @@ hipError_t launch_collide_stream(const T* fOld, const T* force, T* fNew, const G
 template <typename T>
 hipError_t launch_calc_hydro(const T* f, const T* force, T* rho, T* u, const Geom& g, const VForce* vf,
                              hipStream_t s);
+// Collide+stream followed by CalcHydro in one launch, with the results of
+// launch_collide_stream + launch_calc_hydro bit for bit, for lattices where
+// fused_step_supported(g, ty, sizeof(T)): nz % 64 == 0 and ny % ty == 0, ty in
+// {2, 4, 8} (rows per tile), and 15 n sizeof(T) < 4 GiB (32-bit buffer
+// offsets).  `strips` splits the x sweep (0: one workgroup per CU).
+bool fused_step_supported(const Geom& g, int ty, size_t elem);
+template <typename T>
+hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u, const Geom& g, double tau_s,
+                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s);
 template <typename T>
 hipError_t launch_init_from_hydro(const T* rho, const T* u, T* f, const Geom& g, hipStream_t s);
 
@@ -94,6 +104,11 @@ hipError_t launch_spread_lists(int npts, const Geom& g, const SpreadPoint* pts, 
 template <typename T>
 hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
                                T* force, hipStream_t s);
+
+// Force at the first and last site of every list's 64-site run:
+// edge[6 l + 3 e + a] (e = 0 first, 1 last), 6 spread_lists(g) doubles.
+hipError_t launch_edge_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
+                             double* edge, hipStream_t s);
 
 // ---- batched launches over the members of an ensemble
 // One lattice of a batch.  g[cur ^ parity] holds the state at the start of a

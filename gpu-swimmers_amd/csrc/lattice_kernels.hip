@@ -34,6 +34,7 @@
 #include "d3q15.hpp"
 #include "internal.hpp"
 #include "spread.hpp"
+#include "xcd.hpp"
 
 namespace lbs {
 
@@ -502,6 +503,368 @@ __global__ __launch_bounds__(BLOCK) void k_init_from_hydro_f32(const float* __re
 }
 
 // ------------------------------------------------------------------------
+// Fused step: collide+stream (lb.cu:72-218) and CalcHydro (lb.cu:250-295) in
+// one pass, for lattices with nz % 64 == 0 and ny % TY == 0.
+//
+// CalcHydro at site x pulls g'_p(x - c_p), the values the collision has just
+// written around x.  A workgroup owns a tile of TY rows (y) by 64 sites (z)
+// and sweeps it along x over a strip of planes [xa, xb).  Row waves 1..TY
+// collide the tile's own sites; row waves 0 and TY+1 (the rows above and
+// below) and the first 2 (TY+2) lanes of one more wave (the sites just before
+// and after each row in z) collide the ring of neighbour sites; planes xa-1
+// and xb are collided as well.  Each collided value that a tile site pulls
+// goes to LDS at that target site's index, in ring slots by c_x, and
+// CalcHydro of plane x-1 runs once plane x is collided: it reads the c_x = -1
+// values of plane x, the c_x = 0 values of x-1 and the c_x = +1 values of
+// x-2.  Neighbour sites are computed by the same code from the same inputs as
+// their owners compute them, so the distributions and rho/u equal the
+// two-kernel results bit for bit, and the distributions are read from HBM
+// once per step instead of twice.
+// ------------------------------------------------------------------------
+
+constexpr int FUSED_Z = 64;  // sites of a tile along z: one wavefront
+
+// Directions grouped by c_x (5 each): group 0 c_x = -1, 1 c_x = 0, 2 c_x = +1,
+// and the index of p within its group.
+__host__ __device__ constexpr int dir_group(int p) { return CI[p][0] + 1; }
+__host__ __device__ constexpr int dir_index(int p) {
+  int k = 0;
+  for (int q = 0; q < p; ++q) k += CI[q][0] == CI[p][0] ? 1 : 0;
+  return k;
+}
+// LDS ring slots: group 0 slot 0 (read in the iteration that writes it),
+// group 1 slots 1-2 (read one iteration later), group 2 slots 3-5 (two later).
+constexpr int FUSED_SLOTS = 6;
+constexpr int FUSED_TRASH = FUSED_SLOTS * 5;  // slot after the ring: unused writes of the z-halo wave
+
+struct FusedTiles {
+  uint32_t nyt, nzt;  // tiles along y and z
+  uint32_t strip;     // planes per strip along x
+  uint32_t nblk;      // tiles x strips
+};
+
+template <typename T>
+__device__ __forceinline__ void collide_site(const T (&f)[Q], const T (&F)[3], T tau_s, T tau_b, T omega_s,
+                                             T omega_b, T (&out)[Q]) {
+  if constexpr (sizeof(T) == 8) {
+    double mode[NKEEP];
+    fwd_all(f, mode, std::make_integer_sequence<int, NKEEP>{});
+    relax_f64(mode, F, tau_s, tau_b, omega_s, omega_b);
+    [&]<int... P>(std::integer_sequence<int, P...>) {
+      ((out[P] = inv<P>(mode)), ...);
+    }(std::make_integer_sequence<int, Q>{});
+  } else {
+    Moments32 m = moments_dev32(f);
+    relax_dev32(m, F, tau_s, tau_b, omega_s, omega_b);
+    inverse_dev32(m, out);
+  }
+}
+
+// rho (FP32: rho - 1) and u of pulled distributions fl, as hydro_*_body.
+template <typename T>
+__device__ __forceinline__ void hydro_site(const T (&fl)[Q], const T (&F)[3], T& rho, T (&u)[3]) {
+  if constexpr (sizeof(T) == 8) {
+    const double m0 = fwd<0>(fl);
+    const double m1 = fwd<1>(fl);
+    const double m2 = fwd<2>(fl);
+    const double m3 = fwd<3>(fl);
+    rho = m0;
+    u[0] = (m1 + 0.5 * F[0]) / m0;
+    u[1] = (m2 + 0.5 * F[1]) / m0;
+    u[2] = (m3 + 0.5 * F[2]) / m0;
+  } else {
+    const Moments32 m = moments_dev32(fl);
+    const float inv_rho = 1.0f / (1.0f + m.drho);
+    rho = m.drho;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) u[a] = __builtin_fmaf(0.5f, F[a], m.j[a]) * inv_rho;
+  }
+}
+
+// Barrier for LDS only: pending global loads (the next plane's pulls) and
+// stores stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Wavefront roles in a fused tile.  Each role runs its own copy of the sweep,
+// so that every load and store inside the loop is unconditional (the wait
+// counts then let the next plane's pulls stay in flight) and the collision
+// computes only the outputs the role stores.
+enum FusedRole { ROLE_TILE, ROLE_ABOVE, ROLE_BELOW, ROLE_ZHALO };
+
+template <typename T>
+struct FusedArgs {
+  const T* gOld;
+  const T* force;
+  VForce vf;
+  T* gNew;
+  T* rho;
+  T* u;
+  Geom g;
+  T tau_s, tau_b, omega_s, omega_b;
+};
+
+// LDS rows hold tile z = -1..64 (66 words), so a row wavefront writes every
+// z-shifted value without a lane test; the two end words are never read.
+constexpr int FUSED_ROW = FUSED_Z + 2;
+
+// Buffer loads and stores for the fused sweep: one resource per array, a
+// wavefront-uniform byte offset (SGPR) and a 32-bit per-lane byte offset, so
+// the 15 pull addresses cost 3 VGPRs.  Needs every offset below 4 GiB
+// (fused_step_supported checks Q n sizeof(T) < 2^32).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, 0xFFFFFFFFu, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+// Non-temporal store (cache policy bit 1: nt).
+template <typename T>
+__device__ __forceinline__ void buffer_store_nt(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, voff, soff, 2);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 2);
+}
+
+// One wavefront's sweep of the tile.  r: tile row (0..TY+1; uniform except in
+// the z-halo wave), zz: tile z (-1..64), (y, z) the lattice row and z.
+template <typename T, bool VF, int TY, int ROLE>
+__device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict__ lds, int xa, int last, int r,
+                                            int zz, bool active, int y, int z) {
+  constexpr int PLANE = TY * FUSED_ROW;  // LDS words of one direction slot
+  constexpr bool TILE = ROLE == ROLE_TILE;
+  const Geom& g = A.g;
+  const uint32_t n = g.n, nz = uint32_t(g.nz), nyz = g.nyz;
+  const int lane = int(threadIdx.x) % 64;
+  constexpr uint32_t B = sizeof(T);
+  const __amdgpu_buffer_rsrc_t rOld = buffer_rsrc(A.gOld), rNew = buffer_rsrc(A.gNew);
+  // row offsets: uniform in row wavefronts (part of the SGPR offset), per lane
+  // in the z-halo wavefront (part of the lane offset)
+  uint32_t oy[3], oz[3];
+  oy[0] = uint32_t(y == 0 ? g.ny - 1 : y - 1) * nz;
+  oy[1] = uint32_t(y) * nz;
+  oy[2] = uint32_t(y + 1 == g.ny ? 0 : y + 1) * nz;
+  oz[0] = uint32_t(z == 0 ? g.nz - 1 : z - 1);
+  oz[1] = uint32_t(z);
+  oz[2] = uint32_t(z + 1 == g.nz ? 0 : z + 1);
+  constexpr bool ZH = ROLE == ROLE_ZHALO;
+  // site = srow (x, y and the tile's first z: uniform) + zl (lane part)
+  const uint32_t zrow = ZH ? 0u : uint32_t(z - lane);
+  const uint32_t zl = ZH ? oy[1] + oz[1] : uint32_t(lane);
+  const uint32_t yrow = ZH ? 0u : oy[1];
+  auto plane = [&](int it) {
+    int x = (xa - 1 + it) % g.nx;  // the list pipeline looks up to two planes past the strip
+    return x < 0 ? x + g.nx : x;
+  };
+  // Virtual force of a row wavefront: the row's list for plane it is walked
+  // lane-parallel (lanes_row_force) from entries loaded one plane ahead --
+  // list bounds two planes ahead (offv), ids one plane ahead (idv), then the
+  // point records (rec) -- so the dependent loads overlap the previous
+  // plane's work instead of stalling the wavefront.
+  constexpr bool VFROW = VF && !ZH;
+  const bool simple = g.nx >= 4 && g.ny >= 4;
+  uint32_t offv = 0, idv = 0, o0 = 0, m0 = 0, o1 = 0, m1 = 0;
+  LaneEntry rec{};
+  auto list_of = [&](int it) { return (uint32_t(plane(it)) * nyz + yrow + zrow) / WAVE_SITES; };
+  auto load_off = [&](int it) { offv = A.vf.off[list_of(it) + uint32_t(lane & 1)]; };
+  auto take_list = [&](uint32_t& o, uint32_t& m) {
+    o = uint32_t(__builtin_amdgcn_readlane(int(offv), 0));
+    m = uint32_t(__builtin_amdgcn_readlane(int(offv), 1)) - o;
+    idv = A.vf.sorted[uint32_t(lane) < m ? o + uint32_t(lane) : 0u];
+  };
+  // pulls of plane it (and its stored or run-end force)
+  auto pull = [&](int it, T(&f)[Q], T(&sf)[3]) {
+    const int x = plane(it);
+    const uint32_t ox[3] = {uint32_t(x == 0 ? g.nx - 1 : x - 1) * nyz, uint32_t(x) * nyz,
+                            uint32_t(x + 1 == g.nx ? 0 : x + 1) * nyz};
+    [&]<int... P>(std::integer_sequence<int, P...>) {
+      ((f[P] = ZH ? buffer_load<T>(rOld, (oy[1 - CI[P][1]] + oz[1 - CI[P][2]]) * B,
+                                    (uint32_t(P) * n + ox[1 - CI[P][0]]) * B)
+                  : buffer_load<T>(rOld, oz[1 - CI[P][2]] * B,
+                                    (uint32_t(P) * n + ox[1 - CI[P][0]] + oy[1 - CI[P][1]]) * B)),
+       ...);
+    }(std::make_integer_sequence<int, Q>{});
+    if constexpr (!VF) {
+      const __amdgpu_buffer_rsrc_t rF = buffer_rsrc(A.force);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) sf[a] = buffer_load<T>(rF, zl * B, (uint32_t(a) * n + ox[1] + yrow + zrow) * B);
+    } else if constexpr (ZH) {
+      // z-halo sites are the last (z % 64 == 63) or first site of a run
+      const uint32_t site = ox[1] + zl;
+      const uint32_t e = 2 * (site / WAVE_SITES) + (site % WAVE_SITES == WAVE_SITES - 1 ? 1 : 0);
+      const __amdgpu_buffer_rsrc_t rE = buffer_rsrc(A.vf.edge);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) sf[a] = T(buffer_load<double>(rE, (3 * e + uint32_t(a)) * 8u, 0));
+    }
+  };
+  // everything plane it needs, issued while plane it-1 is processed: list
+  // bounds of plane it+1, the pulls, and the list records of plane it (from
+  // the ids take_list loaded one plane earlier)
+  auto load = [&](int it, T(&f)[Q], T(&sf)[3]) {
+    if constexpr (VFROW) load_off(it + 1);
+    pull(it, f, sf);
+    if constexpr (VFROW) load_entry(A.vf.pts, idv, rec);
+  };
+  // force at this lane's site on plane it
+  auto force = [&](int it, const T(&sf)[3], T(&F)[3]) {
+    if constexpr (VFROW) {
+      const int x = plane(it);
+      double acc[3];
+      if (simple)
+        lanes_row_force(rec, o0, m0, A.vf.pts, A.vf.sorted, x, y, z, g, acc);
+      else
+        wave_force(A.vf.pts, A.vf.off, A.vf.sorted, list_of(it), true, x, y, z, g, acc);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) F[a] = T(acc[a]);
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) F[a] = sf[a];
+    }
+  };
+  T Fh[3] = {T(0), T(0), T(0)};  // force at this site on the previous plane
+  // STORE: write the distributions (tile planes); HYDRO: CalcHydro of plane it-1
+  auto process = [&]<bool STORE, bool HYDRO>(int it, const T(&f)[Q], const T(&F)[3]) {
+    const int x = plane(it);
+    const uint32_t srow = uint32_t(x) * nyz + yrow + zrow;
+    T out[Q];
+    collide_site(f, F, A.tau_s, A.tau_b, A.omega_s, A.omega_b, out);
+    uint32_t o2 = 0, m2 = 0;
+    if constexpr (VFROW) take_list(o2, m2);  // plane it+2 (bounds loaded with plane it+1)
+    if constexpr (STORE) {
+#pragma unroll
+      for (int p = 0; p < Q; ++p) buffer_store_nt(out[p], rNew, zl * B, (uint32_t(p) * n + srow) * B);
+    }
+    const int w1 = 1 + it % 2, w2 = 3 + it % 3;
+    [&]<int... P>(std::integer_sequence<int, P...>) {
+      (
+          [&] {
+            constexpr int cy = CI[P][1], cz = CI[P][2];
+            if constexpr (ROLE == ROLE_ZHALO && cz == 0) return;
+            if constexpr (ROLE == ROLE_ABOVE && cy != 1) return;
+            if constexpr (ROLE == ROLE_BELOW && cy != -1) return;
+            constexpr int grp = dir_group(P), k = dir_index(P);
+            const int slot = grp == 0 ? 0 : (grp == 1 ? w1 : w2);
+            const int tr = r + cy;  // target tile row
+            const int at = (slot * 5 + k) * PLANE + (tr - 1) * FUSED_ROW + zz + cz + 1;
+            if constexpr (ROLE == ROLE_ZHALO) {
+              const bool ok = active && tr >= 1 && tr <= TY && zz + cz >= 0 && zz + cz < FUSED_Z;
+              lds[ok ? at : FUSED_TRASH * PLANE + lane] = out[P];
+            } else if constexpr (ROLE == ROLE_TILE) {
+              if (tr >= 1 && tr <= TY) lds[at] = out[P];  // uniform test
+            } else {
+              lds[at] = out[P];
+            }
+          }(),
+          ...);
+    }(std::make_integer_sequence<int, Q>{});
+    lds_barrier();
+    if constexpr (HYDRO) {
+      const int r1 = 1 + (it + 1) % 2, r2 = 3 + (it + 1) % 3;
+      T fl[Q];
+      [&]<int... P>(std::integer_sequence<int, P...>) {
+        ((fl[P] = lds[((dir_group(P) == 0 ? 0 : (dir_group(P) == 1 ? r1 : r2)) * 5 + dir_index(P)) * PLANE +
+                      (r - 1) * FUSED_ROW + lane + 1]),
+         ...);
+      }(std::make_integer_sequence<int, Q>{});
+      T rh, uh[3];
+      hydro_site(fl, Fh, rh, uh);
+      const uint32_t sp = uint32_t(plane(it - 1)) * nyz + yrow + zrow;
+      buffer_store_nt(rh, buffer_rsrc(A.rho), zl * B, sp * B);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) buffer_store_nt(uh[a], buffer_rsrc(A.u), zl * B, (uint32_t(a) * n + sp) * B);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) Fh[a] = F[a];
+    o0 = o1;
+    m0 = m1;
+    o1 = o2;
+    m1 = m2;
+    lds_barrier();
+  };
+  // Iterations: 0 (plane xa-1, halo), 1 (xa: stored, no CalcHydro yet),
+  // 2..last-1 (stored + CalcHydro of the plane before), last (plane xb, halo:
+  // CalcHydro of xb-1 only).  last >= 2.  Two register sets: the next plane's
+  // pulls are in flight while this plane is collided.
+  T fa[Q], fb[Q], sa[3] = {}, sb[3] = {}, F[3];
+  if constexpr (VFROW) {  // prime the list pipeline: plane 0 records, plane 1 ids
+    load_off(0);
+    take_list(o0, m0);
+    load_entry(A.vf.pts, idv, rec);
+    load_off(1);
+    take_list(o1, m1);
+  }
+  pull(0, fa, sa);
+  force(0, sa, F);
+  load(1, fb, sb);
+  process.template operator()<false, false>(0, fa, F);
+  force(1, sb, F);
+  load(2, fa, sa);
+  process.template operator()<TILE, false>(1, fb, F);
+  int it = 2;
+  for (; it + 1 < last; it += 2) {
+    force(it, sa, F);
+    load(it + 1, fb, sb);
+    process.template operator()<TILE, TILE>(it, fa, F);
+    force(it + 1, sb, F);
+    load(it + 2, fa, sa);
+    process.template operator()<TILE, TILE>(it + 1, fb, F);
+  }
+  if (it + 1 == last) {
+    force(it, sa, F);
+    load(last, fb, sb);
+    process.template operator()<TILE, TILE>(it, fa, F);
+    force(last, sb, F);
+    process.template operator()<false, TILE>(last, fb, F);
+  } else {
+    force(last, sa, F);
+    process.template operator()<false, TILE>(last, fa, F);
+  }
+}
+
+template <typename T, bool VF, int TY>
+__global__ __launch_bounds__((TY + 3) * 64) void k_step_hydro(FusedArgs<T> A, FusedTiles t) {
+  constexpr int ROWS = TY + 2;
+  __shared__ T lds[FUSED_TRASH * TY * FUSED_ROW + 64];
+  const uint32_t L = xcd_block(round_xcd(t.nblk));
+  if (L >= t.nblk) return;
+  const Geom& g = A.g;
+  // tiles along y are adjacent in L, so one XCD's L2 serves the row halos
+  const uint32_t yt = L % t.nyt, rest = L / t.nyt;
+  const uint32_t zt = rest % t.nzt, sx = rest / t.nzt;
+  const int xa = int(sx * t.strip), xb = min(xa + int(t.strip), g.nx);
+  const int last = xb - xa + 1;  // iterations 0..last collide planes xa-1..xb
+  const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64), lane = int(threadIdx.x) % 64;
+  // this lane's position: tile row r (0 and ROWS-1 are halo rows) and tile z
+  // index zz (-1 and 64 are halo sites); spare lanes of the z-halo wave take
+  // a valid position and write nothing
+  auto wrap = [](int v, int size) { return v < 0 ? v + size : (v >= size ? v - size : v); };
+  if (wv == ROWS) {
+    const bool active = lane < 2 * ROWS;
+    const int r = active ? lane >> 1 : 0;
+    const int zz = (active && (lane & 1)) ? FUSED_Z : -1;
+    const int y = wrap(int(yt) * TY + r - 1, g.ny), z = wrap(int(zt) * FUSED_Z + zz, g.nz);
+    fused_sweep<T, VF, TY, ROLE_ZHALO>(A, lds, xa, last, r, zz, active, y, z);
+    return;
+  }
+  const int y = wrap(int(yt) * TY + wv - 1, g.ny), z = int(zt) * FUSED_Z + lane;
+  if (wv == 0)
+    fused_sweep<T, VF, TY, ROLE_ABOVE>(A, lds, xa, last, wv, lane, true, y, z);
+  else if (wv == ROWS - 1)
+    fused_sweep<T, VF, TY, ROLE_BELOW>(A, lds, xa, last, wv, lane, true, y, z);
+  else
+    fused_sweep<T, VF, TY, ROLE_TILE>(A, lds, xa, last, wv, lane, true, y, z);
+}
+
+// ------------------------------------------------------------------------
 // Launch wrappers: one lattice, or a batch of lattices described by LatDesc
 // entries with one WorkItem (member, block of that member) per workgroup.
 // ------------------------------------------------------------------------
@@ -700,6 +1063,66 @@ hipError_t launch_collide_stream<float>(const float* fOld, const float* force, f
                        float(tau_s), float(tau_b), os, ob);
   return hipGetLastError();
 }
+
+bool fused_step_supported(const Geom& g, int ty, size_t elem) {
+  return g.n > 0 && g.nz % FUSED_Z == 0 && (ty == 2 || ty == 4 || ty == 8) && g.ny % ty == 0 &&
+         uint64_t(Q) * g.n * elem < (uint64_t(1) << 32);
+}
+
+template <typename T>
+hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u, const Geom& g, double tau_s,
+                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s) {
+  if (!fused_step_supported(g, ty, sizeof(T)) || strips < 0) return hipErrorInvalidValue;
+  FusedTiles t;
+  t.nyt = uint32_t(g.ny / ty);
+  t.nzt = uint32_t(g.nz / FUSED_Z);
+  const uint32_t tiles = t.nyt * t.nzt;
+  if (strips == 0) {
+    // one workgroup per CU (the tile's LDS and VGPRs allow one): enough
+    // strips for every CU to have a tile, each as long as possible
+    static int cus = 0;
+    if (!cus) {
+      int dev = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    }
+    strips = int(std::min<uint32_t>(uint32_t(g.nx), std::max<uint32_t>(1, (uint32_t(cus) + tiles - 1) / tiles)));
+  }
+  strips = std::min(strips, g.nx);
+  t.strip = uint32_t((g.nx + strips - 1) / strips);
+  t.nblk = tiles * uint32_t((g.nx + int(t.strip) - 1) / int(t.strip));
+  FusedArgs<T> A;
+  A.gOld = fOld;
+  A.force = force;
+  A.vf = vf ? *vf : VForce{nullptr, nullptr, nullptr, nullptr};
+  if (vf && !vf->edge) return hipErrorInvalidValue;
+  A.gNew = fNew;
+  A.rho = rho;
+  A.u = u;
+  A.g = g;
+  A.tau_s = T(tau_s);
+  A.tau_b = T(tau_b);
+  A.omega_s = T(1.0 / (tau_s + 0.5));
+  A.omega_b = T(1.0 / (tau_b + 0.5));
+  const dim3 grid(round_xcd(t.nblk));
+#define LBS_FUSED(TYV)                                                                                       \
+  if (ty == TYV) {                                                                                           \
+    if (vf)                                                                                                  \
+      hipLaunchKernelGGL((k_step_hydro<T, true, TYV>), grid, dim3((TYV + 3) * 64), 0, s, A, t);              \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_step_hydro<T, false, TYV>), grid, dim3((TYV + 3) * 64), 0, s, A, t);             \
+  }
+  LBS_FUSED(2)
+  LBS_FUSED(4)
+  LBS_FUSED(8)
+#undef LBS_FUSED
+  return hipGetLastError();
+}
+template hipError_t launch_step_hydro<double>(const double*, const double*, double*, double*, double*, const Geom&,
+                                              double, double, const VForce*, int, int, hipStream_t);
+template hipError_t launch_step_hydro<float>(const float*, const float*, float*, float*, float*, const Geom&, double,
+                                             double, const VForce*, int, int, hipStream_t);
 
 template <>
 hipError_t launch_calc_hydro<double>(const double* f, const double* force, double* rho, double* u, const Geom& g,

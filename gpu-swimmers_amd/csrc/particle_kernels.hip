@@ -14,6 +14,7 @@
 #include "d3q15.hpp"
 #include "internal.hpp"
 #include "spread.hpp"
+#include "xcd.hpp"
 
 namespace lbs {
 namespace {
@@ -395,17 +396,9 @@ __device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r
   sd[size_t(nt) * d + it] = s_old + vel;
 }
 
-// Workgroups are handed to the 8 XCDs (each with its own L2) round-robin by
-// workgroup id.  Particles are stored in spatial order, so each XCD is given
-// one contiguous range of the `nb` particle blocks instead: launched with
-// round_xcd(nb) workgroups, workgroup b takes block (b % 8) * round_xcd(nb)/8
-// + b / 8 (a block >= nb means nothing to do), and neighbouring particles
+// Particles are stored in spatial order, so the move kernels give each XCD one
+// contiguous range of particle blocks (xcd_block): neighbouring particles
 // share their stencils' lines in one L2.
-constexpr uint32_t XCDS = 8;
-__host__ __device__ inline uint32_t round_xcd(uint32_t nb) { return (nb + XCDS - 1) / XCDS * XCDS; }
-__device__ __forceinline__ uint32_t xcd_block(uint32_t nb8) {
-  return (blockIdx.x % XCDS) * (nb8 / XCDS) + blockIdx.x / XCDS;
-}
 
 template <typename T>
 __global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp, double* __restrict__ r,
@@ -748,6 +741,28 @@ __global__ __launch_bounds__(SITE_BLOCK) void k_spread_blocks(Geom g, const Spre
   spread_block_body(g, pts, off, sorted, force, blockIdx.x, blockIdx.x * LISTS_PER_BLOCK, stage);
 }
 
+// Force at the first and the last site of every 64-site run (edge[6 l + 3 e
+// + a]: list l, e = 0 first, 1 last), the sum wave_force gives there.  The
+// fused step's z-halo sites are such sites of neighbouring runs; they load
+// it instead of walking one list per lane.
+__global__ __launch_bounds__(SITE_BLOCK) void k_edge_force(Geom g, uint32_t nlists,
+                                                          const SpreadPoint* __restrict__ pts,
+                                                          const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ sorted,
+                                                          double* __restrict__ edge) {
+  const uint32_t i = blockIdx.x * SITE_BLOCK + threadIdx.x;
+  if (i >= 2 * nlists) return;
+  const uint32_t l = i >> 1;
+  const uint32_t s = l * WAVE_SITES + ((i & 1) ? WAVE_SITES - 1 : 0);
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (s < g.n) {
+    const SiteXYZ c = site_xyz(s, g);
+    lane_force(pts, off, sorted, l, c.x, c.y, c.z, g, acc);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) edge[3 * size_t(i) + a] = acc[a];
+}
+
 template <typename T>
 __global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __restrict__ ldesc,
                                                               const WorkItem* __restrict__ block_work,
@@ -889,6 +904,15 @@ template hipError_t launch_spread_force<double>(const Geom&, const SpreadPoint*,
                                                 double*, hipStream_t);
 template hipError_t launch_spread_force<float>(const Geom&, const SpreadPoint*, const uint32_t*, const uint32_t*,
                                                float*, hipStream_t);
+
+hipError_t launch_edge_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
+                             double* edge, hipStream_t s) {
+  const uint32_t nl = spread_lists(g);
+  if (nl == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_edge_force, dim3((2 * nl + SITE_BLOCK - 1) / SITE_BLOCK), dim3(SITE_BLOCK), 0, s, g, nl, pts,
+                     off, sorted, edge);
+  return hipGetLastError();
+}
 
 hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
                                SpreadPoint* pts, hipStream_t s) {

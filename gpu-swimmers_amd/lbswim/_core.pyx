@@ -71,6 +71,8 @@ cdef extern from "lbswim.h":
     int lbswim_lattice_set_force_mode(lbswim_lattice* lat, int mode)
     int lbswim_lattice_sync_force(lbswim_lattice* lat)
     int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual)
+    int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int strips)
+    int lbswim_lattice_fusion(lbswim_lattice* lat, int* active, int* rows, int* strips)
     int lbswim_lattice_params(lbswim_lattice* lat, const lbswim_params** host)
     int lbswim_lattice_addressing(lbswim_lattice* lat, const lbswim_addressing** host)
     int lbswim_lattice_get_time_step(lbswim_lattice* lat, int* out)
@@ -138,7 +140,7 @@ cdef int check(int rc) except -1:
 
 STORAGE = {"f64": LBSWIM_STORE_F64, "f32": LBSWIM_STORE_F32DEV}
 KERNELS = ("collide_stream", "calc_hydro", "zero_force", "add_forces", "swimmers_move", "tracers_move",
-           "init_from_hydro")
+           "init_from_hydro", "step_hydro")
 
 
 cdef object wrap_view(object owner, void* data, int ndim, cnp.npy_intp* shape, int typenum):
@@ -402,6 +404,20 @@ cdef class Lattice:
     def sync_force(self):
         """Write the force array if the last coupled step left it virtual."""
         check(lbswim_lattice_sync_force(self.handle()))
+
+    def set_fusion(self, enable=True, int rows=0, int strips=0):
+        """Fuse Step and CalcHydro of the coupled step into one launch where the
+        lattice allows it (nz % 64 == 0, ny % rows == 0); rows (2, 4, 8) and
+        strips 0 choose the tile shape from the lattice.  On by default for
+        FP64 storage, off for FP32."""
+        check(lbswim_lattice_set_fusion(self.handle(), 1 if enable else 0, rows, strips))
+
+    @property
+    def fusion(self):
+        """(active, rows, strips) of the coupled step's fused Step + CalcHydro."""
+        cdef int a = 0, r = 0, s = 0
+        check(lbswim_lattice_fusion(self.handle(), &a, &r, &s))
+        return bool(a), r, s
 
     @property
     def force_is_virtual(self):

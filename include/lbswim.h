@@ -76,7 +76,8 @@ extern "C" {
 #define LBSWIM_K_SMOVE 4   /* DoSwimmerArrayMove */
 #define LBSWIM_K_TMOVE 5   /* DoTracerArrayMove */
 #define LBSWIM_K_INIT 6    /* DoInitFromHydro */
-#define LBSWIM_K_COUNT 7
+#define LBSWIM_K_FUSED 7   /* DoStep + DoCalcHydro in one launch (coupled step) */
+#define LBSWIM_K_COUNT 8
 
 /* Same members, order and types as LBParams (lbswim/liblbswim/LBParams.h:4-30). */
 typedef struct lbswim_params {
@@ -225,6 +226,17 @@ int lbswim_lattice_sync_force(lbswim_lattice* lat);
 /* *is_virtual = 1 if the last coupled step left the force virtual (not yet
  * written to the force array). No reference counterpart. */
 int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual);
+/* Step fusion in lbswim_system_step: with `enable` (the default for FP64
+ * storage; FP32 storage runs the two launches unless enabled) and a
+ * lattice with nz % 64 == 0, ny % rows == 0 and 15 n sizeof(storage) < 4 GiB,
+ * Step and CalcHydro run as one launch that reads the distributions once,
+ * with results identical bit for bit to the two launches.  rows: lattice
+ * rows per tile (2, 4 or 8; 0 chooses); strips: x sweeps per tile column (0
+ * chooses).  No reference counterpart. */
+int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int strips);
+/* *active = 1 if the coupled step of this lattice runs fused; *rows and
+ * *strips (either may be NULL) receive the tile shape it uses. */
+int lbswim_lattice_fusion(lbswim_lattice* lat, int* active, int* rows, int* strips);
 
 /* ---- ensembles (no reference counterpart) ------------------------------
  * Many independent systems -- each a lattice with its swimmer and tracer

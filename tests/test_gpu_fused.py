@@ -1,0 +1,103 @@
+"""GPU parity of the fused coupled step: Step + CalcHydro in one launch.
+
+Where a lattice qualifies (nz % 64 == 0, ny % rows == 0) lbswim_system_step
+collides and computes rho/u in one sweep over tiles of rows with a ring of
+neighbour sites recomputed.  Its distributions, rho, u, force and particle
+state must equal the two-launch path bit for bit -- for every tile shape
+(rows, strips), lattices thinner than the halo (nx = 1, ny = rows), both
+storage forms and both force modes -- and the FP64 form must equal the CPU
+oracle bit for bit.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import lbswim_oracle as O
+from cases import build_lbswim, collect_lbswim, make_case, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def oracle_threads():
+    O.set_threads(min(16, os.cpu_count() or 1))
+
+
+def run(case, storage, force_mode, fusion, steps):
+    lat, system, sw, tr = build_lbswim(case, storage, force_mode)
+    if fusion is None:
+        lat.set_fusion(False)
+    else:
+        lat.set_fusion(True, *fusion)
+    active, rows, _ = lat.fusion
+    assert active == (fusion is not None), lat.fusion
+    if fusion is not None and fusion[0]:
+        assert rows == fusion[0]
+    system.Step(steps)
+    return collect_lbswim(lat, sw, tr)
+
+
+def swimmer_case(size, seed, ns=60, nt=300):
+    case = make_case(size, 0, 0.3, 0.7, ns=ns, nt=nt, alpha=0.0, seed=seed, drho=1e-2, du=1e-2, P=1e-2, l=0.6)
+    # swimmers straddling the periodic faces and the 64-site tile boundaries
+    L = np.array(size, float)
+    case.sw_r[:, 0] = [0.05, 0.1, 0.2]
+    case.sw_r[:, 1] = L - 0.05
+    case.sw_r[:, 2] = [L[0] / 2, L[1] - 0.5, 63.7 % L[2]]
+    case.sw_r[:, 3] = [0.5, L[1] / 2, 64.2 % L[2]]
+    return case
+
+
+SHAPES = [
+    # (size, rows, strips)
+    ((4, 8, 64), 4, 0),
+    ((4, 8, 64), 8, 3),
+    ((3, 4, 128), 2, 1),
+    ((3, 4, 128), 4, 2),
+    ((1, 2, 64), 2, 0),     # one plane; the halo rows are the tile's own rows
+    ((2, 6, 64), 2, 2),
+    ((7, 16, 192), 8, 0),
+    ((5, 12, 64), 4, 5),    # one plane per strip
+]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("force_mode", ["stored", "virtual"])
+@pytest.mark.parametrize("size,rows,strips", SHAPES)
+def test_fused_equals_two_launches(size, rows, strips, force_mode, storage):
+    case = swimmer_case(size, seed=size[0] * 100 + rows)
+    say("fused vs two launches: %s rows=%d strips=%d force %s %s" % (size, rows, strips, force_mode, storage))
+    a = run(case, storage, force_mode, None, 4)
+    b = run(case, storage, force_mode, (rows, strips), 4)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("force_mode", ["stored", "virtual"])
+def test_fused_equals_oracle(force_mode):
+    case = make_case((6, 8, 128), 5, 0.25, ns=40, nt=500, alpha=0.0, seed=21, drho=1e-2, du=1e-2, P=1e-2, l=0.5)
+    say("fused vs oracle: (6, 8, 128), 5 steps, force %s" % force_mode)
+    ref, _ = run_oracle(O, case)
+    got = run(case, "f64", force_mode, (0, 0), case.steps)
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), k
+
+
+def test_fusion_skipped_where_unsupported():
+    """nz not a multiple of 64, or ny not a multiple of the requested rows:
+    the coupled step runs the two launches."""
+    from lbswim import lb
+    lat = lb.Lattice(8, 8, 48, 0.25, 0.25)
+    assert lat.fusion[0] is False
+    lat = lb.Lattice(8, 6, 64, 0.25, 0.25)
+    lat.set_fusion(True, 4, 0)
+    assert lat.fusion[0] is False
+    lat.set_fusion(True, 2, 0)
+    assert lat.fusion == (True, 2, 0)

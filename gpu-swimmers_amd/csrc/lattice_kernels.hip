@@ -666,19 +666,28 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     return x < 0 ? x + g.nx : x;
   };
   // Virtual force of a row wavefront: the row's list for plane it is walked
-  // lane-parallel (lanes_row_force) from entries loaded one plane ahead --
-  // list bounds two planes ahead (offv), ids one plane ahead (idv), then the
-  // point records (rec) -- so the dependent loads overlap the previous
-  // plane's work instead of stalling the wavefront.
+  // lane-parallel (lanes_row_force) from entries loaded ahead -- list bounds
+  // (offv) two planes before, ids (idv) and then the point records (rec)
+  // during the previous plane -- so each dependent load has at least half a
+  // plane's work to complete behind.
   constexpr bool VFROW = VF && !ZH;
   const bool simple = g.nx >= 4 && g.ny >= 4;
-  uint32_t offv = 0, idv = 0, o0 = 0, m0 = 0, o1 = 0, m1 = 0;
+  // offv[p & 1]: bounds of plane p (lane 0 first entry, lane 1 end), loaded
+  // two iterations before take_list reads them
+  uint32_t offv[2] = {0, 0}, idv = 0, o0 = 0, m0 = 0, o1 = 0, m1 = 0;
   LaneEntry rec{};
   auto list_of = [&](int it) { return (uint32_t(plane(it)) * nyz + yrow + zrow) / WAVE_SITES; };
-  auto load_off = [&](int it) { offv = A.vf.off[list_of(it) + uint32_t(lane & 1)]; };
-  auto take_list = [&](uint32_t& o, uint32_t& m) {
-    o = uint32_t(__builtin_amdgcn_readlane(int(offv), 0));
-    m = uint32_t(__builtin_amdgcn_readlane(int(offv), 1)) - o;
+  auto load_off = [&](int it) {
+    const uint32_t v = A.vf.off[list_of(it) + uint32_t(lane & 1)];
+    if (it & 1)
+      offv[1] = v;
+    else
+      offv[0] = v;
+  };
+  auto take_list = [&](int it, uint32_t& o, uint32_t& m) {
+    const uint32_t v = (it & 1) ? offv[1] : offv[0];
+    o = uint32_t(__builtin_amdgcn_readlane(int(v), 0));
+    m = uint32_t(__builtin_amdgcn_readlane(int(v), 1)) - o;
     idv = A.vf.sorted[uint32_t(lane) < m ? o + uint32_t(lane) : 0u];
   };
   // pulls of plane it (and its stored or run-end force)
@@ -706,13 +715,12 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
       for (int a = 0; a < 3; ++a) sf[a] = T(buffer_load<double>(rE, (3 * e + uint32_t(a)) * 8u, 0));
     }
   };
-  // everything plane it needs, issued while plane it-1 is processed: list
-  // bounds of plane it+1, the pulls, and the list records of plane it (from
-  // the ids take_list loaded one plane earlier)
+  // issued before plane it-1 is collided: list bounds of plane it+2 and the
+  // pulls of plane it (the list records of plane it follow the collision,
+  // when its registers are free)
   auto load = [&](int it, T(&f)[Q], T(&sf)[3]) {
-    if constexpr (VFROW) load_off(it + 1);
+    if constexpr (VFROW) load_off(it + 2);
     pull(it, f, sf);
-    if constexpr (VFROW) load_entry(A.vf.pts, idv, rec);
   };
   // force at this lane's site on plane it
   auto force = [&](int it, const T(&sf)[3], T(&F)[3]) {
@@ -738,7 +746,10 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     T out[Q];
     collide_site(f, F, A.tau_s, A.tau_b, A.omega_s, A.omega_b, out);
     uint32_t o2 = 0, m2 = 0;
-    if constexpr (VFROW) take_list(o2, m2);  // plane it+2 (bounds loaded with plane it+1)
+    if constexpr (VFROW) {
+      load_entry(A.vf.pts, idv, rec);  // records of plane it+1
+      take_list(it + 2, o2, m2);       // ids of plane it+2, from bounds loaded with plane it
+    }
     if constexpr (STORE) {
 #pragma unroll
       for (int p = 0; p < Q; ++p) buffer_store_nt(out[p], rNew, zl * B, (uint32_t(p) * n + srow) * B);
@@ -795,12 +806,13 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
   // CalcHydro of xb-1 only).  last >= 2.  Two register sets: the next plane's
   // pulls are in flight while this plane is collided.
   T fa[Q], fb[Q], sa[3] = {}, sb[3] = {}, F[3];
-  if constexpr (VFROW) {  // prime the list pipeline: plane 0 records, plane 1 ids
+  if constexpr (VFROW) {  // prime the list pipeline: plane 0 records, plane 1 ids, plane 2 bounds
     load_off(0);
-    take_list(o0, m0);
+    take_list(0, o0, m0);
     load_entry(A.vf.pts, idv, rec);
     load_off(1);
-    take_list(o1, m1);
+    take_list(1, o1, m1);
+    load_off(2);
   }
   pull(0, fa, sa);
   force(0, sa, F);

@@ -74,11 +74,12 @@ def make_workload(size, seed=1234):
     return make_case((size, size, size), 0, 0.25, ns=32768, nt=1_000_000, alpha=0.01, seed=seed)
 
 
-def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200):
+def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, fusion=True):
     from cases import build_lbswim
     import lbswim
 
     lat, system, sw, tr = build_lbswim(case, storage, force_mode)
+    lat.set_fusion(fusion)
     fused = lat.fusion[0]
     main = "step_hydro" if fused else "collide_stream"  # the dominant lattice kernel
     system.Step(warmup)
@@ -285,6 +286,8 @@ def main():
     ap.add_argument("--forms", default="f64,f32")
     ap.add_argument("--force-mode", default="auto", choices=["auto", "stored", "virtual"],
                     help="swimmer force of the coupled step: virtual where sparse (auto), stored, virtual")
+    ap.add_argument("--fusion", default="on", choices=["on", "off"],
+                    help="collide/stream and CalcHydro as one launch where the lattice allows (on) or two launches")
     ap.add_argument("--probe", choices=["copy"], help="run the streaming-copy bandwidth probe instead")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-sim ensemble leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving leg (0: skip)")
@@ -311,7 +314,7 @@ def main():
     res = {}
     for form in forms:
         log("timing %s: %d warm-up + %d steps" % (form, a.warmup, a.steps))
-        res[form] = time_gpu(case, form, a.steps, a.warmup, a.force_mode)
+        res[form] = time_gpu(case, form, a.steps, a.warmup, a.force_mode, fusion=a.fusion == "on")
         log("%s: %.3f ms/step, %.0f MLUPS, %s %.0f GB/s" % (form, res[form]["ms_per_step"], res[form]["mlups"],
                                                             res[form]["main_kernel"], res[form]["main_gbs"]))
     main_form = "f64" if "f64" in res else forms[0]

@@ -351,8 +351,6 @@ int lbswim_lattice_create(int nx, int ny, int nz, double tau_s, double tau_b, in
   if (!lat) return fail(LBSWIM_ENOMEM, "lattice allocation failed");
   lat->storage = storage;
   lat->elem = storage == LBSWIM_STORE_F64 ? 8 : 4;
-  // fusion pays off in FP64; in FP32 the two launches are as fast at 256^3
-  lat->fuse = storage == LBSWIM_STORE_F64;
   fill_params(&lat->params, tau_s, tau_b);
   lat->addr.size[0] = nx;
   lat->addr.size[1] = ny;

@@ -657,8 +657,15 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
   oz[1] = uint32_t(z);
   oz[2] = uint32_t(z + 1 == g.nz ? 0 : z + 1);
   constexpr bool ZH = ROLE == ROLE_ZHALO;
+  // Row wavefronts: y and the tile's first z are wavefront-uniform, and the
+  // buffer instructions need them in SGPRs (a VGPR offset would be looped
+  // over lane by lane).
+  if constexpr (!ZH) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) oy[i] = uint32_t(__builtin_amdgcn_readfirstlane(int(oy[i])));
+  }
   // site = srow (x, y and the tile's first z: uniform) + zl (lane part)
-  const uint32_t zrow = ZH ? 0u : uint32_t(z - lane);
+  const uint32_t zrow = ZH ? 0u : uint32_t(__builtin_amdgcn_readfirstlane(z - lane));
   const uint32_t zl = ZH ? oy[1] + oz[1] : uint32_t(lane);
   const uint32_t yrow = ZH ? 0u : oy[1];
   auto plane = [&](int it) {

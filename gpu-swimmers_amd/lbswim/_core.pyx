@@ -408,8 +408,7 @@ cdef class Lattice:
     def set_fusion(self, enable=True, int rows=0, int strips=0):
         """Fuse Step and CalcHydro of the coupled step into one launch where the
         lattice allows it (nz % 64 == 0, ny % rows == 0); rows (2, 4, 8) and
-        strips 0 choose the tile shape from the lattice.  On by default for
-        FP64 storage, off for FP32."""
+        strips 0 choose the tile shape from the lattice.  On by default."""
         check(lbswim_lattice_set_fusion(self.handle(), 1 if enable else 0, rows, strips))
 
     @property

@@ -226,8 +226,7 @@ int lbswim_lattice_sync_force(lbswim_lattice* lat);
 /* *is_virtual = 1 if the last coupled step left the force virtual (not yet
  * written to the force array). No reference counterpart. */
 int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual);
-/* Step fusion in lbswim_system_step: with `enable` (the default for FP64
- * storage; FP32 storage runs the two launches unless enabled) and a
+/* Step fusion in lbswim_system_step: with `enable` (the default) and a
  * lattice with nz % 64 == 0, ny % rows == 0 and 15 n sizeof(storage) < 4 GiB,
  * Step and CalcHydro run as one launch that reads the distributions once,
  * with results identical bit for bit to the two launches.  rows: lattice

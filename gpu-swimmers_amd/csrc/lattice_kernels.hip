@@ -785,14 +785,21 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
           ...);
     }(std::make_integer_sequence<int, Q>{});
     lds_barrier();
+    // CalcHydro of plane it-1: the LDS reads come before the second barrier
+    // (which only has to keep the next plane's writes off these slots), the
+    // arithmetic and stores after it, overlapping the other wavefronts' next
+    // plane
+    T fl[Q];
     if constexpr (HYDRO) {
       const int r1 = 1 + (it + 1) % 2, r2 = 3 + (it + 1) % 3;
-      T fl[Q];
       [&]<int... P>(std::integer_sequence<int, P...>) {
         ((fl[P] = lds[((dir_group(P) == 0 ? 0 : (dir_group(P) == 1 ? r1 : r2)) * 5 + dir_index(P)) * PLANE +
                       (r - 1) * FUSED_ROW + lane + 1]),
          ...);
       }(std::make_integer_sequence<int, Q>{});
+    }
+    lds_barrier();
+    if constexpr (HYDRO) {
       T rh, uh[3];
       hydro_site(fl, Fh, rh, uh);
       const uint32_t sp = uint32_t(plane(it - 1)) * nyz + yrow + zrow;
@@ -806,7 +813,6 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     m0 = m1;
     o1 = o2;
     m1 = m2;
-    lds_barrier();
   };
   // Iterations: 0 (plane xa-1, halo), 1 (xa: stored, no CalcHydro yet),
   // 2..last-1 (stored + CalcHydro of the plane before), last (plane xb, halo:

@@ -673,7 +673,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     return x < 0 ? x + g.nx : x;
   };
   // Virtual force of a row wavefront: the row's list for plane it is walked
-  // lane-parallel (lanes_row_force) from entries loaded ahead -- list bounds
+  // four lanes per entry (quad_row_force) from entries loaded ahead -- list bounds
   // (offv) two planes before, ids (idv) and then the point records (rec)
   // during the previous plane -- so each dependent load has at least half a
   // plane's work to complete behind.
@@ -682,7 +682,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
   // offv[p & 1]: bounds of plane p (lane 0 first entry, lane 1 end), loaded
   // two iterations before take_list reads them
   uint32_t offv[2] = {0, 0}, idv = 0, o0 = 0, m0 = 0, o1 = 0, m1 = 0;
-  LaneEntry rec{};
+  QuadEntry rec{};
   auto list_of = [&](int it) { return (uint32_t(plane(it)) * nyz + yrow + zrow) / WAVE_SITES; };
   auto load_off = [&](int it) {
     const uint32_t v = A.vf.off[list_of(it) + uint32_t(lane & 1)];
@@ -695,7 +695,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     const uint32_t v = (it & 1) ? offv[1] : offv[0];
     o = uint32_t(__builtin_amdgcn_readlane(int(v), 0));
     m = uint32_t(__builtin_amdgcn_readlane(int(v), 1)) - o;
-    idv = A.vf.sorted[uint32_t(lane) < m ? o + uint32_t(lane) : 0u];
+    idv = A.vf.sorted[uint32_t(lane >> 2) < m ? o + uint32_t(lane >> 2) : 0u];  // four lanes per entry
   };
   // pulls of plane it (and its stored or run-end force)
   auto pull = [&](int it, T(&f)[Q], T(&sf)[3]) {
@@ -735,7 +735,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
       const int x = plane(it);
       double acc[3];
       if (simple)
-        lanes_row_force(rec, o0, m0, A.vf.pts, A.vf.sorted, x, y, z, g, acc);
+        quad_row_force(rec, o0, m0, A.vf.pts, A.vf.sorted, x, y, z, g, acc);
       else
         wave_force(A.vf.pts, A.vf.off, A.vf.sorted, list_of(it), true, x, y, z, g, acc);
 #pragma unroll
@@ -754,7 +754,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     collide_site(f, F, A.tau_s, A.tau_b, A.omega_s, A.omega_b, out);
     uint32_t o2 = 0, m2 = 0;
     if constexpr (VFROW) {
-      load_entry(A.vf.pts, idv, rec);  // records of plane it+1
+      load_quad_entry(A.vf.pts, idv, rec);  // records of plane it+1
       take_list(it + 2, o2, m2);       // ids of plane it+2, from bounds loaded with plane it
     }
     if constexpr (STORE) {
@@ -822,7 +822,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
   if constexpr (VFROW) {  // prime the list pipeline: plane 0 records, plane 1 ids, plane 2 bounds
     load_off(0);
     take_list(0, o0, m0);
-    load_entry(A.vf.pts, idv, rec);
+    load_quad_entry(A.vf.pts, idv, rec);
     load_off(1);
     take_list(1, o1, m1);
     load_off(2);

@@ -245,9 +245,16 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_add_forces(int ns, double P
 // c < 3 (0 on the others).  Every lane of the group must call it.
 constexpr int PLANE_STRIDE = 49;  // 48 products + 1: groups start on different banks
 
+// One lane's share of a stencil: its column's weights and the gathered u.
+struct GroupGather {
+  double dx[4], dy, dz;
+  double uv[4][3];
+};
+
 template <typename T>
-__device__ __forceinline__ double group_interp(const T* __restrict__ u, const Geom& g, const double (&r)[3], int c,
-                                               double* __restrict__ terms) {
+__device__ __forceinline__ GroupGather group_gather(const T* __restrict__ u, const Geom& g, const double (&r)[3],
+                                                    int c) {
+  GroupGather G;
   const int q = c < 12 ? c : 0;
   const int axis = q >> 2;
   const double ra = axis == 0 ? r[0] : (axis == 1 ? r[1] : r[2]);
@@ -257,32 +264,35 @@ __device__ __forceinline__ double group_interp(const T* __restrict__ u, const Ge
   const double mine_del = peskin(ra - x);
   const int j = c >> 2, k = c & 3;
   uint32_t ix[4];
-  double dx[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     ix[i] = uint32_t(__shfl(mine_idx, i, GROUP));
-    dx[i] = __shfl(mine_del, i, GROUP);
+    G.dx[i] = __shfl(mine_del, i, GROUP);
   }
   const uint32_t iyz =
       uint32_t(__shfl(mine_idx, 4 + j, GROUP)) * uint32_t(g.nz) + uint32_t(__shfl(mine_idx, 8 + k, GROUP));
-  const double dy = __shfl(mine_del, 4 + j, GROUP), dz = __shfl(mine_del, 8 + k, GROUP);
+  G.dy = __shfl(mine_del, 4 + j, GROUP);
+  G.dz = __shfl(mine_del, 8 + k, GROUP);
   const size_t n = g.n;
-  double uv[4][3];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint32_t ijk = ix[i] * g.nyz + iyz;
-    uv[i][0] = double(u[ijk]);
-    uv[i][1] = double(u[n + ijk]);
-    uv[i][2] = double(u[2 * n + ijk]);
+    G.uv[i][0] = double(u[ijk]);
+    G.uv[i][1] = double(u[n + ijk]);
+    G.uv[i][2] = double(u[2 * n + ijk]);
   }
+  return G;
+}
+
+__device__ __forceinline__ double group_sum(const GroupGather& G, int c, double* __restrict__ terms) {
   const int d = c < 3 ? c : 0;
   double vel = 0.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const double delta3d = dx[i] * dy * dz;
-    terms[3 * c + 0] = delta3d * uv[i][0];
-    terms[3 * c + 1] = delta3d * uv[i][1];
-    terms[3 * c + 2] = delta3d * uv[i][2];
+    const double delta3d = G.dx[i] * G.dy * G.dz;
+    terms[3 * c + 0] = delta3d * G.uv[i][0];
+    terms[3 * c + 1] = delta3d * G.uv[i][1];
+    terms[3 * c + 2] = delta3d * G.uv[i][2];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -295,6 +305,12 @@ __device__ __forceinline__ double group_interp(const T* __restrict__ u, const Ge
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   return vel;
+}
+
+template <typename T>
+__device__ __forceinline__ double group_interp(const T* __restrict__ u, const Geom& g, const double (&r)[3], int c,
+                                               double* __restrict__ terms) {
+  return group_sum(group_gather(u, g, r, c), c, terms);
 }
 
 // DoSwimmerArrayMove, SwimmerArray.cu:123-210 (norm starts at 0; with
@@ -318,8 +334,11 @@ __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp,
     N[d] = nv[size_t(ns) * d + is];
     rMinus[d] = R[d] - N[d] * cp.l;
   }
-  const double a = group_interp(u, g, R, c, terms);
-  const double b = group_interp(u, g, rMinus, c, terms);
+  // both stencils' gathers in flight together, then the two ordered sums
+  const GroupGather GA = group_gather(u, g, R, c);
+  const GroupGather GB = group_gather(u, g, rMinus, c);
+  const double a = group_sum(GA, c, terms);
+  const double b = group_sum(GB, c, terms);
   double vel[3], vMinus[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {

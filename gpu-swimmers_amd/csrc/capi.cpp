@@ -168,6 +168,10 @@ struct lbswim_lattice {
   // lattice qualifies); rows/strips 0 = chosen from the lattice size
   bool fuse = true;
   int fuse_rows = 0;
+  // u with a site's components together, written by the fused step for the
+  // particle moves; current only while nothing else has written u since
+  void* d_u_aos = nullptr;
+  bool u_aos_valid = false;
   int fuse_strips = 0;
 };
 
@@ -388,6 +392,7 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   if (g_rt.ready) (void)hipStreamSynchronize(stream());
   dev_free(lat->d_rho);
   dev_free(lat->d_u);
+  dev_free(lat->d_u_aos);
   dev_free(lat->d_force);
   dev_free(lat->d_f[0]);
   dev_free(lat->d_f[1]);
@@ -453,6 +458,7 @@ static int lattice_step_impl(lbswim_lattice* lat) {
 }
 
 static int lattice_hydro_impl(lbswim_lattice* lat) {
+  lat->u_aos_valid = false;
   const VForce vf = virtual_force(lat);
   const VForce* pvf = lat->force_virtual ? &vf : nullptr;
   KernelTimer t(lat, LBSWIM_K_HYDRO);
@@ -483,6 +489,8 @@ static int fusion_rows(const lbswim_lattice* lat) {
 // Step + CalcHydro of the coupled step in one launch (fusion_rows(lat) > 0).
 static int lattice_step_hydro_impl(lbswim_lattice* lat, int rows) {
   const double ts = lat->params.tau_s, tb = lat->params.tau_b;
+  if (!lat->d_u_aos)
+    if (int rc = dev_alloc_zero(&lat->d_u_aos, 3 * lat->elem * lat->n)) return rc;
   const VForce vf = virtual_force(lat);
   const VForce* pvf = lat->force_virtual ? &vf : nullptr;
   KernelTimer t(lat, LBSWIM_K_FUSED);
@@ -490,15 +498,17 @@ static int lattice_step_hydro_impl(lbswim_lattice* lat, int rows) {
   if (lat->storage == LBSWIM_STORE_F64)
     e = launch_step_hydro<double>(static_cast<const double*>(lat->d_f[lat->cur]),
                                   static_cast<const double*>(lat->d_force), static_cast<double*>(lat->d_f[1 - lat->cur]),
-                                  static_cast<double*>(lat->d_rho), static_cast<double*>(lat->d_u), lat->geom, ts, tb,
-                                  pvf, rows, lat->fuse_strips, stream());
+                                  static_cast<double*>(lat->d_rho), static_cast<double*>(lat->d_u),
+                                  static_cast<double*>(lat->d_u_aos), lat->geom, ts, tb, pvf, rows, lat->fuse_strips,
+                                  stream());
   else
     e = launch_step_hydro<float>(static_cast<const float*>(lat->d_f[lat->cur]), static_cast<const float*>(lat->d_force),
                                  static_cast<float*>(lat->d_f[1 - lat->cur]), static_cast<float*>(lat->d_rho),
-                                 static_cast<float*>(lat->d_u), lat->geom, ts, tb, pvf, rows, lat->fuse_strips,
-                                 stream());
+                                 static_cast<float*>(lat->d_u), static_cast<float*>(lat->d_u_aos), lat->geom, ts, tb, pvf,
+                                 rows, lat->fuse_strips, stream());
   t.done();
   HIP_TRY(e);
+  lat->u_aos_valid = true;
   lat->cur = 1 - lat->cur;
   lat->time_step++;
   return LBSWIM_OK;
@@ -630,6 +640,8 @@ int lbswim_lattice_field(lbswim_lattice* lat, int field, double** host, void** d
   if (device) {
     if (field == LBSWIM_FORCE)
       if (int rc = materialize_force(lat)) return rc;
+    // the caller may write u through the pointer: the moves' copy is then stale
+    if (field == LBSWIM_U) lat->u_aos_valid = false;
     *device = field_device(lat, field);
   }
   if (size) *size = field_count(lat, field);
@@ -645,6 +657,7 @@ int lbswim_lattice_h2d(lbswim_lattice* lat, int field) {
   void* dev = field_device(lat, field);
   const size_t n = lat->n;
   if (field == LBSWIM_FORCE) lat->force_virtual = false;  // the host array replaces the force
+  if (field == LBSWIM_U) lat->u_aos_valid = false;
   if (lat->storage == LBSWIM_STORE_F64) {
     if (!is_dist(field)) return copy_h2d(dev, h, cnt * sizeof(double));
     std::vector<double> stage(cnt);
@@ -880,12 +893,14 @@ static int swimmers_move_impl(lbswim_swimmers* sw, lbswim_lattice* lat) {
   hipError_t e;
   if (lat->storage == LBSWIM_STORE_F64)
     e = launch_swimmers_move<double>(st.num, p, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_V),
-                                     fld(st, LBSWIM_PART_N), prng, st.perm, static_cast<const double*>(lat->d_u),
-                                     lat->geom, sw->d_tumbles, stream());
+                                     fld(st, LBSWIM_PART_N), prng, st.perm,
+                                     static_cast<const double*>(lat->u_aos_valid ? lat->d_u_aos : lat->d_u),
+                                     lat->u_aos_valid, lat->geom, sw->d_tumbles, stream());
   else
     e = launch_swimmers_move<float>(st.num, p, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_V),
-                                    fld(st, LBSWIM_PART_N), prng, st.perm, static_cast<const float*>(lat->d_u),
-                                    lat->geom, sw->d_tumbles, stream());
+                                    fld(st, LBSWIM_PART_N), prng, st.perm,
+                                    static_cast<const float*>(lat->u_aos_valid ? lat->d_u_aos : lat->d_u),
+                                    lat->u_aos_valid, lat->geom, sw->d_tumbles, stream());
   t.done();
   HIP_TRY(e);
   st.moves_since_order++;
@@ -976,10 +991,12 @@ static int tracers_move_impl(lbswim_tracers* tr, lbswim_lattice* lat) {
   hipError_t e;
   if (lat->storage == LBSWIM_STORE_F64)
     e = launch_tracers_move<double>(st.num, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_S), fld(st, LBSWIM_PART_V),
-                                    static_cast<const double*>(lat->d_u), lat->geom, stream());
+                                    static_cast<const double*>(lat->u_aos_valid ? lat->d_u_aos : lat->d_u),
+                                    lat->u_aos_valid, lat->geom, stream());
   else
     e = launch_tracers_move<float>(st.num, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_S), fld(st, LBSWIM_PART_V),
-                                   static_cast<const float*>(lat->d_u), lat->geom, stream());
+                                   static_cast<const float*>(lat->u_aos_valid ? lat->d_u_aos : lat->d_u),
+                                   lat->u_aos_valid, lat->geom, stream());
   t.done();
   HIP_TRY(e);
   st.moves_since_order++;
@@ -1581,6 +1598,7 @@ void ens_advance(lbswim_ensemble* e, int k) {
     if (k & 1) m.lat->cur ^= 1;
     m.lat->time_step += k;
     m.lat->force_virtual = false;  // the batched step spreads into the force array
+    m.lat->u_aos_valid = false;    // and writes u only in the reference layout
     for (auto* s : m.sw) s->st.moves_since_order += k;
     for (auto* t : m.tr) t->st.moves_since_order += k;
   }

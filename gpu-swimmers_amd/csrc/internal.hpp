@@ -49,9 +49,11 @@ hipError_t launch_calc_hydro(const T* f, const T* force, T* rho, T* u, const Geo
 // {2, 4, 8} (rows per tile), and 15 n sizeof(T) < 4 GiB (32-bit buffer
 // offsets).  `strips` splits the x sweep (0: one workgroup per CU).
 bool fused_step_supported(const Geom& g, int ty, size_t elem);
+// u_aos (3 n values) receives u again with a site's components together,
+// u_aos[3 ijk + a], which the particle moves gather from.
 template <typename T>
-hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u, const Geom& g, double tau_s,
-                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s);
+hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u, T* u_aos, const Geom& g,
+                             double tau_s, double tau_b, const VForce* vf, int ty, int strips, hipStream_t s);
 template <typename T>
 hipError_t launch_init_from_hydro(const T* rho, const T* u, T* f, const Geom& g, hipStream_t s);
 
@@ -67,12 +69,14 @@ hipError_t launch_totals(const T* f, const Geom& g, double* partial, int nblocks
 template <typename T>
 hipError_t launch_swimmers_add_forces(int ns, const SwimParams& p, const double* r, const double* n,
                                       T* force, const Geom& g, hipStream_t s);
+// aos: u holds the three components of a site together (u[3 ijk + a]),
+// the copy the fused step writes for the moves; else u[a n + ijk].
 template <typename T>
 hipError_t launch_swimmers_move(int ns, const SwimParams& p, double* r, double* v, double* n,
-                                uint64_t* prng, const int* perm, const T* u, const Geom& g,
+                                uint64_t* prng, const int* perm, const T* u, bool aos, const Geom& g,
                                 unsigned long long* tumbles, hipStream_t s);
 template <typename T>
-hipError_t launch_tracers_move(int nt, double* r, double* sdisp, double* v, const T* u, const Geom& g,
+hipError_t launch_tracers_move(int nt, double* r, double* sdisp, double* v, const T* u, bool aos, const Geom& g,
                                hipStream_t s);
 
 // ---- deterministic spreading (ZeroForce + AddForces of all swimmer arrays)

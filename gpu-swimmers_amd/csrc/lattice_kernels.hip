@@ -603,6 +603,7 @@ struct FusedArgs {
   T* gNew;
   T* rho;
   T* u;
+  T* u_aos;  // u again with a site's components together, u_aos[3 ijk + a], for the particle moves
   Geom g;
   T tau_s, tau_b, omega_s, omega_b;
 };
@@ -624,6 +625,15 @@ __device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t voff
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
   else
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+// Store through the caches (L2 merges the partial lines of strided stores).
+template <typename T>
+__device__ __forceinline__ void buffer_store(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, voff, soff, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
 }
 // Non-temporal store (cache policy bit 1: nt).
 template <typename T>
@@ -806,6 +816,8 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
       buffer_store_nt(rh, buffer_rsrc(A.rho), zl * B, sp * B);
 #pragma unroll
       for (int a = 0; a < 3; ++a) buffer_store_nt(uh[a], buffer_rsrc(A.u), zl * B, (uint32_t(a) * n + sp) * B);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) buffer_store(uh[a], buffer_rsrc(A.u_aos), (3 * zl + uint32_t(a)) * B, 3 * sp * B);
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) Fh[a] = F[a];
@@ -1095,8 +1107,8 @@ bool fused_step_supported(const Geom& g, int ty, size_t elem) {
 }
 
 template <typename T>
-hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u, const Geom& g, double tau_s,
-                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s) {
+hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u, T* u_aos, const Geom& g,
+                             double tau_s, double tau_b, const VForce* vf, int ty, int strips, hipStream_t s) {
   if (!fused_step_supported(g, ty, sizeof(T)) || strips < 0) return hipErrorInvalidValue;
   FusedTiles t;
   t.nyt = uint32_t(g.ny / ty);
@@ -1125,6 +1137,7 @@ hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* 
   A.gNew = fNew;
   A.rho = rho;
   A.u = u;
+  A.u_aos = u_aos;
   A.g = g;
   A.tau_s = T(tau_s);
   A.tau_b = T(tau_b);
@@ -1144,10 +1157,10 @@ hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* 
 #undef LBS_FUSED
   return hipGetLastError();
 }
-template hipError_t launch_step_hydro<double>(const double*, const double*, double*, double*, double*, const Geom&,
-                                              double, double, const VForce*, int, int, hipStream_t);
-template hipError_t launch_step_hydro<float>(const float*, const float*, float*, float*, float*, const Geom&, double,
-                                             double, const VForce*, int, int, hipStream_t);
+template hipError_t launch_step_hydro<double>(const double*, const double*, double*, double*, double*, double*,
+                                              const Geom&, double, double, const VForce*, int, int, hipStream_t);
+template hipError_t launch_step_hydro<float>(const float*, const float*, float*, float*, float*, float*, const Geom&,
+                                             double, double, const VForce*, int, int, hipStream_t);
 
 template <>
 hipError_t launch_calc_hydro<double>(const double* f, const double* force, double* rho, double* u, const Geom& g,

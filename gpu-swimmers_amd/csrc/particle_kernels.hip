@@ -251,7 +251,9 @@ struct GroupGather {
   double uv[4][3];
 };
 
-template <typename T>
+// AOS: u holds the three components of a site together (u[3 ijk + a], the
+// copy the fused step writes for the moves) instead of u[a n + ijk].
+template <typename T, bool AOS = false>
 __device__ __forceinline__ GroupGather group_gather(const T* __restrict__ u, const Geom& g, const double (&r)[3],
                                                     int c) {
   GroupGather G;
@@ -277,9 +279,16 @@ __device__ __forceinline__ GroupGather group_gather(const T* __restrict__ u, con
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint32_t ijk = ix[i] * g.nyz + iyz;
-    G.uv[i][0] = double(u[ijk]);
-    G.uv[i][1] = double(u[n + ijk]);
-    G.uv[i][2] = double(u[2 * n + ijk]);
+    if constexpr (AOS) {
+      const T* __restrict__ us = u + 3 * size_t(ijk);
+      G.uv[i][0] = double(us[0]);
+      G.uv[i][1] = double(us[1]);
+      G.uv[i][2] = double(us[2]);
+    } else {
+      G.uv[i][0] = double(u[ijk]);
+      G.uv[i][1] = double(u[n + ijk]);
+      G.uv[i][2] = double(u[2 * n + ijk]);
+    }
   }
   return G;
 }
@@ -307,16 +316,16 @@ __device__ __forceinline__ double group_sum(const GroupGather& G, int c, double*
   return vel;
 }
 
-template <typename T>
+template <typename T, bool AOS = false>
 __device__ __forceinline__ double group_interp(const T* __restrict__ u, const Geom& g, const double (&r)[3], int c,
                                                double* __restrict__ terms) {
-  return group_sum(group_gather(u, g, r, c), c, terms);
+  return group_sum(group_gather<T, AOS>(u, g, r, c), c, terms);
 }
 
 // DoSwimmerArrayMove, SwimmerArray.cu:123-210 (norm starts at 0; with
 // rotation off and no tumble, n is kept).  A lane group per swimmer
 // interpolates at r and at r - l n; lane 0 of the group finishes the move.
-template <typename T>
+template <typename T, bool AOS = false>
 __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp, double* __restrict__ r,
                                                    double* __restrict__ v, double* __restrict__ nv,
                                                    uint64_t* __restrict__ prng, const int* __restrict__ perm,
@@ -335,8 +344,8 @@ __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp,
     rMinus[d] = R[d] - N[d] * cp.l;
   }
   // both stencils' gathers in flight together, then the two ordered sums
-  const GroupGather GA = group_gather(u, g, R, c);
-  const GroupGather GB = group_gather(u, g, rMinus, c);
+  const GroupGather GA = group_gather<T, AOS>(u, g, R, c);
+  const GroupGather GB = group_gather<T, AOS>(u, g, rMinus, c);
   const double a = group_sum(GA, c, terms);
   const double b = group_sum(GB, c, terms);
   double vel[3], vMinus[3];
@@ -394,7 +403,7 @@ __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp,
 constexpr int TGROUPS = TRACER_GROUPS;  // tracers per workgroup: 1024 threads, contiguous in storage order
 constexpr int TBLOCK = TGROUPS * GROUP;
 
-template <typename T>
+template <typename T, bool AOS = false>
 __device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r, double* __restrict__ sd,
                                                   double* __restrict__ v, const T* __restrict__ u, const Geom& g,
                                                   uint32_t blk, double* __restrict__ lds) {
@@ -405,7 +414,7 @@ __device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r
   const double R[3] = {r[it], r[size_t(nt) + it], r[2 * size_t(nt) + it]};
   const int d = c < 3 ? c : 0;
   const double s_old = c < 3 ? sd[size_t(nt) * d + it] : 0.0;
-  const double vel = group_interp(u, g, R, c, terms);
+  const double vel = group_interp<T, AOS>(u, g, R, c, terms);
   if (c >= 3) return;
   const int size = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
   const double Rd = d == 0 ? R[0] : (d == 1 ? R[1] : R[2]);
@@ -419,7 +428,7 @@ __device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r
 // contiguous range of particle blocks (xcd_block): neighbouring particles
 // share their stencils' lines in one L2.
 
-template <typename T>
+template <typename T, bool AOS>
 __global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp, double* __restrict__ r,
                                                           double* __restrict__ v, double* __restrict__ nv,
                                                           uint64_t* __restrict__ prng, const int* __restrict__ perm,
@@ -428,17 +437,17 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp,
   __shared__ double lds[GROUPS * PLANE_STRIDE];
   const uint32_t blk = xcd_block(round_xcd(nb));
   if (blk >= nb) return;
-  swimmers_move_body(ns, cp, r, v, nv, prng, perm, u, g, tumbles, blk, lds);
+  swimmers_move_body<T, AOS>(ns, cp, r, v, nv, prng, perm, u, g, tumbles, blk, lds);
 }
 
-template <typename T>
+template <typename T, bool AOS>
 __global__ __launch_bounds__(TBLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
                                                          double* __restrict__ v, const T* __restrict__ u, Geom g,
                                                          uint32_t nb) {
   __shared__ double lds[TGROUPS * PLANE_STRIDE];
   const uint32_t blk = xcd_block(round_xcd(nb));
   if (blk >= nb) return;
-  tracers_move_body(nt, r, sd, v, u, g, blk, lds);
+  tracers_move_body<T, AOS>(nt, r, sd, v, u, g, blk, lds);
 }
 
 // Batched moves: WorkItem.member indexes the particle-array table, unit is
@@ -828,21 +837,28 @@ hipError_t launch_swimmers_add_forces(int ns, const SwimParams& p, const double*
 
 template <typename T>
 hipError_t launch_swimmers_move(int ns, const SwimParams& p, double* r, double* v, double* n, uint64_t* prng,
-                                const int* perm, const T* u, const Geom& g, unsigned long long* tumbles,
+                                const int* perm, const T* u, bool aos, const Geom& g, unsigned long long* tumbles,
                                 hipStream_t s) {
   if (ns <= 0) return hipSuccess;
   const uint32_t nb = uint32_t((ns + GROUPS - 1) / GROUPS);
-  hipLaunchKernelGGL(k_swimmers_move<T>, dim3(round_xcd(nb)), dim3(PBLOCK), 0, s, ns, p, r, v, n, prng, perm, u, g,
-                     tumbles, nb);
+  if (aos)
+    hipLaunchKernelGGL((k_swimmers_move<T, true>), dim3(round_xcd(nb)), dim3(PBLOCK), 0, s, ns, p, r, v, n, prng, perm,
+                       u, g, tumbles, nb);
+  else
+    hipLaunchKernelGGL((k_swimmers_move<T, false>), dim3(round_xcd(nb)), dim3(PBLOCK), 0, s, ns, p, r, v, n, prng,
+                       perm, u, g, tumbles, nb);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_tracers_move(int nt, double* r, double* sd, double* v, const T* u, const Geom& g,
+hipError_t launch_tracers_move(int nt, double* r, double* sd, double* v, const T* u, bool aos, const Geom& g,
                                hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   const uint32_t nb = uint32_t((nt + TGROUPS - 1) / TGROUPS);
-  hipLaunchKernelGGL(k_tracers_move<T>, dim3(round_xcd(nb)), dim3(TBLOCK), 0, s, nt, r, sd, v, u, g, nb);
+  if (aos)
+    hipLaunchKernelGGL((k_tracers_move<T, true>), dim3(round_xcd(nb)), dim3(TBLOCK), 0, s, nt, r, sd, v, u, g, nb);
+  else
+    hipLaunchKernelGGL((k_tracers_move<T, false>), dim3(round_xcd(nb)), dim3(TBLOCK), 0, s, nt, r, sd, v, u, g, nb);
   return hipGetLastError();
 }
 
@@ -851,14 +867,14 @@ template hipError_t launch_swimmers_add_forces<double>(int, const SwimParams&, c
 template hipError_t launch_swimmers_add_forces<float>(int, const SwimParams&, const double*, const double*,
                                                       float*, const Geom&, hipStream_t);
 template hipError_t launch_swimmers_move<double>(int, const SwimParams&, double*, double*, double*, uint64_t*,
-                                                 const int*, const double*, const Geom&, unsigned long long*,
+                                                 const int*, const double*, bool, const Geom&, unsigned long long*,
                                                  hipStream_t);
 template hipError_t launch_swimmers_move<float>(int, const SwimParams&, double*, double*, double*, uint64_t*,
-                                                const int*, const float*, const Geom&, unsigned long long*,
+                                                const int*, const float*, bool, const Geom&, unsigned long long*,
                                                 hipStream_t);
-template hipError_t launch_tracers_move<double>(int, double*, double*, double*, const double*, const Geom&,
+template hipError_t launch_tracers_move<double>(int, double*, double*, double*, const double*, bool, const Geom&,
                                                 hipStream_t);
-template hipError_t launch_tracers_move<float>(int, double*, double*, double*, const float*, const Geom&,
+template hipError_t launch_tracers_move<float>(int, double*, double*, double*, const float*, bool, const Geom&,
                                                hipStream_t);
 
 

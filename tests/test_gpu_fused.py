@@ -101,3 +101,26 @@ def test_fusion_skipped_where_unsupported():
     assert lat.fusion[0] is False
     lat.set_fusion(True, 2, 0)
     assert lat.fusion == (True, 2, 0)
+
+
+def test_moves_use_u_written_after_fused_steps():
+    """The fused step also writes u component-interleaved for the particle
+    moves; a u replaced afterwards (H2D, standalone CalcHydro) must be the one
+    the next moves interpolate -- as with the two launches."""
+    case = swimmer_case((4, 8, 64), seed=77)
+    out = []
+    for fusion in (None, (0, 0)):
+        lat, system, sw, tr = build_lbswim(case, "f64", "virtual")
+        lat.set_fusion(fusion is not None)
+        system.Step(2)
+        lat.u.D2H()
+        lat.u.data[...] = 1.5 * lat.u.data + 1e-3
+        lat.u.H2D()
+        sw.Move(lat)
+        tr.Move(lat)
+        lat.CalcHydro()  # two-launch CalcHydro rewrites u in the reference layout
+        sw.Move(lat)
+        tr.Move(lat)
+        out.append(collect_lbswim(lat, sw, tr))
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k

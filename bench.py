@@ -37,10 +37,10 @@ def collide_bytes(form, virtual):
 
 def fused_bytes(form, virtual):
     """Algorithmic bytes per site of one fused collide+stream+CalcHydro
-    launch: 15 f read, 15 f written, rho and 3 u written, 3 u written again
-    component-interleaved for the particle moves, plus 3 force components
-    read unless the force is virtual."""
-    return (37 + (0 if virtual else 3)) * ELEM[form]
+    launch: 15 f read, 15 f written, rho and 3 u written (u component-
+    interleaved, the layout the particle moves gather from), plus 3 force
+    components read unless the force is virtual."""
+    return (34 + (0 if virtual else 3)) * ELEM[form]
 
 
 def particle_bytes(ns, nt):
@@ -59,12 +59,11 @@ def gathered_bytes(ns, nt, elem):
 
 def step_bytes(form, virtual, fused):
     """Lattice bytes per site of the whole coupled step: collide+stream,
-    CalcHydro (15 f read -- not with the fused launch, which pulls from the
-    values it has just collided but writes u a second time for the moves --
-    rho and 3 u written, 3 force read) and the force array written once; the
-    force terms only with a stored force."""
+    CalcHydro (15 f read -- not with the fused launch, which takes them from
+    the values it has just collided -- rho and 3 u written, 3 force read) and
+    the force array written once; the force terms only with a stored force."""
     force = 0 if virtual else (3 if fused else 3 + 3) + 3
-    return (30 + (3 if fused else 15) + 4 + force) * ELEM[form]
+    return (30 + (0 if fused else 15) + 4 + force) * ELEM[form]
 
 
 def log(msg):

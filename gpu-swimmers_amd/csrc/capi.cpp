@@ -49,10 +49,18 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 // One stream orders the device work of every object, like the reference's
-// default stream.  Created on first use; default priority.
+// default stream.  Created on first use; default priority.  Inside
+// lbswim_system_step the tracer moves run on a second stream (`side`, also
+// default priority) beside the swimmer moves and the next step's spreading;
+// the main stream waits for them before that call returns, so every other
+// entry point sees one ordered stream.
 struct Runtime {
   bool ready = false;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;
+  hipStream_t cur = nullptr;  // where launches go: `stream`, or `side` inside a SideScope
+  hipEvent_t u_ready = nullptr;  // main -> side: the u the tracers move in is written
+  hipEvent_t joined = nullptr;   // side -> main: every tracer move enqueued so far is done
 };
 Runtime g_rt;
 
@@ -62,11 +70,28 @@ int ensure_runtime() {
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev < 1) return fail(LBSWIM_ENODEV, "no HIP device available (%s)", hipGetErrorString(e));
   HIP_TRY(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
+  g_rt.cur = g_rt.stream;
   g_rt.ready = true;
   return LBSWIM_OK;
 }
 
-hipStream_t stream() { return g_rt.stream; }
+hipStream_t stream() { return g_rt.cur; }
+
+int ensure_side_stream() {
+  if (g_rt.side) return LBSWIM_OK;
+  HIP_TRY(hipEventCreateWithFlags(&g_rt.u_ready, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&g_rt.joined, hipEventDisableTiming));
+  HIP_TRY(hipStreamCreateWithFlags(&g_rt.side, hipStreamNonBlocking));
+  return LBSWIM_OK;
+}
+
+// Launches inside the scope go to the side stream.
+struct SideScope {
+  SideScope() { g_rt.cur = g_rt.side; }
+  ~SideScope() { g_rt.cur = g_rt.stream; }
+  SideScope(const SideScope&) = delete;
+  SideScope& operator=(const SideScope&) = delete;
+};
 
 int dev_alloc_zero(void** p, size_t bytes) {
   *p = nullptr;
@@ -160,6 +185,9 @@ struct lbswim_lattice {
   uint32_t* sp_list = nullptr;
   uint32_t* sp_sorted = nullptr;
   double* sp_edge = nullptr;  // 6 sp_lists: force at both ends of each list's run (fused step)
+  size_t sp_rec_cap = 0;      // list entries sp_rec and sp_wz hold (fused step)
+  RunEntry* sp_rec = nullptr;
+  int* sp_wz = nullptr;
   // Virtual force: after a fused step the force lives in the sorted point
   // lists above and d_force is stale until materialize_force writes it.
   int force_mode = LBSWIM_FORCE_AUTO;
@@ -168,10 +196,17 @@ struct lbswim_lattice {
   // lattice qualifies); rows/strips 0 = chosen from the lattice size
   bool fuse = true;
   int fuse_rows = 0;
-  // u with a site's components together, written by the fused step for the
-  // particle moves; current only while nothing else has written u since
+  // u with a site's components together, written by the fused step (the
+  // particle moves gather from it); current only while nothing else has
+  // written u since.  u_virtual: the fused step wrote u only there, and d_u
+  // is stale until materialize_u writes it (u_virtual implies u_aos_valid).
+  // tracers_reading: tracer moves on the side stream may still read d_u_aos;
+  // they are done once tracers_done has fired.
   void* d_u_aos = nullptr;
+  bool tracers_reading = false;
+  hipEvent_t tracers_done = nullptr;
   bool u_aos_valid = false;
+  bool u_virtual = false;
   int fuse_strips = 0;
 };
 
@@ -393,6 +428,7 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->d_rho);
   dev_free(lat->d_u);
   dev_free(lat->d_u_aos);
+  if (lat->tracers_done) (void)hipEventDestroy(lat->tracers_done);
   dev_free(lat->d_force);
   dev_free(lat->d_f[0]);
   dev_free(lat->d_f[1]);
@@ -401,6 +437,8 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->sp_off);
   dev_free(lat->sp_cursor);
   dev_free(lat->sp_edge);
+  dev_free(lat->sp_rec);
+  dev_free(lat->sp_wz);
   dev_free(lat->sp_pts);
   dev_free(lat->sp_list);
   dev_free(lat->sp_sorted);
@@ -432,8 +470,25 @@ static int materialize_force(lbswim_lattice* lat) {
   return LBSWIM_OK;
 }
 
+// u in the reference layout after fused steps, which wrote it only
+// component-interleaved.
+static int materialize_u(lbswim_lattice* lat) {
+  if (!lat->u_virtual) return LBSWIM_OK;
+  hipError_t e;
+  if (lat->storage == LBSWIM_STORE_F64)
+    e = launch_u_from_aos<double>(static_cast<const double*>(lat->d_u_aos), static_cast<double*>(lat->d_u), lat->geom,
+                                  stream());
+  else
+    e = launch_u_from_aos<float>(static_cast<const float*>(lat->d_u_aos), static_cast<float*>(lat->d_u), lat->geom,
+                                 stream());
+  HIP_TRY(e);
+  lat->u_virtual = false;
+  return LBSWIM_OK;
+}
+
 static VForce virtual_force(const lbswim_lattice* lat) {
-  return VForce{static_cast<const SpreadPoint*>(lat->sp_pts), lat->sp_off, lat->sp_sorted, lat->sp_edge};
+  return VForce{static_cast<const SpreadPoint*>(lat->sp_pts), lat->sp_off, lat->sp_sorted, lat->sp_edge, lat->sp_rec,
+                lat->sp_wz};
 }
 
 static int lattice_step_impl(lbswim_lattice* lat) {
@@ -459,6 +514,7 @@ static int lattice_step_impl(lbswim_lattice* lat) {
 
 static int lattice_hydro_impl(lbswim_lattice* lat) {
   lat->u_aos_valid = false;
+  lat->u_virtual = false;  // the two-launch CalcHydro writes the reference layout
   const VForce vf = virtual_force(lat);
   const VForce* pvf = lat->force_virtual ? &vf : nullptr;
   KernelTimer t(lat, LBSWIM_K_HYDRO);
@@ -486,11 +542,16 @@ static int fusion_rows(const lbswim_lattice* lat) {
   return 0;
 }
 
-// Step + CalcHydro of the coupled step in one launch (fusion_rows(lat) > 0).
+// Step + CalcHydro of the coupled step in one launch (fusion_rows(lat) > 0),
+// once the tracer moves still reading the interleaved u (side stream) are done.
 static int lattice_step_hydro_impl(lbswim_lattice* lat, int rows) {
   const double ts = lat->params.tau_s, tb = lat->params.tau_b;
   if (!lat->d_u_aos)
     if (int rc = dev_alloc_zero(&lat->d_u_aos, 3 * lat->elem * lat->n)) return rc;
+  if (lat->tracers_reading) {
+    HIP_TRY(hipStreamWaitEvent(stream(), lat->tracers_done, 0));
+    lat->tracers_reading = false;
+  }
   const VForce vf = virtual_force(lat);
   const VForce* pvf = lat->force_virtual ? &vf : nullptr;
   KernelTimer t(lat, LBSWIM_K_FUSED);
@@ -498,17 +559,17 @@ static int lattice_step_hydro_impl(lbswim_lattice* lat, int rows) {
   if (lat->storage == LBSWIM_STORE_F64)
     e = launch_step_hydro<double>(static_cast<const double*>(lat->d_f[lat->cur]),
                                   static_cast<const double*>(lat->d_force), static_cast<double*>(lat->d_f[1 - lat->cur]),
-                                  static_cast<double*>(lat->d_rho), static_cast<double*>(lat->d_u),
-                                  static_cast<double*>(lat->d_u_aos), lat->geom, ts, tb, pvf, rows, lat->fuse_strips,
-                                  stream());
+                                  static_cast<double*>(lat->d_rho), static_cast<double*>(lat->d_u_aos), lat->geom, ts,
+                                  tb, pvf, rows, lat->fuse_strips, stream());
   else
     e = launch_step_hydro<float>(static_cast<const float*>(lat->d_f[lat->cur]), static_cast<const float*>(lat->d_force),
                                  static_cast<float*>(lat->d_f[1 - lat->cur]), static_cast<float*>(lat->d_rho),
-                                 static_cast<float*>(lat->d_u), static_cast<float*>(lat->d_u_aos), lat->geom, ts, tb, pvf,
-                                 rows, lat->fuse_strips, stream());
+                                 static_cast<float*>(lat->d_u_aos), lat->geom, ts, tb, pvf, rows, lat->fuse_strips,
+                                 stream());
   t.done();
   HIP_TRY(e);
   lat->u_aos_valid = true;
+  lat->u_virtual = true;
   lat->cur = 1 - lat->cur;
   lat->time_step++;
   return LBSWIM_OK;
@@ -557,6 +618,18 @@ int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual) {
   return LBSWIM_OK;
 }
 
+int lbswim_lattice_sync_u(lbswim_lattice* lat) {
+  if (int rc = check_lattice(lat)) return rc;
+  return materialize_u(lat);
+}
+
+int lbswim_lattice_u_is_virtual(lbswim_lattice* lat, int* is_virtual) {
+  if (int rc = check_lattice(lat)) return rc;
+  CHECK_ARG(is_virtual != nullptr, "is_virtual is NULL");
+  *is_virtual = lat->u_virtual ? 1 : 0;
+  return LBSWIM_OK;
+}
+
 int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int strips) {
   if (int rc = check_lattice(lat)) return rc;
   CHECK_ARG(rows == 0 || rows == 2 || rows == 4 || rows == 8, "rows must be 0, 2, 4 or 8 (got %d)", rows);
@@ -579,6 +652,7 @@ int lbswim_lattice_fusion(lbswim_lattice* lat, int* active, int* rows, int* stri
 
 int lbswim_lattice_init_from_hydro(lbswim_lattice* lat) {
   if (int rc = check_lattice(lat)) return rc;
+  if (int rc = materialize_u(lat)) return rc;
   KernelTimer t(lat, LBSWIM_K_INIT);
   hipError_t e;
   if (lat->storage == LBSWIM_STORE_F64)
@@ -640,8 +714,10 @@ int lbswim_lattice_field(lbswim_lattice* lat, int field, double** host, void** d
   if (device) {
     if (field == LBSWIM_FORCE)
       if (int rc = materialize_force(lat)) return rc;
-    // the caller may write u through the pointer: the moves' copy is then stale
-    if (field == LBSWIM_U) lat->u_aos_valid = false;
+    if (field == LBSWIM_U) {
+      if (int rc = materialize_u(lat)) return rc;
+      lat->u_aos_valid = false;  // the caller may write u through the pointer
+    }
     *device = field_device(lat, field);
   }
   if (size) *size = field_count(lat, field);
@@ -657,7 +733,7 @@ int lbswim_lattice_h2d(lbswim_lattice* lat, int field) {
   void* dev = field_device(lat, field);
   const size_t n = lat->n;
   if (field == LBSWIM_FORCE) lat->force_virtual = false;  // the host array replaces the force
-  if (field == LBSWIM_U) lat->u_aos_valid = false;
+  if (field == LBSWIM_U) lat->u_aos_valid = lat->u_virtual = false;  // the host array replaces u
   if (lat->storage == LBSWIM_STORE_F64) {
     if (!is_dist(field)) return copy_h2d(dev, h, cnt * sizeof(double));
     std::vector<double> stage(cnt);
@@ -684,6 +760,8 @@ int lbswim_lattice_d2h(lbswim_lattice* lat, int field) {
   if (int rc = lattice_host(lat, field, &h)) return rc;
   if (field == LBSWIM_FORCE)
     if (int rc = materialize_force(lat)) return rc;
+  if (field == LBSWIM_U)
+    if (int rc = materialize_u(lat)) return rc;
   const size_t cnt = field_count(lat, field);
   const void* dev = field_device(lat, field);
   const size_t n = lat->n;
@@ -1102,14 +1180,22 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
                           fld(st, LBSWIM_PART_N), st.perm, lat->geom, pts, stream()));
     offset += st.num;
   }
-  // the fused step's z-halo sites read the force at the ends of the runs
+  // the fused step's z-halo sites read the force at the ends of the runs, its
+  // row walks the lists' entries as RunEntry records
   const bool edges = virt && fusion_rows(lat) > 0;
   if (edges && !lat->sp_edge)
     if (int rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_edge), 6 * sizeof(double) * lat->sp_lists))
       return rc;
+  if (edges && lat->sp_rec_cap < lcap) {
+    size_t c1 = lat->sp_rec_cap, c2 = lat->sp_rec_cap;
+    if (int rc = grow(reinterpret_cast<void**>(&lat->sp_rec), &c1, lcap, sizeof(RunEntry))) return rc;
+    if (int rc = grow(reinterpret_cast<void**>(&lat->sp_wz), &c2, lcap, sizeof(int))) return rc;
+    lat->sp_rec_cap = lcap;
+  }
   hipError_t e = launch_spread_lists(int(npts), lat->geom, pts, lat->sp_counts, lat->sp_off, lat->sp_cursor,
                                      lat->sp_list, lat->sp_sorted, stream());
-  if (e == hipSuccess && edges) e = launch_edge_force(lat->geom, pts, lat->sp_off, lat->sp_sorted, lat->sp_edge, stream());
+  if (e == hipSuccess && edges)
+    e = launch_edge_force(lat->geom, pts, lat->sp_off, lat->sp_sorted, lat->sp_edge, lat->sp_rec, lat->sp_wz, stream());
   if (e == hipSuccess && !virt) {
     if (lat->storage == LBSWIM_STORE_F64)
       e = launch_spread_force<double>(lat->geom, pts, lat->sp_off, lat->sp_sorted, static_cast<double*>(lat->d_force),
@@ -1134,21 +1220,62 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
   CHECK_ARG(ntr == 0 || tr != nullptr, "tracer list is NULL");
   for (int i = 0; i < nsw; ++i) CHECK_ARG(sw[i] != nullptr, "swimmer handle %d is NULL", i);
   for (int i = 0; i < ntr; ++i) CHECK_ARG(tr[i] != nullptr, "tracer handle %d is NULL", i);
-  for (int s = 0; s < nsteps; ++s) {
-    int rc;
-    if ((rc = spread_forces(lat, sw, nsw))) return rc;
+  // With the fused step the tracer moves of step s run on the side stream
+  // while the main stream goes on with the swimmer moves and the spreading of
+  // step s+1; the fused launch of step s+1, which rewrites the interleaved u
+  // they read, waits for them.  Nothing else reads the tracers inside the
+  // call; the main stream waits for the side stream before returning (also on
+  // error), so later calls are ordered.
+  const bool overlap = ntr > 0 && nsteps > 0 && fusion_rows(lat) > 0;
+  if (overlap)
+    if (int rc = ensure_side_stream()) return rc;
+  bool side_used = false;
+  auto join = [&]() -> int {
+    if (!side_used) return LBSWIM_OK;
+    side_used = false;
+    HIP_TRY(hipEventRecord(g_rt.joined, g_rt.side));
+    HIP_TRY(hipStreamWaitEvent(g_rt.stream, g_rt.joined, 0));
+    lat->tracers_reading = false;
+    return LBSWIM_OK;
+  };
+  int rc = LBSWIM_OK;
+  for (int s = 0; s < nsteps && !rc; ++s) {
+    if ((rc = spread_forces(lat, sw, nsw))) break;
     if (const int rows = fusion_rows(lat)) {
-      if ((rc = lattice_step_hydro_impl(lat, rows))) return rc;
+      if ((rc = lattice_step_hydro_impl(lat, rows))) break;
     } else {
-      if ((rc = lattice_step_impl(lat))) return rc;
-      if ((rc = lattice_hydro_impl(lat))) return rc;
+      if ((rc = lattice_step_impl(lat))) break;
+      if ((rc = lattice_hydro_impl(lat))) break;
     }
-    for (int i = 0; i < nsw; ++i)
-      if ((rc = swimmers_move_impl(sw[i], lat))) return rc;
-    for (int i = 0; i < ntr; ++i)
-      if ((rc = tracers_move_impl(tr[i], lat))) return rc;
+    if (overlap) {
+      if (!lat->tracers_done && hipEventCreateWithFlags(&lat->tracers_done, hipEventDisableTiming) != hipSuccess) {
+        rc = fail(LBSWIM_EHIP, "hipEventCreateWithFlags failed");
+        break;
+      }
+      if (hipEventRecord(g_rt.u_ready, g_rt.stream) != hipSuccess ||
+          hipStreamWaitEvent(g_rt.side, g_rt.u_ready, 0) != hipSuccess) {
+        rc = fail(LBSWIM_EHIP, "ordering the tracer moves after the fused step failed");
+        break;
+      }
+      side_used = true;
+      {
+        SideScope on_side;
+        for (int i = 0; i < ntr && !rc; ++i) rc = tracers_move_impl(tr[i], lat);
+      }
+      if (rc) break;
+      if (hipEventRecord(lat->tracers_done, g_rt.side) != hipSuccess) {
+        rc = fail(LBSWIM_EHIP, "hipEventRecord failed");
+        break;
+      }
+      lat->tracers_reading = true;
+    }
+    for (int i = 0; i < nsw && !rc; ++i) rc = swimmers_move_impl(sw[i], lat);
+    if (rc) break;
+    if (!overlap)
+      for (int i = 0; i < ntr && !rc; ++i) rc = tracers_move_impl(tr[i], lat);
   }
-  return LBSWIM_OK;
+  const int jrc = join();
+  return rc ? rc : jrc;
 }
 
 // ---- diagnostics ----------------------------------------------------------------
@@ -1599,6 +1726,7 @@ void ens_advance(lbswim_ensemble* e, int k) {
     m.lat->time_step += k;
     m.lat->force_virtual = false;  // the batched step spreads into the force array
     m.lat->u_aos_valid = false;    // and writes u only in the reference layout
+    m.lat->u_virtual = false;
     for (auto* s : m.sw) s->st.moves_since_order += k;
     for (auto* t : m.tr) t->st.moves_since_order += k;
   }

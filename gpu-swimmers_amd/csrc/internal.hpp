@@ -27,6 +27,17 @@ Geom make_geom(int nx, int ny, int nz);
 
 // ---- lattice (storage T = double: reference arithmetic; T = float: deviation form)
 struct SpreadPoint;
+// Entry k of the sorted lists for the fused step's row walk (lattices with nz
+// % 64 == 0, where a list's 64-site run is one row x, y): the point's row
+// weight dx_i dy_j at that run, its z weights and its force.  The z of its
+// first stencil site is kept beside it (wz[k]), -1 for an entry the walk
+// skips (a repeated id, or a point whose stencil misses the row).
+struct RunEntry {
+  double dxy;
+  double zw[4];
+  double F[3];
+};
+static_assert(sizeof(RunEntry) == 64, "RunEntry layout");
 // Sorted point lists of the lattice's 64-site runs (see launch_spread_lists).
 // Passed to the collide and CalcHydro launchers instead of a stored force
 // field, they evaluate every site's force from its block's list.
@@ -34,7 +45,9 @@ struct VForce {
   const SpreadPoint* pts;
   const uint32_t* off;
   const uint32_t* sorted;
-  const double* edge;  // launch_edge_force output (fused step only)
+  const double* edge;    // launch_edge_force output (fused step only)
+  const RunEntry* rec;   // launch_edge_force entries (fused step only)
+  const int* wz;
 };
 // `vf` null: the force is read from `force`.
 template <typename T>
@@ -49,11 +62,14 @@ hipError_t launch_calc_hydro(const T* f, const T* force, T* rho, T* u, const Geo
 // {2, 4, 8} (rows per tile), and 15 n sizeof(T) < 4 GiB (32-bit buffer
 // offsets).  `strips` splits the x sweep (0: one workgroup per CU).
 bool fused_step_supported(const Geom& g, int ty, size_t elem);
-// u_aos (3 n values) receives u again with a site's components together,
-// u_aos[3 ijk + a], which the particle moves gather from.
+// u is written only component-interleaved, u_aos[3 ijk + a] (3 n values),
+// which the particle moves gather from; launch_u_from_aos writes the
+// reference layout from it.
 template <typename T>
-hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u, T* u_aos, const Geom& g,
-                             double tau_s, double tau_b, const VForce* vf, int ty, int strips, hipStream_t s);
+hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u_aos, const Geom& g, double tau_s,
+                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s);
+template <typename T>
+hipError_t launch_u_from_aos(const T* aos, T* u, const Geom& g, hipStream_t s);
 template <typename T>
 hipError_t launch_init_from_hydro(const T* rho, const T* u, T* f, const Geom& g, hipStream_t s);
 
@@ -110,9 +126,11 @@ hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint
                                T* force, hipStream_t s);
 
 // Force at the first and last site of every list's 64-site run:
-// edge[6 l + 3 e + a] (e = 0 first, 1 last), 6 spread_lists(g) doubles.
+// edge[6 l + 3 e + a] (e = 0 first, 1 last), 6 spread_lists(g) doubles; and,
+// for g.nz % 64 == 0, the RunEntry rec[k] and wz[k] of every list entry k
+// (capacity spread_list_capacity).
 hipError_t launch_edge_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
-                             double* edge, hipStream_t s);
+                             double* edge, RunEntry* rec, int* wz, hipStream_t s);
 
 // ---- batched launches over the members of an ensemble
 // One lattice of a batch.  g[cur ^ parity] holds the state at the start of a

@@ -602,8 +602,7 @@ struct FusedArgs {
   VForce vf;
   T* gNew;
   T* rho;
-  T* u;
-  T* u_aos;  // u again with a site's components together, u_aos[3 ijk + a], for the particle moves
+  T* u_aos;  // u with a site's components together, u_aos[3 ijk + a]
   Geom g;
   T tau_s, tau_b, omega_s, omega_b;
 };
@@ -683,15 +682,15 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     return x < 0 ? x + g.nx : x;
   };
   // Virtual force of a row wavefront: the row's list for plane it is walked
-  // four lanes per entry (quad_row_force) from entries loaded ahead -- list bounds
-  // (offv) two planes before, ids (idv) and then the point records (rec)
+  // four lanes per entry (quad_row_force) from entries loaded ahead -- list
+  // bounds (offv) two planes before, the entries' RunEntry records (rec)
   // during the previous plane -- so each dependent load has at least half a
   // plane's work to complete behind.
   constexpr bool VFROW = VF && !ZH;
   const bool simple = g.nx >= 4 && g.ny >= 4;
   // offv[p & 1]: bounds of plane p (lane 0 first entry, lane 1 end), loaded
   // two iterations before take_list reads them
-  uint32_t offv[2] = {0, 0}, idv = 0, o0 = 0, m0 = 0, o1 = 0, m1 = 0;
+  uint32_t offv[2] = {0, 0}, kv = 0, o0 = 0, m0 = 0, o1 = 0, m1 = 0;
   QuadEntry rec{};
   auto list_of = [&](int it) { return (uint32_t(plane(it)) * nyz + yrow + zrow) / WAVE_SITES; };
   auto load_off = [&](int it) {
@@ -705,7 +704,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     const uint32_t v = (it & 1) ? offv[1] : offv[0];
     o = uint32_t(__builtin_amdgcn_readlane(int(v), 0));
     m = uint32_t(__builtin_amdgcn_readlane(int(v), 1)) - o;
-    idv = A.vf.sorted[uint32_t(lane >> 2) < m ? o + uint32_t(lane >> 2) : 0u];  // four lanes per entry
+    kv = uint32_t(lane >> 2) < m ? o + uint32_t(lane >> 2) : 0u;  // four lanes per entry
   };
   // pulls of plane it (and its stored or run-end force)
   auto pull = [&](int it, T(&f)[Q], T(&sf)[3]) {
@@ -745,7 +744,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
       const int x = plane(it);
       double acc[3];
       if (simple)
-        quad_row_force(rec, o0, m0, A.vf.pts, A.vf.sorted, x, y, z, g, acc);
+        quad_row_force(rec, o0, m0, A.vf.rec, A.vf.wz, z, g, acc);
       else
         wave_force(A.vf.pts, A.vf.off, A.vf.sorted, list_of(it), true, x, y, z, g, acc);
 #pragma unroll
@@ -764,8 +763,8 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     collide_site(f, F, A.tau_s, A.tau_b, A.omega_s, A.omega_b, out);
     uint32_t o2 = 0, m2 = 0;
     if constexpr (VFROW) {
-      load_quad_entry(A.vf.pts, idv, rec);  // records of plane it+1
-      take_list(it + 2, o2, m2);       // ids of plane it+2, from bounds loaded with plane it
+      load_quad_entry(A.vf.rec, A.vf.wz, kv, rec);  // entries of plane it+1
+      take_list(it + 2, o2, m2);                    // plane it+2's entries, from bounds loaded with plane it
     }
     if constexpr (STORE) {
 #pragma unroll
@@ -814,8 +813,8 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
       hydro_site(fl, Fh, rh, uh);
       const uint32_t sp = uint32_t(plane(it - 1)) * nyz + yrow + zrow;
       buffer_store_nt(rh, buffer_rsrc(A.rho), zl * B, sp * B);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) buffer_store_nt(uh[a], buffer_rsrc(A.u), zl * B, (uint32_t(a) * n + sp) * B);
+      // u goes only to the component-interleaved array; the reference
+      // layout is written from it when asked for (launch_u_from_aos)
 #pragma unroll
       for (int a = 0; a < 3; ++a) buffer_store(uh[a], buffer_rsrc(A.u_aos), (3 * zl + uint32_t(a)) * B, 3 * sp * B);
     }
@@ -829,12 +828,13 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
   // Iterations: 0 (plane xa-1, halo), 1 (xa: stored, no CalcHydro yet),
   // 2..last-1 (stored + CalcHydro of the plane before), last (plane xb, halo:
   // CalcHydro of xb-1 only).  last >= 2.  Two register sets: the next plane's
-  // pulls are in flight while this plane is collided.
+  // pulls are in flight while this plane is collided (a third set, for FP32,
+  // measured slower).
   T fa[Q], fb[Q], sa[3] = {}, sb[3] = {}, F[3];
-  if constexpr (VFROW) {  // prime the list pipeline: plane 0 records, plane 1 ids, plane 2 bounds
+  if constexpr (VFROW) {  // prime the list pipeline: plane 0 entries, plane 1 indices, plane 2 bounds
     load_off(0);
     take_list(0, o0, m0);
-    load_quad_entry(A.vf.pts, idv, rec);
+    load_quad_entry(A.vf.rec, A.vf.wz, kv, rec);
     load_off(1);
     take_list(1, o1, m1);
     load_off(2);
@@ -874,9 +874,12 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_step_hydro(FusedArgs<T> A, Fu
   const uint32_t L = xcd_block(round_xcd(t.nblk));
   if (L >= t.nblk) return;
   const Geom& g = A.g;
-  // tiles along y are adjacent in L, so one XCD's L2 serves the row halos
-  const uint32_t yt = L % t.nyt, rest = L / t.nyt;
-  const uint32_t zt = rest % t.nzt, sx = rest / t.nzt;
+  // Tiles along z, then along y, are adjacent in L, so one XCD sweeps whole
+  // z-rings of neighbouring tiles together: the lines a 64-site run's z-shifted
+  // pulls share with the next run (a third of a FP32 run's lines) and most row
+  // halos come from its own L2.
+  const uint32_t zt = L % t.nzt, rest = L / t.nzt;
+  const uint32_t yt = rest % t.nyt, sx = rest / t.nyt;
   const int xa = int(sx * t.strip), xb = min(xa + int(t.strip), g.nx);
   const int last = xb - xa + 1;  // iterations 0..last collide planes xa-1..xb
   const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64), lane = int(threadIdx.x) % 64;
@@ -1107,8 +1110,8 @@ bool fused_step_supported(const Geom& g, int ty, size_t elem) {
 }
 
 template <typename T>
-hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u, T* u_aos, const Geom& g,
-                             double tau_s, double tau_b, const VForce* vf, int ty, int strips, hipStream_t s) {
+hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u_aos, const Geom& g, double tau_s,
+                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s) {
   if (!fused_step_supported(g, ty, sizeof(T)) || strips < 0) return hipErrorInvalidValue;
   FusedTiles t;
   t.nyt = uint32_t(g.ny / ty);
@@ -1132,11 +1135,10 @@ hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* 
   FusedArgs<T> A;
   A.gOld = fOld;
   A.force = force;
-  A.vf = vf ? *vf : VForce{nullptr, nullptr, nullptr, nullptr};
-  if (vf && !vf->edge) return hipErrorInvalidValue;
+  A.vf = vf ? *vf : VForce{};
+  if (vf && !(vf->edge && vf->rec && vf->wz)) return hipErrorInvalidValue;
   A.gNew = fNew;
   A.rho = rho;
-  A.u = u;
   A.u_aos = u_aos;
   A.g = g;
   A.tau_s = T(tau_s);
@@ -1157,10 +1159,30 @@ hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* 
 #undef LBS_FUSED
   return hipGetLastError();
 }
-template hipError_t launch_step_hydro<double>(const double*, const double*, double*, double*, double*, double*,
-                                              const Geom&, double, double, const VForce*, int, int, hipStream_t);
-template hipError_t launch_step_hydro<float>(const float*, const float*, float*, float*, float*, float*, const Geom&,
-                                             double, double, const VForce*, int, int, hipStream_t);
+template hipError_t launch_step_hydro<double>(const double*, const double*, double*, double*, double*, const Geom&,
+                                              double, double, const VForce*, int, int, hipStream_t);
+template hipError_t launch_step_hydro<float>(const float*, const float*, float*, float*, float*, const Geom&, double,
+                                             double, const VForce*, int, int, hipStream_t);
+
+// u[a n + s] = u_aos[3 s + a]: the reference layout from the fused step's
+// component-interleaved u.  Thread per site; the three loads of a site are
+// contiguous and each component's stores coalesce.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_u_from_aos(const T* __restrict__ aos, T* __restrict__ u, uint32_t n) {
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= n) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) store_stream(&u[size_t(a) * n + s], aos[3 * size_t(s) + a]);
+}
+
+template <typename T>
+hipError_t launch_u_from_aos(const T* aos, T* u, const Geom& g, hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_u_from_aos<T>, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, aos, u, g.n);
+  return hipGetLastError();
+}
+template hipError_t launch_u_from_aos<double>(const double*, double*, const Geom&, hipStream_t);
+template hipError_t launch_u_from_aos<float>(const float*, float*, const Geom&, hipStream_t);
 
 template <>
 hipError_t launch_calc_hydro<double>(const double* f, const double* force, double* rho, double* u, const Geom& g,

@@ -237,13 +237,17 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_add_forces(int ns, double P
 // streamed plane by plane: lane q < 12 forms stencil coordinate q (axis q / 4,
 // offset q % 4) once and the lanes read the ones they need from each other;
 // lane c = 4j + k gathers column (j, k) of every x plane i; for each plane the
-// 16 lanes write their products to the group's 48-double LDS slice and lanes
-// 0..2 add them to their component in t = 16i + 4j + k order -- the i, j, k
-// order of the reference -- then the next plane reuses the slice.  A group's
-// lanes are in one wavefront and LDS operations of a wavefront complete in
-// order, so no workgroup barrier is needed.  Returns component c on lanes
-// c < 3 (0 on the others).  Every lane of the group must call it.
-constexpr int PLANE_STRIDE = 49;  // 48 products + 1: groups start on different banks
+// 16 lanes write their products to the group's LDS slice, SUM_PLANES stencil
+// planes i at a time, component-major (slice[(3p + a) 16 + c]), and lanes
+// 0..2 add the 16 products of their component of each plane in t = 16i + 4j
+// + k order -- the i, j, k order of the reference -- reading them 16 bytes at
+// a time; then the next planes reuse the slice.  A group's lanes are in one
+// wavefront and LDS operations of a wavefront complete in order, so no
+// workgroup barrier is needed.  Returns component c on lanes c < 3 (0 on the
+// others).  Every lane of the group must call it.
+constexpr int SUM_PLANES = 1;
+// products of SUM_PLANES planes + 2: 16-byte aligned, groups start on different banks
+constexpr int PLANE_STRIDE = 48 * SUM_PLANES + 2;
 
 // One lane's share of a stencil: its column's weights and the gathered u.
 struct GroupGather {
@@ -297,17 +301,27 @@ __device__ __forceinline__ double group_sum(const GroupGather& G, int c, double*
   const int d = c < 3 ? c : 0;
   double vel = 0.0;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const double delta3d = G.dx[i] * G.dy * G.dz;
-    terms[3 * c + 0] = delta3d * G.uv[i][0];
-    terms[3 * c + 1] = delta3d * G.uv[i][1];
-    terms[3 * c + 2] = delta3d * G.uv[i][2];
+  for (int i0 = 0; i0 < 4; i0 += SUM_PLANES) {
+#pragma unroll
+    for (int p = 0; p < SUM_PLANES; ++p) {
+      const double delta3d = G.dx[i0 + p] * G.dy * G.dz;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) terms[(3 * p + a) * GROUP + c] = delta3d * G.uv[i0 + p][a];
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (c < 3) {
 #pragma unroll
-      for (int cc = 0; cc < 16; ++cc) vel += terms[3 * cc + d];
+      for (int p = 0; p < SUM_PLANES; ++p) {
+        const double2* __restrict__ row = reinterpret_cast<const double2*>(terms + (3 * p + d) * GROUP);
+#pragma unroll
+        for (int h = 0; h < GROUP / 2; ++h) {
+          const double2 two = row[h];
+          vel += two.x;
+          vel += two.y;
+        }
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -434,7 +448,7 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp,
                                                           uint64_t* __restrict__ prng, const int* __restrict__ perm,
                                                           const T* __restrict__ u, Geom g,
                                                           unsigned long long* __restrict__ tumbles, uint32_t nb) {
-  __shared__ double lds[GROUPS * PLANE_STRIDE];
+  __shared__ __attribute__((aligned(16))) double lds[GROUPS * PLANE_STRIDE];
   const uint32_t blk = xcd_block(round_xcd(nb));
   if (blk >= nb) return;
   swimmers_move_body<T, AOS>(ns, cp, r, v, nv, prng, perm, u, g, tumbles, blk, lds);
@@ -444,7 +458,7 @@ template <typename T, bool AOS>
 __global__ __launch_bounds__(TBLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
                                                          double* __restrict__ v, const T* __restrict__ u, Geom g,
                                                          uint32_t nb) {
-  __shared__ double lds[TGROUPS * PLANE_STRIDE];
+  __shared__ __attribute__((aligned(16))) double lds[TGROUPS * PLANE_STRIDE];
   const uint32_t blk = xcd_block(round_xcd(nb));
   if (blk >= nb) return;
   tracers_move_body<T, AOS>(nt, r, sd, v, u, g, blk, lds);
@@ -457,7 +471,7 @@ template <typename T>
 __global__ __launch_bounds__(PBLOCK) void kb_swimmers_move(const LatDesc* __restrict__ ldesc,
                                                            const SwimDesc* __restrict__ sdesc,
                                                            const WorkItem* __restrict__ work, uint32_t nwork) {
-  __shared__ double lds[GROUPS * PLANE_STRIDE];
+  __shared__ __attribute__((aligned(16))) double lds[GROUPS * PLANE_STRIDE];
   const uint32_t wi = xcd_block(round_xcd(nwork));
   if (wi >= nwork) return;
   const WorkItem w = work[wi];
@@ -471,7 +485,7 @@ template <typename T>
 __global__ __launch_bounds__(TBLOCK) void kb_tracers_move(const LatDesc* __restrict__ ldesc,
                                                           const TracerDesc* __restrict__ tdesc,
                                                           const WorkItem* __restrict__ work, uint32_t nwork) {
-  __shared__ double lds[TGROUPS * PLANE_STRIDE];
+  __shared__ __attribute__((aligned(16))) double lds[TGROUPS * PLANE_STRIDE];
   const uint32_t wi = xcd_block(round_xcd(nwork));
   if (wi >= nwork) return;
   const WorkItem w = work[wi];
@@ -791,6 +805,58 @@ __global__ __launch_bounds__(SITE_BLOCK) void k_edge_force(Geom g, uint32_t nlis
   for (int a = 0; a < 3; ++a) edge[3 * size_t(i) + a] = acc[a];
 }
 
+// RunEntry rec[k] and wz[k] of every list entry k (g.nz % 64 == 0: the run of
+// list l is one row x, y): the row weight dx_i dy_j at that row as the row
+// walk of wave_force forms it, and wz = -1 where that walk skips the entry
+// (the id repeats the previous entry's, or the stencil misses the row).  A
+// wavefront takes ENTRY_LISTS consecutive lists, whose entries are
+// contiguous, one lane per entry, and finds an entry's list among the
+// lists' bounds held by lanes 0..ENTRY_LISTS.
+constexpr uint32_t ENTRY_LISTS = 16;
+__global__ __launch_bounds__(SITE_BLOCK) void k_run_entries(Geom g, uint32_t nlists,
+                                                           const SpreadPoint* __restrict__ pts,
+                                                           const uint32_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ sorted,
+                                                           RunEntry* __restrict__ rec, int* __restrict__ wz) {
+  const uint32_t l0 = (blockIdx.x * SITE_BLOCK + threadIdx.x) / WAVE_SITES * ENTRY_LISTS;
+  if (l0 >= nlists) return;  // whole wavefront
+  const int lane = int(threadIdx.x % WAVE_SITES);
+  // lane j <= ENTRY_LISTS: first entry of list l0 + j (the end for j = ENTRY_LISTS)
+  const uint32_t oj = off[min(l0 + min(uint32_t(lane), ENTRY_LISTS), nlists)];
+  const uint32_t k0 = uint32_t(__builtin_amdgcn_readlane(int(oj), 0));
+  const uint32_t k1 = uint32_t(__builtin_amdgcn_readlane(int(oj), int(ENTRY_LISTS)));
+  for (uint32_t base = k0; base < k1; base += WAVE_SITES) {
+    const uint32_t k = base + uint32_t(lane);
+    // list of entry k: the last j with off[l0 + j] <= k (every lane searches,
+    // so every lane the shuffles read from is active)
+    int j = 0;
+#pragma unroll
+    for (int step = int(ENTRY_LISTS) / 2; step; step >>= 1)
+      if (uint32_t(__shfl(int(oj), j + step)) <= k) j += step;
+    const uint32_t first = uint32_t(__shfl(int(oj), j));
+    if (k >= k1) continue;
+    const uint32_t l = l0 + uint32_t(j);
+    const SiteXYZ c = site_xyz(l * WAVE_SITES, g);
+    const uint32_t id = sorted[k];
+    const uint32_t prev = k > first ? sorted[k - 1] : 0xFFFFFFFFu;
+    const SpreadPoint* __restrict__ p = pts + id;
+    int di = c.x - p->wc[0];
+    di += di < 0 ? g.nx : 0;
+    int dj = c.y - p->wc[4];
+    dj += dj < 0 ? g.ny : 0;
+    const bool skip = id == prev || (di | dj) >= 4;
+    RunEntry r;
+    r.dxy = pick4(p->del[0], p->del[1], p->del[2], p->del[3], di & 3) *
+            pick4(p->del[4], p->del[5], p->del[6], p->del[7], dj & 3);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r.zw[q] = p->del[8 + q];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r.F[a] = p->F[a];
+    rec[k] = r;
+    wz[k] = skip ? -1 : p->wc[8];
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __restrict__ ldesc,
                                                               const WorkItem* __restrict__ block_work,
@@ -941,11 +1007,17 @@ template hipError_t launch_spread_force<float>(const Geom&, const SpreadPoint*, 
                                                float*, hipStream_t);
 
 hipError_t launch_edge_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
-                             double* edge, hipStream_t s) {
+                             double* edge, RunEntry* rec, int* wz, hipStream_t s) {
   const uint32_t nl = spread_lists(g);
   if (nl == 0) return hipSuccess;
   hipLaunchKernelGGL(k_edge_force, dim3((2 * nl + SITE_BLOCK - 1) / SITE_BLOCK), dim3(SITE_BLOCK), 0, s, g, nl, pts,
                      off, sorted, edge);
+  if (rec && wz && g.nz % WAVE_SITES == 0) {
+    const uint32_t waves = (nl + ENTRY_LISTS - 1) / ENTRY_LISTS;
+    const uint32_t per_block = SITE_BLOCK / WAVE_SITES;
+    hipLaunchKernelGGL(k_run_entries, dim3((waves + per_block - 1) / per_block), dim3(SITE_BLOCK), 0, s, g, nl, pts,
+                       off, sorted, rec, wz);
+  }
   return hipGetLastError();
 }
 

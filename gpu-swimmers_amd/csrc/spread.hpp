@@ -257,27 +257,22 @@ __device__ __forceinline__ void lanes_row_force(const LaneEntry& r, uint32_t o, 
 }
 
 // One list entry spread over four lanes: lane 4e + q holds entry e (of the
-// first 16) with its z weight q.
+// first 16) with its z weight q, from the entry's RunEntry.
 struct QuadEntry {
-  double del[8];  // x and y weights
-  double zq;      // z weight q
+  double dxy;  // row weight dx_i dy_j at this run
+  double zq;   // z weight q
   double F[3];
-  int wc0, wc4, wc8;
-  uint32_t id;
+  int wz;      // z of the first stencil site, -1: skipped entry
 };
 
-__device__ __forceinline__ void load_quad_entry(const SpreadPoint* __restrict__ pts, uint32_t id, QuadEntry& r) {
-  const SpreadPoint* __restrict__ p = pts + id;
-  const int q = int(threadIdx.x % 4);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) r.del[k] = p->del[k];
-  r.zq = p->del[8 + q];
+__device__ __forceinline__ void load_quad_entry(const RunEntry* __restrict__ rec, const int* __restrict__ wz,
+                                                uint32_t k, QuadEntry& r) {
+  const RunEntry* __restrict__ p = rec + k;
+  r.dxy = p->dxy;
+  r.zq = p->zw[threadIdx.x % 4];
 #pragma unroll
   for (int a = 0; a < 3; ++a) r.F[a] = p->F[a];
-  r.wc0 = p->wc[0];
-  r.wc4 = p->wc[4];
-  r.wc8 = p->wc[8];
-  r.id = id;
+  r.wz = wz[k];
 }
 
 // The row walk of wave_force (simple lattice, nz % 64 == 0: X and Y uniform)
@@ -288,28 +283,19 @@ __device__ __forceinline__ void load_quad_entry(const SpreadPoint* __restrict__ 
 // whose q is its own offset.  Entries past the 16th are read from memory.
 // Same products, same order, same sum as wave_force.
 __device__ __forceinline__ void quad_row_force(const QuadEntry& r, uint32_t o, uint32_t m,
-                                               const SpreadPoint* __restrict__ pts,
-                                               const uint32_t* __restrict__ sorted, int Xu, int Yu, int Z,
+                                               const RunEntry* __restrict__ rec, const int* __restrict__ wz, int Z,
                                                const Geom& g, double (&acc)[3]) {
   constexpr uint32_t QUADS = WAVE_SITES / 4;
   acc[0] = acc[1] = acc[2] = 0.0;
   const int lane = int(threadIdx.x % WAVE_SITES), e = lane >> 2, q = lane & 3;
-  int di = Xu - r.wc0;
-  di += di < 0 ? g.nx : 0;
-  int dj = Yu - r.wc4;
-  dj += dj < 0 ? g.ny : 0;
-  const uint32_t id_before = uint32_t(__shfl_up(int(r.id), 4, WAVE_SITES));
-  const bool dup = e > 0 && id_before == r.id;
-  const bool use = uint32_t(e) < m && !dup && (di | dj) < 4;
-  const double dxy = pick4(r.del[0], r.del[1], r.del[2], r.del[3], di & 3) *
-                     pick4(r.del[4], r.del[5], r.del[6], r.del[7], dj & 3);
-  const double delta3d = dxy * r.zq;
+  const bool use = uint32_t(e) < m && r.wz >= 0;
+  const double delta3d = r.dxy * r.zq;
   const double c0 = delta3d * r.F[0], c1 = delta3d * r.F[1], c2 = delta3d * r.F[2];
   uint64_t mask = __ballot(use && q == 0);
   while (mask) {
     const int L = __builtin_ctzll(mask);  // lane 4e of the next entry
     mask &= mask - 1;
-    int dk = Z - __builtin_amdgcn_readlane(r.wc8, L);
+    int dk = Z - __builtin_amdgcn_readlane(r.wz, L);
     dk += dk < 0 ? g.nz : 0;
     const int src = L + (dk & 3);
     const double t0 = __shfl(c0, src, WAVE_SITES), t1 = __shfl(c1, src, WAVE_SITES),
@@ -321,30 +307,21 @@ __device__ __forceinline__ void quad_row_force(const QuadEntry& r, uint32_t o, u
     }
   }
   if (m > QUADS) {  // rare: the rest of a long list, one entry at a time
-    uint32_t prev = uint32_t(__builtin_amdgcn_readlane(int(r.id), WAVE_SITES - 4));
     for (uint32_t k = o + QUADS; k < o + m; ++k) {
-      const uint32_t id = sorted[k];
-      if (id == prev) continue;
-      prev = id;
-      const SpreadPoint* __restrict__ p = pts + id;
-      int pi = Xu - p->wc[0];
-      pi += pi < 0 ? g.nx : 0;
-      int pj = Yu - p->wc[4];
-      pj += pj < 0 ? g.ny : 0;
-      if ((pi | pj) >= 4) continue;
-      const double dxy_k = p->del[pi] * p->del[4 + pj];
-      const double F0 = p->F[0], F1 = p->F[1], F2 = p->F[2];
-      int dk = Z - p->wc[8];
+      const int w = wz[k];
+      if (w < 0) continue;
+      const RunEntry* __restrict__ p = rec + k;
+      int dk = Z - w;
       dk += dk < 0 ? g.nz : 0;
       uint64_t zbits = 0;
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq)
-        zbits |= __builtin_bit_cast(uint64_t, p->del[8 + qq]) & (uint64_t(0) - uint64_t(dk == qq));
-      const double d3 = dxy_k * __builtin_bit_cast(double, zbits);
+        zbits |= __builtin_bit_cast(uint64_t, p->zw[qq]) & (uint64_t(0) - uint64_t(dk == qq));
+      const double d3 = p->dxy * __builtin_bit_cast(double, zbits);
       if (dk < 4) {
-        acc[0] += d3 * F0;
-        acc[1] += d3 * F1;
-        acc[2] += d3 * F2;
+        acc[0] += d3 * p->F[0];
+        acc[1] += d3 * p->F[1];
+        acc[2] += d3 * p->F[2];
       }
     }
   }

@@ -71,6 +71,8 @@ cdef extern from "lbswim.h":
     int lbswim_lattice_set_force_mode(lbswim_lattice* lat, int mode)
     int lbswim_lattice_sync_force(lbswim_lattice* lat)
     int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual)
+    int lbswim_lattice_sync_u(lbswim_lattice* lat)
+    int lbswim_lattice_u_is_virtual(lbswim_lattice* lat, int* is_virtual)
     int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int strips)
     int lbswim_lattice_fusion(lbswim_lattice* lat, int* active, int* rows, int* strips)
     int lbswim_lattice_params(lbswim_lattice* lat, const lbswim_params** host)
@@ -404,6 +406,18 @@ cdef class Lattice:
     def sync_force(self):
         """Write the force array if the last coupled step left it virtual."""
         check(lbswim_lattice_sync_force(self.handle()))
+
+    def sync_u(self):
+        """Write u in the reference layout if fused steps left it only in the
+        particle moves' component-interleaved copy."""
+        check(lbswim_lattice_sync_u(self.handle()))
+
+    @property
+    def u_is_virtual(self):
+        """True while u is not yet written in the reference layout."""
+        cdef int v = 0
+        check(lbswim_lattice_u_is_virtual(self.handle(), &v))
+        return bool(v)
 
     def set_fusion(self, enable=True, int rows=0, int strips=0):
         """Fuse Step and CalcHydro of the coupled step into one launch where the

@@ -226,6 +226,16 @@ int lbswim_lattice_sync_force(lbswim_lattice* lat);
 /* *is_virtual = 1 if the last coupled step left the force virtual (not yet
  * written to the force array). No reference counterpart. */
 int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual);
+/* After fused coupled steps (below) u lives only in a component-interleaved
+ * copy that the particle moves read; the reference-layout u array is written
+ * on demand: by lbswim_lattice_d2h(LBSWIM_U), lbswim_lattice_field(LBSWIM_U)
+ * with a device pointer requested, lbswim_lattice_init_from_hydro and
+ * lbswim_lattice_sync_u.  A caller that reads u on the device by a pointer
+ * obtained earlier calls lbswim_lattice_sync_u first.  No reference
+ * counterpart. */
+int lbswim_lattice_sync_u(lbswim_lattice* lat);
+/* *is_virtual = 1 while u is not yet written in the reference layout. */
+int lbswim_lattice_u_is_virtual(lbswim_lattice* lat, int* is_virtual);
 /* Step fusion in lbswim_system_step: with `enable` (the default) and a
  * lattice with nz % 64 == 0, ny % rows == 0 and 15 n sizeof(storage) < 4 GiB,
  * Step and CalcHydro run as one launch that reads the distributions once,

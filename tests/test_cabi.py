@@ -58,6 +58,7 @@ def test_invalid_arguments_rejected_without_device(lib):
     assert lib.lbswim_lattice_create(2048, 2048, 1024, 0.25, 0.25, 0, ctypes.byref(out)) == EINVAL
     assert out.value is None
     assert lib.lbswim_lattice_step(None) == EINVAL
+    assert lib.lbswim_lattice_sync_u(None) == EINVAL
     lib.lbswim_swimmers_create.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     assert lib.lbswim_swimmers_create(-1, None, ctypes.byref(out)) == EINVAL
     lib.lbswim_tracers_create.argtypes = [ctypes.c_int, ctypes.c_void_p]

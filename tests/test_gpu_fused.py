@@ -124,3 +124,56 @@ def test_moves_use_u_written_after_fused_steps():
         out.append(collect_lbswim(lat, sw, tr))
     for k in out[0]:
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_u_written_on_demand_after_fused_steps():
+    """Fused steps leave u only in the moves' interleaved copy; InitFromHydro,
+    D2H and sync_u must see the u the two launches write."""
+    case = swimmer_case((4, 8, 64), seed=91)
+    out = []
+    for fusion in (None, (0, 0)):
+        lat, system, sw, tr = build_lbswim(case, "f64", "virtual")
+        lat.set_fusion(fusion is not None)
+        system.Step(3)
+        assert lat.u_is_virtual == (fusion is not None)
+        lat.InitFromHydro()  # rebuilds f from rho and u
+        assert not lat.u_is_virtual
+        system.Step(2)
+        lat.sync_u()
+        assert not lat.u_is_virtual
+        lat.sync_u()  # idempotent
+        system.Step(1)
+        out.append(collect_lbswim(lat, sw, tr))
+        assert not lat.u_is_virtual
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_tracer_moves_overlapping_the_next_step():
+    """With the fused step the tracer moves run on a second stream beside the
+    swimmer moves and the next step's spreading, and the next fused launch
+    waits for them.  Two tracer arrays, calls of odd and even step counts, and
+    a standalone move between calls must give the two launches' results bit
+    for bit."""
+    from lbswim import tracers
+    case = swimmer_case((4, 8, 64), seed=123, nt=500)
+    out = []
+    for fusion in (None, (0, 0)):
+        lat, system, sw, tr = build_lbswim(case, "f64", "virtual")
+        lat.set_fusion(fusion is not None)
+        tr2 = tracers.Array(200)
+        tr2.r[:] = case.tr_r[:, ::-1][:, :200] * 0.5 + 0.25
+        tr2.v[:] = 0.0
+        tr2.H2D()
+        system.AddThing(tr2)
+        system.Step(3)
+        system.Step(1)
+        tr2.Move(lat)  # reads the current buffer after the call has returned
+        system.Step(4)
+        res = collect_lbswim(lat, sw, tr)
+        tr2.D2H()
+        res["tr2_r"] = np.array(tr2.r).copy()
+        res["tr2_v"] = np.array(tr2.v).copy()
+        out.append(res)
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k

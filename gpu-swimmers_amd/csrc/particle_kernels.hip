@@ -412,23 +412,119 @@ __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp,
   prng[is] = ctr + 1;
 }
 
+// ---- stencil gathers by pairs of columns (tracer moves) ----------------------
+// A group of 8 lanes serves one tracer: lane c = 2j + h owns the stencil
+// columns (j, 2h) and (j, 2h + 1), whose products are consecutive in the
+// reference's i, j, k order, so each lane writes them to the plane's LDS
+// slice together, and a wavefront sums 8 tracers' stencils at once.
+constexpr int PAIR = 8;
+
+struct PairGather {
+  double dx[4], dy, dz[2];
+  double uv[4][2][3];
+};
+
+// The weights and wrapped coordinates are those of group_gather: lane c
+// computes the x (c < 4) or y weight of offset c & 3, and every lane the z
+// weight of offset c & 3, with the same expressions.
+template <typename T, bool AOS>
+__device__ __forceinline__ PairGather pair_gather(const T* __restrict__ u, const Geom& g, const double (&r)[3],
+                                                  int c) {
+  PairGather G;
+  const int axis = c >> 2;
+  const double ra = axis == 0 ? r[0] : r[1];
+  const int na = axis == 0 ? g.nx : g.ny;
+  const double x = ceil(ra - 2.) + (c & 3);
+  const int mine_idx = int(wrap_index(x, na));
+  const double mine_del = peskin(ra - x);
+  const double zq = ceil(r[2] - 2.) + (c & 3);
+  const int z_idx = int(wrap_index(zq, g.nz));
+  const double z_del = peskin(r[2] - zq);
+  const int j = c >> 1, h = c & 1;
+  uint32_t ix[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    ix[i] = uint32_t(__shfl(mine_idx, i, PAIR));
+    G.dx[i] = __shfl(mine_del, i, PAIR);
+  }
+  const uint32_t iy = uint32_t(__shfl(mine_idx, 4 + j, PAIR)) * uint32_t(g.nz);
+  G.dy = __shfl(mine_del, 4 + j, PAIR);
+  uint32_t iz[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    iz[e] = uint32_t(__shfl(z_idx, 2 * h + e, PAIR));
+    G.dz[e] = __shfl(z_del, 2 * h + e, PAIR);
+  }
+  const size_t n = g.n;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const uint32_t ijk = ix[i] * g.nyz + iy + iz[e];
+      if constexpr (AOS) {
+        const T* __restrict__ us = u + 3 * size_t(ijk);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) G.uv[i][e][a] = double(us[a]);
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) G.uv[i][e][a] = double(u[a * n + ijk]);
+      }
+    }
+  }
+  return G;
+}
+
+// group_sum for a pair gather: per plane i the lane's two products of each
+// component go to slice[a 16 + 2c + e] in one 16-byte write, and lanes 0..2
+// add the 16 products of their component in column order.  Same products,
+// same order.
+__device__ __forceinline__ double pair_sum(const PairGather& G, int c, double* __restrict__ terms) {
+  const int d = c < 3 ? c : 0;
+  double vel = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double delta0 = G.dx[i] * G.dy * G.dz[0];
+    const double delta1 = G.dx[i] * G.dy * G.dz[1];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      *reinterpret_cast<double2*>(terms + a * GROUP + 2 * c) = double2{delta0 * G.uv[i][0][a], delta1 * G.uv[i][1][a]};
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (c < 3) {
+      const double2* __restrict__ row = reinterpret_cast<const double2*>(terms + d * GROUP);
+#pragma unroll
+      for (int hh = 0; hh < GROUP / 2; ++hh) {
+        const double2 two = row[hh];
+        vel += two.x;
+        vel += two.y;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  return vel;
+}
+
 // DoTracerArrayMove, TracerArray.cu:12-43 (s updated per particle), a lane
-// group of 16 per tracer (group_interp).
-constexpr int TGROUPS = TRACER_GROUPS;  // tracers per workgroup: 1024 threads, contiguous in storage order
-constexpr int TBLOCK = TGROUPS * GROUP;
+// group of 8 per tracer (pair_gather, pair_sum).
+constexpr int TGROUPS = TRACER_GROUPS;  // tracers per workgroup: 512 threads, contiguous in storage order
+constexpr int TBLOCK = TGROUPS * PAIR;
+static_assert(SUM_PLANES == 1, "pair_sum stages one plane at a time");
 
 template <typename T, bool AOS = false>
 __device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r, double* __restrict__ sd,
                                                   double* __restrict__ v, const T* __restrict__ u, const Geom& g,
                                                   uint32_t blk, double* __restrict__ lds) {
-  const int grp = int(threadIdx.x) / GROUP, c = int(threadIdx.x) % GROUP;
+  const int grp = int(threadIdx.x) / PAIR, c = int(threadIdx.x) % PAIR;
   const int it = int(blk) * TGROUPS + grp;
   if (it >= nt) return;  // whole group: the lanes of a group share `it`
   double* terms = lds + grp * PLANE_STRIDE;
   const double R[3] = {r[it], r[size_t(nt) + it], r[2 * size_t(nt) + it]};
   const int d = c < 3 ? c : 0;
   const double s_old = c < 3 ? sd[size_t(nt) * d + it] : 0.0;
-  const double vel = group_interp<T, AOS>(u, g, R, c, terms);
+  const double vel = pair_sum(pair_gather<T, AOS>(u, g, R, c), c, terms);
   if (c >= 3) return;
   const int size = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
   const double Rd = d == 0 ? R[0] : (d == 1 ? R[1] : R[2]);
@@ -812,12 +908,18 @@ __global__ __launch_bounds__(SITE_BLOCK) void k_edge_force(Geom g, uint32_t nlis
 // wavefront takes ENTRY_LISTS consecutive lists, whose entries are
 // contiguous, one lane per entry, and finds an entry's list among the
 // lists' bounds held by lanes 0..ENTRY_LISTS.
+//
+// With `edge` (simple lattice: nx, ny >= 4) it also writes k_edge_force's
+// output: the entries reaching the first or last site of their run are taken
+// lane by lane in list order and added, with add_point's products, by the
+// lane that holds that list (lane j: list l0 + j).
 constexpr uint32_t ENTRY_LISTS = 16;
 __global__ __launch_bounds__(SITE_BLOCK) void k_run_entries(Geom g, uint32_t nlists,
                                                            const SpreadPoint* __restrict__ pts,
                                                            const uint32_t* __restrict__ off,
                                                            const uint32_t* __restrict__ sorted,
-                                                           RunEntry* __restrict__ rec, int* __restrict__ wz) {
+                                                           RunEntry* __restrict__ rec, int* __restrict__ wz,
+                                                           double* __restrict__ edge) {
   const uint32_t l0 = (blockIdx.x * SITE_BLOCK + threadIdx.x) / WAVE_SITES * ENTRY_LISTS;
   if (l0 >= nlists) return;  // whole wavefront
   const int lane = int(threadIdx.x % WAVE_SITES);
@@ -825,6 +927,7 @@ __global__ __launch_bounds__(SITE_BLOCK) void k_run_entries(Geom g, uint32_t nli
   const uint32_t oj = off[min(l0 + min(uint32_t(lane), ENTRY_LISTS), nlists)];
   const uint32_t k0 = uint32_t(__builtin_amdgcn_readlane(int(oj), 0));
   const uint32_t k1 = uint32_t(__builtin_amdgcn_readlane(int(oj), int(ENTRY_LISTS)));
+  double ef[3] = {0.0, 0.0, 0.0}, el[3] = {0.0, 0.0, 0.0};  // lane j: first and last site of list l0 + j
   for (uint32_t base = k0; base < k1; base += WAVE_SITES) {
     const uint32_t k = base + uint32_t(lane);
     // list of entry k: the last j with off[l0 + j] <= k (every lane searches,
@@ -834,26 +937,62 @@ __global__ __launch_bounds__(SITE_BLOCK) void k_run_entries(Geom g, uint32_t nli
     for (int step = int(ENTRY_LISTS) / 2; step; step >>= 1)
       if (uint32_t(__shfl(int(oj), j + step)) <= k) j += step;
     const uint32_t first = uint32_t(__shfl(int(oj), j));
-    if (k >= k1) continue;
-    const uint32_t l = l0 + uint32_t(j);
-    const SiteXYZ c = site_xyz(l * WAVE_SITES, g);
-    const uint32_t id = sorted[k];
-    const uint32_t prev = k > first ? sorted[k - 1] : 0xFFFFFFFFu;
-    const SpreadPoint* __restrict__ p = pts + id;
-    int di = c.x - p->wc[0];
-    di += di < 0 ? g.nx : 0;
-    int dj = c.y - p->wc[4];
-    dj += dj < 0 ? g.ny : 0;
-    const bool skip = id == prev || (di | dj) >= 4;
-    RunEntry r;
-    r.dxy = pick4(p->del[0], p->del[1], p->del[2], p->del[3], di & 3) *
-            pick4(p->del[4], p->del[5], p->del[6], p->del[7], dj & 3);
+    const bool valid = k < k1;
+    RunEntry r{};
+    int w = -1;
+    if (valid) {
+      const uint32_t l = l0 + uint32_t(j);
+      const SiteXYZ c = site_xyz(l * WAVE_SITES, g);
+      const uint32_t id = sorted[k];
+      const uint32_t prev = k > first ? sorted[k - 1] : 0xFFFFFFFFu;
+      const SpreadPoint* __restrict__ p = pts + id;
+      int di = c.x - p->wc[0];
+      di += di < 0 ? g.nx : 0;
+      int dj = c.y - p->wc[4];
+      dj += dj < 0 ? g.ny : 0;
+      const bool skip = id == prev || (di | dj) >= 4;
+      r.dxy = pick4(p->del[0], p->del[1], p->del[2], p->del[3], di & 3) *
+              pick4(p->del[4], p->del[5], p->del[6], p->del[7], dj & 3);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) r.zw[q] = p->del[8 + q];
+      for (int q = 0; q < 4; ++q) r.zw[q] = p->del[8 + q];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) r.F[a] = p->F[a];
-    rec[k] = r;
-    wz[k] = skip ? -1 : p->wc[8];
+      for (int a = 0; a < 3; ++a) r.F[a] = p->F[a];
+      rec[k] = r;
+      w = skip ? -1 : p->wc[8];
+      wz[k] = w;
+    }
+    if (!edge) continue;
+    // z of the run's first site: l0 + j's runs start at multiples of 64 in z
+    const int z0 = int((l0 + uint32_t(j)) * WAVE_SITES % uint32_t(g.nz));
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      int dk = z0 + (e ? WAVE_SITES - 1 : 0) - w;
+      dk += dk < 0 ? g.nz : 0;
+      const bool reach = w >= 0 && dk < 4;
+      const double d3 = r.dxy * pick4(r.zw[0], r.zw[1], r.zw[2], r.zw[3], dk & 3);
+      const double t[3] = {d3 * r.F[0], d3 * r.F[1], d3 * r.F[2]};
+      uint64_t mask = __ballot(reach);
+      while (mask) {
+        const int L = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int jl = __builtin_amdgcn_readlane(j, L);
+        const double v0 = __shfl(t[0], L), v1 = __shfl(t[1], L), v2 = __shfl(t[2], L);
+        if (lane == jl) {
+          double* acc = e ? el : ef;
+          acc[0] += v0;
+          acc[1] += v1;
+          acc[2] += v2;
+        }
+      }
+    }
+  }
+  if (edge && uint32_t(lane) < ENTRY_LISTS && l0 + uint32_t(lane) < nlists) {
+    double* out = edge + 6 * size_t(l0 + uint32_t(lane));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      out[a] = ef[a];
+      out[3 + a] = el[a];
+    }
   }
 }
 
@@ -1010,13 +1149,18 @@ hipError_t launch_edge_force(const Geom& g, const SpreadPoint* pts, const uint32
                              double* edge, RunEntry* rec, int* wz, hipStream_t s) {
   const uint32_t nl = spread_lists(g);
   if (nl == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_edge_force, dim3((2 * nl + SITE_BLOCK - 1) / SITE_BLOCK), dim3(SITE_BLOCK), 0, s, g, nl, pts,
-                     off, sorted, edge);
-  if (rec && wz && g.nz % WAVE_SITES == 0) {
+  const bool entries = rec && wz && g.nz % WAVE_SITES == 0;
+  // k_run_entries also writes the run-end forces where every axis has a
+  // single matching stencil position (add_point's simple case)
+  const bool merged = entries && g.nx >= 4 && g.ny >= 4;
+  if (!merged)
+    hipLaunchKernelGGL(k_edge_force, dim3((2 * nl + SITE_BLOCK - 1) / SITE_BLOCK), dim3(SITE_BLOCK), 0, s, g, nl,
+                       pts, off, sorted, edge);
+  if (entries) {
     const uint32_t waves = (nl + ENTRY_LISTS - 1) / ENTRY_LISTS;
     const uint32_t per_block = SITE_BLOCK / WAVE_SITES;
     hipLaunchKernelGGL(k_run_entries, dim3((waves + per_block - 1) / per_block), dim3(SITE_BLOCK), 0, s, g, nl, pts,
-                       off, sorted, rec, wz);
+                       off, sorted, rec, wz, merged ? edge : nullptr);
   }
   return hipGetLastError();
 }

@@ -1153,7 +1153,7 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     const size_t nt = size_t(lat->sp_lists);
     int rc;
     if ((rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_counts), 4 * nt)) ||
-        (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_off), 4 * (nt + 2 + (nt + 1023) / 1024))) ||
+        (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_off), 4 * spread_off_entries(uint32_t(nt)))) ||
         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->sp_cursor), 4 * nt)))
       return rc;
   }
@@ -1651,7 +1651,7 @@ int ens_build(lbswim_ensemble* e) {
   const size_t nt = size_t(e->nlists);
   size_t cap_sorted = e->cap_list;
   if ((rc = grow_to(&e->counts, &e->cap_counts, nt, 4)) ||
-      (rc = grow_to(&e->off, &e->cap_off, nt + 2 + (nt + 1023) / 1024, 4)) ||
+      (rc = grow_to(&e->off, &e->cap_off, spread_off_entries(uint32_t(nt)), 4)) ||
       (rc = grow_to(&e->cursor, &e->cap_cursor, nt, 4)) || (rc = grow_to(&e->pts, &e->cap_pts, size_t(e->npts), 176)) ||
       (rc = grow_to(&e->sorted, &cap_sorted, lcap, 4)) || (rc = grow_to(&e->list, &e->cap_list, lcap, 4)))
     return rc;

@@ -111,10 +111,12 @@ static_assert(sizeof(SpreadPoint) == 176, "SpreadPoint layout");
 // kernels' 256-site workgroups: spread_lists(g) = 4 ceil(n / 256) of them.
 uint32_t spread_lists(const Geom& g);
 size_t spread_list_capacity(size_t npts);
+// Entries of the `off` array of launch_spread_lists for nlists lists.
+size_t spread_off_entries(uint32_t nlists);
 // Point records of the swimmers of one array, ids from 2*base.
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
                          const Geom& g, SpreadPoint* pts, hipStream_t s);
-// counts[nl], off[nl + 2 + ceil(nl/1024)] (offsets, then scan and sort scratch),
+// counts[nl], off[spread_off_entries(nl)] (offsets, then scan and sort scratch),
 // cursor[nl], list/sorted[spread_list_capacity]; afterwards
 // sorted[off[l]..off[l+1]) holds the ids of the points reaching sites
 // 64 l .. 64 l + 63 in increasing order.

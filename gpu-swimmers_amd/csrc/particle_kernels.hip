@@ -672,7 +672,9 @@ __global__ __launch_bounds__(PBLOCK) void k_count_rows(uint32_t npts, Geom g, co
 
 // Exclusive scan of counts[0..nt) into off[0..nt] in two coalesced passes
 // over SCAN_BLOCK lists per workgroup; the second pass also clears cursor.
-constexpr int SCAN_BLOCK = 1024;
+// Small workgroups: the scans run beside the tracer moves, and a workgroup
+// has to wait for a CU with room for all of its wavefronts.
+constexpr int SCAN_BLOCK = 256;
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* part, uint32_t* total) {
   const int t = int(threadIdx.x);
@@ -1083,6 +1085,11 @@ template hipError_t launch_tracers_move<float>(int, double*, double*, double*, c
                                                hipStream_t);
 
 
+size_t spread_off_entries(uint32_t nlists) {
+  // nlists + 1 offsets, the scan's per-workgroup sums, the sort's queue counter
+  return size_t(nlists) + 2 + (size_t(nlists) + SCAN_BLOCK - 1) / SCAN_BLOCK;
+}
+
 size_t spread_list_capacity(size_t npts) {
   // A stencil has 16 (x, y) rows of 4 sites; the z-run of a row (one wrap at
   // most) lies in at most 4 lists, so a point is listed at most 64 times.
@@ -1121,8 +1128,9 @@ static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ld
   uint32_t* nqueued = off + nt + 1 + nsb;
   hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(size_t(nt))), dim3(PBLOCK), 0, s, uint32_t(nt), off, list, sorted,
                      cursor, nqueued);
-  // one wavefront per list, at most 512 workgroups
-  const unsigned long_blocks = unsigned(std::min<size_t>(512, (size_t(nt) + 3) / 4));
+  // one wavefront per queued list, at most 128 workgroups (each wavefront takes
+  // several lists when many are long)
+  const unsigned long_blocks = unsigned(std::min<size_t>(128, (size_t(nt) + 3) / 4));
   hipLaunchKernelGGL(k_sort_long, dim3(long_blocks), dim3(LONG_BLOCK), 0, s, off, list, sorted, cursor, nqueued);
   return hipGetLastError();
 }

@@ -154,7 +154,7 @@ int lbswim_lattice_storage(lbswim_lattice* lat, int* out);
  * *size the element count (n, 3n or 15n). Any output pointer may be NULL.
  * The device distribution buffers hold post-collision values g (streaming is
  * done by pulling); the post-stream fOld the host sees is f_p(x) = g_p(x - c_p),
- * and *_h2d/*_d2h apply that shift. */
+ * and lbswim_lattice_h2d / lbswim_lattice_d2h apply that shift. */
 int lbswim_lattice_field(lbswim_lattice* lat, int field, double** host, void** device, size_t* size);
 /* SharedArray::H2D / D2H of one field (Shared.cu:15-26). For fOld/fNew the
  * device side is the buffer currently holding that role, as after the

@@ -177,3 +177,49 @@ def test_tracer_moves_overlapping_the_next_step():
         out.append(res)
     for k in out[0]:
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_interleaved_systems_and_tracers_without_swimmers():
+    """Two systems stepped in alternation share the library's second stream;
+    a system with tracers and no swimmers (zero force) still moves them
+    beside the next step.  Each must give its two-launch result alone, bit
+    for bit."""
+    from lbswim import lb, tracers
+    from lbswim.System import System
+
+    def tracer_only(case, fusion):
+        lat = lb.Lattice(*case.size, case.tau_s, case.tau_b)
+        lat.rho.data.ravel()[:] = case.rho
+        lat.rho.H2D()
+        lat.u.data.ravel()[:] = case.u
+        lat.u.H2D()
+        lat.ZeroForce()
+        lat.InitFromHydro()
+        lat.set_fusion(fusion)
+        system = System(lat)
+        tr = tracers.Array(case.tr_r.shape[1])
+        tr.r[:] = case.tr_r
+        tr.v[:] = 0.0
+        tr.H2D()
+        system.AddThing(tr)
+        return lat, system, tr
+
+    case_a = swimmer_case((4, 8, 64), seed=5)
+    case_b = make_case((5, 4, 128), 0, 0.3, ns=0, nt=700, alpha=0.0, seed=6, drho=1e-2, du=1e-2)
+    lat, system, sw, tr = build_lbswim(case_a, "f64", "virtual")
+    lat.set_fusion(False)
+    system.Step(5)
+    solo = [collect_lbswim(lat, sw, tr)]
+    lat, system, tr = tracer_only(case_b, False)
+    system.Step(5)
+    solo.append(collect_lbswim(lat, None, tr))
+    lat_a, sys_a, sw_a, tr_a = build_lbswim(case_a, "f64", "virtual")
+    lat_b, sys_b, tr_b = tracer_only(case_b, True)
+    assert lat_a.fusion[0] and lat_b.fusion[0]
+    for n in (2, 1, 2):
+        sys_a.Step(n)
+        sys_b.Step(n)
+    got = [collect_lbswim(lat_a, sw_a, tr_a), collect_lbswim(lat_b, None, tr_b)]
+    for want, have in zip(solo, got):
+        for k in want:
+            assert np.array_equal(want[k], have[k]), k

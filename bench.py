@@ -370,6 +370,11 @@ def main():
                              "kernel_frac": round(f["main_gbs"] / HBM_PEAK_GBS, 4),
                              "step_hbm_frac": round(f["step_gbs"] / HBM_PEAK_GBS, 4),
                              "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in f["kernels"].items()}}
+        if m["fused"] == f["fused"]:
+            # the lattice pass alone (collide+stream+CalcHydro launch, timed in
+            # the timed region of each form); the particle work around it is
+            # FP64 in both forms
+            out["fp32_store"]["lattice_pass_speedup_vs_fp64"] = round(m["main_avg_ms"] / f["main_avg_ms"], 3)
     if a.ensemble_steps > 0:
         out["ensemble"] = {}
         for form in forms:

@@ -310,6 +310,18 @@ def test_config2_reduced_500_steps():
         assert np.array_equal(got[k], ref[k]), k
 
 
+def test_config2_recipe_fused_500_steps():
+    """The config2 recipe on a 64^3 lattice, where the coupled step takes the
+    fused launch, the virtual force and the tracer moves on the second
+    stream, for the full 500 steps: bit for bit with the oracle."""
+    case = config2(steps=500, size=64, ns=256, nt=8000)
+    say("config2 recipe at 64^3 (fused step): 256 swimmers, 8000 tracers, 500 steps")
+    ref, _ = run_oracle(O, case)
+    got, _ = run_lbswim(case)
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), k
+
+
 # ------------------------------------------------------------- FP32 storage
 
 @pytest.mark.parametrize("size,steps", [((32, 32, 32), 200), ((64, 48, 40), 100)])

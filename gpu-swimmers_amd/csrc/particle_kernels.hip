@@ -1128,9 +1128,9 @@ static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ld
   uint32_t* nqueued = off + nt + 1 + nsb;
   hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(size_t(nt))), dim3(PBLOCK), 0, s, uint32_t(nt), off, list, sorted,
                      cursor, nqueued);
-  // one wavefront per queued list, at most 128 workgroups (each wavefront takes
-  // several lists when many are long)
-  const unsigned long_blocks = unsigned(std::min<size_t>(128, (size_t(nt) + 3) / 4));
+  // one wavefront per list, at most 512 workgroups (dense ensemble members
+  // queue many long lists)
+  const unsigned long_blocks = unsigned(std::min<size_t>(512, (size_t(nt) + 3) / 4));
   hipLaunchKernelGGL(k_sort_long, dim3(long_blocks), dim3(LONG_BLOCK), 0, s, off, list, sorted, cursor, nqueued);
   return hipGetLastError();
 }

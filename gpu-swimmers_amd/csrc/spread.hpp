@@ -168,94 +168,6 @@ __device__ __forceinline__ void lane_force(const SpreadPoint* __restrict__ pts, 
   }
 }
 
-// One list entry held by one lane: the point record fields the row walk needs.
-struct LaneEntry {
-  double del[12];
-  double F[3];
-  int wc0, wc4, wc8;
-  uint32_t id;
-};
-
-__device__ __forceinline__ void load_entry(const SpreadPoint* __restrict__ pts, uint32_t id, LaneEntry& r) {
-  const SpreadPoint* __restrict__ p = pts + id;
-#pragma unroll
-  for (int q = 0; q < 12; ++q) r.del[q] = p->del[q];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) r.F[a] = p->F[a];
-  r.wc0 = p->wc[0];
-  r.wc4 = p->wc[4];
-  r.wc8 = p->wc[8];
-  r.id = id;
-}
-
-__device__ __forceinline__ double readlane_f64(double v, int e) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b)), e));
-  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b >> 32)), e));
-  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
-}
-
-// The row walk of wave_force (simple lattice, nz % 64 == 0: X and Y uniform)
-// over list entries o .. o+m-1 whose first min(m, 64) entries lane e holds
-// (entry o + e in r): each lane finds its entry's row weight dx_i dy_j, and
-// the wavefront then adds the entries in list order, broadcasting each one's
-// row weight, force and z weights from its lane.  Entries past the 64th are
-// read from memory.  Same products, same order, same sum as wave_force.
-__device__ __forceinline__ void lanes_row_force(const LaneEntry& r, uint32_t o, uint32_t m,
-                                                const SpreadPoint* __restrict__ pts,
-                                                const uint32_t* __restrict__ sorted, int Xu, int Yu, int Z,
-                                                const Geom& g, double (&acc)[3]) {
-  acc[0] = acc[1] = acc[2] = 0.0;
-  const int lane = int(threadIdx.x % WAVE_SITES);
-  int di = Xu - r.wc0;
-  di += di < 0 ? g.nx : 0;
-  int dj = Yu - r.wc4;
-  dj += dj < 0 ? g.ny : 0;
-  const uint32_t id_before = uint32_t(__shfl_up(int(r.id), 1, WAVE_SITES));
-  const bool dup = lane > 0 && id_before == r.id;
-  const bool use = uint32_t(lane) < m && !dup && (di | dj) < 4;
-  const double dxy = pick4(r.del[0], r.del[1], r.del[2], r.del[3], di & 3) *
-                     pick4(r.del[4], r.del[5], r.del[6], r.del[7], dj & 3);
-  auto add = [&](double dxy_e, double F0, double F1, double F2, int wc8, const double (&zw)[4]) {
-    int dk = Z - wc8;
-    dk += dk < 0 ? g.nz : 0;
-    uint64_t zbits = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) zbits |= __builtin_bit_cast(uint64_t, zw[q]) & (uint64_t(0) - uint64_t(dk == q));
-    const double delta3d = dxy_e * __builtin_bit_cast(double, zbits);
-    if (dk < 4) {
-      acc[0] += delta3d * F0;
-      acc[1] += delta3d * F1;
-      acc[2] += delta3d * F2;
-    }
-  };
-  uint64_t mask = __ballot(use);
-  while (mask) {
-    const int e = __builtin_ctzll(mask);
-    mask &= mask - 1;
-    const double zw[4] = {readlane_f64(r.del[8], e), readlane_f64(r.del[9], e), readlane_f64(r.del[10], e),
-                          readlane_f64(r.del[11], e)};
-    add(readlane_f64(dxy, e), readlane_f64(r.F[0], e), readlane_f64(r.F[1], e), readlane_f64(r.F[2], e),
-        __builtin_amdgcn_readlane(r.wc8, e), zw);
-  }
-  if (m > WAVE_SITES) {  // rare: the rest of a long list, one entry at a time
-    uint32_t prev = uint32_t(__builtin_amdgcn_readlane(int(r.id), WAVE_SITES - 1));
-    for (uint32_t e = o + WAVE_SITES; e < o + m; ++e) {
-      const uint32_t id = sorted[e];
-      if (id == prev) continue;
-      prev = id;
-      const SpreadPoint* __restrict__ p = pts + id;
-      int pi = Xu - p->wc[0];
-      pi += pi < 0 ? g.nx : 0;
-      int pj = Yu - p->wc[4];
-      pj += pj < 0 ? g.ny : 0;
-      if ((pi | pj) >= 4) continue;
-      const double zw[4] = {p->del[8], p->del[9], p->del[10], p->del[11]};
-      add(p->del[pi] * p->del[4 + pj], p->F[0], p->F[1], p->F[2], p->wc[8], zw);
-    }
-  }
-}
-
 // One list entry spread over four lanes: lane 4e + q holds entry e (of the
 // first 16) with its z weight q, from the entry's RunEntry.
 struct QuadEntry {

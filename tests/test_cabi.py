@@ -59,6 +59,7 @@ def test_invalid_arguments_rejected_without_device(lib):
     assert out.value is None
     assert lib.lbswim_lattice_step(None) == EINVAL
     assert lib.lbswim_lattice_sync_u(None) == EINVAL
+    assert lib.lbswim_lattice_u_is_virtual(None, None) == EINVAL
     lib.lbswim_swimmers_create.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     assert lib.lbswim_swimmers_create(-1, None, ctypes.byref(out)) == EINVAL
     lib.lbswim_tracers_create.argtypes = [ctypes.c_int, ctypes.c_void_p]

@@ -116,7 +116,8 @@ size_t spread_off_entries(uint32_t nlists);
 // Point records of the swimmers of one array, ids from 2*base.
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
                          const Geom& g, SpreadPoint* pts, hipStream_t s);
-// counts[nl], off[spread_off_entries(nl)] (offsets, then scan and sort scratch),
+// counts[nl] (zero on entry: zeroed at allocation, left zero by every build),
+// off[spread_off_entries(nl)] (offsets, then scan and sort scratch),
 // cursor[nl], list/sorted[spread_list_capacity]; afterwards
 // sorted[off[l]..off[l+1]) holds the ids of the points reaching sites
 // 64 l .. 64 l + 63 in increasing order.

@@ -671,7 +671,8 @@ __global__ __launch_bounds__(PBLOCK) void k_count_rows(uint32_t npts, Geom g, co
 }
 
 // Exclusive scan of counts[0..nt) into off[0..nt] in two coalesced passes
-// over SCAN_BLOCK lists per workgroup; the second pass also clears cursor.
+// over SCAN_BLOCK lists per workgroup; the second pass also clears cursor and
+// resets counts to zero for the next build (its last reader).
 // Small workgroups: the scans run beside the tracer moves, and a workgroup
 // has to wait for a CU with room for all of its wavefronts.
 constexpr int SCAN_BLOCK = 256;
@@ -701,7 +702,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(int nt, const uint32_t
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(int nt, const uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(int nt, uint32_t* __restrict__ counts,
                                                            const uint32_t* __restrict__ sums,
                                                            uint32_t* __restrict__ off, uint32_t* __restrict__ cursor) {
   __shared__ uint32_t part[SCAN_BLOCK];
@@ -720,6 +721,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(int nt, const uint32_
   if (i < nt) {
     off[i] = base + ex;
     cursor[i] = 0u;
+    counts[i] = 0u;
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
     off[nt] = base + total;
@@ -1107,12 +1109,12 @@ hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r,
 }
 
 // Counts, offsets, lists and sorted lists of nt lists from the npts point
-// records (counts zeroed here).
+// records.  counts[0..nt) must be zero on entry: they are zeroed when
+// allocated and reset by k_scan_apply at the end of every build.
 template <bool Batched>
 static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ldesc, const SpreadPoint* pts,
                               uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted,
                               hipStream_t s) {
-  if (hipError_t e = launch_fill_zero(counts, 4 * size_t(nt), s)) return e;
   const unsigned rows = blocks_for(16 * size_t(npts));
   if (npts > 0)
     hipLaunchKernelGGL(k_count_rows<Batched>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc, pts, counts);

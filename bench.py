@@ -297,8 +297,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate of the serving leg (1/s)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (a lattice would be split into "
+              "x-slabs exchanging one plane of 5 distributions per face each step; ensembles would assign "
+              "whole simulations to GPUs); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if a.probe == "copy":

@@ -532,10 +532,17 @@ __host__ __device__ constexpr int dir_index(int p) {
   for (int q = 0; q < p; ++q) k += CI[q][0] == CI[p][0] ? 1 : 0;
   return k;
 }
-// LDS ring slots: group 0 slot 0 (read in the iteration that writes it),
-// group 1 slots 1-2 (read one iteration later), group 2 slots 3-5 (two later).
-constexpr int FUSED_SLOTS = 6;
-constexpr int FUSED_TRASH = FUSED_SLOTS * 5;  // slot after the ring: unused writes of the z-halo wave
+// LDS ring slots.  Six slot-sets (FP64): group 0 slot 0 (read in the
+// iteration that writes it), group 1 slots 1-2 (read one iteration later),
+// group 2 slots 3-5 (two later); a second barrier per plane keeps the next
+// plane's writes off the slots CalcHydro is still reading.  Nine (FP32, whose
+// LDS has room): groups 0, 1, 2 rotate over 2, 3, 4 slots (0-1, 2-4, 5-8), so
+// the next plane writes other slots and one barrier per plane suffices (a
+// wavefront cannot pass the next plane's barrier before every wavefront has
+// read this plane's slots).  The slot-set after the ring takes the unused
+// writes of the z-halo wave.
+template <typename T>
+constexpr int fused_slots() { return sizeof(T) == 4 ? 9 : 6; }
 
 struct FusedTiles {
   uint32_t nyt, nzt;  // tiles along y and z
@@ -650,6 +657,7 @@ template <typename T, bool VF, int TY, int ROLE>
 __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict__ lds, int xa, int last, int r,
                                             int zz, bool active, int y, int z) {
   constexpr int PLANE = TY * FUSED_ROW;  // LDS words of one direction slot
+  constexpr int SLOTS = fused_slots<T>(), TRASH = SLOTS * 5;
   constexpr bool TILE = ROLE == ROLE_TILE;
   const Geom& g = A.g;
   const uint32_t n = g.n, nz = uint32_t(g.nz), nyz = g.nyz;
@@ -770,7 +778,9 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
 #pragma unroll
       for (int p = 0; p < Q; ++p) buffer_store_nt(out[p], rNew, zl * B, (uint32_t(p) * n + srow) * B);
     }
-    const int w1 = 1 + it % 2, w2 = 3 + it % 3;
+    const int w0 = SLOTS == 6 ? 0 : it % 2;
+    const int w1 = SLOTS == 6 ? 1 + it % 2 : 2 + it % 3;
+    const int w2 = SLOTS == 6 ? 3 + it % 3 : 5 + it % 4;
     [&]<int... P>(std::integer_sequence<int, P...>) {
       (
           [&] {
@@ -779,12 +789,12 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
             if constexpr (ROLE == ROLE_ABOVE && cy != 1) return;
             if constexpr (ROLE == ROLE_BELOW && cy != -1) return;
             constexpr int grp = dir_group(P), k = dir_index(P);
-            const int slot = grp == 0 ? 0 : (grp == 1 ? w1 : w2);
+            const int slot = grp == 0 ? w0 : (grp == 1 ? w1 : w2);
             const int tr = r + cy;  // target tile row
             const int at = (slot * 5 + k) * PLANE + (tr - 1) * FUSED_ROW + zz + cz + 1;
             if constexpr (ROLE == ROLE_ZHALO) {
               const bool ok = active && tr >= 1 && tr <= TY && zz + cz >= 0 && zz + cz < FUSED_Z;
-              lds[ok ? at : FUSED_TRASH * PLANE + lane] = out[P];
+              lds[ok ? at : TRASH * PLANE + lane] = out[P];
             } else if constexpr (ROLE == ROLE_TILE) {
               if (tr >= 1 && tr <= TY) lds[at] = out[P];  // uniform test
             } else {
@@ -795,19 +805,22 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     }(std::make_integer_sequence<int, Q>{});
     lds_barrier();
     // CalcHydro of plane it-1: the LDS reads come before the second barrier
-    // (which only has to keep the next plane's writes off these slots), the
-    // arithmetic and stores after it, overlapping the other wavefronts' next
-    // plane
+    // (six slot-sets; it only has to keep the next plane's writes off these
+    // slots), the arithmetic and stores after it, overlapping the other
+    // wavefronts' next plane.  Slots read: group 0 written this iteration,
+    // group 1 the one before, group 2 two before.
     T fl[Q];
     if constexpr (HYDRO) {
-      const int r1 = 1 + (it + 1) % 2, r2 = 3 + (it + 1) % 3;
+      const int r0 = SLOTS == 6 ? 0 : it % 2;
+      const int r1 = SLOTS == 6 ? 1 + (it + 1) % 2 : 2 + (it + 2) % 3;
+      const int r2 = SLOTS == 6 ? 3 + (it + 1) % 3 : 5 + (it + 2) % 4;
       [&]<int... P>(std::integer_sequence<int, P...>) {
-        ((fl[P] = lds[((dir_group(P) == 0 ? 0 : (dir_group(P) == 1 ? r1 : r2)) * 5 + dir_index(P)) * PLANE +
+        ((fl[P] = lds[((dir_group(P) == 0 ? r0 : (dir_group(P) == 1 ? r1 : r2)) * 5 + dir_index(P)) * PLANE +
                       (r - 1) * FUSED_ROW + lane + 1]),
          ...);
       }(std::make_integer_sequence<int, Q>{});
     }
-    lds_barrier();
+    if constexpr (SLOTS == 6) lds_barrier();
     if constexpr (HYDRO) {
       T rh, uh[3];
       hydro_site(fl, Fh, rh, uh);
@@ -870,7 +883,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
 template <typename T, bool VF, int TY>
 __global__ __launch_bounds__((TY + 3) * 64) void k_step_hydro(FusedArgs<T> A, FusedTiles t) {
   constexpr int ROWS = TY + 2;
-  __shared__ T lds[FUSED_TRASH * TY * FUSED_ROW + 64];
+  __shared__ T lds[fused_slots<T>() * 5 * TY * FUSED_ROW + 64];
   const uint32_t L = xcd_block(round_xcd(t.nblk));
   if (L >= t.nblk) return;
   const Geom& g = A.g;

@@ -223,3 +223,34 @@ def test_interleaved_systems_and_tracers_without_swimmers():
     for want, have in zip(solo, got):
         for k in want:
             assert np.array_equal(want[k], have[k]), k
+
+
+@pytest.mark.parametrize("storage,size", [("f64", (272, 256, 512)), ("f32", (544, 256, 512))])
+def test_fused_at_the_32bit_offset_limit(storage, size):
+    """The fused step addresses each distribution buffer with 32-bit byte
+    offsets (15 n sizeof(T) < 4 GiB).  A lattice at 99.6 % of that limit
+    must still give the two launches' distributions, rho and u bit for bit."""
+    from lbswim import _core
+    n = size[0] * size[1] * size[2]
+    elem = 8 if storage == "f64" else 4
+    assert 0.99 * 2 ** 32 < 15 * n * elem < 2 ** 32
+    case = make_case(size, 0, 0.3, ns=256, nt=2000, alpha=0.0, seed=9, drho=1e-2, du=1e-2)
+    say("fused vs two launches at the offset limit: %s %s" % (size, storage))
+    out = []
+    for fusion in (None, (0, 0)):
+        lat, system, sw, tr = build_lbswim(case, storage, "virtual")
+        lat.set_fusion(fusion is not None)
+        assert lat.fusion[0] == (fusion is not None)
+        system.Step(2)
+        _core.synchronize()
+        res = {}
+        for name in ("fOld", "rho", "u"):
+            arr = getattr(lat, name)
+            arr.D2H()
+            res[name] = np.array(arr.data, copy=True)
+        tr.D2H()
+        res["tr_r"] = np.array(tr.r, copy=True)
+        out.append(res)
+        del lat, system, sw, tr
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k

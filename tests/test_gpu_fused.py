@@ -101,6 +101,14 @@ def test_fusion_skipped_where_unsupported():
     assert lat.fusion[0] is False
     lat.set_fusion(True, 2, 0)
     assert lat.fusion == (True, 2, 0)
+    del lat
+    # 15 n sizeof(double) above 4 GiB: the fused launch's 32-bit offsets
+    # cannot reach the whole buffer
+    lat = lb.Lattice(280, 256, 512, 0.25, 0.25)
+    assert lat.fusion[0] is False
+    del lat
+    lat = lb.Lattice(280, 256, 512, 0.25, 0.25, storage="f32")
+    assert lat.fusion[0] is True
 
 
 def test_moves_use_u_written_after_fused_steps():

@@ -82,21 +82,32 @@ def build_cython():
 
 
 EXAMPLE = os.path.join(LIBDIR, "lbswim_readme_coupled")
+DEVPTR = os.path.join(LIBDIR, "lbswim_device_pointers")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 
 def build_example():
-    """The C++ API example used by the GPU tests (tests/cpp/readme_coupled.cpp)."""
+    """The C++ API programs used by the GPU tests (tests/cpp/): the README
+    example, and the device-pointer check, which also reads device memory with
+    the HIP runtime."""
     src = os.path.join(ROOT, "tests", "cpp", "readme_coupled.cpp")
     tmp = EXAMPLE + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", tmp, src, "-L" + LIBDIR, "-llbswim_amd",
           "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, EXAMPLE)
+    src = os.path.join(ROOT, "tests", "cpp", "device_pointers.cpp")
+    tmp = DEVPTR + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + INCLUDE,
+          "-I" + os.path.join(ROCM, "include"), "-o", tmp, src, "-L" + LIBDIR, "-llbswim_amd",
+          "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, DEVPTR)
     return EXAMPLE
 
 
 def build_all(cython=True, jobs=4):
     lib = build_hip(jobs)
-    outs = [lib, build_example()]
+    outs = [lib, build_example(), DEVPTR]
     if cython:
         outs.append(build_cython())
     return outs

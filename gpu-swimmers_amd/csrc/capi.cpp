@@ -212,8 +212,9 @@ struct lbswim_lattice {
 
 // Device storage of one particle array in slot order (particle_order.hip):
 // field f of slot s, component c is at d[f][c*num + s]; perm[s] is the user's
-// index of the particle in slot s.  Every field buffer and the staging buffer
-// hold 3*num 8-byte words, so buffers can trade places after a re-order.
+// index of the particle in slot s.  The staging buffer holds 3*num 8-byte
+// words (the largest field); every buffer keeps its address until the array
+// is destroyed.
 struct PStore {
   int num = 0;
   uint64_t* d[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -870,13 +871,18 @@ int pstore_maybe_reorder(PStore& st, const Geom& g) {
   const double* r = reinterpret_cast<const double*>(st.d[LBSWIM_PART_R]);
   HIP_TRY(launch_spatial_order(st.num, r, g, st.keys, st.keys_alt, st.idx, st.order, st.sort_tmp,
                                st.sort_tmp_bytes, stream()));
+  // Each field is gathered into the staging buffer and copied back, so every
+  // field keeps its device buffer for the array's lifetime (as the
+  // reference's SharedArray::device does, Shared.h:8): a device pointer
+  // obtained once stays valid across re-orderings.
   for (int f = 0; f < 5; ++f) {
     if (!st.has[f]) continue;
+    const size_t bytes = 8 * size_t(pstore_ncomp(f)) * st.num;
     HIP_TRY(launch_gather<uint64_t>(st.num, pstore_ncomp(f), st.order, st.d[f], st.stage, stream()));
-    std::swap(st.d[f], st.stage);
+    HIP_TRY(hipMemcpyAsync(st.d[f], st.stage, bytes, hipMemcpyDeviceToDevice, stream()));
   }
   HIP_TRY(launch_gather<int>(st.num, 1, st.order, st.perm, st.perm_alt, stream()));
-  std::swap(st.perm, st.perm_alt);
+  HIP_TRY(hipMemcpyAsync(st.perm, st.perm_alt, 4 * size_t(st.num), hipMemcpyDeviceToDevice, stream()));
   st.dirty = false;
   st.moves_since_order = 0;
   return LBSWIM_OK;
@@ -1028,6 +1034,12 @@ int lbswim_swimmers_tumbles(lbswim_swimmers* sw, unsigned long long* out) {
   return copy_d2h(out, sw->d_tumbles, sizeof(unsigned long long));
 }
 
+int lbswim_swimmers_slots(lbswim_swimmers* sw, const int** device) {
+  CHECK_ARG(sw && device, "NULL argument");
+  *device = sw->st.perm;
+  return LBSWIM_OK;
+}
+
 // ---- tracers ----------------------------------------------------------------
 
 int lbswim_tracers_create(int n, lbswim_tracers** out) {
@@ -1114,6 +1126,12 @@ int lbswim_tracers_d2h(lbswim_tracers* tr, int field) {
   CHECK_ARG(tr, "tracer handle is NULL");
   if (int rc = tracer_check_field(field)) return rc;
   return pstore_d2h(tr->st, field);
+}
+
+int lbswim_tracers_slots(lbswim_tracers* tr, const int** device) {
+  CHECK_ARG(tr && device, "NULL argument");
+  *device = tr->st.perm;
+  return LBSWIM_OK;
 }
 
 // ---- coupled step -------------------------------------------------------------
@@ -1421,24 +1439,10 @@ struct EnsMember {
   lbswim_lattice* lat = nullptr;  // nullptr: free slot
   std::vector<lbswim_swimmers*> sw;
   std::vector<lbswim_tracers*> tr;
-  // state captured in the device tables: lattice buffer role and particle
-  // storage pointers (a member used outside the ensemble may change them)
+  // lattice buffer role captured in the device tables (a member stepped
+  // outside the ensemble may change it; particle buffers never move)
   int cur_at_build = 0;
-  std::vector<const void*> storage_at_build;
 };
-
-std::vector<const void*> storage_signature(const EnsMember& m) {
-  std::vector<const void*> sig;
-  for (auto* s : m.sw) {
-    sig.push_back(s->st.d[LBSWIM_PART_R]);
-    sig.push_back(s->st.perm);
-  }
-  for (auto* t : m.tr) {
-    sig.push_back(t->st.d[LBSWIM_PART_R]);
-    sig.push_back(t->st.perm);
-  }
-  return sig;
-}
 
 // Workgroup orders the autotuner chooses between (results do not depend on it).
 enum { ORDER_MEMBER_MAJOR = 0, ORDER_ROUND_ROBIN = 1, ORDER_LARGEST_FIRST = 2, ORDER_COUNT = 3 };
@@ -1618,7 +1622,6 @@ int ens_build(lbswim_ensemble* e) {
   for (auto& m : e->members) {
     if (!m.lat) continue;
     m.cur_at_build = m.lat->cur;
-    m.storage_at_build = storage_signature(m);
   }
   const std::vector<WorkItem> lw = order_work(latwork[0], e->order, weight[0]);
   const std::vector<WorkItem> lwv = order_work(latwork[1], e->order, weight[1]);
@@ -1848,7 +1851,7 @@ int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps) {
   if (nsteps == 0) return LBSWIM_OK;
   for (auto& m : e->members) {
     if (!m.lat || e->dirty) continue;
-    if (m.lat->cur != (m.cur_at_build ^ e->parity) || storage_signature(m) != m.storage_at_build) e->dirty = true;
+    if (m.lat->cur != (m.cur_at_build ^ e->parity)) e->dirty = true;
   }
   if (e->dirty)
     if (int rc = ens_build(e)) return rc;

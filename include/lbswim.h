@@ -182,13 +182,21 @@ int lbswim_swimmers_move(lbswim_swimmers* sw, lbswim_lattice* lat);
 /* SwimmerArray::{r,v,n,prng} as SharedArray{host,device,size}; *elem_size is
  * 8 for all four (double, double, double, uint64 counter).  The host mirror is
  * in the user's order; the device copy is kept in a spatially sorted order
- * that the library changes from time to time (h2d/d2h translate). */
+ * that the library changes from time to time (h2d/d2h translate).  Each
+ * device pointer stays the same for the array's life (Shared.h:8). */
 int lbswim_swimmers_field(lbswim_swimmers* sw, int field, void** host, void** device, size_t* count,
                           size_t* elem_size);
 int lbswim_swimmers_h2d(lbswim_swimmers* sw, int field);
 int lbswim_swimmers_d2h(lbswim_swimmers* sw, int field);
 /* Number of tumbles since creation (diagnostic; no reference counterpart). */
 int lbswim_swimmers_tumbles(lbswim_swimmers* sw, unsigned long long* out);
+/* *device: int slots[num] on the device, the user index of the swimmer whose
+ * data occupy storage slot s of the device fields (component c of field f at
+ * device[c*num + s] belongs to swimmer slots[s]).  The pointer is fixed; the
+ * contents change when the library re-orders the storage (at a Move after
+ * positions were written by h2d, and every 64 moves).  No reference
+ * counterpart: the reference keeps the user order on the device. */
+int lbswim_swimmers_slots(lbswim_swimmers* sw, const int** device);
 
 /* ---- TracerArray (lbswim/liblbswim/TracerArray.h:6-14; TracerArray.pxd:4-11) ---- */
 
@@ -202,6 +210,8 @@ int lbswim_tracers_move(lbswim_tracers* tr, lbswim_lattice* lat);
 int lbswim_tracers_field(lbswim_tracers* tr, int field, void** host, void** device, size_t* count);
 int lbswim_tracers_h2d(lbswim_tracers* tr, int field);
 int lbswim_tracers_d2h(lbswim_tracers* tr, int field);
+/* As lbswim_swimmers_slots, for tracers. No reference counterpart. */
+int lbswim_tracers_slots(lbswim_tracers* tr, const int** device);
 
 /* ---- coupled step ------------------------------------------------------ */
 

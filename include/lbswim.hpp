@@ -6,8 +6,11 @@
 // (TracerArray), so code written against the reference's C++ API compiles
 // against this header.  Differences: failures throw lbswim::Error instead of
 // exiting; Lattice takes an optional storage form; swimmer `prng` holds a
-// uint64 draw counter per swimmer; device pointers of particle fields refer
-// to a library-chosen storage order (H2D/D2H translate to the user order).
+// uint64 draw counter per swimmer; device buffers of particle fields hold
+// the particles in a library-chosen storage order (H2D/D2H translate to the
+// user order; slots() gives the user index of each storage slot); the
+// device distributions fOld/fNew hold post-collision values (pull
+// streaming, see lbswim_lattice_field).
 #pragma once
 
 #include <cstddef>
@@ -37,11 +40,18 @@ using LatticeAddressing = lbswim_addressing;  // Lattice.h:8-12
 using CommonParams = lbswim_common_params;    // SwimmerArray.h:8-16
 
 // A host mirror and its device copy (Shared.h:4-15).  H2D/D2H forward to the
-// owning object's C entry points.
+// owning object's C entry points.  `device` is typed as in the reference
+// (Shared.h:8) where the device buffer holds T: every field of the FP64
+// lattice form and every particle field.  A lattice in the FP32 storage form
+// keeps its fields on the device as float (distributions and rho as
+// deviations from the rest state); there `device` is nullptr and
+// `device_raw` is the float buffer.  Both stay valid for the owner's life,
+// except fOld/fNew of a lattice, which trade buffers at every Step.
 template <typename T>
 struct SharedArray {
   T* host = nullptr;
-  void* device = nullptr;
+  T* device = nullptr;
+  void* device_raw = nullptr;
   size_t size = 0;
   void H2D() { check(h2d_(owner_, field_)); }
   void D2H() { check(d2h_(owner_, field_)); }
@@ -140,13 +150,27 @@ class Lattice {
 
   lbswim_lattice* handle() const { return impl_; }
   void refresh_devices() {
-    check(lbswim_lattice_field(impl_, LBSWIM_FOLD, nullptr, &data_.fOld.device, nullptr));
-    check(lbswim_lattice_field(impl_, LBSWIM_FNEW, nullptr, &data_.fNew.device, nullptr));
+    set_device(data_.fOld, LBSWIM_FOLD);
+    set_device(data_.fNew, LBSWIM_FNEW);
   }
+  // After coupled steps (lbswim_system_step) u and the force may be held in
+  // another form on the device; these write the reference-layout arrays that
+  // data->u.device and data->force.device point at.
+  void sync_u() { check(lbswim_lattice_sync_u(impl_)); }
+  void sync_force() { check(lbswim_lattice_sync_force(impl_)); }
 
  private:
+  void set_device(SharedArray<double>& a, int field) {
+    void* dev = nullptr;
+    check(lbswim_lattice_field(impl_, field, nullptr, &dev, nullptr));
+    int storage = LBSWIM_STORE_F64;
+    check(lbswim_lattice_storage(impl_, &storage));
+    a.device_raw = dev;
+    a.device = storage == LBSWIM_STORE_F64 ? static_cast<double*>(dev) : nullptr;
+  }
   void bind(SharedArray<double>& a, int field) {
-    check(lbswim_lattice_field(impl_, field, &a.host, &a.device, &a.size));
+    check(lbswim_lattice_field(impl_, field, &a.host, nullptr, &a.size));
+    set_device(a, field);
     a.bind(impl_, field, detail::lat_h2d, detail::lat_d2h);
   }
   lbswim_lattice* impl_ = nullptr;
@@ -184,14 +208,24 @@ class SwimmerArray {
   SharedArray<uint64_t> prng;
 
   lbswim_swimmers* handle() const { return impl_; }
+  // Device array of num ints: the user index of the swimmer in each storage
+  // slot of the device fields (no reference counterpart).
+  const int* slots() const {
+    const int* p = nullptr;
+    check(lbswim_swimmers_slots(impl_, &p));
+    return p;
+  }
 
  private:
   template <typename T>
   void bind(SharedArray<T>& a, int field) {
     void* host = nullptr;
+    void* dev = nullptr;
     size_t elem = 0;
-    check(lbswim_swimmers_field(impl_, field, &host, &a.device, &a.size, &elem));
+    check(lbswim_swimmers_field(impl_, field, &host, &dev, &a.size, &elem));
     a.host = static_cast<T*>(host);
+    a.device = static_cast<T*>(dev);
+    a.device_raw = dev;
     a.bind(impl_, field, detail::sw_h2d, detail::sw_d2h);
   }
   lbswim_swimmers* impl_ = nullptr;
@@ -220,12 +254,21 @@ class TracerArray {
   SharedArray<double> v;
 
   lbswim_tracers* handle() const { return impl_; }
+  // Device array of num ints: the user index of the tracer in each storage slot.
+  const int* slots() const {
+    const int* p = nullptr;
+    check(lbswim_tracers_slots(impl_, &p));
+    return p;
+  }
 
  private:
   void bind(SharedArray<double>& a, int field) {
     void* host = nullptr;
-    check(lbswim_tracers_field(impl_, field, &host, &a.device, &a.size));
+    void* dev = nullptr;
+    check(lbswim_tracers_field(impl_, field, &host, &dev, &a.size));
     a.host = static_cast<double*>(host);
+    a.device = static_cast<double*>(dev);
+    a.device_raw = dev;
     a.bind(impl_, field, detail::tr_h2d, detail::tr_d2h);
   }
   lbswim_tracers* impl_ = nullptr;

@@ -41,3 +41,15 @@ def test_cpp_readme_example_matches_oracle(tmp_path, alpha):
     assert np.array_equal(sw_r, ref["sw_r"].ravel())
     assert np.array_equal(sw_n, ref["sw_n"].ravel())
     assert np.array_equal(tr_r, ref["tr_r"].ravel())
+
+
+def test_cpp_device_pointers_typed_and_fixed():
+    """lat.data->u.device read as double*, particle device pointers fixed across
+    the storage re-ordering after 64 moves, contents equal to D2H through
+    slots(); FP32 lattices expose their float buffer as device_raw only."""
+    exe = os.path.join(os.path.dirname(EXE), "lbswim_device_pointers")
+    assert os.path.exists(exe), "device-pointer program not built (run gpu-swimmers_amd/build_native.py)"
+    print("C++ device pointers, 70 steps", flush=True)
+    res = subprocess.run([exe, "70"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert res.stdout.strip().endswith("ok"), res.stdout

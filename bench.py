@@ -131,7 +131,8 @@ def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, f
     sw.D2H()
     tr.D2H()
     res["state"] = {"rho": np.array(lat.rho.data).ravel(), "u": np.array(lat.u.data).ravel(),
-                    "sw_r": np.array(sw.r.data), "sw_n": np.array(sw.n.data), "tr_r": np.array(tr.r.data)}
+                    "sw_r": np.array(sw.r.data), "sw_n": np.array(sw.n.data), "tr_r": np.array(tr.r.data),
+                    "steps": res["state_steps"]}
     del lat, system, sw, tr
     return res
 
@@ -236,32 +237,126 @@ def bench_serving(nreq, rate, seed=1234):
             "p99_ratio": round(seq["p99_s"] / batched["p99_s"], 2)}
 
 
-def cpu_baseline(case, min_seconds=10.0, max_steps=64):
-    """The FP64 oracle (OpenMP over the host cores) on the same workload: coupled
-    steps in chunks of 4 until about `min_seconds` of CPU time have passed."""
-    import lbswim_oracle as O
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    cores = min(cores, os.cpu_count() or cores)
-    O.set_threads(cores)
+def oracle_errors(got, ref, L):
+    """FP32-storage state against the FP64 oracle state of the same inputs and
+    step count: ||a-b||_inf/||b||_inf for u and rho - 1, minimum-image
+    absolute error for positions, absolute for orientations."""
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    def traj(a, b):
+        d = np.abs(a - b)
+        return float(np.max(np.minimum(d, L - d))) if d.size else 0.0
+
+    out = {"u_rel": rel(got["u"], ref["u"]), "drho_rel": rel(got["rho"] - 1.0, ref["rho"] - 1.0)}
+    if ref["sw_r"].size:
+        out["swimmer_r_abs"] = traj(got["sw_r"], ref["sw_r"])
+        out["swimmer_n_abs"] = float(np.max(np.abs(got["sw_n"] - ref["sw_n"])))
+    if ref["tr_r"].size:
+        out["tracer_r_abs"] = traj(got["tr_r"], ref["tr_r"])
+    return out
+
+
+def small_config_states(storage):
+    """Final states of the 16^3 (1000 steps, alpha 0.01) and 128^3 (500
+    steps) configurations run with the given storage form."""
+    from cases import config1, config2, run_lbswim
+    out = {}
+    for name, case in (("C1", config1(1000, alpha=0.01)), ("C2", config2(500))):
+        got, _ = run_lbswim(case, storage)
+        out[name] = {k: got[k] for k in ("rho", "u", "sw_r", "sw_n", "tr_r")}
+    return out
+
+
+def _oracle_setup(O, case):
     lat = O.Lattice(*case.size, case.tau_s, case.tau_b)
     lat.rho[:] = case.rho
     lat.u[:] = case.u
     lat.InitFromHydro()
-    sw = O.Swimmers(case.sw_r.shape[1], O.Common(**case.common))
-    sw.r[:] = case.sw_r
-    sw.n[:] = case.sw_n
-    tr = O.Tracers(case.tr_r.shape[1])
-    tr.r[:] = case.tr_r
-    O.run(lat, 1, sw, tr)  # warm-up
-    steps = 0
+    sw = tr = None
+    if case.sw_r.shape[1]:
+        sw = O.Swimmers(case.sw_r.shape[1], O.Common(**case.common))
+        sw.r[:] = case.sw_r
+        sw.n[:] = case.sw_n
+    if case.tr_r.shape[1]:
+        tr = O.Tracers(case.tr_r.shape[1])
+        tr.r[:] = case.tr_r
+    return lat, sw, tr
+
+
+def _oracle_state(lat, sw, tr):
+    e3 = np.zeros((3, 0))
+    return {"rho": lat.rho.copy(), "u": lat.u.copy(), "sw_r": sw.r.copy() if sw else e3,
+            "sw_n": sw.n.copy() if sw else e3, "tr_r": tr.r.copy() if tr else e3}
+
+
+def oracle_legs(case, f64_state, f32_state, f32_small, budget_s=240.0):
+    """The FP64 oracle, built for this host (-O3 -march=native, OpenMP over
+    its cores), on the benchmark workload and the two small configurations.
+
+    - cpu_baseline: 256^3 coupled steps, 10 warm-up steps, then the median of
+      3 timed runs of k steps (k chosen for about 5 s each);
+    - the same run continued to the step count of the GPU states gives the
+      FP64 reference for the FP32 error at 256^3 (and is compared bit for bit
+      with the FP64 GPU state);
+    - 16^3 x 1000 and 128^3 x 500 steps give the FP32 errors of those
+      configurations.
+    The 256^3 continuation is skipped if it would take more than budget_s."""
+    import statistics
+    import tempfile
+    import lbswim_oracle as O
+    build = "oracle built -O3 -march=native on this host"
+    try:
+        O.use_library(O.build_native(os.path.join(tempfile.mkdtemp(), "liblbswim_oracle_native.so")))
+    except Exception as exc:  # no compiler on the host: the prebuilt oracle
+        build = "prebuilt oracle (-O3; native build failed: %s)" % exc
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    cores = min(cores, os.cpu_count() or cores)
+    O.set_threads(cores)
+    lat, sw, tr = _oracle_setup(O, case)
     t0 = time.perf_counter()
-    while steps < max_steps and time.perf_counter() - t0 < min_seconds:
-        O.run(lat, 4, sw, tr)
-        steps += 4
-    dt = time.perf_counter() - t0
-    return {"value": lat.n * steps / dt / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
-            "sample": "%d coupled steps (after 1 warm-up) of the same 256^3 workload, FP64 oracle, "
-                      "OpenMP over %d threads, %.1f s" % (steps, cores, dt)}
+    O.run(lat, 10, sw, tr)
+    per = (time.perf_counter() - t0) / 10
+    k = int(max(1, min(60, round(5.0 / per))))
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.run(lat, k, sw, tr)
+        runs.append(time.perf_counter() - t0)
+    med = statistics.median(runs)
+    res = {"cpu_baseline": {
+        "value": lat.n * k / med / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
+        "sample": "median of 3 runs of %d coupled steps (%.2f/%.2f/%.2f s) after 10 warm-up steps, 256^3 workload "
+                  "(32768 swimmers, 1e6 tracers), FP64 oracle, %s, OpenMP over %d threads"
+                  % (k, runs[0], runs[1], runs[2], build, cores)}}
+    log("CPU baseline: %.1f MLUPS (%d threads)" % (res["cpu_baseline"]["value"], cores))
+    if f32_small is None:
+        return res
+    errs = {}
+    target = f64_state["steps"]
+    left = target - (10 + 3 * k)
+    if left >= 0 and left * med / k <= budget_s:
+        log("oracle: continuing the 256^3 run to %d steps" % target)
+        O.run(lat, left, sw, tr)
+        ref = _oracle_state(lat, sw, tr)
+        L = float(case.size[0])
+        errs["C3"] = oracle_errors(f32_state, ref, L)
+        res["f64_equals_oracle_256cubed"] = all(np.array_equal(f64_state[key], ref[key]) for key in ref)
+    else:
+        errs["C3"] = None
+    del lat, sw, tr
+    from cases import config1, config2
+    for name, c, L in (("C1", config1(1000, alpha=0.01), 16.0), ("C2", config2(500), 128.0)):
+        log("oracle: %s" % name)
+        lat, sw, tr = _oracle_setup(O, c)
+        O.run(lat, c.steps, sw, tr)
+        errs[name] = oracle_errors(f32_small[name], _oracle_state(lat, sw, tr), L)
+        del lat, sw, tr
+    res["fp32_error_vs_fp64_oracle"] = {
+        "C1": errs["C1"], "C2": errs["C2"], "C3": errs["C3"],
+        "note": "FP32-storage state against the FP64 oracle after 1000 (16^3, alpha 0.01), 500 (128^3) and "
+                "%d (256^3, alpha 0.01) coupled steps from identical inputs" % target}
+    return res
 
 
 def load_traffic(kernel):
@@ -376,6 +471,10 @@ def main():
             # the timed region of each form); the particle work around it is
             # FP64 in both forms
             out["fp32_store"]["lattice_pass_speedup_vs_fp64"] = round(m["main_avg_ms"] / f["main_avg_ms"], 3)
+    small32 = None
+    if "f32" in res and "f64" in res and not a.no_cpu and a.size == 256:
+        log("FP32 runs of the 16^3 and 128^3 configurations")
+        small32 = small_config_states("f32")
     if a.ensemble_steps > 0:
         out["ensemble"] = {}
         for form in forms:
@@ -387,8 +486,11 @@ def main():
         out["serving"] = bench_serving(a.serve_requests, a.serve_rate)
         log("serving: %s" % out["serving"])
     if not a.no_cpu:
-        log("CPU baseline (oracle)")
-        out["cpu_baseline"] = cpu_baseline(case)
+        log("CPU baseline and FP64 oracle references")
+        if small32 is not None:
+            out.update(oracle_legs(case, res["f64"]["state"], res["f32"]["state"], small32))
+        else:
+            out.update(oracle_legs(case, res[main_form]["state"], None, None))
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

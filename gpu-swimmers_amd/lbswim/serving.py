@@ -96,6 +96,15 @@ def initial_state(req):
     return rho, u, sw_r, sw_n, tr_r
 
 
+def common_params(req):
+    """Swimmer parameters of a request (CommonParams fields): P = 1e-3,
+    l = 0.1, the request's alpha and seed, and the mobility of README.md's
+    example, (1/a - 1/L_h)/(6 pi eta) with a = 0.1, L_h = 1.5, eta = tau/3."""
+    eta = req.tau / 3
+    return dict(P=1e-3, l=0.1, alpha=req.alpha, mobility=(1.0 / 0.1 - 1.0 / 1.5) / (6.0 * math.pi * eta),
+                translational_advection_off=False, rotational_advection_off=False, seed=req.seed)
+
+
 def build(req):
     """Device objects of a request, initialised as InitFromHydro from its state."""
     from . import lb, swimmers, tracers
@@ -111,9 +120,8 @@ def build(req):
     things = []
     if req.swimmers:
         cp = swimmers.CommonParams()
-        cp.P, cp.l, cp.alpha, cp.seed = 1e-3, 0.1, req.alpha, req.seed
-        eta = req.tau / 3
-        cp.mobility = (1.0 / 0.1 - 1.0 / 1.5) / (6.0 * math.pi * eta)
+        for k, v in common_params(req).items():
+            setattr(cp, k, v)
         sw = swimmers.Array(req.swimmers, cp)
         sw.r[:] = sw_r
         sw.n[:] = sw_n

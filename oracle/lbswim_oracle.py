@@ -61,6 +61,22 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
+def build_native(path):
+    """Compile the timing build (-march=native for the CPU that compiles it)
+    to `path`; same arithmetic as the default build."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "native", "NATIVE_OUT=" + path])
+    return path
+
+
+def use_library(path):
+    """Load the oracle from `path` instead of the default build (before any
+    other call of this module)."""
+    global LIB_PATH
+    if _lib is not None and os.path.abspath(path) != os.path.abspath(LIB_PATH):
+        raise RuntimeError("the oracle library is already loaded from %s" % LIB_PATH)
+    LIB_PATH = path
+
+
 _lib = None
 
 

@@ -389,3 +389,21 @@ def test_coupled_fixture_regression():
     out = coupled_small(O)
     for k in ref.files:
         np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+def test_native_timing_build_same_results(tmp_path):
+    """The CPU baseline times a build compiled with -march=native on its host;
+    it must compute what the pinned build computes (a vectorised reduction
+    or a contracted multiply-add would change the bits)."""
+    import subprocess
+    import sys
+    lib = O.build_native(str(tmp_path / "liblbswim_oracle_native.so"))
+    out = tmp_path / "native.npz"
+    code = ("import sys; sys.path[:0] = %r; import numpy as np, lbswim_oracle as O; O.use_library(%r); "
+            "from golden_cases import coupled_small; np.savez(%r, **coupled_small(O))"
+            % ([HERE, os.path.join(os.path.dirname(HERE), "oracle")], lib, str(out)))
+    subprocess.check_call([sys.executable, "-c", code])
+    ref = np.load(os.path.join(HERE, "golden", "coupled_small.npz"))
+    got = np.load(out)
+    for k in ref.files:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)

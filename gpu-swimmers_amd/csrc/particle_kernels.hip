@@ -147,16 +147,97 @@ __device__ __forceinline__ double u01d(uint32_t a, uint32_t b) {
   return (double(m) + 0.5) * 0x1p-53;
 }
 
+// Box-Muller needs ln(u) and cos/sin(2 pi u).  Library versions of these
+// differ between the device and a host C library in the last bit, so the
+// tumble orientations would differ between a CPU restatement and this code;
+// these are written out from series with +, -, *, / and sqrt only (all
+// correctly rounded on both), so they give the same bits everywhere.  Accuracy
+// about 2e-16 absolute, checked against numpy by the oracle's tests.
+
+// ln(x) for a normal x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)), and
+// ln(m) = 2 atanh(s), s = (m-1)/(m+1), |s| <= 0.172, by its Taylor series to
+// s^23 (the next term is below 1e-19).
+__device__ __forceinline__ double bm_log(double x) {
+  const uint64_t bits = __builtin_bit_cast(uint64_t, x);
+  int e = int((bits >> 52) & 0x7FF) - 1022;
+  double m = __builtin_bit_cast(double, (bits & 0x000FFFFFFFFFFFFFull) | (uint64_t(1022) << 52));  // [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 23.0;
+  p = p * s2 + 1.0 / 21.0;
+  p = p * s2 + 1.0 / 19.0;
+  p = p * s2 + 1.0 / 17.0;
+  p = p * s2 + 1.0 / 15.0;
+  p = p * s2 + 1.0 / 13.0;
+  p = p * s2 + 1.0 / 11.0;
+  p = p * s2 + 1.0 / 9.0;
+  p = p * s2 + 1.0 / 7.0;
+  p = p * s2 + 1.0 / 5.0;
+  p = p * s2 + 1.0 / 3.0;
+  p = p * s2 + 1.0;
+  return double(e) * 0.69314718055994530942 + 2.0 * s * p;
+}
+
+// cos and sin of 2 pi u for u in [0, 1): 4u = q + f (both exact), the angle
+// q pi/2 + f pi/2 is reduced to theta = f pi/2 or (1-f) pi/2 in [0, pi/4],
+// whose sine and cosine come from their Taylor series to theta^19 and
+// theta^18, then rotated by the quadrant q.
+__device__ __forceinline__ void bm_cossin_2pi(double u, double& c, double& s) {
+  const double t = 4.0 * u;
+  const double q = floor(t);
+  double f = t - q;
+  const bool swap = f > 0.5;
+  if (swap) f = 1.0 - f;
+  const double th = f * 1.57079632679489661923;
+  const double t2 = th * th;
+  double ps = -1.0 / 121645100408832000.0;  // -1/19!
+  ps = ps * t2 + 1.0 / 355687428096000.0;    // 1/17!
+  ps = ps * t2 - 1.0 / 1307674368000.0;      // 1/15!
+  ps = ps * t2 + 1.0 / 6227020800.0;         // 1/13!
+  ps = ps * t2 - 1.0 / 39916800.0;           // 1/11!
+  ps = ps * t2 + 1.0 / 362880.0;             // 1/9!
+  ps = ps * t2 - 1.0 / 5040.0;               // 1/7!
+  ps = ps * t2 + 1.0 / 120.0;                // 1/5!
+  ps = ps * t2 - 1.0 / 6.0;                  // 1/3!
+  ps = ps * t2 + 1.0;
+  double pc = -1.0 / 6402373705728000.0;     // -1/18!
+  pc = pc * t2 + 1.0 / 20922789888000.0;     // 1/16!
+  pc = pc * t2 - 1.0 / 87178291200.0;        // 1/14!
+  pc = pc * t2 + 1.0 / 479001600.0;          // 1/12!
+  pc = pc * t2 - 1.0 / 3628800.0;            // 1/10!
+  pc = pc * t2 + 1.0 / 40320.0;              // 1/8!
+  pc = pc * t2 - 1.0 / 720.0;                // 1/6!
+  pc = pc * t2 + 1.0 / 24.0;                 // 1/4!
+  pc = pc * t2 - 1.0 / 2.0;                  // 1/2!
+  pc = pc * t2 + 1.0;
+  double sv = th * ps, cv = pc;
+  if (swap) {
+    const double tmp = sv;
+    sv = cv;
+    cv = tmp;
+  }
+  const int quad = int(q);
+  c = quad == 0 ? cv : (quad == 1 ? -sv : (quad == 2 ? -cv : sv));
+  s = quad == 0 ? sv : (quad == 1 ? cv : (quad == 2 ? -sv : -cv));
+}
+
 // Three N(0,1) normals by Box-Muller from draw blocks 1 and 2.
 __device__ __forceinline__ void normals3(unsigned long long seed, int is, uint64_t ctr, double (&g)[3]) {
   const U4 b1 = draw(seed, is, ctr, 1u), b2 = draw(seed, is, ctr, 2u);
   const double u1 = u01d(b1.x[0], b1.x[1]), u2 = u01d(b1.x[2], b1.x[3]);
   const double u3 = u01d(b2.x[0], b2.x[1]), u4 = u01d(b2.x[2], b2.x[3]);
-  double rad = sqrt(-2.0 * log(u1));
-  g[0] = rad * cos(2.0 * M_PI * u2);
-  g[1] = rad * sin(2.0 * M_PI * u2);
-  rad = sqrt(-2.0 * log(u3));
-  g[2] = rad * cos(2.0 * M_PI * u4);
+  double c, sn;
+  double rad = sqrt(-2.0 * bm_log(u1));
+  bm_cossin_2pi(u2, c, sn);
+  g[0] = rad * c;
+  g[1] = rad * sn;
+  rad = sqrt(-2.0 * bm_log(u3));
+  bm_cossin_2pi(u4, c, sn);
+  g[2] = rad * c;
 }
 
 // ---- force spreading -------------------------------------------------------

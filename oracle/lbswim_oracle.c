@@ -465,21 +465,100 @@ static double orc_u01d(uint32_t a, uint32_t b) {
   return ((double)m + 0.5) * 0x1p-53;
 }
 
+/* ln(x) and cos/sin(2 pi u) for Box-Muller, from series with +, -, *, /
+ * only, so that GPU and CPU give the same bits (library log/sin/cos differ in
+ * the last bit).  ln: x = m 2^e, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s),
+ * s = (m-1)/(m+1), Taylor series to s^23. */
+double orc_bm_log(double x) {
+  uint64_t bits, mb;
+  int e;
+  double m, s, s2, p;
+  memcpy(&bits, &x, 8);
+  e = (int)((bits >> 52) & 0x7FF) - 1022;
+  mb = (bits & 0x000FFFFFFFFFFFFFull) | ((uint64_t)1022 << 52);
+  memcpy(&m, &mb, 8); /* [0.5, 1) */
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  s = (m - 1.0) / (m + 1.0);
+  s2 = s * s;
+  p = 1.0 / 23.0;
+  p = p * s2 + 1.0 / 21.0;
+  p = p * s2 + 1.0 / 19.0;
+  p = p * s2 + 1.0 / 17.0;
+  p = p * s2 + 1.0 / 15.0;
+  p = p * s2 + 1.0 / 13.0;
+  p = p * s2 + 1.0 / 11.0;
+  p = p * s2 + 1.0 / 9.0;
+  p = p * s2 + 1.0 / 7.0;
+  p = p * s2 + 1.0 / 5.0;
+  p = p * s2 + 1.0 / 3.0;
+  p = p * s2 + 1.0;
+  return (double)e * 0.69314718055994530942 + 2.0 * s * p;
+}
+
+/* cos and sin of 2 pi u, u in [0, 1): 4u = q + f, theta = f pi/2 (or
+ * (1-f) pi/2 with sine and cosine swapped) in [0, pi/4], Taylor series to
+ * theta^19 / theta^18, rotated by the quadrant q. */
+void orc_bm_cossin_2pi(double u, double *c, double *s) {
+  const double t = 4.0 * u;
+  const double q = floor(t);
+  double f = t - q, th, t2, ps, pc, sv, cv;
+  const int swap = f > 0.5;
+  int quad;
+  if (swap) f = 1.0 - f;
+  th = f * 1.57079632679489661923;
+  t2 = th * th;
+  ps = -1.0 / 121645100408832000.0;
+  ps = ps * t2 + 1.0 / 355687428096000.0;
+  ps = ps * t2 - 1.0 / 1307674368000.0;
+  ps = ps * t2 + 1.0 / 6227020800.0;
+  ps = ps * t2 - 1.0 / 39916800.0;
+  ps = ps * t2 + 1.0 / 362880.0;
+  ps = ps * t2 - 1.0 / 5040.0;
+  ps = ps * t2 + 1.0 / 120.0;
+  ps = ps * t2 - 1.0 / 6.0;
+  ps = ps * t2 + 1.0;
+  pc = -1.0 / 6402373705728000.0;
+  pc = pc * t2 + 1.0 / 20922789888000.0;
+  pc = pc * t2 - 1.0 / 87178291200.0;
+  pc = pc * t2 + 1.0 / 479001600.0;
+  pc = pc * t2 - 1.0 / 3628800.0;
+  pc = pc * t2 + 1.0 / 40320.0;
+  pc = pc * t2 - 1.0 / 720.0;
+  pc = pc * t2 + 1.0 / 24.0;
+  pc = pc * t2 - 1.0 / 2.0;
+  pc = pc * t2 + 1.0;
+  sv = th * ps;
+  cv = pc;
+  if (swap) {
+    const double tmp = sv;
+    sv = cv;
+    cv = tmp;
+  }
+  quad = (int)q;
+  *c = quad == 0 ? cv : (quad == 1 ? -sv : (quad == 2 ? -cv : sv));
+  *s = quad == 0 ? sv : (quad == 1 ? cv : (quad == 2 ? -sv : -cv));
+}
+
 /* Three N(0,1) normals by Box-Muller from draw blocks 1 and 2. */
 static void orc_normals3(unsigned long long seed, int is, uint64_t ctr, double g[3]) {
   uint32_t b1[4], b2[4];
-  double u1, u2, u3, u4, rad;
+  double u1, u2, u3, u4, rad, c, s;
   orc_draw(seed, is, ctr, 1u, b1);
   orc_draw(seed, is, ctr, 2u, b2);
   u1 = orc_u01d(b1[0], b1[1]);
   u2 = orc_u01d(b1[2], b1[3]);
   u3 = orc_u01d(b2[0], b2[1]);
   u4 = orc_u01d(b2[2], b2[3]);
-  rad = sqrt(-2.0 * log(u1));
-  g[0] = rad * cos(2.0 * M_PI * u2);
-  g[1] = rad * sin(2.0 * M_PI * u2);
-  rad = sqrt(-2.0 * log(u3));
-  g[2] = rad * cos(2.0 * M_PI * u4);
+  rad = sqrt(-2.0 * orc_bm_log(u1));
+  orc_bm_cossin_2pi(u2, &c, &s);
+  g[0] = rad * c;
+  g[1] = rad * s;
+  rad = sqrt(-2.0 * orc_bm_log(u3));
+  orc_bm_cossin_2pi(u4, &c, &s);
+  g[2] = rad * c;
 }
 
 /* Uniform and (if it tumbles) normals for one swimmer step; exported for tests. */

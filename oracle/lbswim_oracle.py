@@ -105,6 +105,9 @@ def lib():
         L.orc_swimmer_uniform.argtypes = [ctypes.c_ulonglong, _I, ctypes.c_uint64]
         L.orc_swimmer_uniform.restype = ctypes.c_float
         L.orc_swimmer_normals.argtypes = [ctypes.c_ulonglong, _I, ctypes.c_uint64, _D]
+        L.orc_bm_log.argtypes = [ctypes.c_double]
+        L.orc_bm_log.restype = ctypes.c_double
+        L.orc_bm_cossin_2pi.argtypes = [ctypes.c_double, _D, _D]
         L.orc_run.argtypes = [_I, _I, _I, ctypes.c_double, ctypes.c_double, _I,
                               _D, _D, _D, _D, _D, _I, ctypes.POINTER(OrcCommon),
                               _D, _D, _D, _U64, _I, _D, _D, _D]
@@ -156,6 +159,18 @@ def swimmer_normals(seed, i, ctr):
     g = np.zeros(3)
     lib().orc_swimmer_normals(seed, i, ctr, _p(g))
     return g
+
+
+def bm_log(x):
+    """ln x as the Box-Muller transform evaluates it (series, same bits on the GPU)."""
+    return lib().orc_bm_log(float(x))
+
+
+def bm_cossin_2pi(u):
+    """(cos 2 pi u, sin 2 pi u) as the Box-Muller transform evaluates them."""
+    c, s = ctypes.c_double(), ctypes.c_double()
+    lib().orc_bm_cossin_2pi(float(u), ctypes.byref(c), ctypes.byref(s))
+    return c.value, s.value
 
 
 class Lattice:

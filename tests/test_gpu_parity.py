@@ -256,7 +256,9 @@ def test_config1_alpha_tumbling_1000_steps():
     # same counter-based stream on both sides: identical tumble decisions
     assert t_gpu == t_ref
     assert np.array_equal(got["sw_prng"], ref["sw_prng"])
-    compare(got, ref, ("fOld", "rho", "u", "sw_r", "sw_n", "tr_r"), 1e-10, "config1 alpha=0.01")
+    # tumble orientations use the same series log/cos/sin as the oracle: bit for bit
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), k
 
 
 def test_tumble_rate_binomial_bound():

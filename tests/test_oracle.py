@@ -351,6 +351,26 @@ def test_philox_known_answer_vectors():
         0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
 
 
+def test_box_muller_log_and_cossin_accuracy():
+    """The series log and cos/sin(2 pi u) of the Box-Muller transform against
+    numpy (a wrong or missing series term shows as an error far above 1e-15;
+    a wrong quadrant as an error of order 1)."""
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(0, 1, 3000), 2.0 ** -rng.uniform(0, 54, 3000),
+                        [2.0 ** -54, 0.5, 0.70710678118654752, 0.7071067811865476, 1 - 2 ** -53]])
+    got = np.array([O.bm_log(v) for v in x])
+    assert np.max(np.abs(got - np.log(x)) / np.maximum(1.0, np.abs(np.log(x)))) < 4e-16
+    u = np.concatenate([rng.uniform(0, 1, 4000), np.arange(8) / 8.0, [2 ** -53, 0.125 - 2 ** -55, 0.375 + 2 ** -54]])
+    cs = np.array([O.bm_cossin_2pi(v) for v in u])
+    # reference in extended precision: 2 pi u rounded to double alone is off by up to 4e-16
+    ang = np.longdouble("6.283185307179586476925286766559005768") * u.astype(np.longdouble)
+    assert np.max(np.abs(cs[:, 0] - np.cos(ang))) < 3e-16
+    assert np.max(np.abs(cs[:, 1] - np.sin(ang))) < 3e-16
+    # exact at the quadrant points
+    assert O.bm_cossin_2pi(0.0) == (1.0, 0.0) and O.bm_cossin_2pi(0.25) == (0.0, 1.0)
+    assert O.bm_cossin_2pi(0.5) == (-1.0, 0.0) and O.bm_cossin_2pi(0.75) == (0.0, -1.0)
+
+
 def test_uniforms_and_normals_statistics():
     U = np.array([O.swimmer_uniform(1234, i, c) for i in range(200) for c in range(100)])
     assert U.min() > 0.0 and U.max() <= 1.0

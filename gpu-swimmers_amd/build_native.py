@@ -24,7 +24,7 @@ LIB = os.path.join(LIBDIR, "liblbswim_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["lattice_kernels.hip", "particle_kernels.hip", "particle_order.hip", "capi.cpp"]
+SOURCES = ["lattice_kernels.hip", "particle_kernels.hip", "particle_order.hip", "noalloc.hip", "capi.cpp"]
 
 # -ffp-contract=off: the FP64 path reproduces the reference's unfused
 #   arithmetic; FP32 kernels ask for fusion explicitly with fmaf.

@@ -70,6 +70,9 @@ int ensure_runtime() {
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev < 1) return fail(LBSWIM_ENODEV, "no HIP device available (%s)", hipGetErrorString(e));
   HIP_TRY(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
+  // the no-allocation step's fixed workspace on this device
+  void* scratch = nullptr;
+  HIP_TRY(noalloc_scratch(&scratch));
   g_rt.cur = g_rt.stream;
   g_rt.ready = true;
   return LBSWIM_OK;
@@ -1294,6 +1297,109 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
   }
   const int jrc = join();
   return rc ? rc : jrc;
+}
+
+// ---- coupled step without allocation -----------------------------------------
+
+namespace {
+
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// The spreading workspace of one lattice carved out of the fixed device
+// workspace: counts, offsets (+ scan and sort scratch), cursor, point
+// records, list and sorted entries.
+struct FixedSpread {
+  uint32_t* counts = nullptr;
+  uint32_t* off = nullptr;
+  uint32_t* cursor = nullptr;
+  SpreadPoint* pts = nullptr;
+  uint32_t* list = nullptr;
+  uint32_t* sorted = nullptr;
+  size_t bytes = 0;
+};
+
+FixedSpread carve_spread(unsigned char* base, uint32_t nl, size_t npts) {
+  FixedSpread w;
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base ? base + at : nullptr;
+    at += align256(bytes);
+    return p;
+  };
+  w.counts = reinterpret_cast<uint32_t*>(take(4 * size_t(nl)));
+  w.off = reinterpret_cast<uint32_t*>(take(4 * spread_off_entries(nl)));
+  w.cursor = reinterpret_cast<uint32_t*>(take(4 * size_t(nl)));
+  w.pts = reinterpret_cast<SpreadPoint*>(take(sizeof(SpreadPoint) * npts));
+  w.list = reinterpret_cast<uint32_t*>(take(4 * spread_list_capacity(npts)));
+  w.sorted = reinterpret_cast<uint32_t*>(take(4 * spread_list_capacity(npts)));
+  w.bytes = at;
+  return w;
+}
+
+}  // namespace
+
+int lbswim_system_step_noalloc(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
+                               int ntr, int nsteps) {
+  if (int rc = check_lattice(lat)) return rc;
+  CHECK_ARG(nsw >= 0 && ntr >= 0 && nsteps >= 0, "negative count");
+  CHECK_ARG(nsw == 0 || sw != nullptr, "swimmer list is NULL");
+  CHECK_ARG(ntr == 0 || tr != nullptr, "tracer list is NULL");
+  for (int i = 0; i < nsw; ++i) CHECK_ARG(sw[i] != nullptr, "swimmer handle %d is NULL", i);
+  for (int i = 0; i < ntr; ++i) CHECK_ARG(tr[i] != nullptr, "tracer handle %d is NULL", i);
+  size_t npts = 0;
+  for (int i = 0; i < nsw; ++i) npts += 2 * size_t(sw[i]->st.num);
+  const uint32_t nl = spread_lists(lat->geom);
+  const size_t need = npts ? carve_spread(nullptr, nl, npts).bytes : 0;
+  CHECK_ARG(need <= NOALLOC_SCRATCH_BYTES,
+            "the no-allocation step needs %zu bytes of workspace for %zu swimmer ends on %u lists; its fixed "
+            "workspace holds %zu bytes (use lbswim_system_step)",
+            need, npts, nl, NOALLOC_SCRATCH_BYTES);
+  if (nsteps == 0) return LBSWIM_OK;
+  void* base = nullptr;
+  HIP_TRY(noalloc_scratch(&base));
+  const FixedSpread w = carve_spread(static_cast<unsigned char*>(base), nl, npts);
+  // The lists' counts must start at zero; the workspace keeps what the last
+  // call (possibly of another lattice) left in it.  A fill kernel, not a
+  // memset, so that a caller may capture the call into a graph.
+  if (npts) HIP_TRY(launch_fill_zero(w.counts, 4 * size_t(nl), stream()));
+  const bool fused = lat->d_u_aos != nullptr && fusion_rows(lat) > 0;
+  int rc = LBSWIM_OK;
+  for (int s = 0; s < nsteps && !rc; ++s) {
+    // ZeroForce + AddForces: the force array written from the sorted lists
+    if (npts) {
+      KernelTimer t(lat, LBSWIM_K_ADDF);
+      int base_id = 0;
+      hipError_t e = hipSuccess;
+      for (int i = 0; i < nsw && e == hipSuccess; ++i) {
+        PStore& st = sw[i]->st;
+        e = launch_points(st.num, base_id, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R),
+                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, w.pts, stream());
+        base_id += st.num;
+      }
+      if (e == hipSuccess)
+        e = launch_spread_lists(int(npts), lat->geom, w.pts, w.counts, w.off, w.cursor, w.list, w.sorted, stream());
+      if (e == hipSuccess)
+        e = lat->storage == LBSWIM_STORE_F64
+                ? launch_spread_force<double>(lat->geom, w.pts, w.off, w.sorted, static_cast<double*>(lat->d_force),
+                                              stream())
+                : launch_spread_force<float>(lat->geom, w.pts, w.off, w.sorted, static_cast<float*>(lat->d_force),
+                                             stream());
+      t.done();
+      HIP_TRY(e);
+      lat->force_virtual = false;
+    } else if ((rc = lattice_zero_force_impl(lat))) {
+      break;
+    }
+    if (fused) {
+      if ((rc = lattice_step_hydro_impl(lat, fusion_rows(lat)))) break;
+    } else {
+      if ((rc = lattice_step_impl(lat))) break;
+      if ((rc = lattice_hydro_impl(lat))) break;
+    }
+    for (int i = 0; i < nsw && !rc; ++i) rc = swimmers_move_impl(sw[i], lat);
+    for (int i = 0; i < ntr && !rc; ++i) rc = tracers_move_impl(tr[i], lat);
+  }
+  return rc;
 }
 
 // ---- diagnostics ----------------------------------------------------------------

@@ -216,6 +216,11 @@ hipError_t launch_gather(int np, int ncomp, const int* order, const T* src, T* d
 template <typename T>
 hipError_t launch_scatter(int np, int ncomp, const int* order, const T* src, T* dst, hipStream_t s);
 
+// ---- fixed workspace of the no-allocation coupled step (noalloc.hip)
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+// Device address of this device's workspace (looked up once per device).
+hipError_t noalloc_scratch(void** out);
+
 // ---- diagnostics
 hipError_t launch_copy(const double* src, double* dst, size_t count, hipStream_t s);
 hipError_t launch_peskin_probe(const double* x, double* out, int count, hipStream_t s);

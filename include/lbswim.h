@@ -229,6 +229,24 @@ int lbswim_tracers_slots(lbswim_tracers* tr, const int** device);
  * obtained earlier calls lbswim_lattice_sync_force first. */
 int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
                        int ntr, int nsteps);
+/* lbswim_system_step for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  No reference counterpart.  It calls no hipMalloc, hipFree or
+ * allocating library routine: the swimmer-force lists live in a fixed
+ * device workspace of 64 MiB per GPU (held from the library's first use on
+ * that GPU until the process ends), and the force is written to the
+ * lattice's own force array (stored, not virtual).  Results are bit for bit
+ * those of lbswim_system_step.  Returns LBSWIM_EINVAL, before enqueueing
+ * anything, when the lists of the system need more than the workspace
+ * (about 1.4 KB per swimmer plus 12 B per 64 lattice sites: e.g. 256^3 with
+ * 32768 swimmers needs 49 MB).  Takes the fused Step+CalcHydro launch only if
+ * the lattice already holds its interleaved u buffer (an earlier
+ * lbswim_system_step allocated it), else the two launches; tracer moves stay
+ * on the library stream.  Every call on a GPU shares that GPU's workspace:
+ * calls whose device work overlaps (other streams, concurrent replays of
+ * graphs that captured it) race. */
+int lbswim_system_step_noalloc(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
+                               int ntr, int nsteps);
 /* LBSWIM_FORCE_STORED, _AUTO (default) or _VIRTUAL. No reference counterpart. */
 int lbswim_lattice_set_force_mode(lbswim_lattice* lat, int mode);
 /* Writes the force array if the last coupled step left it virtual. No reference counterpart. */

@@ -390,6 +390,9 @@ def main():
     ap.add_argument("--ensemble-eager", action="store_true",
                     help="launch the ensemble's steps eagerly instead of replaying captured graphs")
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--legs", default="main,ensemble,serving",
+                    help="legs to run: main (the 256^3 coupled step; required for the JSON line's value), "
+                         "ensemble, serving")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (a lattice would be split into "
@@ -404,6 +407,18 @@ def main():
         gbs = 2 * 8 * count / (ms * 1e-3) / 1e9
         print(json.dumps({"probe": "copy", "doubles": count, "bytes_per_copy": 16 * count, "avg_ms": round(ms, 4),
                           "GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+        return
+    legs = {x for x in a.legs.split(",") if x}
+    if "main" not in legs:  # partial runs (profiling): no headline value
+        out = {"legs": sorted(legs)}
+        if "ensemble" in legs and a.ensemble_steps > 0:
+            out["ensemble"] = {}
+            for form in [f for f in a.forms.split(",") if f]:
+                log("ensemble %s: 64 sims x %d steps" % (form, a.ensemble_steps))
+                out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps, graph=not a.ensemble_eager)
+        if "serving" in legs and a.serve_requests > 0:
+            out["serving"] = bench_serving(a.serve_requests, a.serve_rate)
+        print(json.dumps(out), flush=True)
         return
     log("building workload %d^3" % a.size)
     case = make_workload(a.size)

@@ -1162,6 +1162,15 @@ bool use_virtual_force(int mode, size_t npts, size_t lists) {
   return mode == LBSWIM_FORCE_VIRTUAL || (mode == LBSWIM_FORCE_AUTO && npts * 17 <= VF_MAX_MEAN_ENTRIES * lists);
 }
 
+// How a system's swimmer force is formed: virtual from its lists where
+// sparse; otherwise stored, from all its points when they are few enough
+// that every run walking them all costs less than building lists that would
+// each hold many of them, else from lists.
+int force_path(int mode, size_t npts, const Geom& g) {
+  if (use_virtual_force(mode, npts, spread_lists(g))) return FORCE_LISTS_VIRTUAL;
+  return npts <= RANGE_FORCE_MAX_POINTS ? FORCE_RANGE_STORED : FORCE_LISTS_STORED;
+}
+
 // ZeroForce + AddForces of every swimmer array, as one deterministic gather.
 // With the virtual force the lists are left for the next collide and CalcHydro.
 int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
@@ -1183,6 +1192,29 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     if (int rc = grow(&lat->sp_pts, &cap_bytes, npts * sizeof(SpreadPoint), 1)) return rc;
     lat->sp_pts_cap = npts;
   }
+  SpreadPoint* pts = static_cast<SpreadPoint*>(lat->sp_pts);
+  const int path = force_path(lat->force_mode, npts, lat->geom);
+  if (path == FORCE_RANGE_STORED) {
+    // few points, dense: every run walks all of them, no lists
+    KernelTimer t(lat, LBSWIM_K_ADDF);
+    hipError_t e = hipSuccess;
+    int offset = 0;
+    for (int i = 0; i < nsw && e == hipSuccess; ++i) {
+      PStore& st = sw[i]->st;
+      e = launch_points(st.num, offset, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_N),
+                        st.perm, lat->geom, pts, stream());
+      offset += st.num;
+    }
+    if (e == hipSuccess)
+      e = lat->storage == LBSWIM_STORE_F64
+              ? launch_range_force<double>(lat->geom, pts, uint32_t(npts), static_cast<double*>(lat->d_force),
+                                           stream())
+              : launch_range_force<float>(lat->geom, pts, uint32_t(npts), static_cast<float*>(lat->d_force), stream());
+    t.done();
+    HIP_TRY(e);
+    lat->force_virtual = false;
+    return LBSWIM_OK;
+  }
   const size_t lcap = spread_list_capacity(npts);
   if (lat->sp_list_cap < lcap) {
     size_t c1 = lat->sp_list_cap, c2 = lat->sp_list_cap;
@@ -1190,8 +1222,7 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     if (int rc = grow(reinterpret_cast<void**>(&lat->sp_sorted), &c2, lcap, 4)) return rc;
     lat->sp_list_cap = lcap;
   }
-  SpreadPoint* pts = static_cast<SpreadPoint*>(lat->sp_pts);
-  const bool virt = use_virtual_force(lat->force_mode, npts, lat->sp_lists);
+  const bool virt = path == FORCE_LISTS_VIRTUAL;
 
   KernelTimer t(lat, LBSWIM_K_ADDF);
   int offset = 0;
@@ -1601,13 +1632,18 @@ struct lbswim_ensemble {
   // device tables
   LatDesc* d_lat = nullptr;
   size_t cap_lat = 0;
-  // lattice blocks of the members whose force is stored / virtual
+  // lattice blocks: of every member (collide, CalcHydro), of the members
+  // whose force is stored (written before the collide) and of those whose
+  // force is virtual (written from the lists when a call returns)
   WorkItem* d_latwork = nullptr;
   size_t cap_latwork = 0;
   int n_latwork = 0;
-  WorkItem* d_latwork_vf = nullptr;
-  size_t cap_latwork_vf = 0;
-  int n_latwork_vf = 0;
+  WorkItem* d_storedwork = nullptr;
+  size_t cap_storedwork = 0;
+  int n_storedwork = 0;
+  WorkItem* d_vfwork = nullptr;
+  size_t cap_vfwork = 0;
+  int n_vfwork = 0;
   SwimDesc* d_sw = nullptr;
   size_t cap_sw = 0;
   WorkItem* d_swwork = nullptr;
@@ -1657,9 +1693,10 @@ int ens_build(lbswim_ensemble* e) {
   HIP_TRY(hipStreamSynchronize(stream()));
   ens_drop_graph(e);
   std::vector<LatDesc> lat;
-  // per member: its lattice blocks, in the stored- or the virtual-force group
-  std::vector<std::vector<WorkItem>> latwork[2];
-  std::vector<size_t> weight[2];
+  // per member: its lattice blocks; blocks of stored- and virtual-force members
+  std::vector<std::vector<WorkItem>> latwork;
+  std::vector<size_t> weight;
+  std::vector<WorkItem> storedwork, vfwork;
   std::vector<SwimDesc> sw;
   std::vector<WorkItem> swwork, ptwork, trwork;
   std::vector<TracerDesc> tr;
@@ -1680,17 +1717,21 @@ int ens_build(lbswim_ensemble* e) {
     d.omega_s = 1.0 / (d.tau_s + 0.5);
     d.omega_b = 1.0 / (d.tau_b + 0.5);
     d.cur = L->cur;
-    d.tile_base = tile_base;
-    lat.push_back(d);
     size_t member_pts = 0;
     for (lbswim_swimmers* s : m.sw) member_pts += 2 * size_t(s->st.num);
     // the same density rule as a single lattice, member by member
-    const int group = use_virtual_force(e->force_mode, member_pts, spread_lists(L->geom)) ? 1 : 0;
+    d.force_path = force_path(e->force_mode, member_pts, L->geom);
+    d.tile_base = d.force_path == FORCE_RANGE_STORED ? 0u : tile_base;
+    if (d.force_path != FORCE_RANGE_STORED) tile_base += spread_lists(L->geom);
+    d.point_first = point_base;
+    d.point_count = uint32_t(member_pts);
+    lat.push_back(d);
     std::vector<WorkItem> lw;
     for (uint32_t b = 0; b < lattice_blocks(L->geom); ++b) lw.push_back(WorkItem{mi, b});
-    latwork[group].push_back(lw);
-    weight[group].push_back(L->n);
-    tile_base += spread_lists(L->geom);
+    std::vector<WorkItem>& group = d.force_path == FORCE_LISTS_VIRTUAL ? vfwork : storedwork;
+    group.insert(group.end(), lw.begin(), lw.end());
+    latwork.push_back(lw);
+    weight.push_back(L->n);
     for (lbswim_swimmers* s : m.sw) {
       if (int rc = pstore_maybe_reorder(s->st, L->geom)) return rc;
       const uint32_t ai = uint32_t(sw.size());
@@ -1729,25 +1770,26 @@ int ens_build(lbswim_ensemble* e) {
     if (!m.lat) continue;
     m.cur_at_build = m.lat->cur;
   }
-  const std::vector<WorkItem> lw = order_work(latwork[0], e->order, weight[0]);
-  const std::vector<WorkItem> lwv = order_work(latwork[1], e->order, weight[1]);
+  const std::vector<WorkItem> lw = order_work(latwork, e->order, weight);
   int rc;
   if ((rc = upload(&e->d_lat, &e->cap_lat, lat)) || (rc = upload(&e->d_latwork, &e->cap_latwork, lw)) ||
-      (rc = upload(&e->d_latwork_vf, &e->cap_latwork_vf, lwv)) ||
-      (rc = upload(&e->d_sw, &e->cap_sw, sw)) ||
+      (rc = upload(&e->d_storedwork, &e->cap_storedwork, storedwork)) ||
+      (rc = upload(&e->d_vfwork, &e->cap_vfwork, vfwork)) || (rc = upload(&e->d_sw, &e->cap_sw, sw)) ||
       (rc = upload(&e->d_swwork, &e->cap_swwork, swwork)) || (rc = upload(&e->d_ptwork, &e->cap_ptwork, ptwork)) ||
       (rc = upload(&e->d_tr, &e->cap_tr, tr)) || (rc = upload(&e->d_trwork, &e->cap_trwork, trwork)))
     return rc;
   e->n_latwork = int(lw.size());
-  e->n_latwork_vf = int(lwv.size());
+  e->n_storedwork = int(storedwork.size());
+  e->n_vfwork = int(vfwork.size());
   e->n_swwork = int(swwork.size());
   e->n_ptwork = int(ptwork.size());
   e->n_trwork = int(trwork.size());
   e->nlists = int(tile_base);
   e->npts = int(point_base);
   // spreading workspace
-  size_t lcap = 0;
-  for (const auto& a : sw) lcap += spread_list_capacity(2 * size_t(a.num));
+  size_t lcap = 0;  // list entries of the members with lists
+  for (const auto& a : sw)
+    if (lat[a.member].force_path != FORCE_RANGE_STORED) lcap += spread_list_capacity(2 * size_t(a.num));
   auto grow_to = [&](auto** p, size_t* cap, size_t need, size_t elem) -> int {
     if (*cap >= need) return LBSWIM_OK;
     dev_free(*p);
@@ -1777,12 +1819,11 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   HIP_TRY(launch_batch_lists(e->d_lat, e->npts, e->nlists, e->pts, e->counts, e->off, e->cursor, e->list, e->sorted,
                              s));
   const VForce vf{e->pts, e->off, e->sorted};
-  // members with a stored force: write it, then read it; the others evaluate it from the lists
-  HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_latwork, e->d_latwork, e->pts, e->off, e->sorted, s));
-  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, nullptr, s));
-  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork_vf, e->n_latwork_vf, parity, &vf, s));
-  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, nullptr, s));
-  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork_vf, e->n_latwork_vf, parity, &vf, s));
+  // members with a stored force: write it (from their lists or all their
+  // points); the others evaluate it from the lists in the collide and CalcHydro
+  HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->pts, e->off, e->sorted, s));
+  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, vf, s));
+  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, vf, s));
   HIP_TRY(launch_batch_swimmers_move<T>(e->d_lat, e->d_sw, e->d_swwork, e->n_swwork, s));
   HIP_TRY(launch_batch_tracers_move<T>(e->d_lat, e->d_tr, e->d_trwork, e->n_trwork, s));
   return LBSWIM_OK;
@@ -1796,9 +1837,9 @@ int ens_enqueue(lbswim_ensemble* e, int parity) {
 // array from the last step's lists, as the reference leaves it.
 int ens_materialize_force(lbswim_ensemble* e) {
   hipError_t r = e->storage == LBSWIM_STORE_F64
-                     ? launch_batch_spread_force<double>(e->d_lat, e->n_latwork_vf, e->d_latwork_vf, e->pts, e->off,
+                     ? launch_batch_spread_force<double>(e->d_lat, e->n_vfwork, e->d_vfwork, e->pts, e->off,
                                                          e->sorted, stream())
-                     : launch_batch_spread_force<float>(e->d_lat, e->n_latwork_vf, e->d_latwork_vf, e->pts, e->off,
+                     : launch_batch_spread_force<float>(e->d_lat, e->n_vfwork, e->d_vfwork, e->pts, e->off,
                                                         e->sorted, stream());
   HIP_TRY(r);
   return LBSWIM_OK;
@@ -1870,7 +1911,8 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   ens_drop_graph(e);
   dev_free(e->d_lat);
   dev_free(e->d_latwork);
-  dev_free(e->d_latwork_vf);
+  dev_free(e->d_storedwork);
+  dev_free(e->d_vfwork);
   dev_free(e->d_sw);
   dev_free(e->d_swwork);
   dev_free(e->d_ptwork);

@@ -123,6 +123,10 @@ hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r,
 // 64 l .. 64 l + 63 in increasing order.
 hipError_t launch_spread_lists(int npts, const Geom& g, const SpreadPoint* pts, uint32_t* counts, uint32_t* off,
                                uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s);
+// Writes the force of every site from all npts point records in id order,
+// without lists: the sum the sorted lists give (dense small systems).
+template <typename T>
+hipError_t launch_range_force(const Geom& g, const SpreadPoint* pts, uint32_t npts, T* force, hipStream_t s);
 // Writes the force of every site from the lists (ZeroForce + AddForces).
 template <typename T>
 hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
@@ -146,8 +150,22 @@ struct LatDesc {
   Geom geom;
   double tau_s, tau_b, omega_s, omega_b;
   int cur;
-  uint32_t tile_base;  // first spreading list of this member in the batch
+  uint32_t tile_base;    // first spreading list of this member in the batch
+  uint32_t point_first;  // first point id of this member (its arrays' points are consecutive)
+  uint32_t point_count;
+  // how the member's swimmer force reaches its collide and CalcHydro:
+  // FORCE_LISTS_VIRTUAL (evaluated from its lists there), FORCE_LISTS_STORED
+  // (written from its lists to the force array first), FORCE_RANGE_STORED
+  // (written from all its points in id order, no lists)
+  int force_path;
 };
+constexpr int FORCE_LISTS_VIRTUAL = 0;
+constexpr int FORCE_LISTS_STORED = 1;
+constexpr int FORCE_RANGE_STORED = 2;
+// Points up to which a dense (stored-force) system takes FORCE_RANGE_STORED:
+// every 64-site run walks all of them instead of building lists that would
+// hold a large share of them anyway.
+constexpr size_t RANGE_FORCE_MAX_POINTS = 4096;
 // One workgroup of a batched launch: member index and the block (or tile,
 // or particle group) of that member it processes.
 struct WorkItem {
@@ -177,12 +195,13 @@ struct TracerDesc {
 uint32_t lattice_blocks(const Geom& g);  // workgroups of a one-lattice launch
 constexpr int PARTICLE_GROUPS = 16;      // swimmers per workgroup of the swimmer move kernel
 constexpr int TRACER_GROUPS = 64;        // tracers per workgroup of the tracer move kernel
-// vf: the force of member m's block b is evaluated from lists tile_base_m + 4 b ...
+// Members with FORCE_LISTS_VIRTUAL evaluate the force of block b from lists
+// tile_base_m + 4 b ... of `vf`; the others read their force array.
 template <typename T>
-hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
+hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce& vf,
                                 hipStream_t s);
 template <typename T>
-hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
+hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce& vf,
                               hipStream_t s);
 // Point records of a batch: points of array a get ids point_base_a + 2*user +
 // end; work items are blocks of 128 swimmers (256 points) of an array.
@@ -191,7 +210,8 @@ hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, cons
 // Sorted lists of all nlists lists of the batch (see launch_spread_lists).
 hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const SpreadPoint* pts, uint32_t* counts,
                               uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s);
-// Writes the force arrays of the lattice blocks in block_work from the lists.
+// Writes the force arrays of the lattice blocks in block_work: from the lists
+// (FORCE_LISTS_STORED) or from all the member's points (FORCE_RANGE_STORED).
 template <typename T>
 hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nblocks, const WorkItem* block_work,
                                      const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,

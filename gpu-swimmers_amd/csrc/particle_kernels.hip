@@ -745,6 +745,7 @@ __global__ __launch_bounds__(PBLOCK) void k_count_rows(uint32_t npts, Geom g, co
   uint32_t base = 0;
   if constexpr (Batched) {
     const LatDesc& m = ldesc[p.pad[0]];
+    if (m.force_path == FORCE_RANGE_STORED) return;  // the member has no lists
     g = m.geom;
     base = m.tile_base;
   }
@@ -824,6 +825,7 @@ __global__ __launch_bounds__(PBLOCK) void k_fill_rows(uint32_t npts, Geom g, con
   uint32_t base = 0;
   if constexpr (Batched) {
     const LatDesc& m = ldesc[p.pad[0]];
+    if (m.force_path == FORCE_RANGE_STORED) return;
     g = m.geom;
     base = m.tile_base;
   }
@@ -953,6 +955,29 @@ __device__ __forceinline__ void spread_block_body(const Geom& g, const SpreadPoi
   force[s] = T(acc[0]);
   force[n + s] = T(acc[1]);
   force[2 * n + s] = T(acc[2]);
+}
+
+// Force of the sites of lattice block `blk` from all points first ..
+// first+count-1 (range_force), written to the force array.
+template <typename T>
+__device__ __forceinline__ void range_block_body(const Geom& g, const SpreadPoint* __restrict__ pts, uint32_t first,
+                                                 uint32_t count, T* __restrict__ force, uint32_t blk) {
+  const uint32_t s = blk * SITE_BLOCK + threadIdx.x;
+  const bool ok = s < g.n;
+  const SiteXYZ c = site_xyz(ok ? s : 0u, g);
+  double acc[3];
+  range_force(pts, first, count, ok, c.x, c.y, c.z, g, acc);
+  if (!ok) return;
+  const size_t n = g.n;
+  force[s] = T(acc[0]);
+  force[n + s] = T(acc[1]);
+  force[2 * n + s] = T(acc[2]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(SITE_BLOCK) void k_range_blocks(Geom g, const SpreadPoint* __restrict__ pts,
+                                                            uint32_t npts, T* __restrict__ force) {
+  range_block_body(g, pts, 0u, npts, force, blockIdx.x);
 }
 
 template <typename T>
@@ -1090,8 +1115,11 @@ __global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __
   __shared__ WaveStage stage[LISTS_PER_BLOCK];
   const WorkItem w = block_work[blockIdx.x];
   const LatDesc& m = ldesc[w.member];
-  spread_block_body(m.geom, pts, off, sorted, static_cast<T*>(m.force), w.unit,
-                    m.tile_base + w.unit * LISTS_PER_BLOCK, stage);
+  if (m.force_path == FORCE_RANGE_STORED)
+    range_block_body(m.geom, pts, m.point_first, m.point_count, static_cast<T*>(m.force), w.unit);
+  else
+    spread_block_body(m.geom, pts, off, sorted, static_cast<T*>(m.force), w.unit,
+                      m.tile_base + w.unit * LISTS_PER_BLOCK, stage);
 }
 
 // ---- device-math probes used by the parity tests ----
@@ -1222,6 +1250,16 @@ hipError_t launch_spread_lists(int npts, const Geom& g, const SpreadPoint* pts, 
                                uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s) {
   return build_lists<false>(npts, int(spread_lists(g)), g, nullptr, pts, counts, off, cursor, list, sorted, s);
 }
+
+template <typename T>
+hipError_t launch_range_force(const Geom& g, const SpreadPoint* pts, uint32_t npts, T* force, hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_range_blocks<T>, dim3((g.n + SITE_BLOCK - 1) / SITE_BLOCK), dim3(SITE_BLOCK), 0, s, g, pts, npts,
+                     force);
+  return hipGetLastError();
+}
+template hipError_t launch_range_force<double>(const Geom&, const SpreadPoint*, uint32_t, double*, hipStream_t);
+template hipError_t launch_range_force<float>(const Geom&, const SpreadPoint*, uint32_t, float*, hipStream_t);
 
 template <typename T>
 hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,

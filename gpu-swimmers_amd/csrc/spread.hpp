@@ -150,6 +150,22 @@ __device__ __forceinline__ void wave_force(const SpreadPoint* __restrict__ pts, 
   }
 }
 
+// Force at this lane's site from every point first .. first+count-1 in id
+// order: exactly the sum of the site's sorted list (a point not in the list
+// reaches no site of its run and adds nothing; a listed point is added once),
+// without the list.  The point index is uniform: records come through the
+// scalar cache.  For dense small systems, whose lists would each hold a large
+// share of all points.
+__device__ __forceinline__ void range_force(const SpreadPoint* __restrict__ pts, uint32_t first, uint32_t count,
+                                            bool ok, int X, int Y, int Z, const Geom& g, double (&acc)[3]) {
+  acc[0] = acc[1] = acc[2] = 0.0;
+  first = __builtin_amdgcn_readfirstlane(first);
+  count = __builtin_amdgcn_readfirstlane(count);
+  const bool simple = g.nx >= 4 && g.ny >= 4 && g.nz >= 4;
+  for (uint32_t id = first; id < first + count; ++id)
+    if (ok) add_point(pts + id, X, Y, Z, g, simple, acc);
+}
+
 // wave_force for a lane whose list differs from the other lanes' (`wl` per
 // lane, vector loads): the sum of add_point over the list, the same products
 // in the same order as wave_force gives for that site.

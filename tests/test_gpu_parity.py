@@ -135,11 +135,14 @@ def test_system_step_single_ops_bitwise(force_mode):
 
 @pytest.mark.parametrize("force_mode", ["stored", "virtual"])
 @pytest.mark.parametrize("size,ns,l", [((3, 2, 5), 7, 0.4), ((1, 6, 2), 3, 1.5), ((9, 33, 40), 300, 0.1),
-                                       ((40, 20, 70), 2000, 2.5), ((5, 3, 300), 40, 0.8), ((6, 4, 257), 60, 1.2)])
+                                       ((40, 20, 70), 2000, 2.5), ((5, 3, 300), 40, 0.8), ((6, 4, 257), 60, 1.2),
+                                       ((20, 24, 30), 2600, 0.3)])
 def test_spreading_exact_on_awkward_shapes(size, ns, l, force_mode):
     """Stencils wrapping several times onto lattices thinner than 4, partial
     blocks, z-runs crossing block boundaries and the periodic face, dense
-    overlapping swimmers, long swimmers spanning blocks."""
+    overlapping swimmers, long swimmers spanning blocks.  With the stored
+    force, systems of up to 4096 swimmer ends walk all points per run and
+    denser ones (2600 swimmers) build sorted lists."""
     case = make_case(size, 3, 0.3, ns=ns, nt=0, alpha=0.0, seed=ns, drho=1e-2, du=1e-2, P=1e-2, l=l)
     say("spreading exact: %s ns=%d l=%g force %s" % (size, ns, l, force_mode))
     ref, _ = run_oracle(O, case)

@@ -1632,12 +1632,10 @@ struct lbswim_ensemble {
   // device tables
   LatDesc* d_lat = nullptr;
   size_t cap_lat = 0;
-  // lattice blocks: of every member (collide, CalcHydro), of the members
-  // whose force is stored (written before the collide) and of those whose
-  // force is virtual (written from the lists when a call returns)
-  WorkItem* d_latwork = nullptr;
-  size_t cap_latwork = 0;
-  int n_latwork = 0;
+  // lattice blocks of the members whose force is stored (written before the
+  // collide) and of those whose force is virtual (evaluated in the collide
+  // and CalcHydro, written from the lists when a call returns), in the order
+  // the autotuner chose
   WorkItem* d_storedwork = nullptr;
   size_t cap_storedwork = 0;
   int n_storedwork = 0;
@@ -1672,6 +1670,10 @@ struct lbswim_ensemble {
   size_t cap_list = 0;
   SpreadPoint* pts = nullptr;
   size_t cap_pts = 0;
+  // stored-force members' chain forked to the side stream (see ens_enqueue_step)
+  bool fork_stored = false;
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join = nullptr;
   // two captured steps (parities 0,1 from `graph_parity`)
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -1693,10 +1695,9 @@ int ens_build(lbswim_ensemble* e) {
   HIP_TRY(hipStreamSynchronize(stream()));
   ens_drop_graph(e);
   std::vector<LatDesc> lat;
-  // per member: its lattice blocks; blocks of stored- and virtual-force members
-  std::vector<std::vector<WorkItem>> latwork;
-  std::vector<size_t> weight;
-  std::vector<WorkItem> storedwork, vfwork;
+  // per member: its lattice blocks, in the stored- (0) or virtual-force (1) group
+  std::vector<std::vector<WorkItem>> latwork[2];
+  std::vector<size_t> weight[2];
   std::vector<SwimDesc> sw;
   std::vector<WorkItem> swwork, ptwork, trwork;
   std::vector<TracerDesc> tr;
@@ -1728,10 +1729,9 @@ int ens_build(lbswim_ensemble* e) {
     lat.push_back(d);
     std::vector<WorkItem> lw;
     for (uint32_t b = 0; b < lattice_blocks(L->geom); ++b) lw.push_back(WorkItem{mi, b});
-    std::vector<WorkItem>& group = d.force_path == FORCE_LISTS_VIRTUAL ? vfwork : storedwork;
-    group.insert(group.end(), lw.begin(), lw.end());
-    latwork.push_back(lw);
-    weight.push_back(L->n);
+    const int group = d.force_path == FORCE_LISTS_VIRTUAL ? 1 : 0;
+    latwork[group].push_back(lw);
+    weight[group].push_back(L->n);
     for (lbswim_swimmers* s : m.sw) {
       if (int rc = pstore_maybe_reorder(s->st, L->geom)) return rc;
       const uint32_t ai = uint32_t(sw.size());
@@ -1770,17 +1770,24 @@ int ens_build(lbswim_ensemble* e) {
     if (!m.lat) continue;
     m.cur_at_build = m.lat->cur;
   }
-  const std::vector<WorkItem> lw = order_work(latwork, e->order, weight);
+  const std::vector<WorkItem> storedwork = order_work(latwork[0], e->order, weight[0]);
+  const std::vector<WorkItem> vfwork = order_work(latwork[1], e->order, weight[1]);
   int rc;
-  if ((rc = upload(&e->d_lat, &e->cap_lat, lat)) || (rc = upload(&e->d_latwork, &e->cap_latwork, lw)) ||
-      (rc = upload(&e->d_storedwork, &e->cap_storedwork, storedwork)) ||
+  if ((rc = upload(&e->d_lat, &e->cap_lat, lat)) || (rc = upload(&e->d_storedwork, &e->cap_storedwork, storedwork)) ||
       (rc = upload(&e->d_vfwork, &e->cap_vfwork, vfwork)) || (rc = upload(&e->d_sw, &e->cap_sw, sw)) ||
       (rc = upload(&e->d_swwork, &e->cap_swwork, swwork)) || (rc = upload(&e->d_ptwork, &e->cap_ptwork, ptwork)) ||
       (rc = upload(&e->d_tr, &e->cap_tr, tr)) || (rc = upload(&e->d_trwork, &e->cap_trwork, trwork)))
     return rc;
-  e->n_latwork = int(lw.size());
   e->n_storedwork = int(storedwork.size());
   e->n_vfwork = int(vfwork.size());
+  bool stored_lists = false;
+  for (const LatDesc& d : lat) stored_lists = stored_lists || d.force_path == FORCE_LISTS_STORED;
+  e->fork_stored = !storedwork.empty() && !vfwork.empty() && !stored_lists;
+  if (e->fork_stored) {
+    if (int r = ensure_side_stream()) return r;
+    if (!e->ev_fork) HIP_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    if (!e->ev_join) HIP_TRY(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+  }
   e->n_swwork = int(swwork.size());
   e->n_ptwork = int(ptwork.size());
   e->n_trwork = int(trwork.size());
@@ -1816,14 +1823,37 @@ template <typename T>
 int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   hipStream_t s = stream();
   HIP_TRY(launch_batch_points(e->d_lat, e->d_sw, e->d_ptwork, e->n_ptwork, e->pts, s));
+  // Members with a stored force write it (from their lists or from all their
+  // points), then collide and CalcHydro; the others evaluate it from the
+  // lists in the collide and CalcHydro.  When no stored member needs lists,
+  // the stored members' chain runs on the side stream beside the list
+  // building and the virtual-force members' lattice launches (a fork and a
+  // join by events, which a graph capture records as dependencies).
+  const bool fork = e->fork_stored;
+  hipStream_t sb = fork ? g_rt.side : s;
+  if (fork) {
+    HIP_TRY(hipEventRecord(e->ev_fork, s));
+    HIP_TRY(hipStreamWaitEvent(sb, e->ev_fork, 0));
+  }
+  auto stored_chain = [&]() -> int {
+    HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->pts, e->off, e->sorted, sb));
+    HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, parity, nullptr, sb));
+    HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_storedwork, e->n_storedwork, parity, nullptr, sb));
+    return LBSWIM_OK;
+  };
+  if (fork)
+    if (int rc = stored_chain()) return rc;
   HIP_TRY(launch_batch_lists(e->d_lat, e->npts, e->nlists, e->pts, e->counts, e->off, e->cursor, e->list, e->sorted,
                              s));
+  if (!fork)
+    if (int rc = stored_chain()) return rc;
   const VForce vf{e->pts, e->off, e->sorted};
-  // members with a stored force: write it (from their lists or all their
-  // points); the others evaluate it from the lists in the collide and CalcHydro
-  HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->pts, e->off, e->sorted, s));
-  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, vf, s));
-  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_latwork, e->n_latwork, parity, vf, s));
+  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_vfwork, e->n_vfwork, parity, &vf, s));
+  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_vfwork, e->n_vfwork, parity, &vf, s));
+  if (fork) {
+    HIP_TRY(hipEventRecord(e->ev_join, sb));
+    HIP_TRY(hipStreamWaitEvent(s, e->ev_join, 0));
+  }
   HIP_TRY(launch_batch_swimmers_move<T>(e->d_lat, e->d_sw, e->d_swwork, e->n_swwork, s));
   HIP_TRY(launch_batch_tracers_move<T>(e->d_lat, e->d_tr, e->d_trwork, e->n_trwork, s));
   return LBSWIM_OK;
@@ -1909,8 +1939,9 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   if (!e) return;
   if (g_rt.ready) (void)hipStreamSynchronize(stream());
   ens_drop_graph(e);
+  if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+  if (e->ev_join) (void)hipEventDestroy(e->ev_join);
   dev_free(e->d_lat);
-  dev_free(e->d_latwork);
   dev_free(e->d_storedwork);
   dev_free(e->d_vfwork);
   dev_free(e->d_sw);

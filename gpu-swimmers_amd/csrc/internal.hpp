@@ -195,13 +195,14 @@ struct TracerDesc {
 uint32_t lattice_blocks(const Geom& g);  // workgroups of a one-lattice launch
 constexpr int PARTICLE_GROUPS = 16;      // swimmers per workgroup of the swimmer move kernel
 constexpr int TRACER_GROUPS = 64;        // tracers per workgroup of the tracer move kernel
-// Members with FORCE_LISTS_VIRTUAL evaluate the force of block b from lists
-// tile_base_m + 4 b ... of `vf`; the others read their force array.
+// vf: the work covers members with FORCE_LISTS_VIRTUAL, whose force of block
+// b comes from lists tile_base_m + 4 b ...; null: members that read their
+// force array.
 template <typename T>
-hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce& vf,
+hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
                                 hipStream_t s);
 template <typename T>
-hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce& vf,
+hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
                               hipStream_t s);
 // Point records of a batch: points of array a get ids point_base_a + 2*user +
 // end; work items are blocks of 128 swimmers (256 points) of an array.

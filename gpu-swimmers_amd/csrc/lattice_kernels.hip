@@ -966,58 +966,40 @@ __global__ __launch_bounds__(BLOCK) void k_calc_hydro_vf_f32(const float* __rest
 
 // At the start of a batched step with `parity` p the state of member m is in
 // g[cur_m ^ p]; the collision writes g[cur_m ^ p ^ 1], which CalcHydro reads.
-// Members with FORCE_LISTS_VIRTUAL evaluate the force of block `unit` from
-// lists tile_base_m + 4 unit ...; the others read their force array.  The
-// branch is uniform per workgroup.
-template <typename T>
+// VF: the launch covers members with FORCE_LISTS_VIRTUAL, whose force of
+// block `unit` comes from lists tile_base_m + 4 unit ...; otherwise members
+// whose force array holds it.  (One kernel for both would need more registers
+// than either.)
+template <typename T, bool VF>
 __global__ __launch_bounds__(BLOCK) void kb_collide(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
                                                     int parity, VForce vf) {
   const WorkItem w = work[blockIdx.x];
   const LatDesc& m = desc[w.member];
   const int c = m.cur ^ parity;
   const uint32_t vlist = m.tile_base + w.unit * LISTS_PER_BLOCK;
-  if constexpr (sizeof(T) == 8) {
-    const double* gi = static_cast<const double*>(m.g[c]);
-    double* go = static_cast<double*>(m.g[c ^ 1]);
-    const double* f = static_cast<const double*>(m.force);
-    if (m.force_path == FORCE_LISTS_VIRTUAL)
-      collide_f64_body<true>(gi, f, go, m.geom, m.tau_s, m.tau_b, m.omega_s, m.omega_b, w.unit, vf, vlist);
-    else
-      collide_f64_body<false>(gi, f, go, m.geom, m.tau_s, m.tau_b, m.omega_s, m.omega_b, w.unit, vf, vlist);
-  } else {
-    const float* gi = static_cast<const float*>(m.g[c]);
-    float* go = static_cast<float*>(m.g[c ^ 1]);
-    const float* f = static_cast<const float*>(m.force);
-    const float ts = float(m.tau_s), tb = float(m.tau_b), os = float(m.omega_s), ob = float(m.omega_b);
-    if (m.force_path == FORCE_LISTS_VIRTUAL)
-      collide_f32_body<true>(gi, f, go, m.geom, ts, tb, os, ob, w.unit, vf, vlist);
-    else
-      collide_f32_body<false>(gi, f, go, m.geom, ts, tb, os, ob, w.unit, vf, vlist);
-  }
+  if constexpr (sizeof(T) == 8)
+    collide_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
+                         static_cast<double*>(m.g[c ^ 1]), m.geom, m.tau_s, m.tau_b, m.omega_s, m.omega_b, w.unit, vf,
+                         vlist);
+  else
+    collide_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
+                         static_cast<float*>(m.g[c ^ 1]), m.geom, float(m.tau_s), float(m.tau_b), float(m.omega_s),
+                         float(m.omega_b), w.unit, vf, vlist);
 }
 
-template <typename T>
+template <typename T, bool VF>
 __global__ __launch_bounds__(BLOCK) void kb_hydro(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
                                                   int parity, VForce vf) {
   const WorkItem w = work[blockIdx.x];
   const LatDesc& m = desc[w.member];
   const int c = m.cur ^ parity ^ 1;
   const uint32_t vlist = m.tile_base + w.unit * LISTS_PER_BLOCK;
-  T* rho = static_cast<T*>(m.rho);
-  T* u = static_cast<T*>(m.u);
-  const T* gi = static_cast<const T*>(m.g[c]);
-  const T* f = static_cast<const T*>(m.force);
-  if constexpr (sizeof(T) == 8) {
-    if (m.force_path == FORCE_LISTS_VIRTUAL)
-      hydro_f64_body<true>(gi, f, rho, u, m.geom, w.unit, vf, vlist);
-    else
-      hydro_f64_body<false>(gi, f, rho, u, m.geom, w.unit, vf, vlist);
-  } else {
-    if (m.force_path == FORCE_LISTS_VIRTUAL)
-      hydro_f32_body<true>(gi, f, rho, u, m.geom, w.unit, vf, vlist);
-    else
-      hydro_f32_body<false>(gi, f, rho, u, m.geom, w.unit, vf, vlist);
-  }
+  if constexpr (sizeof(T) == 8)
+    hydro_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
+                       static_cast<double*>(m.rho), static_cast<double*>(m.u), m.geom, w.unit, vf, vlist);
+  else
+    hydro_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
+                       static_cast<float*>(m.rho), static_cast<float*>(m.u), m.geom, w.unit, vf, vlist);
 }
 
 // ------------------------------------------------------------------------
@@ -1236,23 +1218,29 @@ hipError_t launch_init_from_hydro<float>(const float* rho, const float* u, float
 }
 
 template <typename T>
-hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce& vf,
+hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
                                 hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_collide<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, vf);
+  if (vf)
+    hipLaunchKernelGGL((kb_collide<T, true>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf);
+  else
+    hipLaunchKernelGGL((kb_collide<T, false>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, VForce{});
   return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce& vf,
+hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
                               hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_hydro<T>, dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, vf);
+  if (vf)
+    hipLaunchKernelGGL((kb_hydro<T, true>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf);
+  else
+    hipLaunchKernelGGL((kb_hydro<T, false>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, VForce{});
   return hipGetLastError();
 }
-template hipError_t launch_batch_collide<double>(const LatDesc*, const WorkItem*, int, int, const VForce&, hipStream_t);
-template hipError_t launch_batch_collide<float>(const LatDesc*, const WorkItem*, int, int, const VForce&, hipStream_t);
-template hipError_t launch_batch_hydro<double>(const LatDesc*, const WorkItem*, int, int, const VForce&, hipStream_t);
-template hipError_t launch_batch_hydro<float>(const LatDesc*, const WorkItem*, int, int, const VForce&, hipStream_t);
+template hipError_t launch_batch_collide<double>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
+template hipError_t launch_batch_collide<float>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
+template hipError_t launch_batch_hydro<double>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
+template hipError_t launch_batch_hydro<float>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
 
 uint32_t lattice_blocks(const Geom& g) { return grid_for(g.n); }
 

@@ -959,14 +959,16 @@ __device__ __forceinline__ void spread_block_body(const Geom& g, const SpreadPoi
 
 // Force of the sites of lattice block `blk` from all points first ..
 // first+count-1 (range_force), written to the force array.
+// The whole workgroup calls it (range_force_staged holds barriers).
 template <typename T>
 __device__ __forceinline__ void range_block_body(const Geom& g, const SpreadPoint* __restrict__ pts, uint32_t first,
-                                                 uint32_t count, T* __restrict__ force, uint32_t blk) {
+                                                 uint32_t count, T* __restrict__ force, uint32_t blk,
+                                                 RangeStage* __restrict__ st) {
   const uint32_t s = blk * SITE_BLOCK + threadIdx.x;
   const bool ok = s < g.n;
   const SiteXYZ c = site_xyz(ok ? s : 0u, g);
   double acc[3];
-  range_force(pts, first, count, ok, c.x, c.y, c.z, g, acc);
+  range_force_staged(pts, first, count, ok, c.x, c.y, c.z, g, st, acc);
   if (!ok) return;
   const size_t n = g.n;
   force[s] = T(acc[0]);
@@ -977,7 +979,8 @@ __device__ __forceinline__ void range_block_body(const Geom& g, const SpreadPoin
 template <typename T>
 __global__ __launch_bounds__(SITE_BLOCK) void k_range_blocks(Geom g, const SpreadPoint* __restrict__ pts,
                                                             uint32_t npts, T* __restrict__ force) {
-  range_block_body(g, pts, 0u, npts, force, blockIdx.x);
+  __shared__ RangeStage st;
+  range_block_body(g, pts, 0u, npts, force, blockIdx.x, &st);
 }
 
 template <typename T>
@@ -1112,14 +1115,17 @@ __global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __
                                                               const SpreadPoint* __restrict__ pts,
                                                               const uint32_t* __restrict__ off,
                                                               const uint32_t* __restrict__ sorted) {
-  __shared__ WaveStage stage[LISTS_PER_BLOCK];
+  __shared__ union {
+    WaveStage list[LISTS_PER_BLOCK];
+    RangeStage range;
+  } stage;
   const WorkItem w = block_work[blockIdx.x];
   const LatDesc& m = ldesc[w.member];
   if (m.force_path == FORCE_RANGE_STORED)
-    range_block_body(m.geom, pts, m.point_first, m.point_count, static_cast<T*>(m.force), w.unit);
+    range_block_body(m.geom, pts, m.point_first, m.point_count, static_cast<T*>(m.force), w.unit, &stage.range);
   else
     spread_block_body(m.geom, pts, off, sorted, static_cast<T*>(m.force), w.unit,
-                      m.tile_base + w.unit * LISTS_PER_BLOCK, stage);
+                      m.tile_base + w.unit * LISTS_PER_BLOCK, stage.list);
 }
 
 // ---- device-math probes used by the parity tests ----

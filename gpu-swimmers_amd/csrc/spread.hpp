@@ -150,20 +150,69 @@ __device__ __forceinline__ void wave_force(const SpreadPoint* __restrict__ pts, 
   }
 }
 
-// Force at this lane's site from every point first .. first+count-1 in id
-// order: exactly the sum of the site's sorted list (a point not in the list
-// reaches no site of its run and adds nothing; a listed point is added once),
-// without the list.  The point index is uniform: records come through the
-// scalar cache.  For dense small systems, whose lists would each hold a large
-// share of all points.
-__device__ __forceinline__ void range_force(const SpreadPoint* __restrict__ pts, uint32_t first, uint32_t count,
-                                            bool ok, int X, int Y, int Z, const Geom& g, double (&acc)[3]) {
+// Force at this lane's site from every point first .. first+count-1, added
+// in id order: exactly the sum of the site's sorted list (a point not in the
+// list reaches no site of its run and adds nothing; a listed point is added
+// once), without the list -- for dense small systems, whose lists would each
+// hold a large share of all points.  For a whole workgroup (it holds
+// barriers): chunks of 256 consecutive point records are copied to `st` with
+// coalesced loads (one round trip per chunk); each lane then tests one record
+// of every 64 against the x range (and, when the wavefront's sites lie in one
+// x plane, the y range) of its wavefront's sites, and the wavefront adds, in
+// id order, only the points that pass -- one that fails reaches none of its
+// sites.
+constexpr int RANGE_STAGE = 256;
+struct RangeStage {
+  SpreadPoint pt[RANGE_STAGE];
+};
+
+__device__ __forceinline__ void range_force_staged(const SpreadPoint* __restrict__ pts, uint32_t first,
+                                                   uint32_t count, bool ok, int X, int Y, int Z, const Geom& g,
+                                                   RangeStage* __restrict__ st, double (&acc)[3]) {
   acc[0] = acc[1] = acc[2] = 0.0;
   first = __builtin_amdgcn_readfirstlane(first);
   count = __builtin_amdgcn_readfirstlane(count);
   const bool simple = g.nx >= 4 && g.ny >= 4 && g.nz >= 4;
-  for (uint32_t id = first; id < first + count; ++id)
-    if (ok) add_point(pts + id, X, Y, Z, g, simple, acc);
+  const int lane = int(threadIdx.x % WAVE_SITES);
+  // the wavefront's sites are consecutive, so its valid lanes are a prefix
+  // and their x (and, within a plane, y) do not decrease
+  const uint64_t okm = __ballot(ok);
+  int Xlo = 0, Xhi = -1, Ylo = 0, Yhi = -1;
+  if (okm) {
+    const int last = 63 - __builtin_clzll(okm);
+    Xlo = __builtin_amdgcn_readfirstlane(X);
+    Xhi = __builtin_amdgcn_readlane(X, last);
+    Ylo = __builtin_amdgcn_readfirstlane(Y);
+    Yhi = __builtin_amdgcn_readlane(Y, last);
+  }
+  const bool one_plane = Xlo == Xhi;
+  for (uint32_t base = first; base < first + count; base += RANGE_STAGE) {
+    const uint32_t m = min(uint32_t(RANGE_STAGE), first + count - base);
+    __syncthreads();  // the previous chunk is consumed
+    uint64_t* __restrict__ dst = reinterpret_cast<uint64_t*>(st->pt);
+    const uint64_t* __restrict__ src = reinterpret_cast<const uint64_t*>(pts + base);
+    for (uint32_t w = threadIdx.x; w < m * uint32_t(sizeof(SpreadPoint) / 8); w += blockDim.x) dst[w] = src[w];
+    __syncthreads();
+    for (uint32_t sub = 0; sub < m; sub += WAVE_SITES) {
+      bool cand = false;
+      if (sub + uint32_t(lane) < m && okm) {
+        const SpreadPoint& p = st->pt[sub + lane];
+        bool hx = false, hy = !one_plane;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          hx = hx || (p.wc[k] >= Xlo && p.wc[k] <= Xhi);
+          if (one_plane) hy = hy || (p.wc[4 + k] >= Ylo && p.wc[4 + k] <= Yhi);
+        }
+        cand = hx && hy;
+      }
+      uint64_t mask = __ballot(cand);
+      while (mask) {
+        const int k = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        if (ok) add_point(&st->pt[sub + k], X, Y, Z, g, simple, acc);
+      }
+    }
+  }
 }
 
 // wave_force for a lane whose list differs from the other lanes' (`wl` per

@@ -183,7 +183,8 @@ struct lbswim_lattice {
   uint32_t* sp_off = nullptr;     // sp_lists + 1 offsets + scan and sort scratch
   uint32_t* sp_cursor = nullptr;  // sp_lists
   size_t sp_pts_cap = 0;
-  void* sp_pts = nullptr;  // SpreadPoint records for sp_pts_cap points
+  void* sp_pts = nullptr;       // SpreadPoint records for sp_pts_cap points
+  uint32_t* sp_slot = nullptr;  // point ids in storage order (sp_pts_cap)
   size_t sp_list_cap = 0;
   uint32_t* sp_list = nullptr;
   uint32_t* sp_sorted = nullptr;
@@ -444,6 +445,7 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->sp_rec);
   dev_free(lat->sp_wz);
   dev_free(lat->sp_pts);
+  dev_free(lat->sp_slot);
   dev_free(lat->sp_list);
   dev_free(lat->sp_sorted);
   for (double* h : lat->h) std::free(h);
@@ -1189,7 +1191,9 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
   }
   size_t cap_bytes = lat->sp_pts_cap * sizeof(SpreadPoint);
   if (lat->sp_pts_cap < npts) {
+    size_t slot_cap = lat->sp_pts_cap;
     if (int rc = grow(&lat->sp_pts, &cap_bytes, npts * sizeof(SpreadPoint), 1)) return rc;
+    if (int rc = grow(reinterpret_cast<void**>(&lat->sp_slot), &slot_cap, npts, 4)) return rc;
     lat->sp_pts_cap = npts;
   }
   SpreadPoint* pts = static_cast<SpreadPoint*>(lat->sp_pts);
@@ -1202,7 +1206,7 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     for (int i = 0; i < nsw && e == hipSuccess; ++i) {
       PStore& st = sw[i]->st;
       e = launch_points(st.num, offset, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_N),
-                        st.perm, lat->geom, pts, stream());
+                        st.perm, lat->geom, pts, lat->sp_slot, stream());
       offset += st.num;
     }
     if (e == hipSuccess)
@@ -1229,7 +1233,7 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
   for (int i = 0; i < nsw; ++i) {
     PStore& st = sw[i]->st;
     HIP_TRY(launch_points(st.num, offset, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R),
-                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, pts, stream()));
+                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, pts, lat->sp_slot, stream()));
     offset += st.num;
   }
   // the fused step's z-halo sites read the force at the ends of the runs, its
@@ -1244,7 +1248,8 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     if (int rc = grow(reinterpret_cast<void**>(&lat->sp_wz), &c2, lcap, sizeof(int))) return rc;
     lat->sp_rec_cap = lcap;
   }
-  hipError_t e = launch_spread_lists(int(npts), lat->geom, pts, lat->sp_counts, lat->sp_off, lat->sp_cursor,
+  hipError_t e = launch_spread_lists(int(npts), lat->geom, pts, lat->sp_slot, lat->sp_counts, lat->sp_off,
+                                     lat->sp_cursor,
                                      lat->sp_list, lat->sp_sorted, stream());
   if (e == hipSuccess && edges)
     e = launch_edge_force(lat->geom, pts, lat->sp_off, lat->sp_sorted, lat->sp_edge, lat->sp_rec, lat->sp_wz, stream());
@@ -1344,6 +1349,7 @@ struct FixedSpread {
   uint32_t* off = nullptr;
   uint32_t* cursor = nullptr;
   SpreadPoint* pts = nullptr;
+  uint32_t* slot = nullptr;
   uint32_t* list = nullptr;
   uint32_t* sorted = nullptr;
   size_t bytes = 0;
@@ -1361,6 +1367,7 @@ FixedSpread carve_spread(unsigned char* base, uint32_t nl, size_t npts) {
   w.off = reinterpret_cast<uint32_t*>(take(4 * spread_off_entries(nl)));
   w.cursor = reinterpret_cast<uint32_t*>(take(4 * size_t(nl)));
   w.pts = reinterpret_cast<SpreadPoint*>(take(sizeof(SpreadPoint) * npts));
+  w.slot = reinterpret_cast<uint32_t*>(take(4 * npts));
   w.list = reinterpret_cast<uint32_t*>(take(4 * spread_list_capacity(npts)));
   w.sorted = reinterpret_cast<uint32_t*>(take(4 * spread_list_capacity(npts)));
   w.bytes = at;
@@ -1404,11 +1411,12 @@ int lbswim_system_step_noalloc(lbswim_lattice* lat, lbswim_swimmers* const* sw, 
       for (int i = 0; i < nsw && e == hipSuccess; ++i) {
         PStore& st = sw[i]->st;
         e = launch_points(st.num, base_id, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R),
-                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, w.pts, stream());
+                          fld(st, LBSWIM_PART_N), st.perm, lat->geom, w.pts, w.slot, stream());
         base_id += st.num;
       }
       if (e == hipSuccess)
-        e = launch_spread_lists(int(npts), lat->geom, w.pts, w.counts, w.off, w.cursor, w.list, w.sorted, stream());
+        e = launch_spread_lists(int(npts), lat->geom, w.pts, w.slot, w.counts, w.off, w.cursor, w.list, w.sorted,
+                                stream());
       if (e == hipSuccess)
         e = lat->storage == LBSWIM_STORE_F64
                 ? launch_spread_force<double>(lat->geom, w.pts, w.off, w.sorted, static_cast<double*>(lat->d_force),
@@ -1670,25 +1678,68 @@ struct lbswim_ensemble {
   size_t cap_list = 0;
   SpreadPoint* pts = nullptr;
   size_t cap_pts = 0;
+  uint32_t* slot_ids = nullptr;  // point ids in storage order
+  size_t cap_slot = 0;
+  bool dense_lists = false;      // aggregate the list building's atomics
   // stored-force members' chain forked to the side stream (see ens_enqueue_step)
   bool fork_stored = false;
+  bool stored_lists = false;  // some stored-force member builds lists
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join = nullptr;
-  // two captured steps (parities 0,1 from `graph_parity`)
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
-  bool graph_valid = false;
-  int graph_parity = 0;
+  // two captured steps, one graph for each starting parity (the parity
+  // changes when a call steps an odd number of times)
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  // per-phase event timing (steps launched eagerly while on)
+  unsigned timing = 0;
+  std::vector<TimeRec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t sites_virtual = 0, sites_stored = 0;
 };
 
 namespace {
 
+// Records an event pair around one batched phase on stream `s` when timing is on.
+struct EnsTimer {
+  lbswim_ensemble* e;
+  int kernel;
+  hipStream_t s;
+  hipEvent_t start = nullptr;
+  static hipEvent_t take(lbswim_ensemble* e) {
+    if (!e->pool.empty()) {
+      hipEvent_t ev = e->pool.back();
+      e->pool.pop_back();
+      return ev;
+    }
+    hipEvent_t ev = nullptr;
+    return hipEventCreate(&ev) == hipSuccess ? ev : nullptr;
+  }
+  EnsTimer(lbswim_ensemble* en, int k, hipStream_t st) : e(en), kernel(k), s(st) {
+    if ((e->timing >> k) & 1u) {
+      start = take(e);
+      if (start) (void)hipEventRecord(start, s);
+    }
+  }
+  void done() {
+    if (!start) return;
+    hipEvent_t stop = take(e);
+    if (!stop) return;
+    (void)hipEventRecord(stop, s);
+    e->recs.push_back(TimeRec{kernel, start, stop});
+    start = nullptr;
+  }
+};
+
+// Destroys the captured graphs once every launch of them has finished.
 void ens_drop_graph(lbswim_ensemble* e) {
-  if (e->exec) (void)hipGraphExecDestroy(e->exec);
-  if (e->graph) (void)hipGraphDestroy(e->graph);
-  e->exec = nullptr;
-  e->graph = nullptr;
-  e->graph_valid = false;
+  if (!e->exec[0] && !e->exec[1] && !e->graph[0] && !e->graph[1]) return;
+  (void)hipStreamSynchronize(stream());
+  for (int p = 0; p < 2; ++p) {
+    if (e->exec[p]) (void)hipGraphExecDestroy(e->exec[p]);
+    if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+    e->exec[p] = nullptr;
+    e->graph[p] = nullptr;
+  }
 }
 
 int ens_build(lbswim_ensemble* e) {
@@ -1780,9 +1831,17 @@ int ens_build(lbswim_ensemble* e) {
     return rc;
   e->n_storedwork = int(storedwork.size());
   e->n_vfwork = int(vfwork.size());
-  bool stored_lists = false;
-  for (const LatDesc& d : lat) stored_lists = stored_lists || d.force_path == FORCE_LISTS_STORED;
-  e->fork_stored = !storedwork.empty() && !vfwork.empty() && !stored_lists;
+  // aggregated atomics when any member with lists has dense ones
+  e->dense_lists = false;
+  for (size_t i = 0; i < lat.size(); ++i) {
+    if (lat[i].force_path == FORCE_RANGE_STORED) continue;
+    e->dense_lists = e->dense_lists || dense_lists(lat[i].point_count, spread_lists(lat[i].geom));
+  }
+  e->sites_virtual = e->sites_stored = 0;
+  for (const LatDesc& d : lat) (d.force_path == FORCE_LISTS_VIRTUAL ? e->sites_virtual : e->sites_stored) += d.geom.n;
+  e->stored_lists = false;
+  for (const LatDesc& d : lat) e->stored_lists = e->stored_lists || d.force_path == FORCE_LISTS_STORED;
+  e->fork_stored = !storedwork.empty() && !vfwork.empty();
   if (e->fork_stored) {
     if (int r = ensure_side_stream()) return r;
     if (!e->ev_fork) HIP_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
@@ -1811,6 +1870,7 @@ int ens_build(lbswim_ensemble* e) {
   if ((rc = grow_to(&e->counts, &e->cap_counts, nt, 4)) ||
       (rc = grow_to(&e->off, &e->cap_off, spread_off_entries(uint32_t(nt)), 4)) ||
       (rc = grow_to(&e->cursor, &e->cap_cursor, nt, 4)) || (rc = grow_to(&e->pts, &e->cap_pts, size_t(e->npts), 176)) ||
+      (rc = grow_to(&e->slot_ids, &e->cap_slot, size_t(e->npts), 4)) ||
       (rc = grow_to(&e->sorted, &cap_sorted, lcap, 4)) || (rc = grow_to(&e->list, &e->cap_list, lcap, 4)))
     return rc;
   HIP_TRY(hipStreamSynchronize(stream()));
@@ -1822,40 +1882,66 @@ int ens_build(lbswim_ensemble* e) {
 template <typename T>
 int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   hipStream_t s = stream();
-  HIP_TRY(launch_batch_points(e->d_lat, e->d_sw, e->d_ptwork, e->n_ptwork, e->pts, s));
+  EnsTimer t_lists(e, LBSWIM_K_ADDF, s);
+  HIP_TRY(launch_batch_points(e->d_lat, e->d_sw, e->d_ptwork, e->n_ptwork, e->pts, e->slot_ids, s));
   // Members with a stored force write it (from their lists or from all their
   // points), then collide and CalcHydro; the others evaluate it from the
-  // lists in the collide and CalcHydro.  When no stored member needs lists,
-  // the stored members' chain runs on the side stream beside the list
-  // building and the virtual-force members' lattice launches (a fork and a
-  // join by events, which a graph capture records as dependencies).
+  // lists in the collide and CalcHydro.  With both kinds present the stored
+  // members' chain runs on the side stream beside the virtual-force members'
+  // lattice launches, and, when no stored member needs lists, beside the
+  // list building too (a fork and a join by events, which a graph capture
+  // records as dependencies).
   const bool fork = e->fork_stored;
+  const bool early = fork && !e->stored_lists;
   hipStream_t sb = fork ? g_rt.side : s;
-  if (fork) {
+  auto fork_now = [&]() -> int {
     HIP_TRY(hipEventRecord(e->ev_fork, s));
     HIP_TRY(hipStreamWaitEvent(sb, e->ev_fork, 0));
-  }
+    return LBSWIM_OK;
+  };
+  if (early)
+    if (int rc = fork_now()) return rc;
   auto stored_chain = [&]() -> int {
+    EnsTimer t(e, LBSWIM_K_STORED, sb);
     HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->pts, e->off, e->sorted, sb));
     HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, parity, nullptr, sb));
     HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_storedwork, e->n_storedwork, parity, nullptr, sb));
+    t.done();
     return LBSWIM_OK;
   };
-  if (fork)
+  if (early)
     if (int rc = stored_chain()) return rc;
-  HIP_TRY(launch_batch_lists(e->d_lat, e->npts, e->nlists, e->pts, e->counts, e->off, e->cursor, e->list, e->sorted,
-                             s));
-  if (!fork)
+  HIP_TRY(launch_batch_lists(e->d_lat, e->npts, e->nlists, e->pts, e->slot_ids, e->dense_lists, e->counts, e->off,
+                             e->cursor, e->list, e->sorted, s));
+  t_lists.done();
+  if (!early) {
+    if (fork)
+      if (int rc = fork_now()) return rc;
     if (int rc = stored_chain()) return rc;
+  }
   const VForce vf{e->pts, e->off, e->sorted};
-  HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_vfwork, e->n_vfwork, parity, &vf, s));
-  HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_vfwork, e->n_vfwork, parity, &vf, s));
+  {
+    EnsTimer t(e, LBSWIM_K_COLLIDE, s);
+    HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_vfwork, e->n_vfwork, parity, &vf, s));
+    t.done();
+  }
+  {
+    EnsTimer t(e, LBSWIM_K_HYDRO, s);
+    HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_vfwork, e->n_vfwork, parity, &vf, s));
+    t.done();
+  }
   if (fork) {
     HIP_TRY(hipEventRecord(e->ev_join, sb));
     HIP_TRY(hipStreamWaitEvent(s, e->ev_join, 0));
   }
-  HIP_TRY(launch_batch_swimmers_move<T>(e->d_lat, e->d_sw, e->d_swwork, e->n_swwork, s));
+  {
+    EnsTimer t(e, LBSWIM_K_SMOVE, s);
+    HIP_TRY(launch_batch_swimmers_move<T>(e->d_lat, e->d_sw, e->d_swwork, e->n_swwork, s));
+    t.done();
+  }
+  EnsTimer t(e, LBSWIM_K_TMOVE, s);
   HIP_TRY(launch_batch_tracers_move<T>(e->d_lat, e->d_tr, e->d_trwork, e->n_trwork, s));
+  t.done();
   return LBSWIM_OK;
 }
 
@@ -1875,11 +1961,11 @@ int ens_materialize_force(lbswim_ensemble* e) {
   return LBSWIM_OK;
 }
 
-int ens_capture(lbswim_ensemble* e) {
-  ens_drop_graph(e);
+// Captures two steps starting with parity p into graph slot p.
+int ens_capture(lbswim_ensemble* e, int p) {
   HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-  int rc = ens_enqueue(e, e->parity);
-  if (!rc) rc = ens_enqueue(e, e->parity ^ 1);
+  int rc = ens_enqueue(e, p);
+  if (!rc) rc = ens_enqueue(e, p ^ 1);
   hipGraph_t g = nullptr;
   hipError_t he = hipStreamEndCapture(stream(), &g);
   if (rc) {
@@ -1887,14 +1973,14 @@ int ens_capture(lbswim_ensemble* e) {
     return rc;
   }
   if (he != hipSuccess) return fail(LBSWIM_EHIP, "graph capture failed: %s", hipGetErrorString(he));
-  e->graph = g;
-  he = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+  hipGraphExec_t x = nullptr;
+  he = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   if (he != hipSuccess) {
-    ens_drop_graph(e);
+    (void)hipGraphDestroy(g);
     return fail(LBSWIM_EHIP, "graph instantiation failed: %s", hipGetErrorString(he));
   }
-  e->graph_valid = true;
-  e->graph_parity = e->parity;
+  e->graph[p] = g;
+  e->exec[p] = x;
   return LBSWIM_OK;
 }
 
@@ -1939,6 +2025,11 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   if (!e) return;
   if (g_rt.ready) (void)hipStreamSynchronize(stream());
   ens_drop_graph(e);
+  for (auto& r : e->recs) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  for (hipEvent_t ev : e->pool) (void)hipEventDestroy(ev);
   if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
   if (e->ev_join) (void)hipEventDestroy(e->ev_join);
   dev_free(e->d_lat);
@@ -1955,6 +2046,7 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   dev_free(e->list);
   dev_free(e->sorted);
   dev_free(e->pts);
+  dev_free(e->slot_ids);
   delete e;
 }
 
@@ -2035,10 +2127,11 @@ int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps) {
   if (e->dirty)
     if (int rc = ens_build(e)) return rc;
   int done = 0;
-  if (e->use_graph && nsteps >= 2) {
-    if (!e->graph_valid || e->graph_parity != e->parity)
-      if (int rc = ens_capture(e)) return rc;
-    for (; done + 2 <= nsteps; done += 2) HIP_TRY(hipGraphLaunch(e->exec, stream()));
+  if (e->use_graph && !e->timing && nsteps >= 2) {
+    const int p = e->parity;
+    if (!e->exec[p])
+      if (int rc = ens_capture(e, p)) return rc;
+    for (; done + 2 <= nsteps; done += 2) HIP_TRY(hipGraphLaunch(e->exec[p], stream()));
     ens_advance(e, done);
   }
   for (; done < nsteps; ++done) {
@@ -2046,6 +2139,47 @@ int lbswim_ensemble_step(lbswim_ensemble* e, int nsteps) {
     ens_advance(e, 1);
   }
   return ens_materialize_force(e);
+}
+
+int lbswim_ensemble_set_timing(lbswim_ensemble* e, unsigned mask) {
+  CHECK_ARG(e, "ensemble is NULL");
+  if (mask) {
+    HIP_TRY(hipStreamSynchronize(stream()));
+    for (auto& r : e->recs) {
+      e->pool.push_back(r.start);
+      e->pool.push_back(r.stop);
+    }
+    e->recs.clear();
+  }
+  e->timing = mask & ((1u << LBSWIM_K_COUNT) - 1u);
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_kernel_time(lbswim_ensemble* e, int kernel, double* total_ms, long* count) {
+  CHECK_ARG(e && total_ms && count, "NULL argument");
+  CHECK_ARG(kernel >= 0 && kernel < LBSWIM_K_COUNT, "unknown kernel id %d", kernel);
+  HIP_TRY(hipStreamSynchronize(stream()));
+  double tot = 0.0;
+  long cnt = 0;
+  for (const auto& r : e->recs) {
+    if (r.kernel != kernel) continue;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, r.start, r.stop));
+    tot += ms;
+    ++cnt;
+  }
+  *total_ms = tot;
+  *count = cnt;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_sites(lbswim_ensemble* e, size_t* virtual_sites, size_t* stored_sites) {
+  CHECK_ARG(e && virtual_sites && stored_sites, "NULL argument");
+  if (e->dirty)
+    if (int rc = ens_build(e)) return rc;
+  *virtual_sites = e->sites_virtual;
+  *stored_sites = e->sites_stored;
+  return LBSWIM_OK;
 }
 
 int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice) {

@@ -113,16 +113,21 @@ uint32_t spread_lists(const Geom& g);
 size_t spread_list_capacity(size_t npts);
 // Entries of the `off` array of launch_spread_lists for nlists lists.
 size_t spread_off_entries(uint32_t nlists);
-// Point records of the swimmers of one array, ids from 2*base.
+// Point records of the swimmers of one array, ids from 2*base;
+// slot_ids[2 base + t] receives the id of the t-th point in storage order.
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
-                         const Geom& g, SpreadPoint* pts, hipStream_t s);
+                         const Geom& g, SpreadPoint* pts, uint32_t* slot_ids, hipStream_t s);
+// Lists averaging more than 8 entries: the list building aggregates its
+// atomics per wavefront.
+inline bool dense_lists(size_t npts, uint32_t nlists) { return npts * 17 > size_t(8) * nlists; }
 // counts[nl] (zero on entry: zeroed at allocation, left zero by every build),
 // off[spread_off_entries(nl)] (offsets, then scan and sort scratch),
 // cursor[nl], list/sorted[spread_list_capacity]; afterwards
 // sorted[off[l]..off[l+1]) holds the ids of the points reaching sites
 // 64 l .. 64 l + 63 in increasing order.
-hipError_t launch_spread_lists(int npts, const Geom& g, const SpreadPoint* pts, uint32_t* counts, uint32_t* off,
-                               uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s);
+hipError_t launch_spread_lists(int npts, const Geom& g, const SpreadPoint* pts, const uint32_t* slot_ids,
+                               uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted,
+                               hipStream_t s);
 // Writes the force of every site from all npts point records in id order,
 // without lists: the sum the sorted lists give (dense small systems).
 template <typename T>
@@ -207,10 +212,12 @@ hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwo
 // Point records of a batch: points of array a get ids point_base_a + 2*user +
 // end; work items are blocks of 128 swimmers (256 points) of an array.
 hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
-                               SpreadPoint* pts, hipStream_t s);
-// Sorted lists of all nlists lists of the batch (see launch_spread_lists).
-hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const SpreadPoint* pts, uint32_t* counts,
-                              uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s);
+                               SpreadPoint* pts, uint32_t* slot_ids, hipStream_t s);
+// Sorted lists of all nlists lists of the batch (see launch_spread_lists);
+// dense: aggregate the atomics per wavefront.
+hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const SpreadPoint* pts,
+                              const uint32_t* slot_ids, bool dense, uint32_t* counts, uint32_t* off, uint32_t* cursor,
+                              uint32_t* list, uint32_t* sorted, hipStream_t s);
 // Writes the force arrays of the lattice blocks in block_work: from the lists
 // (FORCE_LISTS_STORED) or from all the member's points (FORCE_RANGE_STORED).
 template <typename T>

@@ -689,10 +689,13 @@ __global__ __launch_bounds__(TBLOCK) void kb_tracers_move(const LatDesc* __restr
 //      of their own sites from the same lists (wave_force).
 
 // One thread per point: thread t is end t & 1 of the swimmer in slot t >> 1.
+// slot_ids[point_base + t] = the id of the point thread t forms: points in
+// storage (spatial) order, which the list building walks.
 __device__ __forceinline__ void points_body(int ns, uint32_t point_base, double P, double l,
                                             const double* __restrict__ r, const double* __restrict__ nv,
                                             const int* __restrict__ perm, const Geom& g, int member,
-                                            SpreadPoint* __restrict__ pts, uint32_t blk) {
+                                            SpreadPoint* __restrict__ pts, uint32_t* __restrict__ slot_ids,
+                                            uint32_t blk) {
   const int t = int(blk * PBLOCK + threadIdx.x);
   if (t >= 2 * ns) return;
   const int slot = t >> 1, end = t & 1;
@@ -714,42 +717,92 @@ __device__ __forceinline__ void points_body(int ns, uint32_t point_base, double 
   }
   sp.pad[0] = member;
   sp.pad[1] = 0;
-  pts[size_t(point_base) + 2 * size_t(perm[slot]) + end] = sp;
+  const uint32_t id = point_base + 2 * uint32_t(perm[slot]) + uint32_t(end);
+  pts[id] = sp;
+  slot_ids[point_base + uint32_t(t)] = id;
 }
 
 __global__ __launch_bounds__(PBLOCK) void k_points(int ns, int base, double P, double l, const double* __restrict__ r,
                                                   const double* __restrict__ nv, const int* __restrict__ perm,
-                                                  Geom g, SpreadPoint* __restrict__ pts) {
-  points_body(ns, 2 * uint32_t(base), P, l, r, nv, perm, g, 0, pts, blockIdx.x);
+                                                  Geom g, SpreadPoint* __restrict__ pts,
+                                                  uint32_t* __restrict__ slot_ids) {
+  points_body(ns, 2 * uint32_t(base), P, l, r, nv, perm, g, 0, pts, slot_ids, blockIdx.x);
 }
 
 // Batched: work items are blocks of 128 swimmers (256 points) of an array.
 __global__ __launch_bounds__(PBLOCK) void kb_points(const LatDesc* __restrict__ ldesc,
                                                    const SwimDesc* __restrict__ sdesc,
-                                                   const WorkItem* __restrict__ work, SpreadPoint* __restrict__ pts) {
+                                                   const WorkItem* __restrict__ work, SpreadPoint* __restrict__ pts,
+                                                   uint32_t* __restrict__ slot_ids) {
   const WorkItem w = work[blockIdx.x];
   const SwimDesc& a = sdesc[w.member];
   const LatDesc& m = ldesc[a.member];
-  points_body(a.num, a.point_base, a.p.P, a.p.l, a.r, a.n, a.perm, m.geom, a.member, pts, w.unit);
+  points_body(a.num, a.point_base, a.p.P, a.p.l, a.r, a.n, a.perm, m.geom, a.member, pts, slot_ids, w.unit);
 }
 
-// One thread per (point, stencil row): counts the lists the row reaches.
-// `Batched`: geometry and first list from the point's member.
-template <bool Batched>
+// Up to 4 lists (each once) that stencil row `row` of point p reaches.
+struct RowLists {
+  uint32_t l[4];
+  int count;
+};
+__device__ __forceinline__ RowLists row_lists(const SpreadPoint& p, int row, const Geom& g, uint32_t base) {
+  RowLists r{{0u, 0u, 0u, 0u}, 0};
+  for_each_row_list(p, row, g, [&](uint32_t b) { r.l[r.count++] = base + b; });
+  return r;
+}
+
+// Dense systems: the lanes of a wavefront (points in storage order, so
+// spatial neighbours) often hit the same list; one lane per distinct list
+// adds the wavefront's total with one atomic and hands each lane its slot.
+// Returns the value the atomic returned plus the lane's rank among the lanes
+// with the same list.  Every lane of the wavefront calls it; lanes with
+// `key` ~0u take part without a list.
+__device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t* __restrict__ ctr, uint32_t key) {
+  const int lane = int(threadIdx.x % WAVE_SITES);
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  uint32_t mine = 0;
+  uint64_t pending = __ballot(key != 0xFFFFFFFFu);
+  while (pending) {
+    const int leader = __builtin_ctzll(pending);
+    const uint32_t lk = uint32_t(__builtin_amdgcn_readlane(int(key), leader));
+    const uint64_t same = __ballot(key == lk);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[lk], uint32_t(__builtin_popcountll(same)));
+    base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
+    if (key == lk) mine = base + uint32_t(__builtin_popcountll(same & below));
+    pending &= ~same;
+  }
+  return mine;
+}
+
+// One thread per (point, stencil row), points in storage order (slot_ids):
+// counts the lists the row reaches.  `Batched`: geometry and first list from
+// the point's member.  AGG: wavefront-aggregated atomics (dense systems).
+template <bool Batched, bool AGG>
 __global__ __launch_bounds__(PBLOCK) void k_count_rows(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
                                                       const SpreadPoint* __restrict__ pts,
+                                                      const uint32_t* __restrict__ slot_ids,
                                                       uint32_t* __restrict__ counts) {
   const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
-  if (t >= 16 * npts) return;
-  const SpreadPoint& p = pts[t >> 4];
-  uint32_t base = 0;
-  if constexpr (Batched) {
-    const LatDesc& m = ldesc[p.pad[0]];
-    if (m.force_path == FORCE_RANGE_STORED) return;  // the member has no lists
-    g = m.geom;
-    base = m.tile_base;
+  RowLists rl{{0u, 0u, 0u, 0u}, 0};
+  if (t < 16 * npts) {
+    const SpreadPoint& p = pts[slot_ids[t >> 4]];
+    uint32_t base = 0;
+    bool has = true;
+    if constexpr (Batched) {
+      const LatDesc& m = ldesc[p.pad[0]];
+      has = m.force_path != FORCE_RANGE_STORED;  // a member without lists
+      g = m.geom;
+      base = m.tile_base;
+    }
+    if (has) rl = row_lists(p, int(t & 15), g, base);
   }
-  for_each_row_list(p, int(t & 15), g, [&](uint32_t b) { atomicAdd(&counts[base + b], 1u); });
+  if constexpr (AGG) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) (void)wave_aggregated_add(counts, k < rl.count ? rl.l[k] : 0xFFFFFFFFu);
+  } else {
+    for (int k = 0; k < rl.count; ++k) atomicAdd(&counts[rl.l[k]], 1u);
+  }
 }
 
 // Exclusive scan of counts[0..nt) into off[0..nt] in two coalesced passes
@@ -811,29 +864,41 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(int nt, uint32_t* __r
   }
 }
 
-// One thread per (point, stencil row): appends the point id to the lists the
-// row reaches.
-template <bool Batched>
+// One thread per (point, stencil row), points in storage order: appends the
+// point id to the lists the row reaches (in any order; k_sort_lists orders
+// them).  AGG as for k_count_rows.
+template <bool Batched, bool AGG>
 __global__ __launch_bounds__(PBLOCK) void k_fill_rows(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
                                                      const SpreadPoint* __restrict__ pts,
+                                                     const uint32_t* __restrict__ slot_ids,
                                                      const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
                                                      uint32_t* __restrict__ list) {
   const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
-  if (t >= 16 * npts) return;
-  const uint32_t id = t >> 4;
-  const SpreadPoint& p = pts[id];
-  uint32_t base = 0;
-  if constexpr (Batched) {
-    const LatDesc& m = ldesc[p.pad[0]];
-    if (m.force_path == FORCE_RANGE_STORED) return;
-    g = m.geom;
-    base = m.tile_base;
+  RowLists rl{{0u, 0u, 0u, 0u}, 0};
+  uint32_t id = 0;
+  if (t < 16 * npts) {
+    id = slot_ids[t >> 4];
+    const SpreadPoint& p = pts[id];
+    uint32_t base = 0;
+    bool has = true;
+    if constexpr (Batched) {
+      const LatDesc& m = ldesc[p.pad[0]];
+      has = m.force_path != FORCE_RANGE_STORED;
+      g = m.geom;
+      base = m.tile_base;
+    }
+    if (has) rl = row_lists(p, int(t & 15), g, base);
   }
-  for_each_row_list(p, int(t & 15), g, [&](uint32_t b) {
-    const uint32_t tile = base + b;
-    const uint32_t pos = atomicAdd(&cursor[tile], 1u);
-    list[off[tile] + pos] = id;
-  });
+  if constexpr (AGG) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t key = k < rl.count ? rl.l[k] : 0xFFFFFFFFu;
+      const uint32_t pos = wave_aggregated_add(cursor, key);
+      if (key != 0xFFFFFFFFu) list[off[key] + pos] = id;
+    }
+  } else {
+    for (int k = 0; k < rl.count; ++k) list[off[rl.l[k]] + atomicAdd(&cursor[rl.l[k]], 1u)] = id;
+  }
 }
 
 // Sort of every list by point id, in two passes.  k_sort_lists: one lane per
@@ -907,7 +972,39 @@ __device__ __forceinline__ void rank_list(const uint32_t* __restrict__ src, uint
   }
 }
 
-// Fixed grid; wavefront w takes queued lists w, w + waves, ...
+// Sorts v[0..cnt) in place by one wavefront with a bitonic network over the
+// next power of two (padded with the largest id); the lanes take disjoint
+// pairs of each stage, and a wavefront's LDS accesses complete in order.
+__device__ __forceinline__ void bitonic_lds(uint32_t* __restrict__ v, uint32_t cnt, uint32_t lane) {
+  uint32_t P = 1;
+  while (P < cnt) P <<= 1;
+  for (uint32_t e = cnt + lane; e < P; e += WAVE_SITES) v[e] = 0xFFFFFFFFu;
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (uint32_t i = lane; i < P; i += WAVE_SITES) {
+        const uint32_t x = i ^ j;
+        if (x > i) {
+          const uint32_t a = v[i], b = v[x];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            v[i] = b;
+            v[x] = a;
+          }
+        }
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Fixed grid; wavefront w takes queued lists w, w + waves, ...  Lists that
+// fit in the wavefront's LDS slice are sorted there (bitonic, equal ids are
+// interchangeable), longer ones ranked from memory.
 __global__ __launch_bounds__(LONG_BLOCK) void k_sort_long(const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ list,
                                                          uint32_t* __restrict__ sorted,
@@ -923,11 +1020,9 @@ __global__ __launch_bounds__(LONG_BLOCK) void k_sort_long(const uint32_t* __rest
     const uint32_t o = off[wl], cnt = off[wl + 1] - o;
     if (cnt <= LONG_LDS) {
       for (uint32_t e = lane; e < cnt; e += WAVE_SITES) mine[e] = list[o + e];
-      // the wavefront's LDS accesses complete in order
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      rank_list(mine, cnt, lane, sorted + o);
+      bitonic_lds(mine, cnt, lane);
+      for (uint32_t e = lane; e < cnt; e += WAVE_SITES) sorted[o + e] = mine[e];
+      // the next list reuses the slice
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1216,10 +1311,10 @@ size_t spread_list_capacity(size_t npts) {
 uint32_t spread_lists(const Geom& g) { return (g.n + SITE_BLOCK - 1) / SITE_BLOCK * LISTS_PER_BLOCK; }
 
 hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r, const double* n, const int* perm,
-                         const Geom& g, SpreadPoint* pts, hipStream_t s) {
+                         const Geom& g, SpreadPoint* pts, uint32_t* slot_ids, hipStream_t s) {
   if (ns <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_points, dim3(blocks_for(2 * size_t(ns))), dim3(PBLOCK), 0, s, ns, base, p.P, p.l, r, n, perm, g,
-                     pts);
+                     pts, slot_ids);
   return hipGetLastError();
 }
 
@@ -1228,19 +1323,30 @@ hipError_t launch_points(int ns, int base, const SwimParams& p, const double* r,
 // allocated and reset by k_scan_apply at the end of every build.
 template <bool Batched>
 static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ldesc, const SpreadPoint* pts,
-                              uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted,
-                              hipStream_t s) {
+                              const uint32_t* slot_ids, bool dense, uint32_t* counts, uint32_t* off, uint32_t* cursor,
+                              uint32_t* list, uint32_t* sorted, hipStream_t s) {
   const unsigned rows = blocks_for(16 * size_t(npts));
-  if (npts > 0)
-    hipLaunchKernelGGL(k_count_rows<Batched>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc, pts, counts);
+  if (npts > 0) {
+    if (dense)
+      hipLaunchKernelGGL((k_count_rows<Batched, true>), dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc, pts,
+                         slot_ids, counts);
+    else
+      hipLaunchKernelGGL((k_count_rows<Batched, false>), dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc,
+                         pts, slot_ids, counts);
+  }
   // the per-workgroup totals of the scan live after the nt+1 offsets
   const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
   uint32_t* sums = off + nt + 1;
   hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, nt, counts, sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, nt, counts, sums, off, cursor);
-  if (npts > 0)
-    hipLaunchKernelGGL(k_fill_rows<Batched>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc, pts, off,
-                       cursor, list);
+  if (npts > 0) {
+    if (dense)
+      hipLaunchKernelGGL((k_fill_rows<Batched, true>), dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc, pts,
+                         slot_ids, off, cursor, list);
+    else
+      hipLaunchKernelGGL((k_fill_rows<Batched, false>), dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, ldesc, pts,
+                         slot_ids, off, cursor, list);
+  }
   // lists longer than 16 are queued in `cursor` (free once the lists are filled)
   uint32_t* nqueued = off + nt + 1 + nsb;
   hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(size_t(nt))), dim3(PBLOCK), 0, s, uint32_t(nt), off, list, sorted,
@@ -1252,9 +1358,12 @@ static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ld
   return hipGetLastError();
 }
 
-hipError_t launch_spread_lists(int npts, const Geom& g, const SpreadPoint* pts, uint32_t* counts, uint32_t* off,
-                               uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s) {
-  return build_lists<false>(npts, int(spread_lists(g)), g, nullptr, pts, counts, off, cursor, list, sorted, s);
+hipError_t launch_spread_lists(int npts, const Geom& g, const SpreadPoint* pts, const uint32_t* slot_ids,
+                               uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted,
+                               hipStream_t s) {
+  const uint32_t nt = spread_lists(g);
+  return build_lists<false>(npts, int(nt), g, nullptr, pts, slot_ids, dense_lists(size_t(npts), nt), counts, off,
+                            cursor, list, sorted, s);
 }
 
 template <typename T>
@@ -1301,18 +1410,19 @@ hipError_t launch_edge_force(const Geom& g, const SpreadPoint* pts, const uint32
 }
 
 hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
-                               SpreadPoint* pts, hipStream_t s) {
+                               SpreadPoint* pts, uint32_t* slot_ids, hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_points, dim3(nwork), dim3(PBLOCK), 0, s, ldesc, sdesc, work, pts);
+  hipLaunchKernelGGL(kb_points, dim3(nwork), dim3(PBLOCK), 0, s, ldesc, sdesc, work, pts, slot_ids);
   return hipGetLastError();
 }
 
 // Lists of all members are numbered consecutively (member m from its
 // tile_base); block_work lists (member, lattice block) for every block.
-hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const SpreadPoint* pts, uint32_t* counts,
-                              uint32_t* off, uint32_t* cursor, uint32_t* list, uint32_t* sorted, hipStream_t s) {
+hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const SpreadPoint* pts,
+                              const uint32_t* slot_ids, bool dense, uint32_t* counts, uint32_t* off, uint32_t* cursor,
+                              uint32_t* list, uint32_t* sorted, hipStream_t s) {
   if (nlists <= 0) return hipSuccess;
-  return build_lists<true>(npts, nlists, Geom{}, ldesc, pts, counts, off, cursor, list, sorted, s);
+  return build_lists<true>(npts, nlists, Geom{}, ldesc, pts, slot_ids, dense, counts, off, cursor, list, sorted, s);
 }
 
 template <typename T>

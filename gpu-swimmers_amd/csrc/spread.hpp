@@ -92,6 +92,28 @@ __device__ __forceinline__ void add_point(const SpreadPoint* __restrict__ p, int
   }
 }
 
+// add_point for a record held in LDS: in the simple case each lane reads the
+// three weights it selects by index (one LDS read per axis) instead of all
+// twelve -- the same factors in the same order, so the same bits.
+__device__ __forceinline__ void add_point_lds(const SpreadPoint* __restrict__ p, int X, int Y, int Z, const Geom& g,
+                                              bool simple, double (&acc)[3]) {
+  if (!simple) {
+    add_point(p, X, Y, Z, g, simple, acc);
+    return;
+  }
+  int di = X - p->wc[0];
+  di += di < 0 ? g.nx : 0;
+  int dj = Y - p->wc[4];
+  dj += dj < 0 ? g.ny : 0;
+  int dk = Z - p->wc[8];
+  dk += dk < 0 ? g.nz : 0;
+  if ((di | dj | dk) >= 4) return;
+  const double delta3d = p->del[di] * p->del[4 + dj] * p->del[8 + dk];
+  acc[0] += delta3d * p->F[0];
+  acc[1] += delta3d * p->F[1];
+  acc[2] += delta3d * p->F[2];
+}
+
 // Force at this lane's site (X, Y, Z) from list `wl`, the list of the
 // wavefront's 64 sites (ok = false for lanes past the lattice end).  The list
 // index is wavefront-uniform, so ids and point records are read through the
@@ -209,7 +231,7 @@ __device__ __forceinline__ void range_force_staged(const SpreadPoint* __restrict
       while (mask) {
         const int k = __builtin_ctzll(mask);
         mask &= mask - 1;
-        if (ok) add_point(&st->pt[sub + k], X, Y, Z, g, simple, acc);
+        if (ok) add_point_lds(&st->pt[sub + k], X, Y, Z, g, simple, acc);
       }
     }
   }
@@ -347,7 +369,7 @@ __device__ __forceinline__ void wave_force_staged(const SpreadPoint* __restrict_
       const uint32_t id = st->id[e];
       if (id == prev) continue;
       prev = id;
-      if (ok) add_point(&st->pt[e], X, Y, Z, g, simple, acc);
+      if (ok) add_point_lds(&st->pt[e], X, Y, Z, g, simple, acc);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

@@ -114,6 +114,9 @@ cdef extern from "lbswim.h":
     int lbswim_ensemble_set_graph(lbswim_ensemble* e, int on)
     int lbswim_ensemble_set_order(lbswim_ensemble* e, int order)
     int lbswim_ensemble_set_force_mode(lbswim_ensemble* e, int mode)
+    int lbswim_ensemble_set_timing(lbswim_ensemble* e, unsigned mask)
+    int lbswim_ensemble_kernel_time(lbswim_ensemble* e, int kernel, double* total_ms, long* count)
+    int lbswim_ensemble_sites(lbswim_ensemble* e, size_t* virtual_sites, size_t* stored_sites)
     int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice)
     int lbswim_lattice_totals(lbswim_lattice* lat, double* out)
     int lbswim_lattice_set_timing(lbswim_lattice* lat, int on)
@@ -142,7 +145,7 @@ cdef int check(int rc) except -1:
 
 STORAGE = {"f64": LBSWIM_STORE_F64, "f32": LBSWIM_STORE_F32DEV}
 KERNELS = ("collide_stream", "calc_hydro", "zero_force", "add_forces", "swimmers_move", "tracers_move",
-           "init_from_hydro", "step_hydro")
+           "init_from_hydro", "step_hydro", "stored_chain")
 
 
 cdef object wrap_view(object owner, void* data, int ndim, cnp.npy_intp* shape, int typenum):
@@ -1016,3 +1019,32 @@ cdef class Ensemble:
         cdef int choice = 0
         check(lbswim_ensemble_autotune(self.impl, steps_per_trial, &choice))
         return choice
+
+    def set_timing(self, on):
+        """False: no timing (graphs replayed); True or an iterable of phase
+        names from KERNELS ("add_forces", "collide_stream", "calc_hydro",
+        "stored_chain", "swimmers_move", "tracers_move"): steps launched
+        eagerly with HIP events around those phases."""
+        cdef unsigned mask = 0
+        if on is True:
+            mask = (1 << len(KERNELS)) - 1
+        elif on:
+            for name in on:
+                mask |= 1 << KERNELS.index(name)
+        check(lbswim_ensemble_set_timing(self.impl, mask))
+
+    def kernel_times(self):
+        """{phase: (total_ms, count)} recorded since set_timing."""
+        cdef double ms = 0
+        cdef long cnt = 0
+        out = {}
+        for k, name in enumerate(KERNELS):
+            check(lbswim_ensemble_kernel_time(self.impl, k, &ms, &cnt))
+            out[name] = (ms, cnt)
+        return out
+
+    def sites(self):
+        """(sites of virtual-force members, sites of stored-force members)."""
+        cdef size_t v = 0, st = 0
+        check(lbswim_ensemble_sites(self.impl, &v, &st))
+        return v, st

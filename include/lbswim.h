@@ -77,7 +77,8 @@ extern "C" {
 #define LBSWIM_K_TMOVE 5   /* DoTracerArrayMove */
 #define LBSWIM_K_INIT 6    /* DoInitFromHydro */
 #define LBSWIM_K_FUSED 7   /* DoStep + DoCalcHydro in one launch (coupled step) */
-#define LBSWIM_K_COUNT 8
+#define LBSWIM_K_STORED 8  /* ensembles: stored-force members' force, collide and CalcHydro */
+#define LBSWIM_K_COUNT 9
 
 /* Same members, order and types as LBParams (lbswim/liblbswim/LBParams.h:4-30). */
 typedef struct lbswim_params {
@@ -306,6 +307,18 @@ int lbswim_ensemble_set_force_mode(lbswim_ensemble* e, int mode);
 /* Times each order over steps_per_trial (+2 warm-up) real steps of the
  * ensemble -- the members advance by those steps -- and keeps the fastest. */
 int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice);
+/* Per-phase timing of batched steps with HIP events on the streams the
+ * launches go to; while any bit of `mask` (1 << LBSWIM_K_*) is set, steps are
+ * launched eagerly instead of from graphs.  Recorded: LBSWIM_K_ADDF (swimmer
+ * points and lists), _COLLIDE and _HYDRO (the virtual-force members' batched
+ * launches), _STORED (the stored-force members' chain), _SMOVE, _TMOVE.  A
+ * nonzero mask clears earlier records. */
+int lbswim_ensemble_set_timing(lbswim_ensemble* e, unsigned mask);
+/* Sum of recorded durations (ms) and launch count of one phase. */
+int lbswim_ensemble_kernel_time(lbswim_ensemble* e, int kernel, double* total_ms, long* count);
+/* Lattice sites of the members whose swimmer force is virtual (covered by
+ * the batched _COLLIDE/_HYDRO launches) and of the others. */
+int lbswim_ensemble_sites(lbswim_ensemble* e, size_t* virtual_sites, size_t* stored_sites);
 
 /* ---- diagnostics (no reference counterpart) ---------------------------- */
 

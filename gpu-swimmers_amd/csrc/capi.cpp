@@ -198,7 +198,7 @@ struct lbswim_lattice {
   bool force_virtual = false;
   // Step + CalcHydro fused into one launch in the coupled step (where the
   // lattice qualifies); rows/strips 0 = chosen from the lattice size
-  bool fuse = true;
+  int fuse = LBSWIM_FUSION_AUTO;
   int fuse_rows = 0;
   // u with a site's components together, written by the fused step (the
   // particle moves gather from it); current only while nothing else has
@@ -538,13 +538,23 @@ static int lattice_hydro_impl(lbswim_lattice* lat) {
   return LBSWIM_OK;
 }
 
-// Tile rows of the fused step: the requested value, else the largest of 8, 4,
-// 2 dividing ny (8 runs fastest at 256^3); 0 if the lattice does not qualify.
+// Tile rows of the fused step, 0 if the coupled step runs the two launches.
+// LBSWIM_FUSION_AUTO fuses lattices whose tiles are full (nz % 64 == 0, rows
+// dividing ny), where one launch is the faster; LBSWIM_FUSION_ON any lattice
+// the launch supports.  The requested rows, else the largest of 8, 4, 2 that
+// qualifies (full tiles first).
 static int fusion_rows(const lbswim_lattice* lat) {
-  if (!lat->fuse) return 0;
-  if (lat->fuse_rows) return fused_step_supported(lat->geom, lat->fuse_rows, lat->elem) ? lat->fuse_rows : 0;
+  if (lat->fuse == LBSWIM_FUSION_OFF) return 0;
+  const bool any = lat->fuse == LBSWIM_FUSION_ON;
+  auto ok = [&](int ty, bool need_full) {
+    return fused_step_supported(lat->geom, ty, lat->elem) && (!need_full || fused_tiles_full(lat->geom, ty));
+  };
+  if (lat->fuse_rows) return ok(lat->fuse_rows, !any) ? lat->fuse_rows : 0;
   for (int ty : {8, 4, 2})
-    if (fused_step_supported(lat->geom, ty, lat->elem)) return ty;
+    if (ok(ty, true)) return ty;
+  if (any)
+    for (int ty : {8, 4, 2})
+      if (ok(ty, false)) return ty;
   return 0;
 }
 
@@ -638,9 +648,11 @@ int lbswim_lattice_u_is_virtual(lbswim_lattice* lat, int* is_virtual) {
 
 int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int strips) {
   if (int rc = check_lattice(lat)) return rc;
+  CHECK_ARG(enable == LBSWIM_FUSION_OFF || enable == LBSWIM_FUSION_AUTO || enable == LBSWIM_FUSION_ON,
+            "fusion must be LBSWIM_FUSION_OFF, _AUTO or _ON (got %d)", enable);
   CHECK_ARG(rows == 0 || rows == 2 || rows == 4 || rows == 8, "rows must be 0, 2, 4 or 8 (got %d)", rows);
   CHECK_ARG(strips >= 0, "negative strips");
-  lat->fuse = enable != 0;
+  lat->fuse = enable;
   lat->fuse_rows = rows;
   lat->fuse_strips = strips;
   return LBSWIM_OK;
@@ -1197,7 +1209,10 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     lat->sp_pts_cap = npts;
   }
   SpreadPoint* pts = static_cast<SpreadPoint*>(lat->sp_pts);
-  const int path = force_path(lat->force_mode, npts, lat->geom);
+  int path = force_path(lat->force_mode, npts, lat->geom);
+  // the fused launch walks a virtual force only where each 64-site run is one z-row
+  if (path == FORCE_LISTS_VIRTUAL && lat->geom.nz % 64 != 0 && fusion_rows(lat) > 0)
+    path = npts <= RANGE_FORCE_MAX_POINTS ? FORCE_RANGE_STORED : FORCE_LISTS_STORED;
   if (path == FORCE_RANGE_STORED) {
     // few points, dense: every run walks all of them, no lists
     KernelTimer t(lat, LBSWIM_K_ADDF);

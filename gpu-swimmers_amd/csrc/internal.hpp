@@ -58,10 +58,15 @@ hipError_t launch_calc_hydro(const T* f, const T* force, T* rho, T* u, const Geo
                              hipStream_t s);
 // Collide+stream followed by CalcHydro in one launch, with the results of
 // launch_collide_stream + launch_calc_hydro bit for bit, for lattices where
-// fused_step_supported(g, ty, sizeof(T)): nz % 64 == 0 and ny % ty == 0, ty in
-// {2, 4, 8} (rows per tile), and 15 n sizeof(T) < 4 GiB (32-bit buffer
-// offsets).  `strips` splits the x sweep (0: one workgroup per CU).
+// fused_step_supported(g, ty, sizeof(T)): ny >= ty, ty in {2, 4, 8} (rows
+// per tile), and 15 n sizeof(T) < 4 GiB (32-bit buffer offsets); a virtual
+// force `vf` also needs nz % 64 == 0.  `strips` splits the x sweep (0: one
+// workgroup per CU).
 bool fused_step_supported(const Geom& g, int ty, size_t elem);
+// nz % 64 == 0 and ny % ty == 0: no tile overlaps another and no lane idles
+// (the fused launch is then the faster one; the virtual force's row walk
+// needs nz % 64 == 0).
+bool fused_tiles_full(const Geom& g, int ty);
 // u is written only component-interleaved, u_aos[3 ijk + a] (3 n values),
 // which the particle moves gather from; launch_u_from_aos writes the
 // reference layout from it.

@@ -544,10 +544,16 @@ __host__ __device__ constexpr int dir_index(int p) {
 template <typename T>
 constexpr int fused_slots() { return sizeof(T) == 4 ? 9 : 6; }
 
+// Tiles of TY rows by `width` z sites (64, or nz below 64).  Where TY does
+// not divide ny, or 64 does not divide nz (>= 64), the last tile along that
+// axis is moved back to end at the lattice edge and overlaps its neighbour:
+// the sites both compute get the same values from the same inputs, so both
+// stores write the same bits.
 struct FusedTiles {
   uint32_t nyt, nzt;  // tiles along y and z
   uint32_t strip;     // planes per strip along x
   uint32_t nblk;      // tiles x strips
+  int width;          // z sites of a tile
 };
 
 template <typename T>
@@ -652,10 +658,12 @@ __device__ __forceinline__ void buffer_store_nt(T v, __amdgpu_buffer_rsrc_t r, u
 }
 
 // One wavefront's sweep of the tile.  r: tile row (0..TY+1; uniform except in
-// the z-halo wave), zz: tile z (-1..64), (y, z) the lattice row and z.
-template <typename T, bool VF, int TY, int ROLE>
+// the z-halo wave), zz: tile z (-1..W), (y, z) the lattice row and z.
+// PARTIAL (nz < 64, tiles W = nz sites wide): lanes with `active` false take
+// a valid site (zz = W - 1) and write nothing.
+template <typename T, bool VF, int TY, int ROLE, bool PARTIAL>
 __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict__ lds, int xa, int last, int r,
-                                            int zz, bool active, int y, int z) {
+                                            int zz, bool active, int y, int z, int W) {
   constexpr int PLANE = TY * FUSED_ROW;  // LDS words of one direction slot
   constexpr int SLOTS = fused_slots<T>(), TRASH = SLOTS * 5;
   constexpr bool TILE = ROLE == ROLE_TILE;
@@ -682,8 +690,10 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
     for (int i = 0; i < 3; ++i) oy[i] = uint32_t(__builtin_amdgcn_readfirstlane(int(oy[i])));
   }
   // site = srow (x, y and the tile's first z: uniform) + zl (lane part)
-  const uint32_t zrow = ZH ? 0u : uint32_t(__builtin_amdgcn_readfirstlane(z - lane));
-  const uint32_t zl = ZH ? oy[1] + oz[1] : uint32_t(lane);
+  const uint32_t zrow = ZH ? 0u : uint32_t(__builtin_amdgcn_readfirstlane(z - zz));
+  const uint32_t zl = ZH ? oy[1] + oz[1] : uint32_t(zz);
+  // lanes that store (every lane of a row wavefront unless PARTIAL)
+  const bool put = ZH || !PARTIAL || active;
   const uint32_t yrow = ZH ? 0u : oy[1];
   auto plane = [&](int it) {
     int x = (xa - 1 + it) % g.nx;  // the list pipeline looks up to two planes past the strip
@@ -775,8 +785,10 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
       take_list(it + 2, o2, m2);                    // plane it+2's entries, from bounds loaded with plane it
     }
     if constexpr (STORE) {
+      if (put) {
 #pragma unroll
-      for (int p = 0; p < Q; ++p) buffer_store_nt(out[p], rNew, zl * B, (uint32_t(p) * n + srow) * B);
+        for (int p = 0; p < Q; ++p) buffer_store_nt(out[p], rNew, zl * B, (uint32_t(p) * n + srow) * B);
+      }
     }
     const int w0 = SLOTS == 6 ? 0 : it % 2;
     const int w1 = SLOTS == 6 ? 1 + it % 2 : 2 + it % 3;
@@ -793,12 +805,12 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
             const int tr = r + cy;  // target tile row
             const int at = (slot * 5 + k) * PLANE + (tr - 1) * FUSED_ROW + zz + cz + 1;
             if constexpr (ROLE == ROLE_ZHALO) {
-              const bool ok = active && tr >= 1 && tr <= TY && zz + cz >= 0 && zz + cz < FUSED_Z;
+              const bool ok = active && tr >= 1 && tr <= TY && zz + cz >= 0 && zz + cz < W;
               lds[ok ? at : TRASH * PLANE + lane] = out[P];
             } else if constexpr (ROLE == ROLE_TILE) {
-              if (tr >= 1 && tr <= TY) lds[at] = out[P];  // uniform test
+              if (tr >= 1 && tr <= TY && put) lds[at] = out[P];  // uniform test unless PARTIAL
             } else {
-              lds[at] = out[P];
+              if (put) lds[at] = out[P];
             }
           }(),
           ...);
@@ -816,7 +828,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
       const int r2 = SLOTS == 6 ? 3 + (it + 1) % 3 : 5 + (it + 2) % 4;
       [&]<int... P>(std::integer_sequence<int, P...>) {
         ((fl[P] = lds[((dir_group(P) == 0 ? r0 : (dir_group(P) == 1 ? r1 : r2)) * 5 + dir_index(P)) * PLANE +
-                      (r - 1) * FUSED_ROW + lane + 1]),
+                      (r - 1) * FUSED_ROW + zz + 1]),
          ...);
       }(std::make_integer_sequence<int, Q>{});
     }
@@ -825,11 +837,14 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
       T rh, uh[3];
       hydro_site(fl, Fh, rh, uh);
       const uint32_t sp = uint32_t(plane(it - 1)) * nyz + yrow + zrow;
-      buffer_store_nt(rh, buffer_rsrc(A.rho), zl * B, sp * B);
-      // u goes only to the component-interleaved array; the reference
-      // layout is written from it when asked for (launch_u_from_aos)
+      if (put) {
+        buffer_store_nt(rh, buffer_rsrc(A.rho), zl * B, sp * B);
+        // u goes only to the component-interleaved array; the reference
+        // layout is written from it when asked for (launch_u_from_aos)
 #pragma unroll
-      for (int a = 0; a < 3; ++a) buffer_store(uh[a], buffer_rsrc(A.u_aos), (3 * zl + uint32_t(a)) * B, 3 * sp * B);
+        for (int a = 0; a < 3; ++a)
+          buffer_store(uh[a], buffer_rsrc(A.u_aos), (3 * zl + uint32_t(a)) * B, 3 * sp * B);
+      }
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) Fh[a] = F[a];
@@ -880,7 +895,7 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
   }
 }
 
-template <typename T, bool VF, int TY>
+template <typename T, bool VF, int TY, bool PARTIAL>
 __global__ __launch_bounds__((TY + 3) * 64) void k_step_hydro(FusedArgs<T> A, FusedTiles t) {
   constexpr int ROWS = TY + 2;
   __shared__ T lds[fused_slots<T>() * 5 * TY * FUSED_ROW + 64];
@@ -895,26 +910,30 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_step_hydro(FusedArgs<T> A, Fu
   const uint32_t yt = rest % t.nyt, sx = rest / t.nyt;
   const int xa = int(sx * t.strip), xb = min(xa + int(t.strip), g.nx);
   const int last = xb - xa + 1;  // iterations 0..last collide planes xa-1..xb
+  const int W = t.width;
+  const int z0 = min(int(zt) * FUSED_Z, g.nz - W), y0 = min(int(yt) * TY, g.ny - TY);
   const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64), lane = int(threadIdx.x) % 64;
   // this lane's position: tile row r (0 and ROWS-1 are halo rows) and tile z
-  // index zz (-1 and 64 are halo sites); spare lanes of the z-halo wave take
+  // index zz (-1 and W are halo sites); spare lanes of the z-halo wave take
   // a valid position and write nothing
   auto wrap = [](int v, int size) { return v < 0 ? v + size : (v >= size ? v - size : v); };
   if (wv == ROWS) {
     const bool active = lane < 2 * ROWS;
     const int r = active ? lane >> 1 : 0;
-    const int zz = (active && (lane & 1)) ? FUSED_Z : -1;
-    const int y = wrap(int(yt) * TY + r - 1, g.ny), z = wrap(int(zt) * FUSED_Z + zz, g.nz);
-    fused_sweep<T, VF, TY, ROLE_ZHALO>(A, lds, xa, last, r, zz, active, y, z);
+    const int zz = (active && (lane & 1)) ? W : -1;
+    const int y = wrap(y0 + r - 1, g.ny), z = wrap(z0 + zz, g.nz);
+    fused_sweep<T, VF, TY, ROLE_ZHALO, PARTIAL>(A, lds, xa, last, r, zz, active, y, z, W);
     return;
   }
-  const int y = wrap(int(yt) * TY + wv - 1, g.ny), z = int(zt) * FUSED_Z + lane;
+  const bool active = lane < W;
+  const int zz = active ? lane : W - 1;
+  const int y = wrap(y0 + wv - 1, g.ny), z = z0 + zz;
   if (wv == 0)
-    fused_sweep<T, VF, TY, ROLE_ABOVE>(A, lds, xa, last, wv, lane, true, y, z);
+    fused_sweep<T, VF, TY, ROLE_ABOVE, PARTIAL>(A, lds, xa, last, wv, zz, active, y, z, W);
   else if (wv == ROWS - 1)
-    fused_sweep<T, VF, TY, ROLE_BELOW>(A, lds, xa, last, wv, lane, true, y, z);
+    fused_sweep<T, VF, TY, ROLE_BELOW, PARTIAL>(A, lds, xa, last, wv, zz, active, y, z, W);
   else
-    fused_sweep<T, VF, TY, ROLE_TILE>(A, lds, xa, last, wv, lane, true, y, z);
+    fused_sweep<T, VF, TY, ROLE_TILE, PARTIAL>(A, lds, xa, last, wv, zz, active, y, z, W);
 }
 
 // ------------------------------------------------------------------------
@@ -1100,17 +1119,22 @@ hipError_t launch_collide_stream<float>(const float* fOld, const float* force, f
 }
 
 bool fused_step_supported(const Geom& g, int ty, size_t elem) {
-  return g.n > 0 && g.nz % FUSED_Z == 0 && (ty == 2 || ty == 4 || ty == 8) && g.ny % ty == 0 &&
-         uint64_t(Q) * g.n * elem < (uint64_t(1) << 32);
+  return g.n > 0 && (ty == 2 || ty == 4 || ty == 8) && g.ny >= ty && uint64_t(Q) * g.n * elem < (uint64_t(1) << 32);
 }
+
+bool fused_tiles_full(const Geom& g, int ty) { return g.nz % FUSED_Z == 0 && g.ny % ty == 0; }
 
 template <typename T>
 hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u_aos, const Geom& g, double tau_s,
                              double tau_b, const VForce* vf, int ty, int strips, hipStream_t s) {
   if (!fused_step_supported(g, ty, sizeof(T)) || strips < 0) return hipErrorInvalidValue;
+  // the virtual force's row walk needs each 64-site run to be one z-row
+  if (vf && g.nz % FUSED_Z != 0) return hipErrorInvalidValue;
   FusedTiles t;
-  t.nyt = uint32_t(g.ny / ty);
-  t.nzt = uint32_t(g.nz / FUSED_Z);
+  t.nyt = uint32_t((g.ny + ty - 1) / ty);
+  t.nzt = uint32_t((g.nz + FUSED_Z - 1) / FUSED_Z);
+  t.width = std::min(g.nz, FUSED_Z);
+  const bool partial = g.nz < FUSED_Z;
   const uint32_t tiles = t.nyt * t.nzt;
   if (strips == 0) {
     // one workgroup per CU (the tile's LDS and VGPRs allow one): enough
@@ -1144,9 +1168,11 @@ hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* 
 #define LBS_FUSED(TYV)                                                                                       \
   if (ty == TYV) {                                                                                           \
     if (vf)                                                                                                  \
-      hipLaunchKernelGGL((k_step_hydro<T, true, TYV>), grid, dim3((TYV + 3) * 64), 0, s, A, t);              \
+      hipLaunchKernelGGL((k_step_hydro<T, true, TYV, false>), grid, dim3((TYV + 3) * 64), 0, s, A, t);       \
+    else if (partial)                                                                                        \
+      hipLaunchKernelGGL((k_step_hydro<T, false, TYV, true>), grid, dim3((TYV + 3) * 64), 0, s, A, t);       \
     else                                                                                                     \
-      hipLaunchKernelGGL((k_step_hydro<T, false, TYV>), grid, dim3((TYV + 3) * 64), 0, s, A, t);             \
+      hipLaunchKernelGGL((k_step_hydro<T, false, TYV, false>), grid, dim3((TYV + 3) * 64), 0, s, A, t);      \
   }
   LBS_FUSED(2)
   LBS_FUSED(4)

@@ -423,10 +423,12 @@ cdef class Lattice:
         return bool(v)
 
     def set_fusion(self, enable=True, int rows=0, int strips=0):
-        """Fuse Step and CalcHydro of the coupled step into one launch where the
-        lattice allows it (nz % 64 == 0, ny % rows == 0); rows (2, 4, 8) and
-        strips 0 choose the tile shape from the lattice.  On by default."""
-        check(lbswim_lattice_set_fusion(self.handle(), 1 if enable else 0, rows, strips))
+        """Fuse Step and CalcHydro of the coupled step into one launch.
+        enable: True (the default) where the tiles are full (nz % 64 == 0,
+        rows dividing ny), "always" on every lattice with ny >= rows, False
+        never.  rows (2, 4, 8) and strips 0 choose the tile shape."""
+        mode = 2 if enable == "always" else (1 if enable else 0)
+        check(lbswim_lattice_set_fusion(self.handle(), mode, rows, strips))
 
     @property
     def fusion(self):

@@ -265,12 +265,19 @@ int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual);
 int lbswim_lattice_sync_u(lbswim_lattice* lat);
 /* *is_virtual = 1 while u is not yet written in the reference layout. */
 int lbswim_lattice_u_is_virtual(lbswim_lattice* lat, int* is_virtual);
-/* Step fusion in lbswim_system_step: with `enable` (the default) and a
- * lattice with nz % 64 == 0, ny % rows == 0 and 15 n sizeof(storage) < 4 GiB,
- * Step and CalcHydro run as one launch that reads the distributions once,
- * with results identical bit for bit to the two launches.  rows: lattice
- * rows per tile (2, 4 or 8; 0 chooses); strips: x sweeps per tile column (0
- * chooses).  No reference counterpart. */
+/* Step fusion in lbswim_system_step: Step and CalcHydro run as one launch
+ * that reads the distributions once, with results identical bit for bit to
+ * the two launches, on lattices with ny >= rows and 15 n sizeof(storage) <
+ * 4 GiB.  LBSWIM_FUSION_AUTO (the default) fuses where the tiles are full
+ * (nz % 64 == 0, rows dividing ny), where it is the faster; LBSWIM_FUSION_ON
+ * fuses every such lattice (tiles overlap at the edges, lanes idle where nz
+ * < 64, and the swimmer force is stored unless nz % 64 == 0);
+ * LBSWIM_FUSION_OFF never.  rows: lattice rows per tile (2, 4 or 8; 0
+ * chooses); strips: x sweeps per tile column (0 chooses).  No reference
+ * counterpart. */
+#define LBSWIM_FUSION_OFF 0
+#define LBSWIM_FUSION_AUTO 1
+#define LBSWIM_FUSION_ON 2
 int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int strips);
 /* *active = 1 if the coupled step of this lattice runs fused; *rows and
  * *strips (either may be NULL) receive the tile shape it uses. */

@@ -1,8 +1,9 @@
 """GPU parity of the fused coupled step: Step + CalcHydro in one launch.
 
-Where a lattice qualifies (nz % 64 == 0, ny % rows == 0) lbswim_system_step
-collides and computes rho/u in one sweep over tiles of rows with a ring of
-neighbour sites recomputed.  Its distributions, rho, u, force and particle
+Where a lattice qualifies (by default nz % 64 == 0 and ny % rows == 0; with
+fusion "always" any lattice with ny >= rows) lbswim_system_step collides and
+computes rho/u in one sweep over tiles of rows with a ring of neighbour
+sites recomputed.  Its distributions, rho, u, force and particle
 state must equal the two-launch path bit for bit -- for every tile shape
 (rows, strips), lattices thinner than the halo (nx = 1, ny = rows), both
 storage forms and both force modes -- and the FP64 form must equal the CPU
@@ -30,12 +31,12 @@ def oracle_threads():
     O.set_threads(min(16, os.cpu_count() or 1))
 
 
-def run(case, storage, force_mode, fusion, steps):
+def run(case, storage, force_mode, fusion, steps, mode=True):
     lat, system, sw, tr = build_lbswim(case, storage, force_mode)
     if fusion is None:
         lat.set_fusion(False)
     else:
-        lat.set_fusion(True, *fusion)
+        lat.set_fusion(mode, *fusion)
     active, rows, _ = lat.fusion
     assert active == (fusion is not None), lat.fusion
     if fusion is not None and fusion[0]:
@@ -80,6 +81,42 @@ def test_fused_equals_two_launches(size, rows, strips, force_mode, storage):
         assert np.array_equal(a[k], b[k]), k
 
 
+# Lattices whose tiles are not full: nz below 64 (idle lanes), nz above 64
+# but not a multiple of it and ny not a multiple of the rows (the last tile
+# overlaps its neighbour), a single z site.  Fusion "always".
+ANY_SHAPES = [
+    ((4, 8, 16), 8, 0),
+    ((5, 12, 24), 4, 0),
+    ((3, 13, 48), 8, 2),
+    ((4, 8, 96), 8, 0),
+    ((3, 9, 70), 4, 2),
+    ((2, 6, 1), 2, 0),
+    ((6, 10, 33), 8, 0),
+]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("force_mode", ["stored", "virtual"])
+@pytest.mark.parametrize("size,rows,strips", ANY_SHAPES)
+def test_fused_any_shape_equals_two_launches(size, rows, strips, force_mode, storage):
+    case = swimmer_case(size, seed=size[2] * 10 + rows)
+    say("fused (any shape) vs two launches: %s rows=%d strips=%d force %s %s"
+        % (size, rows, strips, force_mode, storage))
+    a = run(case, storage, force_mode, None, 4)
+    b = run(case, storage, force_mode, (rows, strips), 4, mode="always")
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_fused_any_shape_equals_oracle():
+    case = make_case((7, 11, 48), 6, 0.25, ns=50, nt=400, alpha=0.02, seed=31, drho=1e-2, du=1e-2, P=1e-2, l=0.5)
+    say("fused (any shape) vs oracle: (7, 11, 48), 6 steps")
+    ref, _ = run_oracle(O, case)
+    got = run(case, "f64", "auto", (0, 0), case.steps, mode="always")
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), k
+
+
 @pytest.mark.parametrize("force_mode", ["stored", "virtual"])
 def test_fused_equals_oracle(force_mode):
     case = make_case((6, 8, 128), 5, 0.25, ns=40, nt=500, alpha=0.0, seed=21, drho=1e-2, du=1e-2, P=1e-2, l=0.5)
@@ -101,6 +138,10 @@ def test_fusion_skipped_where_unsupported():
     assert lat.fusion[0] is False
     lat.set_fusion(True, 2, 0)
     assert lat.fusion == (True, 2, 0)
+    lat.set_fusion("always", 4, 0)   # tiles of 4 rows overlap on 6 rows
+    assert lat.fusion == (True, 4, 0)
+    lat.set_fusion("always", 8, 0)   # more rows than the lattice has
+    assert lat.fusion[0] is False
     del lat
     # 15 n sizeof(double) above 4 GiB: the fused launch's 32-bit offsets
     # cannot reach the whole buffer

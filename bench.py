@@ -80,7 +80,13 @@ def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, f
     import lbswim
 
     lat, system, sw, tr = build_lbswim(case, storage, force_mode)
-    lat.set_fusion(fusion)
+    if fusion:
+        # the autotuner picks two launches or a fused tile shape on real steps
+        tuned = system.Autotune(5)
+        log("%s: autotuned (rows, strips) = %s" % (storage, tuned))
+    else:
+        lat.set_fusion(False)
+        tuned = None
     fused = lat.fusion[0]
     main = "step_hydro" if fused else "collide_stream"  # the dominant lattice kernel
     system.Step(warmup)
@@ -120,9 +126,10 @@ def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, f
     res["main_gbs"] = res["main_bytes"] * n / (avg * 1e-3) / 1e9
     res["step_gbs"] = res["step_bytes"] * n / (ms_step * 1e-3) / 1e9
     res["tumbles"] = sw.tumbles()
+    res["autotuned"] = list(tuned) if tuned is not None else None
     # final state for the FP32-vs-FP64 error report (same inputs, same step
-    # count: untimed steps up to total_steps)
-    done = warmup + steps + breakdown_steps
+    # count: untimed steps up to total_steps, the tuner's steps included)
+    done = lat.time_step
     if total_steps > done:
         system.Step(total_steps - done)
     res["state_steps"] = max(total_steps, done)
@@ -197,18 +204,38 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True):
         lat, system, sw, tr = build_lbswim(c, storage)
         ens.add(lat, [sw])
         keep.append((lat, sw))
-    choice = ens.autotune(tune_steps)
+    ens.autotune(tune_steps)
+    order, per = ens.tuning
     ens.step(2)
     lbswim.synchronize()
     t0 = time.perf_counter()
     ens.step(steps)
     lbswim.synchronize()
     t_ens = time.perf_counter() - t0
+    # roofline of the dominant launch: the batched collide+stream of the
+    # virtual-force members, timed with HIP events in an eager pass
+    ens.set_timing(["collide_stream", "calc_hydro"])
+    ens.step(20)
+    kt = ens.kernel_times()
+    ens.set_timing(False)
+    vsites, _ = ens.sites()
+    c_ms, c_n = kt["collide_stream"]
+    h_ms, h_n = kt["calc_hydro"]
+    roof = None
+    if c_n:
+        avg = c_ms / c_n
+        byts = vsites * collide_bytes(storage, True)
+        roof = {"kernel": "kb_collide<%s, virtual force>" % storage, "bound": "hbm",
+                "achieved": round(byts / (avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(byts / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": byts, "avg_launch_ms": round(avg, 4),
+                "calc_hydro_avg_ms": round(h_ms / h_n, 4) if h_n else None,
+                "calc_hydro_gbs": round(vsites * (19 * ELEM[storage]) / (h_ms / h_n * 1e-3) / 1e9, 1) if h_n else None}
     return {"sims": nsims, "steps": steps, "sites": sites, "ms_batched": round(t_ens * 1e3, 2),
             "ms_sequential": round(t_seq * 1e3, 2), "sims_per_s": round(nsims / t_ens, 2),
             "sims_per_s_sequential": round(nsims / t_seq, 2), "speedup_vs_sequential": round(t_seq / t_ens, 2),
-            "mlups": round(sites * steps / t_ens / 1e6, 1), "order": ORDERS[choice],
-            "graph": graph}
+            "mlups": round(sites * steps / t_ens / 1e6, 1), "order": ORDERS[order], "blocks_per_workgroup": per,
+            "graph": graph, "roofline": roof}
 
 
 def bench_serving(nreq, rate, seed=1234):
@@ -461,6 +488,7 @@ def main():
             "algorithmic_bytes_per_launch": m["main_bytes"] * n,
             "virtual_force": m["virtual_force"],
             "fusion": m["fusion"],
+            "autotuned_rows_strips": m["autotuned"],
             "avg_launch_ms": round(m["main_avg_ms"], 4),
         },
         "step_hbm_frac": round(m["step_gbs"] / HBM_PEAK_GBS, 4),

@@ -1350,6 +1350,79 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
   return rc ? rc : jrc;
 }
 
+// ---- autotuning of the coupled step ------------------------------------------
+
+int lbswim_system_autotune(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
+                           int ntr, int steps_per_trial, int* rows, int* strips) {
+  if (int rc = check_lattice(lat)) return rc;
+  CHECK_ARG(steps_per_trial > 0, "steps_per_trial must be > 0");
+  int cus = 0, dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  // candidates: the two launches (rows 0), then fused tiles of 8, 4, 2 rows
+  // with 1, 2 and 4 times the strips that give every CU one tile
+  struct Choice {
+    int rows, strips;
+  };
+  std::vector<Choice> cand{{0, 0}};
+  const Geom& g = lat->geom;
+  for (int ty : {8, 4, 2}) {
+    if (!fused_step_supported(g, ty, lat->elem)) continue;
+    const int tiles = (g.ny + ty - 1) / ty * std::max(1, (g.nz + 63) / 64);
+    const int base = std::min(g.nx, std::max(1, (cus + tiles - 1) / tiles));
+    int prev = 0;
+    for (int k : {1, 2, 4}) {
+      const int st = std::min(g.nx, base * k);
+      if (st == prev) continue;
+      prev = st;
+      cand.push_back(Choice{ty, st});
+    }
+  }
+  const int saved_fuse = lat->fuse, saved_rows = lat->fuse_rows, saved_strips = lat->fuse_strips;
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail(LBSWIM_EHIP, "hipEventCreate failed");
+  }
+  int rc = LBSWIM_OK;
+  float best_ms = 0.f;
+  Choice best = cand[0];
+  for (size_t i = 0; i < cand.size() && !rc; ++i) {
+    lat->fuse = cand[i].rows ? LBSWIM_FUSION_ON : LBSWIM_FUSION_OFF;
+    lat->fuse_rows = cand[i].rows;
+    lat->fuse_strips = cand[i].strips;
+    if ((rc = lbswim_system_step(lat, sw, nsw, tr, ntr, 2))) break;  // warm-up, workspace
+    (void)hipEventRecord(a, stream());
+    if ((rc = lbswim_system_step(lat, sw, nsw, tr, ntr, steps_per_trial))) break;
+    (void)hipEventRecord(b, stream());
+    float ms = 0.f;
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+      rc = fail(LBSWIM_EHIP, "timing the coupled step failed");
+      break;
+    }
+    if (i == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = cand[i];
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc) {
+    lat->fuse = saved_fuse;
+    lat->fuse_rows = saved_rows;
+    lat->fuse_strips = saved_strips;
+    return rc;
+  }
+  lat->fuse = best.rows ? LBSWIM_FUSION_ON : LBSWIM_FUSION_OFF;
+  lat->fuse_rows = best.rows;
+  lat->fuse_strips = best.strips;
+  if (rows) *rows = best.rows;
+  if (strips) *strips = best.strips;
+  return LBSWIM_OK;
+}
+
 // ---- coupled step without allocation -----------------------------------------
 
 namespace {
@@ -1651,6 +1724,7 @@ struct lbswim_ensemble {
   bool dirty = true;  // device tables out of date
   int parity = 0;
   int order = ORDER_MEMBER_MAJOR;
+  int per = 1;  // 256-site lattice blocks per work item (workgroup) of the batched launches
   bool use_graph = true;
   // device tables
   LatDesc* d_lat = nullptr;
@@ -1794,7 +1868,7 @@ int ens_build(lbswim_ensemble* e) {
     d.point_count = uint32_t(member_pts);
     lat.push_back(d);
     std::vector<WorkItem> lw;
-    for (uint32_t b = 0; b < lattice_blocks(L->geom); ++b) lw.push_back(WorkItem{mi, b});
+    for (uint32_t b = 0; b < lattice_blocks(L->geom); b += uint32_t(e->per)) lw.push_back(WorkItem{mi, b});
     const int group = d.force_path == FORCE_LISTS_VIRTUAL ? 1 : 0;
     latwork[group].push_back(lw);
     weight[group].push_back(L->n);
@@ -1918,9 +1992,10 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
     if (int rc = fork_now()) return rc;
   auto stored_chain = [&]() -> int {
     EnsTimer t(e, LBSWIM_K_STORED, sb);
-    HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->pts, e->off, e->sorted, sb));
-    HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, parity, nullptr, sb));
-    HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_storedwork, e->n_storedwork, parity, nullptr, sb));
+    HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->per, e->pts, e->off,
+                                         e->sorted, sb));
+    HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb));
+    HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb));
     t.done();
     return LBSWIM_OK;
   };
@@ -1937,12 +2012,12 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   const VForce vf{e->pts, e->off, e->sorted};
   {
     EnsTimer t(e, LBSWIM_K_COLLIDE, s);
-    HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_vfwork, e->n_vfwork, parity, &vf, s));
+    HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_vfwork, e->n_vfwork, e->per, parity, &vf, s));
     t.done();
   }
   {
     EnsTimer t(e, LBSWIM_K_HYDRO, s);
-    HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_vfwork, e->n_vfwork, parity, &vf, s));
+    HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_vfwork, e->n_vfwork, e->per, parity, &vf, s));
     t.done();
   }
   if (fork) {
@@ -1968,9 +2043,9 @@ int ens_enqueue(lbswim_ensemble* e, int parity) {
 // array from the last step's lists, as the reference leaves it.
 int ens_materialize_force(lbswim_ensemble* e) {
   hipError_t r = e->storage == LBSWIM_STORE_F64
-                     ? launch_batch_spread_force<double>(e->d_lat, e->n_vfwork, e->d_vfwork, e->pts, e->off,
-                                                         e->sorted, stream())
-                     : launch_batch_spread_force<float>(e->d_lat, e->n_vfwork, e->d_vfwork, e->pts, e->off,
+                     ? launch_batch_spread_force<double>(e->d_lat, e->n_vfwork, e->d_vfwork, e->per, e->pts,
+                                                         e->off, e->sorted, stream())
+                     : launch_batch_spread_force<float>(e->d_lat, e->n_vfwork, e->d_vfwork, e->per, e->pts, e->off,
                                                         e->sorted, stream());
   HIP_TRY(r);
   return LBSWIM_OK;
@@ -2202,32 +2277,53 @@ int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choic
   hipEvent_t a = nullptr, b = nullptr;
   HIP_TRY(hipEventCreate(&a));
   HIP_TRY(hipEventCreate(&b));
-  int best = e->order;
+  int best_order = e->order, best_per = e->per;
   float best_ms = 0.f;
+  bool first = true;
   int rc = LBSWIM_OK;
-  for (int o = 0; o < ORDER_COUNT && !rc; ++o) {
-    e->order = o;
-    e->dirty = true;
-    if ((rc = lbswim_ensemble_step(e, 2))) break;  // build + capture + warm-up
-    (void)hipEventRecord(a, stream());
-    if ((rc = lbswim_ensemble_step(e, steps_per_trial))) break;
-    (void)hipEventRecord(b, stream());
-    (void)hipEventSynchronize(b);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, a, b);
-    if (o == 0 || ms < best_ms) {
-      best_ms = ms;
-      best = o;
+  // workgroup order x work-item size (1, 2, 4 lattice blocks of 256 sites)
+  for (int per : {1, 2, 4}) {
+    for (int o = 0; o < ORDER_COUNT && !rc; ++o) {
+      e->order = o;
+      e->per = per;
+      e->dirty = true;
+      if ((rc = lbswim_ensemble_step(e, 2))) break;  // build + capture + warm-up
+      (void)hipEventRecord(a, stream());
+      if ((rc = lbswim_ensemble_step(e, steps_per_trial))) break;
+      (void)hipEventRecord(b, stream());
+      (void)hipEventSynchronize(b);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      if (first || ms < best_ms) {
+        best_ms = ms;
+        best_order = o;
+        best_per = per;
+        first = false;
+      }
     }
+    if (rc) break;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc) return rc;
-  if (e->order != best) {
-    e->order = best;
-    e->dirty = true;
-  }
-  if (choice) *choice = best;
+  if (e->order != best_order || e->per != best_per) e->dirty = true;
+  e->order = best_order;
+  e->per = best_per;
+  if (choice) *choice = best_order;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_set_work(lbswim_ensemble* e, int blocks_per_item) {
+  CHECK_ARG(e && blocks_per_item >= 1 && blocks_per_item <= 64, "blocks per work item must be 1..64");
+  if (e->per != blocks_per_item) e->dirty = true;
+  e->per = blocks_per_item;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_tuning(lbswim_ensemble* e, int* order, int* blocks_per_item) {
+  CHECK_ARG(e, "ensemble is NULL");
+  if (order) *order = e->order;
+  if (blocks_per_item) *blocks_per_item = e->per;
   return LBSWIM_OK;
 }
 

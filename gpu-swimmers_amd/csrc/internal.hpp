@@ -208,12 +208,13 @@ constexpr int TRACER_GROUPS = 64;        // tracers per workgroup of the tracer 
 // vf: the work covers members with FORCE_LISTS_VIRTUAL, whose force of block
 // b comes from lists tile_base_m + 4 b ...; null: members that read their
 // force array.
+// Each work item covers `per` consecutive 256-site blocks of its member.
 template <typename T>
-hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
-                                hipStream_t s);
+hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int per, int parity,
+                                const VForce* vf, hipStream_t s);
 template <typename T>
-hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
-                              hipStream_t s);
+hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int per, int parity,
+                              const VForce* vf, hipStream_t s);
 // Point records of a batch: points of array a get ids point_base_a + 2*user +
 // end; work items are blocks of 128 swimmers (256 points) of an array.
 hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
@@ -226,7 +227,7 @@ hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const 
 // Writes the force arrays of the lattice blocks in block_work: from the lists
 // (FORCE_LISTS_STORED) or from all the member's points (FORCE_RANGE_STORED).
 template <typename T>
-hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nblocks, const WorkItem* block_work,
+hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nwork, const WorkItem* block_work, int per,
                                      const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
                                      hipStream_t s);
 template <typename T>

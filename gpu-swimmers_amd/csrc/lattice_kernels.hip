@@ -54,6 +54,7 @@ namespace {
 
 constexpr int BLOCK = 256;
 static_assert(BLOCK == SITE_BLOCK, "force lists are numbered by lattice workgroup");
+__host__ __device__ inline uint32_t grid_blocks(uint32_t n) { return (n + BLOCK - 1) / BLOCK; }
 
 struct Site {
   uint32_t i, j, k;
@@ -989,36 +990,48 @@ __global__ __launch_bounds__(BLOCK) void k_calc_hydro_vf_f32(const float* __rest
 // block `unit` comes from lists tile_base_m + 4 unit ...; otherwise members
 // whose force array holds it.  (One kernel for both would need more registers
 // than either.)
+// A work item covers `per` consecutive 256-site blocks of its member (the
+// work-unit size the ensemble autotuner chooses).
 template <typename T, bool VF>
 __global__ __launch_bounds__(BLOCK) void kb_collide(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
-                                                    int parity, VForce vf) {
+                                                    int parity, VForce vf, int per) {
   const WorkItem w = work[blockIdx.x];
   const LatDesc& m = desc[w.member];
   const int c = m.cur ^ parity;
-  const uint32_t vlist = m.tile_base + w.unit * LISTS_PER_BLOCK;
-  if constexpr (sizeof(T) == 8)
-    collide_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
-                         static_cast<double*>(m.g[c ^ 1]), m.geom, m.tau_s, m.tau_b, m.omega_s, m.omega_b, w.unit, vf,
-                         vlist);
-  else
-    collide_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
-                         static_cast<float*>(m.g[c ^ 1]), m.geom, float(m.tau_s), float(m.tau_b), float(m.omega_s),
-                         float(m.omega_b), w.unit, vf, vlist);
+  const uint32_t nb = grid_blocks(m.geom.n);
+  for (int b = 0; b < per; ++b) {
+    const uint32_t blk = w.unit + uint32_t(b);
+    if (blk >= nb) break;
+    const uint32_t vlist = m.tile_base + blk * LISTS_PER_BLOCK;
+    if constexpr (sizeof(T) == 8)
+      collide_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
+                           static_cast<double*>(m.g[c ^ 1]), m.geom, m.tau_s, m.tau_b, m.omega_s, m.omega_b, blk, vf,
+                           vlist);
+    else
+      collide_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
+                           static_cast<float*>(m.g[c ^ 1]), m.geom, float(m.tau_s), float(m.tau_b), float(m.omega_s),
+                           float(m.omega_b), blk, vf, vlist);
+  }
 }
 
 template <typename T, bool VF>
 __global__ __launch_bounds__(BLOCK) void kb_hydro(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
-                                                  int parity, VForce vf) {
+                                                  int parity, VForce vf, int per) {
   const WorkItem w = work[blockIdx.x];
   const LatDesc& m = desc[w.member];
   const int c = m.cur ^ parity ^ 1;
-  const uint32_t vlist = m.tile_base + w.unit * LISTS_PER_BLOCK;
-  if constexpr (sizeof(T) == 8)
-    hydro_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
-                       static_cast<double*>(m.rho), static_cast<double*>(m.u), m.geom, w.unit, vf, vlist);
-  else
-    hydro_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
-                       static_cast<float*>(m.rho), static_cast<float*>(m.u), m.geom, w.unit, vf, vlist);
+  const uint32_t nb = grid_blocks(m.geom.n);
+  for (int b = 0; b < per; ++b) {
+    const uint32_t blk = w.unit + uint32_t(b);
+    if (blk >= nb) break;
+    const uint32_t vlist = m.tile_base + blk * LISTS_PER_BLOCK;
+    if constexpr (sizeof(T) == 8)
+      hydro_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
+                         static_cast<double*>(m.rho), static_cast<double*>(m.u), m.geom, blk, vf, vlist);
+    else
+      hydro_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
+                         static_cast<float*>(m.rho), static_cast<float*>(m.u), m.geom, blk, vf, vlist);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -1244,29 +1257,33 @@ hipError_t launch_init_from_hydro<float>(const float* rho, const float* u, float
 }
 
 template <typename T>
-hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
-                                hipStream_t s) {
+hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int per, int parity,
+                                const VForce* vf, hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
   if (vf)
-    hipLaunchKernelGGL((kb_collide<T, true>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf);
+    hipLaunchKernelGGL((kb_collide<T, true>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf, per);
   else
-    hipLaunchKernelGGL((kb_collide<T, false>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, VForce{});
+    hipLaunchKernelGGL((kb_collide<T, false>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, VForce{}, per);
   return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce* vf,
-                              hipStream_t s) {
+hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int per, int parity,
+                              const VForce* vf, hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
   if (vf)
-    hipLaunchKernelGGL((kb_hydro<T, true>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf);
+    hipLaunchKernelGGL((kb_hydro<T, true>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf, per);
   else
-    hipLaunchKernelGGL((kb_hydro<T, false>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, VForce{});
+    hipLaunchKernelGGL((kb_hydro<T, false>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, VForce{}, per);
   return hipGetLastError();
 }
-template hipError_t launch_batch_collide<double>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
-template hipError_t launch_batch_collide<float>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
-template hipError_t launch_batch_hydro<double>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
-template hipError_t launch_batch_hydro<float>(const LatDesc*, const WorkItem*, int, int, const VForce*, hipStream_t);
+template hipError_t launch_batch_collide<double>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,
+                                                 hipStream_t);
+template hipError_t launch_batch_collide<float>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,
+                                                hipStream_t);
+template hipError_t launch_batch_hydro<double>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,
+                                               hipStream_t);
+template hipError_t launch_batch_hydro<float>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,
+                                              hipStream_t);
 
 uint32_t lattice_blocks(const Geom& g) { return grid_for(g.n); }
 

@@ -1209,18 +1209,23 @@ __global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __
                                                               const WorkItem* __restrict__ block_work,
                                                               const SpreadPoint* __restrict__ pts,
                                                               const uint32_t* __restrict__ off,
-                                                              const uint32_t* __restrict__ sorted) {
+                                                              const uint32_t* __restrict__ sorted, int per) {
   __shared__ union {
     WaveStage list[LISTS_PER_BLOCK];
     RangeStage range;
   } stage;
   const WorkItem w = block_work[blockIdx.x];
   const LatDesc& m = ldesc[w.member];
-  if (m.force_path == FORCE_RANGE_STORED)
-    range_block_body(m.geom, pts, m.point_first, m.point_count, static_cast<T*>(m.force), w.unit, &stage.range);
-  else
-    spread_block_body(m.geom, pts, off, sorted, static_cast<T*>(m.force), w.unit,
-                      m.tile_base + w.unit * LISTS_PER_BLOCK, stage.list);
+  const uint32_t nb = (m.geom.n + SITE_BLOCK - 1) / SITE_BLOCK;
+  for (int b = 0; b < per; ++b) {  // the work item's `per` consecutive blocks
+    const uint32_t blk = w.unit + uint32_t(b);
+    if (blk >= nb) break;
+    if (m.force_path == FORCE_RANGE_STORED)
+      range_block_body(m.geom, pts, m.point_first, m.point_count, static_cast<T*>(m.force), blk, &stage.range);
+    else
+      spread_block_body(m.geom, pts, off, sorted, static_cast<T*>(m.force), blk, m.tile_base + blk * LISTS_PER_BLOCK,
+                        stage.list);
+  }
 }
 
 // ---- device-math probes used by the parity tests ----
@@ -1426,17 +1431,17 @@ hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const 
 }
 
 template <typename T>
-hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nblocks, const WorkItem* block_work,
+hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nwork, const WorkItem* block_work, int per,
                                      const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
                                      hipStream_t s) {
-  if (nblocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_spread_blocks<T>, dim3(unsigned(nblocks)), dim3(SITE_BLOCK), 0, s, ldesc, block_work, pts,
-                     off, sorted);
+  if (nwork <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_spread_blocks<T>, dim3(unsigned(nwork)), dim3(SITE_BLOCK), 0, s, ldesc, block_work, pts, off,
+                     sorted, per);
   return hipGetLastError();
 }
-template hipError_t launch_batch_spread_force<double>(const LatDesc*, int, const WorkItem*, const SpreadPoint*,
+template hipError_t launch_batch_spread_force<double>(const LatDesc*, int, const WorkItem*, int, const SpreadPoint*,
                                                       const uint32_t*, const uint32_t*, hipStream_t);
-template hipError_t launch_batch_spread_force<float>(const LatDesc*, int, const WorkItem*, const SpreadPoint*,
+template hipError_t launch_batch_spread_force<float>(const LatDesc*, int, const WorkItem*, int, const SpreadPoint*,
                                                      const uint32_t*, const uint32_t*, hipStream_t);
 
 template <typename T>

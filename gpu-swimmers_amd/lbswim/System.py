@@ -35,6 +35,16 @@ class System:
             return None
         return sw, tr
 
+    def Autotune(self, steps_per_trial=5):
+        """Times the native coupled step's launch choices (two launches, fused
+        tile shapes) on real steps -- the system advances by them -- and
+        keeps the fastest; returns (rows, strips), rows 0 for two launches.
+        Systems with non-native things are left as they are (returns None)."""
+        native = self._native_split() if isinstance(self.lat, _core.Lattice) else None
+        if native is None:
+            return None
+        return _core.system_autotune(self.lat, native[0], native[1], steps_per_trial)
+
     def Step(self, nsteps=1):
         native = self._native_split() if isinstance(self.lat, _core.Lattice) else None
         if native is not None:

@@ -100,6 +100,8 @@ cdef extern from "lbswim.h":
     int lbswim_tracers_field(lbswim_tracers* tr, int field, void** host, void** device, size_t* count)
     int lbswim_tracers_h2d(lbswim_tracers* tr, int field)
     int lbswim_tracers_d2h(lbswim_tracers* tr, int field)
+    int lbswim_system_autotune(lbswim_lattice* lat, lbswim_swimmers** sw, int nsw, lbswim_tracers** tr, int ntr,
+                               int steps_per_trial, int* rows, int* strips)
     int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers** sw, int nsw, lbswim_tracers** tr, int ntr,
                            int nsteps)
     ctypedef struct lbswim_ensemble:
@@ -114,6 +116,8 @@ cdef extern from "lbswim.h":
     int lbswim_ensemble_set_graph(lbswim_ensemble* e, int on)
     int lbswim_ensemble_set_order(lbswim_ensemble* e, int order)
     int lbswim_ensemble_set_force_mode(lbswim_ensemble* e, int mode)
+    int lbswim_ensemble_set_work(lbswim_ensemble* e, int blocks_per_item)
+    int lbswim_ensemble_tuning(lbswim_ensemble* e, int* order, int* blocks_per_item)
     int lbswim_ensemble_set_timing(lbswim_ensemble* e, unsigned mask)
     int lbswim_ensemble_kernel_time(lbswim_ensemble* e, int kernel, double* total_ms, long* count)
     int lbswim_ensemble_sites(lbswim_ensemble* e, size_t* virtual_sites, size_t* stored_sites)
@@ -903,6 +907,23 @@ def system_step(Lattice lat, list swimmers, list tracers, int nsteps=1):
     check(lbswim_system_step(lat.handle(), sw, nsw, tr, ntr, nsteps))
 
 
+def system_autotune(Lattice lat, list swimmers, list tracers, int steps_per_trial=5):
+    """Times the two launches and the fused launch's tile shapes on real
+    coupled steps (the system advances by them) and keeps the fastest;
+    returns (rows, strips), rows 0 for the two launches."""
+    cdef int nsw = len(swimmers), ntr = len(tracers), i, rows = 0, strips = 0
+    cdef lbswim_swimmers* sw[64]
+    cdef lbswim_tracers* tr[64]
+    if nsw > 64 or ntr > 64:
+        raise ValueError("at most 64 swimmer and 64 tracer arrays per system")
+    for i in range(nsw):
+        sw[i] = (<SwimmerArray?>swimmers[i]).handle()
+    for i in range(ntr):
+        tr[i] = (<TracerArray?>tracers[i]).handle()
+    check(lbswim_system_autotune(lat.handle(), sw, nsw, tr, ntr, steps_per_trial, &rows, &strips))
+    return rows, strips
+
+
 def device_memory():
     """(free, total) device memory in bytes."""
     cdef size_t f = 0, t = 0
@@ -1021,6 +1042,17 @@ cdef class Ensemble:
         cdef int choice = 0
         check(lbswim_ensemble_autotune(self.impl, steps_per_trial, &choice))
         return choice
+
+    def set_work(self, int blocks_per_item):
+        """Lattice blocks (256 sites) per workgroup of the batched launches."""
+        check(lbswim_ensemble_set_work(self.impl, blocks_per_item))
+
+    @property
+    def tuning(self):
+        """(order, blocks per work item) in use."""
+        cdef int o = 0, p = 0
+        check(lbswim_ensemble_tuning(self.impl, &o, &p))
+        return o, p
 
     def set_timing(self, on):
         """False: no timing (graphs replayed); True or an iterable of phase

@@ -279,6 +279,16 @@ int lbswim_lattice_u_is_virtual(lbswim_lattice* lat, int* is_virtual);
 #define LBSWIM_FUSION_AUTO 1
 #define LBSWIM_FUSION_ON 2
 int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int strips);
+/* Chooses how lbswim_system_step runs this system by timing it: the two
+ * launches, and the fused launch with tiles of 8, 4 and 2 rows and 1, 2 and
+ * 4 times the x strips that give every CU one tile, each over
+ * steps_per_trial (+2 warm-up) real coupled steps -- the system advances by
+ * those steps exactly as lbswim_system_step advances it -- then keeps the
+ * fastest (set as by lbswim_lattice_set_fusion: LBSWIM_FUSION_OFF, or _ON
+ * with those rows and strips).  *rows (0: two launches) and *strips receive
+ * the choice; either may be NULL.  No reference counterpart. */
+int lbswim_system_autotune(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
+                           int ntr, int steps_per_trial, int* rows, int* strips);
 /* *active = 1 if the coupled step of this lattice runs fused; *rows and
  * *strips (either may be NULL) receive the tile shape it uses. */
 int lbswim_lattice_fusion(lbswim_lattice* lat, int* active, int* rows, int* strips);
@@ -311,9 +321,16 @@ int lbswim_ensemble_set_graph(lbswim_ensemble* e, int on);
 int lbswim_ensemble_set_order(lbswim_ensemble* e, int order);
 /* LBSWIM_FORCE_STORED, _AUTO (default) or _VIRTUAL for the batched steps. */
 int lbswim_ensemble_set_force_mode(lbswim_ensemble* e, int mode);
-/* Times each order over steps_per_trial (+2 warm-up) real steps of the
- * ensemble -- the members advance by those steps -- and keeps the fastest. */
+/* Times each workgroup order with work items of 1, 2 and 4 lattice blocks
+ * (256 sites each) over steps_per_trial (+2 warm-up) real steps of the
+ * ensemble -- the members advance by those steps -- and keeps the fastest
+ * pair; *choice receives the order (lbswim_ensemble_tuning gives both). */
 int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice);
+/* Lattice blocks of 256 sites per work item (workgroup) of the batched
+ * lattice launches, 1..64 (default 1). Results do not depend on it. */
+int lbswim_ensemble_set_work(lbswim_ensemble* e, int blocks_per_item);
+/* The order and work-item size in use. */
+int lbswim_ensemble_tuning(lbswim_ensemble* e, int* order, int* blocks_per_item);
 /* Per-phase timing of batched steps with HIP events on the streams the
  * launches go to; while any bit of `mask` (1 << LBSWIM_K_*) is set, steps are
  * launched eagerly instead of from graphs.  Recorded: LBSWIM_K_ADDF (swimmer

@@ -108,10 +108,34 @@ def test_autotune_keeps_results():
         lat, system, sw, tr, things = build_member(c, "f64")
         ens.add(lat, things)
         built.append((lat, sw, tr))
-    choice = ens.autotune(3)          # 3 orders x (2 + 3) real steps
+    choice = ens.autotune(3)          # 3 orders x 3 work-item sizes x (2 + 3) real steps
     assert choice in (0, 1, 2)
+    order, per = ens.tuning
+    assert order == choice and per in (1, 2, 4)
+    assert built[0][0].time_step == 9 * (2 + 3)
     ens.step(5)
-    total = 3 * (2 + 3) + 5
+    total = 9 * (2 + 3) + 5
     for c, (lat, sw, tr) in zip(cases, built):
         assert lat.time_step == total
         assert_same(collect_lbswim(lat, sw, tr), run_alone(c, "f64", total), "after autotune")
+
+
+@pytest.mark.parametrize("per", [3, 8])
+def test_work_items_of_several_blocks(per):
+    """Batched launches whose workgroups each take `per` consecutive lattice
+    blocks of a member (the autotuner's work-item size) give the same
+    results as running each member alone."""
+    from lbswim.ensemble import Ensemble
+    cases = uneven_cases(6, seed=4)
+    say("ensemble with %d blocks per work item" % per)
+    ens = Ensemble("f64")
+    ens.set_work(per)
+    built = []
+    for c in cases:
+        lat, system, sw, tr, things = build_member(c, "f64")
+        ens.add(lat, things)
+        built.append((lat, sw, tr))
+    ens.step(5)
+    assert ens.tuning[1] == per
+    for c, (lat, sw, tr) in zip(cases, built):
+        assert_same(collect_lbswim(lat, sw, tr), run_alone(c, "f64", 5), "per %d" % per)

@@ -303,3 +303,25 @@ def test_fused_at_the_32bit_offset_limit(storage, size):
         del lat, system, sw, tr
     for k in out[0]:
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+@pytest.mark.parametrize("size", [(4, 8, 64), (5, 12, 48)])
+def test_autotune_keeps_results(size):
+    """System.Autotune times the two launches and the fused tile shapes on
+    real coupled steps and keeps the fastest; the system's state after the
+    tuning and further steps equals plain stepping for the same count."""
+    case = swimmer_case(size, seed=size[2] + 3)
+    say("coupled-step autotune: %s" % (size,))
+    lat, system, sw, tr = build_lbswim(case, "f64", "auto")
+    rows, strips = system.Autotune(2)
+    assert rows in (0, 2, 4, 8) and strips >= 0
+    assert lat.fusion[0] == (rows > 0)
+    tuned = lat.time_step
+    assert tuned >= 4 * (2 + 2)  # two launches and at least three tile shapes
+    system.Step(3)
+    got = collect_lbswim(lat, sw, tr)
+    lat2, system2, sw2, tr2 = build_lbswim(case, "f64", "auto")
+    system2.Step(tuned + 3)
+    want = collect_lbswim(lat2, sw2, tr2)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k

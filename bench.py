@@ -75,18 +75,22 @@ def make_workload(size, seed=1234):
     return make_case((size, size, size), 0, 0.25, ns=32768, nt=1_000_000, alpha=0.01, seed=seed)
 
 
-def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, fusion=True):
+def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, fusion=True, tiles=None):
     from cases import build_lbswim
     import lbswim
 
     lat, system, sw, tr = build_lbswim(case, storage, force_mode)
-    if fusion:
+    tuned = None
+    if not fusion:
+        lat.set_fusion(False)
+    elif tiles is not None:
+        # a fixed launch choice (profiling runs: counters distort the tuner's timings)
+        lat.set_fusion(tiles[0] > 0, tiles[0], tiles[1])
+        log("%s: fixed (rows, strips) = %s" % (storage, tiles))
+    else:
         # the autotuner picks two launches or a fused tile shape on real steps
         tuned = system.Autotune(5)
         log("%s: autotuned (rows, strips) = %s" % (storage, tuned))
-    else:
-        lat.set_fusion(False)
-        tuned = None
     fused = lat.fusion[0]
     main = "step_hydro" if fused else "collide_stream"  # the dominant lattice kernel
     system.Step(warmup)
@@ -225,9 +229,11 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True):
     if c_n:
         avg = c_ms / c_n
         byts = vsites * collide_bytes(storage, True)
-        roof = {"kernel": "kb_collide<%s, virtual force>" % storage, "bound": "hbm",
+        kname = "kb_collide<%s, virtual force>" % storage
+        roof = {"kernel": kname, "bound": "hbm",
                 "achieved": round(byts / (avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(byts / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(byts / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic": load_ensemble_traffic(storage, kname) if sites == 6069760 else None,
                 "algorithmic_bytes_per_launch": byts, "avg_launch_ms": round(avg, 4),
                 "calc_hydro_avg_ms": round(h_ms / h_n, 4) if h_n else None,
                 "calc_hydro_gbs": round(vsites * (19 * ELEM[storage]) / (h_ms / h_n * 1e-3) / 1e9, 1) if h_n else None}
@@ -386,6 +392,18 @@ def oracle_legs(case, f64_state, f32_state, f32_small, budget_s=240.0):
     return res
 
 
+def load_ensemble_traffic(storage, kernel):
+    """HBM bytes per launch of the ensemble's batched collide from the
+    committed rocprofv3 --pmc summary of it (null if there is none)."""
+    path = os.path.join(ROOT, "profiles", "kb_collide_%s_ensemble_pmc.json" % storage)
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return d.get("hbm_bytes_per_launch") if d.get("kernel") == kernel else None
+
+
 def load_traffic(kernel):
     """HBM bytes per launch of `kernel` (k_<name>_[vf_]<form>) from the
     committed rocprofv3 --pmc summary of it (null if there is none)."""
@@ -411,6 +429,9 @@ def main():
                     help="swimmer force of the coupled step: virtual where sparse (auto), stored, virtual")
     ap.add_argument("--fusion", default="on", choices=["on", "off"],
                     help="collide/stream and CalcHydro as one launch where the lattice allows (on) or two launches")
+    ap.add_argument("--tiles", default="auto",
+                    help="fused tile shape 'rows,strips' of the coupled step (rows 0: two launches), or auto "
+                         "(autotuned on real steps)")
     ap.add_argument("--probe", choices=["copy"], help="run the streaming-copy bandwidth probe instead")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-sim ensemble leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving leg (0: skip)")
@@ -453,7 +474,8 @@ def main():
     res = {}
     for form in forms:
         log("timing %s: %d warm-up + %d steps" % (form, a.warmup, a.steps))
-        res[form] = time_gpu(case, form, a.steps, a.warmup, a.force_mode, fusion=a.fusion == "on")
+        tiles = None if a.tiles == "auto" else tuple(int(x) for x in a.tiles.split(","))
+        res[form] = time_gpu(case, form, a.steps, a.warmup, a.force_mode, fusion=a.fusion == "on", tiles=tiles)
         log("%s: %.3f ms/step, %.0f MLUPS, %s %.0f GB/s" % (form, res[form]["ms_per_step"], res[form]["mlups"],
                                                             res[form]["main_kernel"], res[form]["main_gbs"]))
     main_form = "f64" if "f64" in res else forms[0]
@@ -518,13 +540,13 @@ def main():
     if "f32" in res and "f64" in res and not a.no_cpu and a.size == 256:
         log("FP32 runs of the 16^3 and 128^3 configurations")
         small32 = small_config_states("f32")
-    if a.ensemble_steps > 0:
+    if "ensemble" in legs and a.ensemble_steps > 0:
         out["ensemble"] = {}
         for form in forms:
             log("ensemble %s: 64 sims x %d steps, batched vs sequential" % (form, a.ensemble_steps))
             out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps, graph=not a.ensemble_eager)
             log("ensemble %s: %s" % (form, out["ensemble"][form]))
-    if a.serve_requests > 0:
+    if "serving" in legs and a.serve_requests > 0:
         log("serving: %d requests at %.1f/s, continuous batching vs sequential" % (a.serve_requests, a.serve_rate))
         out["serving"] = bench_serving(a.serve_requests, a.serve_rate)
         log("serving: %s" % out["serving"])

@@ -180,7 +180,7 @@ def ensemble_cases(nsims=64, seed=1234):
     return out
 
 
-def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True):
+def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True, sequential=True):
     """Batched (one launch per phase, graph replay, autotuned order) against
     the same sims run one after another with the plain per-system step."""
     from cases import build_lbswim
@@ -191,7 +191,7 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True):
     sites = sum(c.size[0] * c.size[1] * c.size[2] for c in cases)
     # sequential plain runs
     t_seq = 0.0
-    for c in cases:
+    for c in (cases if sequential else []):
         lat, system, sw, tr = build_lbswim(c, storage)
         system.Step(2)
         lbswim.synchronize()
@@ -216,32 +216,58 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True):
     ens.step(steps)
     lbswim.synchronize()
     t_ens = time.perf_counter() - t0
-    # roofline of the dominant launch: the batched collide+stream of the
-    # virtual-force members, timed with HIP events in an eager pass
-    ens.set_timing(["collide_stream", "calc_hydro"])
+    # roofline of the dominant lattice launch (the batched fused step of the
+    # members with full 64-site rows, or the batched collide+stream of the
+    # other virtual-force members, whichever covers more sites), each phase
+    # timed with HIP events in an eager pass
+    ens.set_timing(True)
     ens.step(20)
     kt = ens.kernel_times()
     ens.set_timing(False)
-    vsites, _ = ens.sites()
+    vsites, ssites = ens.sites()
+    fsites = ens.fused_sites()
+    phases = {k: round(ms / cnt, 4) for k, (ms, cnt) in kt.items() if cnt}
+    roof = None
+
+    def gbs(byts, ms):
+        return round(byts / (ms * 1e-3) / 1e9, 1)
+
+    f_ms, f_n = kt["step_hydro"]
     c_ms, c_n = kt["collide_stream"]
     h_ms, h_n = kt["calc_hydro"]
-    roof = None
-    if c_n:
+    if f_n and fsites >= vsites:
+        avg = f_ms / f_n
+        byts = fsites * fused_bytes(storage, True)
+        roof = {"kernel": "kb_step_hydro<%s, virtual force>" % storage, "bound": "hbm", "achieved": gbs(byts, avg),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs(byts, avg) / HBM_PEAK_GBS, 4),
+                "traffic": load_ensemble_traffic(storage, "kb_step_hydro<%s, virtual force>" % storage)
+                if sites == 6069760 else None,
+                "algorithmic_bytes_per_launch": byts, "avg_launch_ms": round(avg, 4)}
+        if c_n:
+            roof["collide_avg_ms"] = round(c_ms / c_n, 4)
+            roof["collide_gbs"] = gbs(vsites * collide_bytes(storage, True), c_ms / c_n)
+    elif c_n:
         avg = c_ms / c_n
         byts = vsites * collide_bytes(storage, True)
         kname = "kb_collide<%s, virtual force>" % storage
-        roof = {"kernel": kname, "bound": "hbm",
-                "achieved": round(byts / (avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(byts / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        roof = {"kernel": kname, "bound": "hbm", "achieved": gbs(byts, avg), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs(byts, avg) / HBM_PEAK_GBS, 4),
                 "traffic": load_ensemble_traffic(storage, kname) if sites == 6069760 else None,
                 "algorithmic_bytes_per_launch": byts, "avg_launch_ms": round(avg, 4),
                 "calc_hydro_avg_ms": round(h_ms / h_n, 4) if h_n else None,
-                "calc_hydro_gbs": round(vsites * (19 * ELEM[storage]) / (h_ms / h_n * 1e-3) / 1e9, 1) if h_n else None}
+                "calc_hydro_gbs": gbs(vsites * (19 * ELEM[storage]), h_ms / h_n) if h_n else None}
+    # main-stream phases of one eager step (the stored-force chain runs beside
+    # them on the side stream) against the replayed graphs' time per step
+    main = sum(v for k, v in phases.items() if k != "stored_chain")
     return {"sims": nsims, "steps": steps, "sites": sites, "ms_batched": round(t_ens * 1e3, 2),
-            "ms_sequential": round(t_seq * 1e3, 2), "sims_per_s": round(nsims / t_ens, 2),
-            "sims_per_s_sequential": round(nsims / t_seq, 2), "speedup_vs_sequential": round(t_seq / t_ens, 2),
+            "ms_sequential": round(t_seq * 1e3, 2) if t_seq else None,
+            "sims_per_s": round(nsims / t_ens, 2),
+            "sims_per_s_sequential": round(nsims / t_seq, 2) if t_seq else None,
+            "speedup_vs_sequential": round(t_seq / t_ens, 2) if t_seq else None,
             "mlups": round(sites * steps / t_ens / 1e6, 1), "order": ORDERS[order], "blocks_per_workgroup": per,
-            "graph": graph, "roofline": roof}
+            "fusion_rows_strip": list(ens.fusion), "sites_fused_virtual_stored": [fsites, vsites, ssites],
+            "graph": graph, "roofline": roof, "phases_ms": phases,
+            "graph_ms_per_step": round(t_ens * 1e3 / steps, 4), "main_phases_ms_per_step": round(main, 4)}
 
 
 def bench_serving(nreq, rate, seed=1234):
@@ -437,6 +463,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving leg (0: skip)")
     ap.add_argument("--ensemble-eager", action="store_true",
                     help="launch the ensemble's steps eagerly instead of replaying captured graphs")
+    ap.add_argument("--ensemble-no-sequential", action="store_true",
+                    help="skip the one-after-another baseline of the ensemble leg (profiling runs)")
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--legs", default="main,ensemble,serving",
                     help="legs to run: main (the 256^3 coupled step; required for the JSON line's value), "
@@ -463,7 +491,8 @@ def main():
             out["ensemble"] = {}
             for form in [f for f in a.forms.split(",") if f]:
                 log("ensemble %s: 64 sims x %d steps" % (form, a.ensemble_steps))
-                out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps, graph=not a.ensemble_eager)
+                out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps, graph=not a.ensemble_eager,
+                                                       sequential=not a.ensemble_no_sequential)
         if "serving" in legs and a.serve_requests > 0:
             out["serving"] = bench_serving(a.serve_requests, a.serve_rate)
         print(json.dumps(out), flush=True)
@@ -544,7 +573,8 @@ def main():
         out["ensemble"] = {}
         for form in forms:
             log("ensemble %s: 64 sims x %d steps, batched vs sequential" % (form, a.ensemble_steps))
-            out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps, graph=not a.ensemble_eager)
+            out["ensemble"][form] = bench_ensemble(form, a.ensemble_steps, graph=not a.ensemble_eager,
+                                                       sequential=not a.ensemble_no_sequential)
             log("ensemble %s: %s" % (form, out["ensemble"][form]))
     if "serving" in legs and a.serve_requests > 0:
         log("serving: %d requests at %.1f/s, continuous batching vs sequential" % (a.serve_requests, a.serve_rate))

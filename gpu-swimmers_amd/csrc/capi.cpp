@@ -1675,6 +1675,7 @@ struct EnsMember {
   // lattice buffer role captured in the device tables (a member stepped
   // outside the ensemble may change it; particle buffers never move)
   int cur_at_build = 0;
+  bool fused = false;  // advanced by the batched fused step (u written interleaved)
 };
 
 // Workgroup orders the autotuner chooses between (results do not depend on it).
@@ -1752,6 +1753,27 @@ struct lbswim_ensemble {
   WorkItem* d_trwork = nullptr;
   size_t cap_trwork = 0;
   int n_trwork = 0;
+  // Batched fused step (collide/stream + CalcHydro in one launch) of the
+  // virtual-force members with full tiles: rows per tile (8 or 4, 0 off) and
+  // planes per strip.  Work: tile-strips, 64-list chunks for the RunEntry
+  // records, and their lattice blocks (force written when a call returns).
+  int fuse_rows = 8;
+  int fuse_strip = 32;
+  WorkItem* d_fusework = nullptr;
+  size_t cap_fusework = 0;
+  int n_fusework = 0;
+  WorkItem* d_entrywork = nullptr;
+  size_t cap_entrywork = 0;
+  int n_entrywork = 0;
+  WorkItem* d_fuseblk = nullptr;
+  size_t cap_fuseblk = 0;
+  int n_fuseblk = 0;
+  RunEntry* rec = nullptr;
+  int* wz = nullptr;
+  size_t cap_rec = 0;
+  double* edge = nullptr;
+  size_t cap_edge = 0;
+  size_t sites_fused = 0;
   int nlists = 0;  // spreading lists of all members
   // spreading workspace
   int npts = 0;
@@ -1835,9 +1857,12 @@ int ens_build(lbswim_ensemble* e) {
   HIP_TRY(hipStreamSynchronize(stream()));
   ens_drop_graph(e);
   std::vector<LatDesc> lat;
-  // per member: its lattice blocks, in the stored- (0) or virtual-force (1) group
-  std::vector<std::vector<WorkItem>> latwork[2];
-  std::vector<size_t> weight[2];
+  // per member: its lattice blocks, in the stored-force (0), virtual-force
+  // two-launch (1) or virtual-force fused (2) group
+  std::vector<std::vector<WorkItem>> latwork[3];
+  std::vector<size_t> weight[3];
+  std::vector<std::vector<WorkItem>> fusetiles;  // per fused member: its tile-strips
+  std::vector<WorkItem> entrywork;
   std::vector<SwimDesc> sw;
   std::vector<WorkItem> swwork, ptwork, trwork;
   std::vector<TracerDesc> tr;
@@ -1866,12 +1891,26 @@ int ens_build(lbswim_ensemble* e) {
     if (d.force_path != FORCE_RANGE_STORED) tile_base += spread_lists(L->geom);
     d.point_first = point_base;
     d.point_count = uint32_t(member_pts);
+    const bool fused = e->fuse_rows > 0 && d.force_path == FORCE_LISTS_VIRTUAL &&
+                       batch_fused_member(L->geom, e->fuse_rows, L->elem);
+    if (fused && !L->d_u_aos)
+      if (int rc = dev_alloc_zero(&L->d_u_aos, 3 * L->elem * L->n)) return rc;
+    d.u_aos = fused ? L->d_u_aos : nullptr;
+    m.fused = fused;
     lat.push_back(d);
     std::vector<WorkItem> lw;
     for (uint32_t b = 0; b < lattice_blocks(L->geom); b += uint32_t(e->per)) lw.push_back(WorkItem{mi, b});
-    const int group = d.force_path == FORCE_LISTS_VIRTUAL ? 1 : 0;
+    const int group = d.force_path != FORCE_LISTS_VIRTUAL ? 0 : (fused ? 2 : 1);
     latwork[group].push_back(lw);
     weight[group].push_back(L->n);
+    if (fused) {
+      const uint32_t tiles = uint32_t(L->geom.ny / e->fuse_rows) * uint32_t(L->geom.nz / 64);
+      const uint32_t strips = uint32_t((L->geom.nx + e->fuse_strip - 1) / e->fuse_strip);
+      std::vector<WorkItem> fw;
+      for (uint32_t u = 0; u < tiles * strips; ++u) fw.push_back(WorkItem{mi, u});
+      fusetiles.push_back(fw);
+      for (uint32_t u = 0; 64 * u < spread_lists(L->geom); ++u) entrywork.push_back(WorkItem{mi, u});
+    }
     for (lbswim_swimmers* s : m.sw) {
       if (int rc = pstore_maybe_reorder(s->st, L->geom)) return rc;
       const uint32_t ai = uint32_t(sw.size());
@@ -1912,25 +1951,35 @@ int ens_build(lbswim_ensemble* e) {
   }
   const std::vector<WorkItem> storedwork = order_work(latwork[0], e->order, weight[0]);
   const std::vector<WorkItem> vfwork = order_work(latwork[1], e->order, weight[1]);
+  const std::vector<WorkItem> fuseblk = order_work(latwork[2], e->order, weight[2]);
+  const std::vector<WorkItem> fusework = order_work(fusetiles, e->order, weight[2]);
   int rc;
   if ((rc = upload(&e->d_lat, &e->cap_lat, lat)) || (rc = upload(&e->d_storedwork, &e->cap_storedwork, storedwork)) ||
       (rc = upload(&e->d_vfwork, &e->cap_vfwork, vfwork)) || (rc = upload(&e->d_sw, &e->cap_sw, sw)) ||
       (rc = upload(&e->d_swwork, &e->cap_swwork, swwork)) || (rc = upload(&e->d_ptwork, &e->cap_ptwork, ptwork)) ||
-      (rc = upload(&e->d_tr, &e->cap_tr, tr)) || (rc = upload(&e->d_trwork, &e->cap_trwork, trwork)))
+      (rc = upload(&e->d_tr, &e->cap_tr, tr)) || (rc = upload(&e->d_trwork, &e->cap_trwork, trwork)) ||
+      (rc = upload(&e->d_fusework, &e->cap_fusework, fusework)) ||
+      (rc = upload(&e->d_entrywork, &e->cap_entrywork, entrywork)) ||
+      (rc = upload(&e->d_fuseblk, &e->cap_fuseblk, fuseblk)))
     return rc;
   e->n_storedwork = int(storedwork.size());
   e->n_vfwork = int(vfwork.size());
+  e->n_fusework = int(fusework.size());
+  e->n_entrywork = int(entrywork.size());
+  e->n_fuseblk = int(fuseblk.size());
   // aggregated atomics when any member with lists has dense ones
   e->dense_lists = false;
   for (size_t i = 0; i < lat.size(); ++i) {
     if (lat[i].force_path == FORCE_RANGE_STORED) continue;
     e->dense_lists = e->dense_lists || dense_lists(lat[i].point_count, spread_lists(lat[i].geom));
   }
-  e->sites_virtual = e->sites_stored = 0;
-  for (const LatDesc& d : lat) (d.force_path == FORCE_LISTS_VIRTUAL ? e->sites_virtual : e->sites_stored) += d.geom.n;
+  e->sites_virtual = e->sites_stored = e->sites_fused = 0;
+  for (const LatDesc& d : lat)
+    (d.force_path != FORCE_LISTS_VIRTUAL ? e->sites_stored : (d.u_aos ? e->sites_fused : e->sites_virtual)) +=
+        d.geom.n;
   e->stored_lists = false;
   for (const LatDesc& d : lat) e->stored_lists = e->stored_lists || d.force_path == FORCE_LISTS_STORED;
-  e->fork_stored = !storedwork.empty() && !vfwork.empty();
+  e->fork_stored = !storedwork.empty() && !(vfwork.empty() && fusework.empty());
   if (e->fork_stored) {
     if (int r = ensure_side_stream()) return r;
     if (!e->ev_fork) HIP_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
@@ -1962,6 +2011,12 @@ int ens_build(lbswim_ensemble* e) {
       (rc = grow_to(&e->slot_ids, &e->cap_slot, size_t(e->npts), 4)) ||
       (rc = grow_to(&e->sorted, &cap_sorted, lcap, 4)) || (rc = grow_to(&e->list, &e->cap_list, lcap, 4)))
     return rc;
+  if (!fusework.empty()) {  // RunEntry records of every list entry, run-end forces of every list
+    size_t cap_wz = e->cap_rec;
+    if ((rc = grow_to(&e->rec, &e->cap_rec, lcap, sizeof(RunEntry))) || (rc = grow_to(&e->wz, &cap_wz, lcap, 4)) ||
+        (rc = grow_to(&e->edge, &e->cap_edge, 6 * nt, 8)))
+      return rc;
+  }
   HIP_TRY(hipStreamSynchronize(stream()));
   e->parity = 0;
   e->dirty = false;
@@ -2003,6 +2058,8 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
     if (int rc = stored_chain()) return rc;
   HIP_TRY(launch_batch_lists(e->d_lat, e->npts, e->nlists, e->pts, e->slot_ids, e->dense_lists, e->counts, e->off,
                              e->cursor, e->list, e->sorted, s));
+  HIP_TRY(launch_batch_run_entries(e->d_lat, e->d_entrywork, e->n_entrywork, e->pts, e->off, e->sorted, e->rec, e->wz,
+                                   e->edge, s));
   t_lists.done();
   if (!early) {
     if (fork)
@@ -2010,6 +2067,13 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
     if (int rc = stored_chain()) return rc;
   }
   const VForce vf{e->pts, e->off, e->sorted};
+  if (e->n_fusework) {
+    EnsTimer t(e, LBSWIM_K_FUSED, s);
+    const VForce fvf{e->pts, e->off, e->sorted, e->edge, e->rec, e->wz};
+    HIP_TRY(launch_batch_step_hydro<T>(e->d_lat, e->d_fusework, e->n_fusework, parity, fvf, e->fuse_rows,
+                                       e->fuse_strip, s));
+    t.done();
+  }
   {
     EnsTimer t(e, LBSWIM_K_COLLIDE, s);
     HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_vfwork, e->n_vfwork, e->per, parity, &vf, s));
@@ -2042,12 +2106,14 @@ int ens_enqueue(lbswim_ensemble* e, int parity) {
 // After batched steps with the virtual force: write every member's force
 // array from the last step's lists, as the reference leaves it.
 int ens_materialize_force(lbswim_ensemble* e) {
-  hipError_t r = e->storage == LBSWIM_STORE_F64
-                     ? launch_batch_spread_force<double>(e->d_lat, e->n_vfwork, e->d_vfwork, e->per, e->pts,
-                                                         e->off, e->sorted, stream())
-                     : launch_batch_spread_force<float>(e->d_lat, e->n_vfwork, e->d_vfwork, e->per, e->pts, e->off,
-                                                        e->sorted, stream());
-  HIP_TRY(r);
+  for (int k = 0; k < 2; ++k) {
+    const int nw = k ? e->n_fuseblk : e->n_vfwork;
+    const WorkItem* w = k ? e->d_fuseblk : e->d_vfwork;
+    hipError_t r = e->storage == LBSWIM_STORE_F64
+                       ? launch_batch_spread_force<double>(e->d_lat, nw, w, e->per, e->pts, e->off, e->sorted, stream())
+                       : launch_batch_spread_force<float>(e->d_lat, nw, w, e->per, e->pts, e->off, e->sorted, stream());
+    HIP_TRY(r);
+  }
   return LBSWIM_OK;
 }
 
@@ -2081,8 +2147,9 @@ void ens_advance(lbswim_ensemble* e, int k) {
     if (k & 1) m.lat->cur ^= 1;
     m.lat->time_step += k;
     m.lat->force_virtual = false;  // the batched step spreads into the force array
-    m.lat->u_aos_valid = false;    // and writes u only in the reference layout
-    m.lat->u_virtual = false;
+    // and writes u in the reference layout, or (fused) only interleaved
+    m.lat->u_aos_valid = m.fused;
+    m.lat->u_virtual = m.fused;
     for (auto* s : m.sw) s->st.moves_since_order += k;
     for (auto* t : m.tr) t->st.moves_since_order += k;
   }
@@ -2137,6 +2204,12 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   dev_free(e->sorted);
   dev_free(e->pts);
   dev_free(e->slot_ids);
+  dev_free(e->d_fusework);
+  dev_free(e->d_entrywork);
+  dev_free(e->d_fuseblk);
+  dev_free(e->rec);
+  dev_free(e->wz);
+  dev_free(e->edge);
   delete e;
 }
 
@@ -2303,12 +2376,53 @@ int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choic
     }
     if (rc) break;
   }
+  // then the fused step's tile shape with that order and work-item size:
+  // off, or tiles of 8 rows with strips of 16, 32, 64 planes, or of 4 rows
+  // with strips of 32 (only where some member qualifies)
+  if (!rc) {
+    e->order = best_order;
+    e->per = best_per;
+    e->dirty = true;  // the tables on the device are the last trial's
+    bool any = false;
+    for (auto& m : e->members) {
+      if (!m.lat) continue;
+      size_t npts = 0;
+      for (lbswim_swimmers* sw : m.sw) npts += 2 * size_t(sw->st.num);
+      any = any || (force_path(e->force_mode, npts, m.lat->geom) == FORCE_LISTS_VIRTUAL &&
+                    batch_fused_member(m.lat->geom, 4, m.lat->elem));
+    }
+    if (any) {
+      const int cand[5][2] = {{0, 32}, {8, 16}, {8, 32}, {8, 64}, {4, 32}};
+      float fbest = 0.f;
+      int brows = e->fuse_rows, bstrip = e->fuse_strip;
+      for (int i = 0; i < 5 && !rc; ++i) {
+        e->fuse_rows = cand[i][0];
+        e->fuse_strip = cand[i][1];
+        e->dirty = true;
+        if ((rc = lbswim_ensemble_step(e, 2))) break;
+        (void)hipEventRecord(a, stream());
+        if ((rc = lbswim_ensemble_step(e, steps_per_trial))) break;
+        (void)hipEventRecord(b, stream());
+        (void)hipEventSynchronize(b);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (i == 0 || ms < fbest) {
+          fbest = ms;
+          brows = cand[i][0];
+          bstrip = cand[i][1];
+        }
+      }
+      e->fuse_rows = brows;
+      e->fuse_strip = bstrip;
+      e->dirty = true;
+    }
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc) return rc;
-  if (e->order != best_order || e->per != best_per) e->dirty = true;
   e->order = best_order;
   e->per = best_per;
+  e->dirty = true;  // built for the last trial
   if (choice) *choice = best_order;
   return LBSWIM_OK;
 }
@@ -2324,6 +2438,31 @@ int lbswim_ensemble_tuning(lbswim_ensemble* e, int* order, int* blocks_per_item)
   CHECK_ARG(e, "ensemble is NULL");
   if (order) *order = e->order;
   if (blocks_per_item) *blocks_per_item = e->per;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_set_fusion(lbswim_ensemble* e, int rows, int strip) {
+  CHECK_ARG(e, "ensemble is NULL");
+  CHECK_ARG(rows == 0 || rows == 4 || rows == 8, "rows must be 0, 4 or 8 (got %d)", rows);
+  CHECK_ARG(strip >= 1, "strip must be >= 1 plane (got %d)", strip);
+  if (e->fuse_rows != rows || e->fuse_strip != strip) e->dirty = true;
+  e->fuse_rows = rows;
+  e->fuse_strip = strip;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_fusion(lbswim_ensemble* e, int* rows, int* strip) {
+  CHECK_ARG(e, "ensemble is NULL");
+  if (rows) *rows = e->fuse_rows;
+  if (strip) *strip = e->fuse_strip;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_fused_sites(lbswim_ensemble* e, size_t* fused_sites) {
+  CHECK_ARG(e && fused_sites, "NULL argument");
+  if (e->dirty)
+    if (int rc = ens_build(e)) return rc;
+  *fused_sites = e->sites_fused;
   return LBSWIM_OK;
 }
 

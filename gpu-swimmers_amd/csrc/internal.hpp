@@ -168,6 +168,10 @@ struct LatDesc {
   // (written from its lists to the force array first), FORCE_RANGE_STORED
   // (written from all its points in id order, no lists)
   int force_path;
+  // the interleaved u of a member the batched fused step advances (it then
+  // writes u only there, and the batched moves gather from it); null for the
+  // others
+  void* u_aos;
 };
 constexpr int FORCE_LISTS_VIRTUAL = 0;
 constexpr int FORCE_LISTS_STORED = 1;
@@ -215,6 +219,24 @@ hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int n
 template <typename T>
 hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int per, int parity,
                               const VForce* vf, hipStream_t s);
+// Members the batched fused step can advance with `rows`-row tiles (8 or 4):
+// full tiles (nz % 64 == 0, rows dividing ny), nx and ny >= 4 (the row walk
+// over RunEntry records), offsets below 4 GiB.
+bool batch_fused_member(const Geom& g, int rows, size_t elem);
+// Collide/stream + CalcHydro of those members in one launch, bit for bit the
+// two batched launches; each work item is one tile-strip (k_step_hydro's
+// index within its member) with strips of `strip` planes.  vf holds the
+// batch's point records, list bounds and sorted ids and the RunEntry records
+// and run-end forces of launch_batch_run_entries.
+template <typename T>
+hipError_t launch_batch_step_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce& vf,
+                                   int rows, int strip, hipStream_t s);
+// RunEntry records, wz and run-end forces (launch_edge_force's outputs) of
+// the lists of the members in `work`: each work item is 64 consecutive lists
+// (unit 64 l ...) of its member.
+hipError_t launch_batch_run_entries(const LatDesc* ldesc, const WorkItem* work, int nwork, const SpreadPoint* pts,
+                                    const uint32_t* off, const uint32_t* sorted, RunEntry* rec, int* wz, double* edge,
+                                    hipStream_t s);
 // Point records of a batch: points of array a get ids point_base_a + 2*user +
 // end; work items are blocks of 128 swimmers (256 points) of an array.
 hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,

@@ -896,12 +896,14 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
   }
 }
 
+template <typename T, int TY>
+constexpr int fused_lds_words() { return fused_slots<T>() * 5 * TY * FUSED_ROW + 64; }
+
+// Tile-strip L of the sweep (z tiles fastest, then y tiles, then strips).
 template <typename T, bool VF, int TY, bool PARTIAL>
-__global__ __launch_bounds__((TY + 3) * 64) void k_step_hydro(FusedArgs<T> A, FusedTiles t) {
+__device__ __forceinline__ void step_hydro_tile(const FusedArgs<T>& A, const FusedTiles& t, uint32_t L,
+                                                T* __restrict__ lds) {
   constexpr int ROWS = TY + 2;
-  __shared__ T lds[fused_slots<T>() * 5 * TY * FUSED_ROW + 64];
-  const uint32_t L = xcd_block(round_xcd(t.nblk));
-  if (L >= t.nblk) return;
   const Geom& g = A.g;
   // Tiles along z, then along y, are adjacent in L, so one XCD sweeps whole
   // z-rings of neighbouring tiles together: the lines a 64-site run's z-shifted
@@ -935,6 +937,52 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_step_hydro(FusedArgs<T> A, Fu
     fused_sweep<T, VF, TY, ROLE_BELOW, PARTIAL>(A, lds, xa, last, wv, zz, active, y, z, W);
   else
     fused_sweep<T, VF, TY, ROLE_TILE, PARTIAL>(A, lds, xa, last, wv, zz, active, y, z, W);
+}
+
+template <typename T, bool VF, int TY, bool PARTIAL>
+__global__ __launch_bounds__((TY + 3) * 64) void k_step_hydro(FusedArgs<T> A, FusedTiles t) {
+  __shared__ T lds[fused_lds_words<T, TY>()];
+  const uint32_t L = xcd_block(round_xcd(t.nblk));
+  if (L >= t.nblk) return;
+  step_hydro_tile<T, VF, TY, PARTIAL>(A, t, L, lds);
+}
+
+// Batched fused step over members with full tiles (nz % 64 == 0, TY dividing
+// ny, nx and ny >= 4) and a virtual force: WorkItem.unit is k_step_hydro's
+// tile-strip index within member `member`, whose x sweep is split into
+// strips of `strip` planes.  The member's lists start at its tile_base (the
+// list bounds and run-end forces are offset by it; the entries, point
+// records and point ids are numbered across the batch).  State in g[cur ^
+// parity], as kb_collide.
+template <typename T, int TY>
+__global__ __launch_bounds__((TY + 3) * 64) void kb_step_hydro(const LatDesc* __restrict__ desc,
+                                                               const WorkItem* __restrict__ work, uint32_t nwork,
+                                                               int parity, VForce vf, int strip) {
+  __shared__ T lds[fused_lds_words<T, TY>()];
+  const uint32_t wi = xcd_block(round_xcd(nwork));
+  if (wi >= nwork) return;
+  const WorkItem w = work[wi];
+  const LatDesc& m = desc[w.member];
+  const int c = m.cur ^ parity;
+  FusedArgs<T> A;
+  A.gOld = static_cast<const T*>(m.g[c]);
+  A.force = nullptr;
+  A.vf = VForce{vf.pts, vf.off + m.tile_base, vf.sorted, vf.edge + 6 * size_t(m.tile_base), vf.rec, vf.wz};
+  A.gNew = static_cast<T*>(m.g[c ^ 1]);
+  A.rho = static_cast<T*>(m.rho);
+  A.u_aos = static_cast<T*>(m.u_aos);
+  A.g = m.geom;
+  A.tau_s = T(m.tau_s);
+  A.tau_b = T(m.tau_b);
+  A.omega_s = T(m.omega_s);
+  A.omega_b = T(m.omega_b);
+  FusedTiles t;
+  t.nyt = uint32_t(m.geom.ny / TY);
+  t.nzt = uint32_t(m.geom.nz / FUSED_Z);
+  t.strip = uint32_t(strip);
+  t.width = FUSED_Z;
+  t.nblk = 0;  // not read by the tile sweep
+  step_hydro_tile<T, true, TY, false>(A, t, w.unit, lds);
 }
 
 // ------------------------------------------------------------------------
@@ -1276,6 +1324,32 @@ hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwo
     hipLaunchKernelGGL((kb_hydro<T, false>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, VForce{}, per);
   return hipGetLastError();
 }
+template <typename T>
+hipError_t launch_batch_step_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int parity, const VForce& vf,
+                                   int rows, int strip, hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  if (strip < 1 || !(vf.edge && vf.rec && vf.wz)) return hipErrorInvalidValue;
+  const dim3 grid(round_xcd(uint32_t(nwork)));
+  if (rows == 8)
+    hipLaunchKernelGGL((kb_step_hydro<T, 8>), grid, dim3((8 + 3) * 64), 0, s, desc, work, uint32_t(nwork), parity,
+                       vf, strip);
+  else if (rows == 4)
+    hipLaunchKernelGGL((kb_step_hydro<T, 4>), grid, dim3((4 + 3) * 64), 0, s, desc, work, uint32_t(nwork), parity,
+                       vf, strip);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+template hipError_t launch_batch_step_hydro<double>(const LatDesc*, const WorkItem*, int, int, const VForce&, int,
+                                                    int, hipStream_t);
+template hipError_t launch_batch_step_hydro<float>(const LatDesc*, const WorkItem*, int, int, const VForce&, int,
+                                                   int, hipStream_t);
+
+bool batch_fused_member(const Geom& g, int rows, size_t elem) {
+  return (rows == 8 || rows == 4) && g.nx >= 4 && g.ny >= 4 && g.nz % FUSED_Z == 0 && g.ny % rows == 0 &&
+         fused_step_supported(g, rows, elem);
+}
+
 template hipError_t launch_batch_collide<double>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,
                                                  hipStream_t);
 template hipError_t launch_batch_collide<float>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,

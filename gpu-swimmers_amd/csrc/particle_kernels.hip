@@ -654,8 +654,12 @@ __global__ __launch_bounds__(PBLOCK) void kb_swimmers_move(const LatDesc* __rest
   const WorkItem w = work[wi];
   const SwimDesc& a = sdesc[w.member];
   const LatDesc& m = ldesc[a.member];
-  swimmers_move_body(a.num, a.p, a.r, a.v, a.n, a.prng, a.perm, static_cast<const T*>(m.u), m.geom, a.tumbles,
-                     w.unit, lds);
+  if (m.u_aos)  // the batched fused step wrote u only component-interleaved
+    swimmers_move_body<T, true>(a.num, a.p, a.r, a.v, a.n, a.prng, a.perm, static_cast<const T*>(m.u_aos), m.geom,
+                                a.tumbles, w.unit, lds);
+  else
+    swimmers_move_body<T, false>(a.num, a.p, a.r, a.v, a.n, a.prng, a.perm, static_cast<const T*>(m.u), m.geom,
+                                 a.tumbles, w.unit, lds);
 }
 
 template <typename T>
@@ -668,7 +672,10 @@ __global__ __launch_bounds__(TBLOCK) void kb_tracers_move(const LatDesc* __restr
   const WorkItem w = work[wi];
   const TracerDesc& a = tdesc[w.member];
   const LatDesc& m = ldesc[a.member];
-  tracers_move_body(a.num, a.r, a.s, a.v, static_cast<const T*>(m.u), m.geom, w.unit, lds);
+  if (m.u_aos)
+    tracers_move_body<T, true>(a.num, a.r, a.s, a.v, static_cast<const T*>(m.u_aos), m.geom, w.unit, lds);
+  else
+    tracers_move_body<T, false>(a.num, a.r, a.s, a.v, static_cast<const T*>(m.u), m.geom, w.unit, lds);
 }
 
 // ---- deterministic force spreading -----------------------------------------
@@ -1122,14 +1129,12 @@ __global__ __launch_bounds__(SITE_BLOCK) void k_edge_force(Geom g, uint32_t nlis
 // lane by lane in list order and added, with add_point's products, by the
 // lane that holds that list (lane j: list l0 + j).
 constexpr uint32_t ENTRY_LISTS = 16;
-__global__ __launch_bounds__(SITE_BLOCK) void k_run_entries(Geom g, uint32_t nlists,
-                                                           const SpreadPoint* __restrict__ pts,
-                                                           const uint32_t* __restrict__ off,
-                                                           const uint32_t* __restrict__ sorted,
-                                                           RunEntry* __restrict__ rec, int* __restrict__ wz,
-                                                           double* __restrict__ edge) {
-  const uint32_t l0 = (blockIdx.x * SITE_BLOCK + threadIdx.x) / WAVE_SITES * ENTRY_LISTS;
-  if (l0 >= nlists) return;  // whole wavefront
+// One wavefront's ENTRY_LISTS lists from l0 (< nlists).
+__device__ __forceinline__ void run_entries_wave(const Geom& g, uint32_t nlists, uint32_t l0,
+                                                 const SpreadPoint* __restrict__ pts,
+                                                 const uint32_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ sorted, RunEntry* __restrict__ rec,
+                                                 int* __restrict__ wz, double* __restrict__ edge) {
   const int lane = int(threadIdx.x % WAVE_SITES);
   // lane j <= ENTRY_LISTS: first entry of list l0 + j (the end for j = ENTRY_LISTS)
   const uint32_t oj = off[min(l0 + min(uint32_t(lane), ENTRY_LISTS), nlists)];
@@ -1202,6 +1207,37 @@ __global__ __launch_bounds__(SITE_BLOCK) void k_run_entries(Geom g, uint32_t nli
       out[3 + a] = el[a];
     }
   }
+}
+
+__global__ __launch_bounds__(SITE_BLOCK) void k_run_entries(Geom g, uint32_t nlists,
+                                                           const SpreadPoint* __restrict__ pts,
+                                                           const uint32_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ sorted,
+                                                           RunEntry* __restrict__ rec, int* __restrict__ wz,
+                                                           double* __restrict__ edge) {
+  const uint32_t l0 = (blockIdx.x * SITE_BLOCK + threadIdx.x) / WAVE_SITES * ENTRY_LISTS;
+  if (l0 >= nlists) return;  // whole wavefront
+  run_entries_wave(g, nlists, l0, pts, off, sorted, rec, wz, edge);
+}
+
+// Batched: work item (member, unit) takes the member's lists 64 unit ...
+// (ENTRY_LISTS per wavefront); list bounds and run-end forces are offset by
+// the member's tile_base, entries and point ids are numbered across the batch.
+constexpr uint32_t BATCH_ENTRY_LISTS = SITE_BLOCK / WAVE_SITES * ENTRY_LISTS;
+static_assert(BATCH_ENTRY_LISTS == 64, "lists per work item of kb_run_entries");
+__global__ __launch_bounds__(SITE_BLOCK) void kb_run_entries(const LatDesc* __restrict__ ldesc,
+                                                            const WorkItem* __restrict__ work,
+                                                            const SpreadPoint* __restrict__ pts,
+                                                            const uint32_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ sorted,
+                                                            RunEntry* __restrict__ rec, int* __restrict__ wz,
+                                                            double* __restrict__ edge) {
+  const WorkItem w = work[blockIdx.x];
+  const LatDesc& m = ldesc[w.member];
+  const uint32_t nlists = 4 * ((m.geom.n + SITE_BLOCK - 1) / SITE_BLOCK);
+  const uint32_t l0 = w.unit * BATCH_ENTRY_LISTS + threadIdx.x / WAVE_SITES * ENTRY_LISTS;
+  if (l0 >= nlists) return;  // whole wavefront
+  run_entries_wave(m.geom, nlists, l0, pts, off + m.tile_base, sorted, rec, wz, edge + 6 * size_t(m.tile_base));
 }
 
 template <typename T>
@@ -1411,6 +1447,15 @@ hipError_t launch_edge_force(const Geom& g, const SpreadPoint* pts, const uint32
     hipLaunchKernelGGL(k_run_entries, dim3((waves + per_block - 1) / per_block), dim3(SITE_BLOCK), 0, s, g, nl, pts,
                        off, sorted, rec, wz, merged ? edge : nullptr);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_run_entries(const LatDesc* ldesc, const WorkItem* work, int nwork, const SpreadPoint* pts,
+                                    const uint32_t* off, const uint32_t* sorted, RunEntry* rec, int* wz, double* edge,
+                                    hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_run_entries, dim3(nwork), dim3(SITE_BLOCK), 0, s, ldesc, work, pts, off, sorted, rec, wz,
+                     edge);
   return hipGetLastError();
 }
 

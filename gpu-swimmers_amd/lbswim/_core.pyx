@@ -122,6 +122,9 @@ cdef extern from "lbswim.h":
     int lbswim_ensemble_kernel_time(lbswim_ensemble* e, int kernel, double* total_ms, long* count)
     int lbswim_ensemble_sites(lbswim_ensemble* e, size_t* virtual_sites, size_t* stored_sites)
     int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice)
+    int lbswim_ensemble_set_fusion(lbswim_ensemble* e, int rows, int strip)
+    int lbswim_ensemble_fusion(lbswim_ensemble* e, int* rows, int* strip)
+    int lbswim_ensemble_fused_sites(lbswim_ensemble* e, size_t* fused_sites)
     int lbswim_lattice_totals(lbswim_lattice* lat, double* out)
     int lbswim_lattice_set_timing(lbswim_lattice* lat, int on)
     int lbswim_lattice_set_timing_mask(lbswim_lattice* lat, unsigned mask)
@@ -1054,11 +1057,30 @@ cdef class Ensemble:
         check(lbswim_ensemble_tuning(self.impl, &o, &p))
         return o, p
 
+    def set_fusion(self, int rows, int strip=32):
+        """Batched fused step for the virtual-force members with full tiles:
+        tiles of `rows` (8 or 4; 0 never) lattice rows x 64 sites, strips of
+        `strip` planes.  Results do not depend on it."""
+        check(lbswim_ensemble_set_fusion(self.impl, rows, strip))
+
+    @property
+    def fusion(self):
+        """(rows, strip) of the batched fused step (rows 0: off)."""
+        cdef int r = 0, st = 0
+        check(lbswim_ensemble_fusion(self.impl, &r, &st))
+        return r, st
+
+    def fused_sites(self):
+        """Lattice sites of the members the batched fused step advances."""
+        cdef size_t f = 0
+        check(lbswim_ensemble_fused_sites(self.impl, &f))
+        return f
+
     def set_timing(self, on):
         """False: no timing (graphs replayed); True or an iterable of phase
-        names from KERNELS ("add_forces", "collide_stream", "calc_hydro",
-        "stored_chain", "swimmers_move", "tracers_move"): steps launched
-        eagerly with HIP events around those phases."""
+        names from KERNELS ("add_forces", "step_hydro", "collide_stream",
+        "calc_hydro", "stored_chain", "swimmers_move", "tracers_move"): steps
+        launched eagerly with HIP events around those phases."""
         cdef unsigned mask = 0
         if on is True:
             mask = (1 << len(KERNELS)) - 1
@@ -1078,7 +1100,9 @@ cdef class Ensemble:
         return out
 
     def sites(self):
-        """(sites of virtual-force members, sites of stored-force members)."""
+        """(sites of the virtual-force members the batched collide and
+        CalcHydro launches advance, sites of stored-force members); the
+        fused members' sites: fused_sites()."""
         cdef size_t v = 0, st = 0
         check(lbswim_ensemble_sites(self.impl, &v, &st))
         return v, st

@@ -324,24 +324,40 @@ int lbswim_ensemble_set_force_mode(lbswim_ensemble* e, int mode);
 /* Times each workgroup order with work items of 1, 2 and 4 lattice blocks
  * (256 sites each) over steps_per_trial (+2 warm-up) real steps of the
  * ensemble -- the members advance by those steps -- and keeps the fastest
- * pair; *choice receives the order (lbswim_ensemble_tuning gives both). */
+ * pair, then, with it, the batched fused step's tile shapes
+ * (lbswim_ensemble_set_fusion) where some member qualifies; *choice receives
+ * the order (lbswim_ensemble_tuning and lbswim_ensemble_fusion give the rest). */
 int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice);
 /* Lattice blocks of 256 sites per work item (workgroup) of the batched
  * lattice launches, 1..64 (default 1). Results do not depend on it. */
 int lbswim_ensemble_set_work(lbswim_ensemble* e, int blocks_per_item);
 /* The order and work-item size in use. */
 int lbswim_ensemble_tuning(lbswim_ensemble* e, int* order, int* blocks_per_item);
+/* Batched fused step: members whose force is virtual, with nz % 64 == 0,
+ * rows dividing ny and nx, ny >= 4, advance by one launch of collide/stream
+ * + CalcHydro over tiles of `rows` (8 or 4; 0: never) lattice rows x 64
+ * sites, their x sweep split into strips of `strip` planes (default 8, 32).
+ * Their u is then written component-interleaved and the reference layout is
+ * formed on demand (as a single system's fused step).  Results do not
+ * depend on it.  lbswim_ensemble_autotune also chooses among off, 8 rows x
+ * 16, 32, 64 planes and 4 rows x 32 planes. */
+int lbswim_ensemble_set_fusion(lbswim_ensemble* e, int rows, int strip);
+int lbswim_ensemble_fusion(lbswim_ensemble* e, int* rows, int* strip);
+/* Lattice sites of the members the batched fused step advances. */
+int lbswim_ensemble_fused_sites(lbswim_ensemble* e, size_t* fused_sites);
 /* Per-phase timing of batched steps with HIP events on the streams the
  * launches go to; while any bit of `mask` (1 << LBSWIM_K_*) is set, steps are
  * launched eagerly instead of from graphs.  Recorded: LBSWIM_K_ADDF (swimmer
- * points and lists), _COLLIDE and _HYDRO (the virtual-force members' batched
- * launches), _STORED (the stored-force members' chain), _SMOVE, _TMOVE.  A
- * nonzero mask clears earlier records. */
+ * points and lists), _FUSED (the batched fused step), _COLLIDE and _HYDRO
+ * (the other virtual-force members' batched launches), _STORED (the
+ * stored-force members' chain), _SMOVE, _TMOVE.  A nonzero mask clears
+ * earlier records. */
 int lbswim_ensemble_set_timing(lbswim_ensemble* e, unsigned mask);
 /* Sum of recorded durations (ms) and launch count of one phase. */
 int lbswim_ensemble_kernel_time(lbswim_ensemble* e, int kernel, double* total_ms, long* count);
-/* Lattice sites of the members whose swimmer force is virtual (covered by
- * the batched _COLLIDE/_HYDRO launches) and of the others. */
+/* Lattice sites of the members whose swimmer force is virtual and that the
+ * batched _COLLIDE/_HYDRO launches advance, and of the stored-force members
+ * (the fused members: lbswim_ensemble_fused_sites). */
 int lbswim_ensemble_sites(lbswim_ensemble* e, size_t* virtual_sites, size_t* stored_sites);
 
 /* ---- diagnostics (no reference counterpart) ---------------------------- */

@@ -139,3 +139,42 @@ def test_work_items_of_several_blocks(per):
     assert ens.tuning[1] == per
     for c, (lat, sw, tr) in zip(cases, built):
         assert_same(collect_lbswim(lat, sw, tr), run_alone(c, "f64", 5), "per %d" % per)
+
+
+# Members the batched fused step takes (virtual force, nz % 64 == 0, rows
+# dividing ny) beside members it does not (nz not a multiple of 64, a dense
+# stored-force member, ny not a multiple of 8).
+FUSED_CASES = [((8, 16, 64), 30, 200), ((5, 8, 128), 40, 0), ((6, 24, 64), 10, 100), ((4, 12, 64), 12, 50),
+               ((16, 16, 16), 8, 60), ((12, 12, 24), 200, 0)]
+
+
+@pytest.mark.parametrize("storage,graph,fusion", [("f64", True, (8, 32)), ("f64", False, (8, 3)),
+                                                  ("f64", True, (4, 5)), ("f32", True, (8, 32)),
+                                                  ("f32", False, (4, 64)), ("f64", True, (0, 32))])
+def test_fused_members_match_alone(storage, graph, fusion):
+    from lbswim.ensemble import Ensemble
+    cases = [make_case(size, 0, 0.3 + 0.1 * i, ns=ns, nt=nt, alpha=0.02, seed=77 + i)
+             for i, (size, ns, nt) in enumerate(FUSED_CASES)]
+    say("ensemble with fused members: %s graph=%s fusion=%s" % (storage, graph, fusion))
+    ens = Ensemble(storage)
+    ens.set_graph(graph)
+    ens.set_fusion(*fusion)
+    built = []
+    for c in cases:
+        lat, system, sw, tr, things = build_member(c, storage)
+        ens.add(lat, things)
+        built.append((lat, system, sw, tr))
+    steps = 7
+    ens.step(steps)
+    assert ens.fusion == fusion
+    fused = ens.fused_sites()
+    want = sum(c.size[0] * c.size[1] * c.size[2] for c in cases[:4] if fusion[0] and c.size[1] % fusion[0] == 0)
+    assert fused == want, (fused, want)
+    for i, (c, (lat, system, sw, tr)) in enumerate(zip(cases, built)):
+        assert lat.time_step == steps
+        assert_same(collect_lbswim(lat, sw, tr), run_alone(c, storage, steps), "member %d" % i)
+    # the members go on alone from the ensemble's state (u written interleaved
+    # by the fused launch, the force array as the call left it)
+    lat, system, sw, tr = built[0]
+    system.Step(2)
+    assert_same(collect_lbswim(lat, sw, tr), run_alone(cases[0], storage, steps + 2), "member 0 continued")

@@ -288,12 +288,42 @@ def bench_serving(nreq, rate, seed=1234):
         return on_done
 
     batched = srv.run(reqs, realtime=True, on_done=progress("batched"))
+    srv.close()
+    del srv
     count["n"] = 0
     seq = run_sequential(poisson_requests(nreq, rate, seed), realtime=True, on_done=progress("sequential"))
+    mix = serving_mix_bound(poisson_requests(nreq, rate, seed))
     r = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
     return {"rate_per_s": rate, "batched": r(batched), "sequential": r(seq),
             "throughput_ratio": round(batched["mlups"] / seq["mlups"], 2),
-            "p99_ratio": round(seq["p99_s"] / batched["p99_s"], 2)}
+            "p99_ratio": round(seq["p99_s"] / batched["p99_s"], 2),
+            "mix_resident_mlups": round(mix, 1), "batched_vs_mix_resident": round(batched["mlups"] / mix, 3)}
+
+
+def serving_mix_bound(reqs, steps=40):
+    """Throughput of the same requests when all of them are resident at once
+    and stepped together (two ensembles, graph replay): what the device gives
+    this mix of lattice sizes and particle loads with no arrivals,
+    admissions or collections around it."""
+    from lbswim.ensemble import Ensemble
+    from lbswim.serving import build
+    import lbswim
+    ens = {"f64": Ensemble("f64"), "f32": Ensemble("f32")}
+    keep = []
+    sites = 0
+    for req in reqs:
+        lat, things = build(req)
+        ens[req.storage].add(lat, things)
+        keep.append((lat, things))
+        sites += req.sites
+    for e in ens.values():
+        e.step(2)
+    lbswim.synchronize()
+    t0 = time.perf_counter()
+    for e in ens.values():
+        e.step(steps)
+    lbswim.synchronize()
+    return sites * steps / (time.perf_counter() - t0) / 1e6
 
 
 def oracle_errors(got, ref, L):

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -12,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -55,16 +57,21 @@ int fail(int code, const char* fmt, ...) {
 // the main stream waits for them before that call returns, so every other
 // entry point sees one ordered stream.
 struct Runtime {
-  bool ready = false;
+  std::atomic<bool> ready{false};
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;
-  hipStream_t cur = nullptr;  // where launches go: `stream`, or `side` inside a SideScope
   hipEvent_t u_ready = nullptr;  // main -> side: the u the tracers move in is written
   hipEvent_t joined = nullptr;   // side -> main: every tracer move enqueued so far is done
 };
 Runtime g_rt;
+std::mutex g_rt_mu;  // creation of the runtime's streams and events
+// Where this thread's launches go: the library stream, or another stream
+// inside a StreamScope (the side stream, an ensemble's capture stream).
+thread_local hipStream_t t_cur = nullptr;
 
 int ensure_runtime() {
+  if (g_rt.ready) return LBSWIM_OK;
+  std::lock_guard<std::mutex> lock(g_rt_mu);
   if (g_rt.ready) return LBSWIM_OK;
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
@@ -73,14 +80,14 @@ int ensure_runtime() {
   // the no-allocation step's fixed workspace on this device
   void* scratch = nullptr;
   HIP_TRY(noalloc_scratch(&scratch));
-  g_rt.cur = g_rt.stream;
   g_rt.ready = true;
   return LBSWIM_OK;
 }
 
-hipStream_t stream() { return g_rt.cur; }
+hipStream_t stream() { return t_cur ? t_cur : g_rt.stream; }
 
 int ensure_side_stream() {
+  std::lock_guard<std::mutex> lock(g_rt_mu);
   if (g_rt.side) return LBSWIM_OK;
   HIP_TRY(hipEventCreateWithFlags(&g_rt.u_ready, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&g_rt.joined, hipEventDisableTiming));
@@ -88,12 +95,13 @@ int ensure_side_stream() {
   return LBSWIM_OK;
 }
 
-// Launches inside the scope go to the side stream.
-struct SideScope {
-  SideScope() { g_rt.cur = g_rt.side; }
-  ~SideScope() { g_rt.cur = g_rt.stream; }
-  SideScope(const SideScope&) = delete;
-  SideScope& operator=(const SideScope&) = delete;
+// This thread's launches inside the scope go to stream `s`.
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(t_cur) { t_cur = s; }
+  ~StreamScope() { t_cur = prev; }
+  StreamScope(const StreamScope&) = delete;
+  StreamScope& operator=(const StreamScope&) = delete;
 };
 
 int dev_alloc_zero(void** p, size_t bytes) {
@@ -375,6 +383,55 @@ int lbswim_synchronize(void) {
   if (!g_rt.ready) return LBSWIM_OK;
   HIP_TRY(hipStreamSynchronize(stream()));
   return LBSWIM_OK;
+}
+
+struct lbswim_mark {
+  hipEvent_t ev = nullptr;
+};
+
+int lbswim_mark_create(lbswim_mark** out) {
+  CHECK_ARG(out != nullptr, "out is NULL");
+  *out = nullptr;
+  if (int rc = ensure_runtime()) return rc;
+  lbswim_mark* m = new (std::nothrow) lbswim_mark();
+  if (!m) return fail(LBSWIM_ENOMEM, "mark allocation failed");
+  hipError_t e = hipEventCreate(&m->ev);
+  if (e == hipSuccess) e = hipEventRecord(m->ev, stream());
+  if (e != hipSuccess) {
+    if (m->ev) (void)hipEventDestroy(m->ev);
+    delete m;
+    return fail(LBSWIM_EHIP, "recording a mark failed: %s", hipGetErrorString(e));
+  }
+  *out = m;
+  return LBSWIM_OK;
+}
+
+int lbswim_mark_wait(lbswim_mark* m) {
+  CHECK_ARG(m != nullptr, "mark is NULL");
+  HIP_TRY(hipEventSynchronize(m->ev));
+  return LBSWIM_OK;
+}
+
+int lbswim_mark_reached(lbswim_mark* m, int* reached) {
+  CHECK_ARG(m != nullptr && reached != nullptr, "NULL argument");
+  const hipError_t e = hipEventQuery(m->ev);
+  if (e != hipSuccess && e != hipErrorNotReady) return fail(LBSWIM_EHIP, "mark query failed: %s", hipGetErrorString(e));
+  *reached = e == hipSuccess ? 1 : 0;
+  return LBSWIM_OK;
+}
+
+int lbswim_mark_elapsed(lbswim_mark* a, lbswim_mark* b, double* ms) {
+  CHECK_ARG(a && b && ms, "NULL argument");
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, a->ev, b->ev));
+  *ms = t;
+  return LBSWIM_OK;
+}
+
+void lbswim_mark_destroy(lbswim_mark* m) {
+  if (!m) return;
+  (void)hipEventDestroy(m->ev);
+  delete m;
 }
 
 // ---- lattice ----------------------------------------------------------------
@@ -1331,7 +1388,7 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
       }
       side_used = true;
       {
-        SideScope on_side;
+        StreamScope on_side(g_rt.side);
         for (int i = 0; i < ntr && !rc; ++i) rc = tracers_move_impl(tr[i], lat);
       }
       if (rc) break;
@@ -1792,11 +1849,16 @@ struct lbswim_ensemble {
   uint32_t* slot_ids = nullptr;  // point ids in storage order
   size_t cap_slot = 0;
   bool dense_lists = false;      // aggregate the list building's atomics
-  // stored-force members' chain forked to the side stream (see ens_enqueue_step)
+  // stored-force members' chain forked to the ensemble's side stream (see
+  // ens_enqueue_step)
   bool fork_stored = false;
   bool stored_lists = false;  // some stored-force member builds lists
+  hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join = nullptr;
+  // graphs are captured on a stream of the ensemble's own, so that other
+  // threads' work on the library stream cannot enter or break a capture
+  hipStream_t cap = nullptr;
   // two captured steps, one graph for each starting parity (the parity
   // changes when a call steps an odd number of times)
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -1981,7 +2043,7 @@ int ens_build(lbswim_ensemble* e) {
   for (const LatDesc& d : lat) e->stored_lists = e->stored_lists || d.force_path == FORCE_LISTS_STORED;
   e->fork_stored = !storedwork.empty() && !(vfwork.empty() && fusework.empty());
   if (e->fork_stored) {
-    if (int r = ensure_side_stream()) return r;
+    if (!e->side) HIP_TRY(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
     if (!e->ev_fork) HIP_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
     if (!e->ev_join) HIP_TRY(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
   }
@@ -2037,7 +2099,7 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   // records as dependencies).
   const bool fork = e->fork_stored;
   const bool early = fork && !e->stored_lists;
-  hipStream_t sb = fork ? g_rt.side : s;
+  hipStream_t sb = fork ? e->side : s;
   auto fork_now = [&]() -> int {
     HIP_TRY(hipEventRecord(e->ev_fork, s));
     HIP_TRY(hipStreamWaitEvent(sb, e->ev_fork, 0));
@@ -2119,11 +2181,13 @@ int ens_materialize_force(lbswim_ensemble* e) {
 
 // Captures two steps starting with parity p into graph slot p.
 int ens_capture(lbswim_ensemble* e, int p) {
-  HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+  if (!e->cap) HIP_TRY(hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking));
+  StreamScope on_cap(e->cap);
+  HIP_TRY(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
   int rc = ens_enqueue(e, p);
   if (!rc) rc = ens_enqueue(e, p ^ 1);
   hipGraph_t g = nullptr;
-  hipError_t he = hipStreamEndCapture(stream(), &g);
+  hipError_t he = hipStreamEndCapture(e->cap, &g);
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
@@ -2189,6 +2253,8 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   for (hipEvent_t ev : e->pool) (void)hipEventDestroy(ev);
   if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
   if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+  if (e->side) (void)hipStreamDestroy(e->side);
+  if (e->cap) (void)hipStreamDestroy(e->cap);
   dev_free(e->d_lat);
   dev_free(e->d_storedwork);
   dev_free(e->d_vfwork);
@@ -2243,7 +2309,9 @@ int lbswim_ensemble_add(lbswim_ensemble* e, lbswim_lattice* lat, lbswim_swimmers
 int lbswim_ensemble_remove(lbswim_ensemble* e, int member) {
   CHECK_ARG(e, "NULL argument");
   if (int rc = ens_check_member(e, member)) return rc;
-  HIP_TRY(hipStreamSynchronize(stream()));
+  // No wait: steps already enqueued still advance the member, and anything
+  // the caller does with it afterwards (copies, destroy, which waits for the
+  // stream) is ordered after them on the library stream.
   e->members[member] = EnsMember();
   e->dirty = true;
   return LBSWIM_OK;

@@ -17,7 +17,7 @@ import numpy as np
 
 cnp.import_array()
 
-cdef extern from "lbswim.h":
+cdef extern from "lbswim.h" nogil:
     int LBSWIM_OK, LBSWIM_EINVAL, LBSWIM_ENOMEM, LBSWIM_EHIP, LBSWIM_ENODEV
     int LBSWIM_STORE_F64, LBSWIM_STORE_F32DEV
     int LBSWIM_FORCE_STORED, LBSWIM_FORCE_AUTO, LBSWIM_FORCE_VIRTUAL
@@ -61,6 +61,13 @@ cdef extern from "lbswim.h":
 
     const char* lbswim_last_error()
     int lbswim_synchronize()
+    ctypedef struct lbswim_mark:
+        pass
+    int lbswim_mark_create(lbswim_mark** out)
+    int lbswim_mark_wait(lbswim_mark* m)
+    int lbswim_mark_reached(lbswim_mark* m, int* reached)
+    int lbswim_mark_elapsed(lbswim_mark* a, lbswim_mark* b, double* ms)
+    void lbswim_mark_destroy(lbswim_mark* m)
     int lbswim_lattice_create(int nx, int ny, int nz, double tau_s, double tau_b, int storage,
                               lbswim_lattice** out)
     void lbswim_lattice_destroy(lbswim_lattice* lat)
@@ -163,8 +170,47 @@ cdef object wrap_view(object owner, void* data, int ndim, cnp.npy_intp* shape, i
 
 
 def synchronize():
-    """Wait until all enqueued device work has finished."""
-    check(lbswim_synchronize())
+    """Wait until all enqueued device work has finished (other Python
+    threads run meanwhile)."""
+    cdef int rc
+    with nogil:
+        rc = lbswim_synchronize()
+    check(rc)
+
+
+cdef class Mark:
+    """A point in the library's stream: all device work enqueued before the
+    mark was made.  wait() blocks until the device has reached it (other
+    Python threads run meanwhile); reached tests it; elapsed_ms times the
+    device between two marks."""
+    cdef lbswim_mark* impl
+
+    def __cinit__(self):
+        self.impl = NULL
+        check(lbswim_mark_create(&self.impl))
+
+    def __dealloc__(self):
+        if self.impl is not NULL:
+            lbswim_mark_destroy(self.impl)
+            self.impl = NULL
+
+    def wait(self):
+        cdef int rc
+        with nogil:
+            rc = lbswim_mark_wait(self.impl)
+        check(rc)
+
+    @property
+    def reached(self):
+        cdef int r = 0
+        check(lbswim_mark_reached(self.impl, &r))
+        return bool(r)
+
+    def elapsed_ms(self, Mark later):
+        """Device time from this mark to `later` (both reached)."""
+        cdef double ms = 0
+        check(lbswim_mark_elapsed(self.impl, later.impl, &ms))
+        return ms
 
 
 FORCE_MODES = ("stored", "auto", "virtual")
@@ -386,9 +432,11 @@ cdef class Lattice:
         self._storage = storage
 
     def __dealloc__(self):
-        if self.impl is not NULL:
-            lbswim_lattice_destroy(self.impl)
-            self.impl = NULL
+        cdef lbswim_lattice* p = self.impl
+        self.impl = NULL
+        if p is not NULL:
+            with nogil:
+                lbswim_lattice_destroy(p)
 
     cdef lbswim_lattice* handle(self) except NULL:
         if self.impl is NULL:
@@ -498,10 +546,18 @@ cdef class Lattice:
         return self._views[name]
 
     def _field_h2d(self, int field):
-        check(lbswim_lattice_h2d(self.handle(), field))
+        cdef lbswim_lattice* h = self.handle()
+        cdef int rc
+        with nogil:
+            rc = lbswim_lattice_h2d(h, field)
+        check(rc)
 
     def _field_d2h(self, int field):
-        check(lbswim_lattice_d2h(self.handle(), field))
+        cdef lbswim_lattice* h = self.handle()
+        cdef int rc
+        with nogil:
+            rc = lbswim_lattice_d2h(h, field)
+        check(rc)
 
     @property
     def rho(self):
@@ -707,9 +763,11 @@ cdef class SwimmerArray:
         self._common.adopt(self.impl, self)
 
     def __dealloc__(self):
-        if self.impl is not NULL:
-            lbswim_swimmers_destroy(self.impl)
-            self.impl = NULL
+        cdef lbswim_swimmers* p = self.impl
+        self.impl = NULL
+        if p is not NULL:
+            with nogil:
+                lbswim_swimmers_destroy(p)
 
     cdef Array _make(self, int field, int typenum, cls):
         cdef void* host = NULL
@@ -726,10 +784,18 @@ cdef class SwimmerArray:
         return make_array(self, view, field, cls)
 
     def _field_h2d(self, int field):
-        check(lbswim_swimmers_h2d(self.handle(), field))
+        cdef lbswim_swimmers* h = self.handle()
+        cdef int rc
+        with nogil:
+            rc = lbswim_swimmers_h2d(h, field)
+        check(rc)
 
     def _field_d2h(self, int field):
-        check(lbswim_swimmers_d2h(self.handle(), field))
+        cdef lbswim_swimmers* h = self.handle()
+        cdef int rc
+        with nogil:
+            rc = lbswim_swimmers_d2h(h, field)
+        check(rc)
 
     cdef lbswim_swimmers* handle(self) except NULL:
         if self.impl is NULL:
@@ -822,9 +888,11 @@ cdef class TracerArray:
         self._v = self._make(LBSWIM_PART_V)
 
     def __dealloc__(self):
-        if self.impl is not NULL:
-            lbswim_tracers_destroy(self.impl)
-            self.impl = NULL
+        cdef lbswim_tracers* p = self.impl
+        self.impl = NULL
+        if p is not NULL:
+            with nogil:
+                lbswim_tracers_destroy(p)
 
     cdef Array _make(self, int field):
         cdef void* host = NULL
@@ -837,10 +905,18 @@ cdef class TracerArray:
         return make_array(self, view, field)
 
     def _field_h2d(self, int field):
-        check(lbswim_tracers_h2d(self.handle(), field))
+        cdef lbswim_tracers* h = self.handle()
+        cdef int rc
+        with nogil:
+            rc = lbswim_tracers_h2d(h, field)
+        check(rc)
 
     def _field_d2h(self, int field):
-        check(lbswim_tracers_d2h(self.handle(), field))
+        cdef lbswim_tracers* h = self.handle()
+        cdef int rc
+        with nogil:
+            rc = lbswim_tracers_d2h(h, field)
+        check(rc)
 
     cdef lbswim_tracers* handle(self) except NULL:
         if self.impl is NULL:
@@ -907,7 +983,11 @@ def system_step(Lattice lat, list swimmers, list tracers, int nsteps=1):
         sw[i] = (<SwimmerArray?>swimmers[i]).handle()
     for i in range(ntr):
         tr[i] = (<TracerArray?>tracers[i]).handle()
-    check(lbswim_system_step(lat.handle(), sw, nsw, tr, ntr, nsteps))
+    cdef lbswim_lattice* h = lat.handle()
+    cdef int rc
+    with nogil:
+        rc = lbswim_system_step(h, sw, nsw, tr, ntr, nsteps)
+    check(rc)
 
 
 def system_autotune(Lattice lat, list swimmers, list tracers, int steps_per_trial=5):
@@ -1029,7 +1109,10 @@ cdef class Ensemble:
         return k
 
     def step(self, int nsteps=1):
-        check(lbswim_ensemble_step(self.impl, nsteps))
+        cdef int rc
+        with nogil:
+            rc = lbswim_ensemble_step(self.impl, nsteps)
+        check(rc)
 
     def set_graph(self, bint on):
         check(lbswim_ensemble_set_graph(self.impl, on))

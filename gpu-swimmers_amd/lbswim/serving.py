@@ -14,9 +14,12 @@ bit-identical to running the request alone (ensemble members do not interact).
 Reported: per-request time to result (arrival to result on the host), its p50
 and p99, and aggregate MLUPS over the wall time of the run.
 """
+import collections
+import concurrent.futures
 import dataclasses
 import heapq
 import math
+import threading
 import time
 
 import numpy as np
@@ -165,47 +168,96 @@ def run_alone(req):
 
 
 class Server:
-    """Continuous batching of requests on one GPU under an HBM budget."""
+    """Continuous batching of requests on one GPU under an HBM budget.
 
-    def __init__(self, hbm_budget=32 << 30, max_members=64, chunk=100, graph=True, keep_results=True):
+    The loop thread only enqueues device work: it adds prepared requests to
+    the ensemble of their storage form, steps the ensembles in chunks (graph
+    replays), and takes finished members out, running at most `ahead` chunks
+    ahead of the device (Mark).  Preparing a request (host fields, device
+    objects, copies, InitFromHydro) and collecting a result (copies back,
+    freeing its memory) run on worker threads; their device calls wait on the
+    library stream with the GIL released, so the device has the next chunks
+    queued while they run."""
+
+    def __init__(self, hbm_budget=32 << 30, max_members=64, chunk=100, graph=True, keep_results=True, workers=2,
+                 ahead=2):
         self.budget = int(hbm_budget)
         self.max_members = max_members
         self.chunk = chunk
         self.keep_results = keep_results
+        self.ahead = ahead
         self.ens = {"f64": Ensemble("f64"), "f32": Ensemble("f32")}
         for e in self.ens.values():
             e.set_graph(graph)
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
+        self.lock = threading.Lock()
         self.used = 0
-        self.active = {}  # rid -> [req, lattice, things, ensemble key, member, remaining]
+        self.active = {}     # rid -> [req, lattice, things, ensemble key, member, remaining]
+        self.preparing = {}  # rid -> (req, future of build)
+        self.finishing = {}  # rid -> future of the collection
+        self.marks = collections.deque()
+        self.done = []
         self.rejected = []
         self.cancelled = []
+        self._clock = time.perf_counter
+        self._on_done = None
 
     def fits(self, req):
-        return len(self.active) < self.max_members and self.used + estimate_bytes(req) <= self.budget
+        busy = len(self.active) + len(self.preparing)
+        with self.lock:
+            return busy < self.max_members and self.used + estimate_bytes(req) <= self.budget
 
-    def admit(self, req, now):
-        lat, things = build(req)
-        key = req.storage
-        member = self.ens[key].add(lat, things)
-        self.used += estimate_bytes(req)
+    def submit(self, req, now):
+        """Reserve the request's memory and prepare it on a worker thread."""
+        with self.lock:
+            self.used += estimate_bytes(req)
         req.admitted = now
-        self.active[req.rid] = [req, lat, things, key, member, req.steps]
+        self.preparing[req.rid] = (req, self.pool.submit(build, req))
 
-    def release(self, rid, now, keep):
+    def admit_ready(self):
+        """Add every prepared request to its ensemble."""
+        for rid, (req, fut) in list(self.preparing.items()):
+            if not fut.done():
+                continue
+            del self.preparing[rid]
+            lat, things = fut.result()
+            member = self.ens[req.storage].add(lat, things)
+            self.active[rid] = [req, lat, things, req.storage, member, req.steps]
+
+    def _collect(self, req, lat, things, keep):
+        try:
+            if keep:
+                req.result = collect(lat, things) if self.keep_results else {}
+                req.finished = self._clock()
+        finally:
+            del lat, things  # the device memory is freed here (after the stream has passed it)
+            with self.lock:
+                self.used -= estimate_bytes(req)
+                if keep:
+                    self.done.append(req)
+            if keep and self._on_done is not None:
+                self._on_done(req)
+
+    def release(self, rid, keep):
+        """Take a member out (no wait) and collect it on a worker thread."""
         req, lat, things, key, member, _ = self.active.pop(rid)
-        if keep:
-            req.result = collect(lat, things) if self.keep_results else {}
-            req.finished = now
         self.ens[key].remove(member)
-        self.used -= estimate_bytes(req)
+        self.finishing[rid] = self.pool.submit(self._collect, req, lat, things, keep)
 
     def cancel(self, rid, now=0.0):
-        """Drop a running request: its slot and memory are freed, the others continue unchanged."""
-        self.release(rid, now, keep=False)
+        """Drop a running or preparing request: its slot and memory are
+        freed, the others continue unchanged."""
+        if rid in self.preparing:
+            req, fut = self.preparing.pop(rid)
+            lat, things = fut.result()
+            self.finishing[rid] = self.pool.submit(self._collect, req, lat, things, False)
+        else:
+            self.release(rid, keep=False)
         self.cancelled.append(rid)
 
     def step_chunk(self):
-        """Advance every ensemble by the steps its closest-to-done member needs (<= chunk)."""
+        """Advance every ensemble by the steps its closest-to-done member needs
+        (<= chunk), then wait until the device is at most `ahead` chunks behind."""
         for key, ens in self.ens.items():
             rem = [a[5] for a in self.active.values() if a[3] == key]
             if not rem:
@@ -215,18 +267,28 @@ class Server:
             for a in self.active.values():
                 if a[3] == key:
                     a[5] -= k
+        self.marks.append(_core.Mark())
+        while len(self.marks) > self.ahead:
+            self.marks.popleft().wait()
+
+    def _reap(self):
+        for rid, fut in list(self.finishing.items()):
+            if fut.done():
+                del self.finishing[rid]
+                fut.result()  # a failed collection raises here
 
     def run(self, requests, realtime=True, on_chunk=None, on_done=None):
         """Serve `requests` (arrival times in seconds from the start).  With
         realtime=False arrivals are ignored and everything is admitted as
-        budget allows.  on_done(request) is called as each result arrives.
-        Returns summary statistics."""
+        budget allows.  on_done(request) is called (from a worker thread) as
+        each result arrives.  Returns summary statistics."""
         queue = [(r.arrival if realtime else 0.0, r.rid, r) for r in requests]
         heapq.heapify(queue)
         t0 = time.perf_counter()
-        clock = lambda: time.perf_counter() - t0  # noqa: E731
-        done = []
-        while queue or self.active:
+        self._clock = lambda: time.perf_counter() - t0  # noqa: E731
+        self._on_done = on_done
+        clock = self._clock
+        while queue or self.active or self.preparing or self.finishing:
             now = clock()
             while queue and queue[0][0] <= now:
                 req = queue[0][2]
@@ -237,24 +299,28 @@ class Server:
                 if not self.fits(req):
                     break
                 heapq.heappop(queue)
-                self.admit(req, now)
+                self.submit(req, now)
+            self.admit_ready()
+            self._reap()
             if not self.active:
-                if queue:
-                    time.sleep(max(0.0, min(0.01, queue[0][0] - clock())))
+                # nothing to step: wait for a preparation, a collection or the next arrival
+                waiting = [f for _, f in self.preparing.values()] + list(self.finishing.values())
+                timeout = 0.01 if not queue else max(0.0, min(0.01, queue[0][0] - clock()))
+                if waiting:
+                    concurrent.futures.wait(waiting, timeout=timeout, return_when=concurrent.futures.FIRST_COMPLETED)
+                elif queue:
+                    time.sleep(timeout)
                 continue
             self.step_chunk()
             if on_chunk is not None:
                 on_chunk(self)
-            finished = [rid for rid, a in self.active.items() if a[5] == 0]
-            if finished:
-                _core.synchronize()
-                now = clock()
-                for rid in finished:
-                    done.append(self.active[rid][0])
-                    self.release(rid, now, keep=True)
-                    if on_done is not None:
-                        on_done(done[-1])
+            for rid in [rid for rid, a in self.active.items() if a[5] == 0]:
+                self.release(rid, keep=True)
+        for m in self.marks:
+            m.wait()
+        self.marks.clear()
         wall = clock()
+        done = list(self.done)
         lat = np.array([r.latency() for r in done]) if done else np.zeros(0)
         updates = sum(r.sites * r.steps for r in done)
         return {
@@ -265,6 +331,9 @@ class Server:
             "mean_s": float(lat.mean()) if lat.size else float("nan"),
             "mlups": updates / wall / 1e6 if wall > 0 else 0.0,
         }
+
+    def close(self):
+        self.pool.shutdown(wait=True)
 
 
 def run_sequential(requests, realtime=True, on_done=None):

@@ -123,6 +123,18 @@ typedef struct lbswim_tracers lbswim_tracers;
 const char* lbswim_last_error(void);
 /* Wait for all enqueued device work (the reference waits inside Step, lb.cu:239). */
 int lbswim_synchronize(void);
+/* A point in the library's stream: everything enqueued before
+ * lbswim_mark_create.  _wait blocks until the device has reached it,
+ * _reached tests it, _elapsed times the device between two marks.  Lets
+ * a host loop run a bounded distance ahead of the device without draining
+ * it.  No reference counterpart. */
+typedef struct lbswim_mark lbswim_mark;
+int lbswim_mark_create(lbswim_mark** out);
+int lbswim_mark_wait(lbswim_mark* m);
+int lbswim_mark_reached(lbswim_mark* m, int* reached);
+/* Device time (ms) from mark a to mark b, both reached. */
+int lbswim_mark_elapsed(lbswim_mark* a, lbswim_mark* b, double* ms);
+void lbswim_mark_destroy(lbswim_mark* m);
 
 /* ---- Lattice (lbswim/liblbswim/Lattice.h:53-67; lbswim/_lb.pxd:22-30) ---- */
 
@@ -308,6 +320,9 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e);
 /* Adds a system; *member receives its slot id (slots of removed members are reused). */
 int lbswim_ensemble_add(lbswim_ensemble* e, lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
                         lbswim_tracers* const* tr, int ntr, int* member);
+/* Takes a member out without waiting for the device: steps already enqueued
+ * still complete for it, and later operations on it (copies, a destroy,
+ * which waits for the library stream) are ordered after them. */
 int lbswim_ensemble_remove(lbswim_ensemble* e, int member);
 int lbswim_ensemble_size(lbswim_ensemble* e, int* members);
 /* nsteps coupled steps of every member.  The batched steps use the virtual

@@ -294,10 +294,12 @@ int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int str
 /* Chooses how lbswim_system_step runs this system by timing it: the two
  * launches, and the fused launch with tiles of 8, 4 and 2 rows and 1, 2 and
  * 4 times the x strips that give every CU one tile, each over
- * steps_per_trial (+2 warm-up) real coupled steps -- the system advances by
- * those steps exactly as lbswim_system_step advances it -- then keeps the
- * fastest (set as by lbswim_lattice_set_fusion: LBSWIM_FUSION_OFF, or _ON
- * with those rows and strips).  *rows (0: two launches) and *strips receive
+ * steps_per_trial (+2 warm-up) real coupled steps, then the three fastest
+ * three more times each over 2 steps_per_trial (+2) steps -- the system
+ * advances by all those steps exactly as lbswim_system_step advances it --
+ * and keeps the finalist with the lowest time (set as by
+ * lbswim_lattice_set_fusion: LBSWIM_FUSION_OFF, or _ON with those rows and
+ * strips).  *rows (0: two launches) and *strips receive
  * the choice; either may be NULL.  No reference counterpart. */
 int lbswim_system_autotune(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
                            int ntr, int steps_per_trial, int* rows, int* strips);

@@ -88,9 +88,10 @@ def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, f
         lat.set_fusion(tiles[0] > 0, tiles[0], tiles[1])
         log("%s: fixed (rows, strips) = %s" % (storage, tiles))
     else:
-        # the autotuner picks two launches or a fused tile shape on real steps
+        # the autotuner picks two launches or a fused tile shape (and, for
+        # FP32, the list or the scatter force) on real steps
         tuned = system.Autotune(5)
-        log("%s: autotuned (rows, strips) = %s" % (storage, tuned))
+        log("%s: autotuned (rows, strips) = %s, force %s" % (storage, tuned, lat.force_mode))
     fused = lat.fusion[0]
     main = "step_hydro" if fused else "collide_stream"  # the dominant lattice kernel
     system.Step(warmup)
@@ -123,9 +124,15 @@ def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, f
         if cnt:
             res["kernels"][name] = {"avg_ms": ms / cnt, "launches": cnt}
     avg = main_ms / main_cnt
-    res["main_kernel"] = "k_%s_%s%s" % (main, "vf_" if virtual else "", storage)
+    # scatter force: the fused launch reads the force the moves scattered and
+    # clears the other scatter buffer (3 components read, 3 written per site)
+    scatter = lat.force_mode == "scatter" and fused
+    res["force_mode"] = lat.force_mode
+    if scatter:
+        virtual = False
+    res["main_kernel"] = "k_%s_%s%s" % (main, "scatter_" if scatter else ("vf_" if virtual else ""), storage)
     res["main_avg_ms"] = avg
-    res["main_bytes"] = (fused_bytes if fused else collide_bytes)(storage, virtual)
+    res["main_bytes"] = (fused_bytes if fused else collide_bytes)(storage, virtual) + (3 * ELEM[storage] if scatter else 0)
     res["step_bytes"] = step_bytes(storage, virtual, fused)
     res["main_gbs"] = res["main_bytes"] * n / (avg * 1e-3) / 1e9
     res["step_gbs"] = res["step_bytes"] * n / (ms_step * 1e-3) / 1e9
@@ -585,7 +592,8 @@ def main():
     if "f32" in res and main_form != "f32":
         f = res["f32"]
         out["fp32_store"] = {"mlups": round(f["mlups"], 2), "ms_per_step": round(f["ms_per_step"], 4),
-                             "speedup_vs_fp64": round(f["mlups"] / m["mlups"], 3),
+                             "speedup_vs_fp64": round(f["mlups"] / m["mlups"], 3), "force_mode": f["force_mode"],
+                             "fusion": f["fusion"], "autotuned_rows_strips": f["autotuned"],
                              "kernel": f["main_kernel"], "kernel_gbs": round(f["main_gbs"], 1),
                              "kernel_frac": round(f["main_gbs"] / HBM_PEAK_GBS, 4),
                              "step_hbm_frac": round(f["step_gbs"] / HBM_PEAK_GBS, 4),

@@ -220,7 +220,24 @@ struct lbswim_lattice {
   bool u_aos_valid = false;
   bool u_virtual = false;
   int fuse_strips = 0;
+  // Scatter force (LBSWIM_FORCE_SCATTER, FP32 storage, fused step): the
+  // swimmer moves add the next step's ZeroForce + AddForces into d_fsc[fsc_cur]
+  // with atomics, and the fused launch reads it there while it clears the
+  // other buffer for the moves after it.  fsc_valid: d_fsc[fsc_cur] holds that
+  // force for the swimmer arrays (and their versions) in fsc_set.  After a
+  // scatter step the force is virtual with force_src FORCE_SRC_SCATTER: the
+  // force array is copied from d_fsc[fsc_last] on demand.
+  void* d_fsc[2] = {nullptr, nullptr};
+  int fsc_cur = 0;
+  int fsc_last = 0;
+  bool fsc_valid = false;
+  std::vector<std::pair<const lbswim_swimmers*, uint64_t>> fsc_set;
+  int force_src = 0;
 };
+
+// What a virtual force is made from.
+constexpr int FORCE_SRC_LISTS = 0;    // the sorted point lists
+constexpr int FORCE_SRC_SCATTER = 1;  // the scatter buffer d_fsc[fsc_last]
 
 // Device storage of one particle array in slot order (particle_order.hip):
 // field f of slot s, component c is at d[f][c*num + s]; perm[s] is the user's
@@ -250,7 +267,16 @@ struct lbswim_swimmers {
   lbswim_common_params common_dev{};  // the copy kernels see (SharedItem::device)
   PStore st;
   unsigned long long* d_tumbles = nullptr;
+  // Changes whenever r, n or the parameters may have changed (a value no
+  // other array has had): a lattice's scatter force records it.
+  uint64_t version = 0;
 };
+
+// Fresh swimmer version numbers, unique over the process.
+static uint64_t next_version() {
+  static std::atomic<uint64_t> counter{0};
+  return ++counter;
+}
 
 struct lbswim_tracers {
   PStore st;
@@ -492,6 +518,8 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->d_u_aos);
   if (lat->tracers_done) (void)hipEventDestroy(lat->tracers_done);
   dev_free(lat->d_force);
+  dev_free(lat->d_fsc[0]);
+  dev_free(lat->d_fsc[1]);
   dev_free(lat->d_f[0]);
   dev_free(lat->d_f[1]);
   dev_free(lat->d_partial);
@@ -518,6 +546,13 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
 // from the point lists of the last fused step (ZeroForce + AddForces).
 static int materialize_force(lbswim_lattice* lat) {
   if (!lat->force_virtual) return LBSWIM_OK;
+  if (lat->force_src == FORCE_SRC_SCATTER) {
+    // the force the last scatter step read
+    HIP_TRY(hipMemcpyAsync(lat->d_force, lat->d_fsc[lat->fsc_last], 3 * lat->elem * lat->n, hipMemcpyDeviceToDevice,
+                           stream()));
+    lat->force_virtual = false;
+    return LBSWIM_OK;
+  }
   KernelTimer t(lat, LBSWIM_K_ADDF);
   const SpreadPoint* pts = static_cast<const SpreadPoint*>(lat->sp_pts);
   hipError_t e;
@@ -617,7 +652,9 @@ static int fusion_rows(const lbswim_lattice* lat) {
 
 // Step + CalcHydro of the coupled step in one launch (fusion_rows(lat) > 0),
 // once the tracer moves still reading the interleaved u (side stream) are done.
-static int lattice_step_hydro_impl(lbswim_lattice* lat, int rows) {
+// scatter: the force is read from the scatter buffer d_fsc[fsc_cur], and the
+// other one is cleared for the moves after the launch.
+static int lattice_step_hydro_impl(lbswim_lattice* lat, int rows, bool scatter = false) {
   const double ts = lat->params.tau_s, tb = lat->params.tau_b;
   if (!lat->d_u_aos)
     if (int rc = dev_alloc_zero(&lat->d_u_aos, 3 * lat->elem * lat->n)) return rc;
@@ -626,19 +663,21 @@ static int lattice_step_hydro_impl(lbswim_lattice* lat, int rows) {
     lat->tracers_reading = false;
   }
   const VForce vf = virtual_force(lat);
-  const VForce* pvf = lat->force_virtual ? &vf : nullptr;
+  const VForce* pvf = lat->force_virtual && !scatter ? &vf : nullptr;
+  void* force = scatter ? lat->d_fsc[lat->fsc_cur] : lat->d_force;
+  void* zero = scatter ? lat->d_fsc[lat->fsc_cur ^ 1] : nullptr;
   KernelTimer t(lat, LBSWIM_K_FUSED);
   hipError_t e;
   if (lat->storage == LBSWIM_STORE_F64)
-    e = launch_step_hydro<double>(static_cast<const double*>(lat->d_f[lat->cur]),
-                                  static_cast<const double*>(lat->d_force), static_cast<double*>(lat->d_f[1 - lat->cur]),
-                                  static_cast<double*>(lat->d_rho), static_cast<double*>(lat->d_u_aos), lat->geom, ts,
-                                  tb, pvf, rows, lat->fuse_strips, stream());
+    e = launch_step_hydro<double>(static_cast<const double*>(lat->d_f[lat->cur]), static_cast<const double*>(force),
+                                  static_cast<double*>(lat->d_f[1 - lat->cur]), static_cast<double*>(lat->d_rho),
+                                  static_cast<double*>(lat->d_u_aos), lat->geom, ts, tb, pvf, rows, lat->fuse_strips,
+                                  stream(), static_cast<double*>(zero));
   else
-    e = launch_step_hydro<float>(static_cast<const float*>(lat->d_f[lat->cur]), static_cast<const float*>(lat->d_force),
+    e = launch_step_hydro<float>(static_cast<const float*>(lat->d_f[lat->cur]), static_cast<const float*>(force),
                                  static_cast<float*>(lat->d_f[1 - lat->cur]), static_cast<float*>(lat->d_rho),
                                  static_cast<float*>(lat->d_u_aos), lat->geom, ts, tb, pvf, rows, lat->fuse_strips,
-                                 stream());
+                                 stream(), static_cast<float*>(zero));
   t.done();
   HIP_TRY(e);
   lat->u_aos_valid = true;
@@ -671,11 +710,21 @@ int lbswim_lattice_calc_hydro(lbswim_lattice* lat) {
 
 int lbswim_lattice_set_force_mode(lbswim_lattice* lat, int mode) {
   if (int rc = check_lattice(lat)) return rc;
-  CHECK_ARG(mode == LBSWIM_FORCE_STORED || mode == LBSWIM_FORCE_AUTO || mode == LBSWIM_FORCE_VIRTUAL,
+  CHECK_ARG(mode == LBSWIM_FORCE_STORED || mode == LBSWIM_FORCE_AUTO || mode == LBSWIM_FORCE_VIRTUAL ||
+                mode == LBSWIM_FORCE_SCATTER,
             "unknown force mode %d", mode);
+  CHECK_ARG(mode != LBSWIM_FORCE_SCATTER || lat->storage == LBSWIM_STORE_F32DEV,
+            "the scatter force is for FP32 storage (its atomics round in a varying order)");
   if (mode == LBSWIM_FORCE_STORED)
     if (int rc = materialize_force(lat)) return rc;
   lat->force_mode = mode;
+  return LBSWIM_OK;
+}
+
+int lbswim_lattice_get_force_mode(lbswim_lattice* lat, int* mode) {
+  if (int rc = check_lattice(lat)) return rc;
+  CHECK_ARG(mode != nullptr, "mode is NULL");
+  *mode = lat->force_mode;
   return LBSWIM_OK;
 }
 
@@ -979,6 +1028,7 @@ int lbswim_swimmers_create(int n, const lbswim_common_params* p, lbswim_swimmers
   if (!sw) return fail(LBSWIM_ENOMEM, "swimmer array allocation failed");
   sw->common_host = *p;
   sw->common_dev = *p;
+  sw->version = next_version();
   if ((rc = pstore_alloc(sw->st, n, {LBSWIM_PART_R, LBSWIM_PART_V, LBSWIM_PART_N, LBSWIM_PART_PRNG})) ||
       (rc = dev_alloc_zero(reinterpret_cast<void**>(&sw->d_tumbles), sizeof(unsigned long long)))) {
     std::string msg = g_err;
@@ -1014,6 +1064,7 @@ int lbswim_swimmers_common(lbswim_swimmers* sw, lbswim_common_params** host) {
 int lbswim_swimmers_common_h2d(lbswim_swimmers* sw) {
   CHECK_ARG(sw, "NULL argument");
   sw->common_dev = sw->common_host;
+  sw->version = next_version();
   return LBSWIM_OK;
 }
 
@@ -1042,7 +1093,9 @@ int lbswim_swimmers_add_forces(lbswim_swimmers* sw, lbswim_lattice* lat) {
   return LBSWIM_OK;
 }
 
-static int swimmers_move_impl(lbswim_swimmers* sw, lbswim_lattice* lat) {
+// scatter (FP32, after a fused step): the moved swimmers' dipoles are also
+// added into this force array.
+static int swimmers_move_impl(lbswim_swimmers* sw, lbswim_lattice* lat, void* scatter = nullptr) {
   PStore& st = sw->st;
   if (int rc = pstore_maybe_reorder(st, lat->geom)) return rc;
   KernelTimer t(lat, LBSWIM_K_SMOVE);
@@ -1053,15 +1106,18 @@ static int swimmers_move_impl(lbswim_swimmers* sw, lbswim_lattice* lat) {
     e = launch_swimmers_move<double>(st.num, p, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_V),
                                      fld(st, LBSWIM_PART_N), prng, st.perm,
                                      static_cast<const double*>(lat->u_aos_valid ? lat->d_u_aos : lat->d_u),
-                                     lat->u_aos_valid, lat->geom, sw->d_tumbles, stream());
+                                     lat->u_aos_valid, lat->geom, sw->d_tumbles, stream(),
+                                     static_cast<double*>(scatter));
   else
     e = launch_swimmers_move<float>(st.num, p, fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_V),
                                     fld(st, LBSWIM_PART_N), prng, st.perm,
                                     static_cast<const float*>(lat->u_aos_valid ? lat->d_u_aos : lat->d_u),
-                                    lat->u_aos_valid, lat->geom, sw->d_tumbles, stream());
+                                    lat->u_aos_valid, lat->geom, sw->d_tumbles, stream(),
+                                    static_cast<float*>(scatter));
   t.done();
   HIP_TRY(e);
   st.moves_since_order++;
+  sw->version = next_version();
   return LBSWIM_OK;
 }
 
@@ -1085,7 +1141,11 @@ int lbswim_swimmers_field(lbswim_swimmers* sw, int field, void** host, void** de
   if (host) {
     if (int rc = pstore_host(sw->st, field, host)) return rc;
   }
-  if (device) *device = sw->st.d[field];
+  if (device) {
+    *device = sw->st.d[field];
+    // the caller may write r or n through it
+    if (field == LBSWIM_PART_R || field == LBSWIM_PART_N) sw->version = next_version();
+  }
   if (count) *count = size_t(pstore_ncomp(field)) * sw->st.num;
   if (elem_size) *elem_size = 8;
   return LBSWIM_OK;
@@ -1094,6 +1154,7 @@ int lbswim_swimmers_field(lbswim_swimmers* sw, int field, void** host, void** de
 int lbswim_swimmers_h2d(lbswim_swimmers* sw, int field) {
   CHECK_ARG(sw, "swimmer handle is NULL");
   if (int rc = swimmer_check_field(field)) return rc;
+  if (field == LBSWIM_PART_R || field == LBSWIM_PART_N) sw->version = next_version();
   return pstore_h2d(sw->st, field);
 }
 
@@ -1336,6 +1397,51 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
   t.done();
   HIP_TRY(e);
   lat->force_virtual = virt;
+  lat->force_src = FORCE_SRC_LISTS;
+  return LBSWIM_OK;
+}
+
+// The coupled step takes the scatter force: an FP32 lattice set to
+// LBSWIM_FORCE_SCATTER, stepped by the fused launch, with swimmers.
+bool use_scatter(const lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
+  if (lat->force_mode != LBSWIM_FORCE_SCATTER || lat->storage != LBSWIM_STORE_F32DEV || fusion_rows(lat) == 0)
+    return false;
+  for (int i = 0; i < nsw; ++i)
+    if (sw[i]->st.num > 0) return true;
+  return false;
+}
+
+void scatter_record(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
+  lat->fsc_set.clear();
+  for (int i = 0; i < nsw; ++i) lat->fsc_set.emplace_back(sw[i], sw[i]->version);
+  lat->fsc_valid = true;
+}
+
+// Makes d_fsc[fsc_cur] hold ZeroForce + AddForces of the swimmers as they
+// are now: kept from the last step's moves if no array has changed since
+// (same arrays, same versions), else cleared and scattered here.
+int scatter_prepare(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
+  const size_t bytes = 3 * lat->elem * lat->n;
+  for (int k = 0; k < 2; ++k)
+    if (!lat->d_fsc[k]) {
+      if (int rc = dev_alloc_zero(&lat->d_fsc[k], bytes)) return rc;
+      lat->fsc_valid = false;
+    }
+  bool same = lat->fsc_valid && lat->fsc_set.size() == size_t(nsw);
+  for (int i = 0; same && i < nsw; ++i) same = lat->fsc_set[i].first == sw[i] && lat->fsc_set[i].second == sw[i]->version;
+  if (same) return LBSWIM_OK;
+  lat->fsc_valid = false;
+  KernelTimer t(lat, LBSWIM_K_ADDF);
+  float* f = static_cast<float*>(lat->d_fsc[lat->fsc_cur]);
+  hipError_t e = launch_fill_zero(f, bytes, stream());
+  for (int i = 0; i < nsw && e == hipSuccess; ++i) {
+    PStore& st = sw[i]->st;
+    e = launch_swimmers_add_forces<float>(st.num, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R),
+                                          fld(st, LBSWIM_PART_N), f, lat->geom, stream());
+  }
+  t.done();
+  HIP_TRY(e);
+  scatter_record(lat, sw, nsw);
   return LBSWIM_OK;
 }
 
@@ -1367,14 +1473,27 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
     lat->tracers_reading = false;
     return LBSWIM_OK;
   };
+  // Scatter force (FP32): each step's force was added by the previous step's
+  // swimmer moves into d_fsc[fsc_cur]; the fused launch reads it and clears
+  // the other buffer, into which this step's moves add the next force.
+  const bool scatter = use_scatter(lat, sw, nsw);
   int rc = LBSWIM_OK;
   for (int s = 0; s < nsteps && !rc; ++s) {
-    if ((rc = spread_forces(lat, sw, nsw))) break;
-    if (const int rows = fusion_rows(lat)) {
-      if ((rc = lattice_step_hydro_impl(lat, rows))) break;
+    if (scatter) {
+      if ((rc = scatter_prepare(lat, sw, nsw))) break;
+      if ((rc = lattice_step_hydro_impl(lat, fusion_rows(lat), true))) break;
+      lat->fsc_last = lat->fsc_cur;
+      lat->force_virtual = true;
+      lat->force_src = FORCE_SRC_SCATTER;
+      lat->fsc_valid = false;  // until this step's moves have added the next force
     } else {
-      if ((rc = lattice_step_impl(lat))) break;
-      if ((rc = lattice_hydro_impl(lat))) break;
+      if ((rc = spread_forces(lat, sw, nsw))) break;
+      if (const int rows = fusion_rows(lat)) {
+        if ((rc = lattice_step_hydro_impl(lat, rows))) break;
+      } else {
+        if ((rc = lattice_step_impl(lat))) break;
+        if ((rc = lattice_hydro_impl(lat))) break;
+      }
     }
     if (overlap) {
       if (!lat->tracers_done && hipEventCreateWithFlags(&lat->tracers_done, hipEventDisableTiming) != hipSuccess) {
@@ -1398,8 +1517,12 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
       }
       lat->tracers_reading = true;
     }
-    for (int i = 0; i < nsw && !rc; ++i) rc = swimmers_move_impl(sw[i], lat);
+    for (int i = 0; i < nsw && !rc; ++i) rc = swimmers_move_impl(sw[i], lat, scatter ? lat->d_fsc[lat->fsc_cur ^ 1] : nullptr);
     if (rc) break;
+    if (scatter) {
+      lat->fsc_cur ^= 1;
+      scatter_record(lat, sw, nsw);
+    }
     if (!overlap)
       for (int i = 0; i < ntr && !rc; ++i) rc = tracers_move_impl(tr[i], lat);
   }
@@ -1418,11 +1541,18 @@ int lbswim_system_autotune(lbswim_lattice* lat, lbswim_swimmers* const* sw, int 
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   // candidates: the two launches (rows 0), then fused tiles of 8, 4, 2 rows
-  // with 1, 2 and 4 times the strips that give every CU one tile
+  // with 1, 2 and 4 times the strips that give every CU one tile; for an FP32
+  // lattice left to choose (LBSWIM_FORCE_AUTO or _SCATTER) with swimmers,
+  // each fused shape also with the scatter force
   struct Choice {
-    int rows, strips;
+    int rows, strips, force;
   };
-  std::vector<Choice> cand{{0, 0}};
+  const int mode0 = lat->force_mode == LBSWIM_FORCE_SCATTER ? LBSWIM_FORCE_AUTO : lat->force_mode;
+  bool any_swimmers = false;
+  for (int i = 0; i < nsw; ++i) any_swimmers = any_swimmers || (sw[i] && sw[i]->st.num > 0);
+  const bool try_scatter = lat->storage == LBSWIM_STORE_F32DEV && any_swimmers &&
+                           (lat->force_mode == LBSWIM_FORCE_AUTO || lat->force_mode == LBSWIM_FORCE_SCATTER);
+  std::vector<Choice> cand{{0, 0, mode0}};
   const Geom& g = lat->geom;
   for (int ty : {8, 4, 2}) {
     if (!fused_step_supported(g, ty, lat->elem)) continue;
@@ -1433,10 +1563,12 @@ int lbswim_system_autotune(lbswim_lattice* lat, lbswim_swimmers* const* sw, int 
       const int st = std::min(g.nx, base * k);
       if (st == prev) continue;
       prev = st;
-      cand.push_back(Choice{ty, st});
+      cand.push_back(Choice{ty, st, mode0});
+      if (try_scatter) cand.push_back(Choice{ty, st, LBSWIM_FORCE_SCATTER});
     }
   }
   const int saved_fuse = lat->fuse, saved_rows = lat->fuse_rows, saved_strips = lat->fuse_strips;
+  const int saved_mode = lat->force_mode;
   hipEvent_t a = nullptr, b = nullptr;
   HIP_TRY(hipEventCreate(&a));
   if (hipEventCreate(&b) != hipSuccess) {
@@ -1448,6 +1580,7 @@ int lbswim_system_autotune(lbswim_lattice* lat, lbswim_swimmers* const* sw, int 
     lat->fuse = c.rows ? LBSWIM_FUSION_ON : LBSWIM_FUSION_OFF;
     lat->fuse_rows = c.rows;
     lat->fuse_strips = c.strips;
+    lat->force_mode = c.force;
     if (int r = lbswim_system_step(lat, sw, nsw, tr, ntr, 2)) return r;  // warm-up, workspace
     (void)hipEventRecord(a, stream());
     if (int r = lbswim_system_step(lat, sw, nsw, tr, ntr, steps)) return r;
@@ -1486,11 +1619,13 @@ int lbswim_system_autotune(lbswim_lattice* lat, lbswim_swimmers* const* sw, int 
     lat->fuse = saved_fuse;
     lat->fuse_rows = saved_rows;
     lat->fuse_strips = saved_strips;
+    lat->force_mode = saved_mode;
     return rc;
   }
   lat->fuse = best.rows ? LBSWIM_FUSION_ON : LBSWIM_FUSION_OFF;
   lat->fuse_rows = best.rows;
   lat->fuse_strips = best.strips;
+  lat->force_mode = best.force;
   if (rows) *rows = best.rows;
   if (strips) *strips = best.strips;
   return LBSWIM_OK;
@@ -2230,7 +2365,10 @@ void ens_advance(lbswim_ensemble* e, int k) {
     // and writes u in the reference layout, or (fused) only interleaved
     m.lat->u_aos_valid = m.fused;
     m.lat->u_virtual = m.fused;
-    for (auto* s : m.sw) s->st.moves_since_order += k;
+    for (auto* s : m.sw) {
+      s->st.moves_since_order += k;
+      s->version = next_version();
+    }
     for (auto* t : m.tr) t->st.moves_since_order += k;
   }
   if (k & 1) e->parity ^= 1;

@@ -69,10 +69,12 @@ bool fused_step_supported(const Geom& g, int ty, size_t elem);
 bool fused_tiles_full(const Geom& g, int ty);
 // u is written only component-interleaved, u_aos[3 ijk + a] (3 n values),
 // which the particle moves gather from; launch_u_from_aos writes the
-// reference layout from it.
+// reference layout from it.  zero_next (stored force only, not `force`):
+// a force array the launch clears, for the scatter of the next step's force.
 template <typename T>
 hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u_aos, const Geom& g, double tau_s,
-                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s);
+                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s,
+                             T* zero_next = nullptr);
 template <typename T>
 hipError_t launch_u_from_aos(const T* aos, T* u, const Geom& g, hipStream_t s);
 template <typename T>
@@ -92,10 +94,13 @@ hipError_t launch_swimmers_add_forces(int ns, const SwimParams& p, const double*
                                       T* force, const Geom& g, hipStream_t s);
 // aos: u holds the three components of a site together (u[3 ijk + a]),
 // the copy the fused step writes for the moves; else u[a n + ijk].
+// scatter (FP32 storage with aos only): also adds each moved swimmer's dipole
+// -- the next step's ZeroForce + AddForces -- into this force array with
+// atomics (the array must hold zeros, or other arrays' dipoles, before).
 template <typename T>
 hipError_t launch_swimmers_move(int ns, const SwimParams& p, double* r, double* v, double* n,
                                 uint64_t* prng, const int* perm, const T* u, bool aos, const Geom& g,
-                                unsigned long long* tumbles, hipStream_t s);
+                                unsigned long long* tumbles, hipStream_t s, T* scatter = nullptr);
 template <typename T>
 hipError_t launch_tracers_move(int nt, double* r, double* sdisp, double* v, const T* u, bool aos, const Geom& g,
                                hipStream_t s);

@@ -617,6 +617,9 @@ struct FusedArgs {
   T* gNew;
   T* rho;
   T* u_aos;  // u with a site's components together, u_aos[3 ijk + a]
+  // stored force only: where non-null, the tile's sites of this force array
+  // are cleared (for the next step's scatter into it)
+  T* zero_next;
   Geom g;
   T tau_s, tau_b, omega_s, omega_b;
 };
@@ -789,6 +792,13 @@ __device__ __forceinline__ void fused_sweep(const FusedArgs<T>& A, T* __restrict
       if (put) {
 #pragma unroll
         for (int p = 0; p < Q; ++p) buffer_store_nt(out[p], rNew, zl * B, (uint32_t(p) * n + srow) * B);
+      }
+      if constexpr (!VF) {
+        if (A.zero_next && put) {
+          const __amdgpu_buffer_rsrc_t rZ = buffer_rsrc(A.zero_next);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) buffer_store_nt(T(0), rZ, zl * B, (uint32_t(a) * n + srow) * B);
+        }
       }
     }
     const int w0 = SLOTS == 6 ? 0 : it % 2;
@@ -971,6 +981,7 @@ __global__ __launch_bounds__((TY + 3) * 64) void kb_step_hydro(const LatDesc* __
   A.gNew = static_cast<T*>(m.g[c ^ 1]);
   A.rho = static_cast<T*>(m.rho);
   A.u_aos = static_cast<T*>(m.u_aos);
+  A.zero_next = nullptr;
   A.g = m.geom;
   A.tau_s = T(m.tau_s);
   A.tau_b = T(m.tau_b);
@@ -1187,8 +1198,9 @@ bool fused_tiles_full(const Geom& g, int ty) { return g.nz % FUSED_Z == 0 && g.n
 
 template <typename T>
 hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* u_aos, const Geom& g, double tau_s,
-                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s) {
+                             double tau_b, const VForce* vf, int ty, int strips, hipStream_t s, T* zero_next) {
   if (!fused_step_supported(g, ty, sizeof(T)) || strips < 0) return hipErrorInvalidValue;
+  if (zero_next && (vf || zero_next == force)) return hipErrorInvalidValue;
   // the virtual force's row walk needs each 64-site run to be one z-row
   if (vf && g.nz % FUSED_Z != 0) return hipErrorInvalidValue;
   FusedTiles t;
@@ -1220,6 +1232,7 @@ hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* 
   A.gNew = fNew;
   A.rho = rho;
   A.u_aos = u_aos;
+  A.zero_next = zero_next;
   A.g = g;
   A.tau_s = T(tau_s);
   A.tau_b = T(tau_b);
@@ -1242,9 +1255,9 @@ hipError_t launch_step_hydro(const T* fOld, const T* force, T* fNew, T* rho, T* 
   return hipGetLastError();
 }
 template hipError_t launch_step_hydro<double>(const double*, const double*, double*, double*, double*, const Geom&,
-                                              double, double, const VForce*, int, int, hipStream_t);
+                                              double, double, const VForce*, int, int, hipStream_t, double*);
 template hipError_t launch_step_hydro<float>(const float*, const float*, float*, float*, float*, const Geom&, double,
-                                             double, const VForce*, int, int, hipStream_t);
+                                             double, const VForce*, int, int, hipStream_t, float*);
 
 // u[a n + s] = u_aos[3 s + a]: the reference layout from the fused step's
 // component-interleaved u.  Thread per site; the three loads of a site are

@@ -417,16 +417,93 @@ __device__ __forceinline__ double group_interp(const T* __restrict__ u, const Ge
   return group_sum(group_gather<T, AOS>(u, g, r, c), c, terms);
 }
 
+// ZeroForce + AddForces of the swimmer at R with orientation N, added into
+// `force` with hardware atomics (the reference's own scatter,
+// SwimmerArray.cu:42-112: tail at R - l N with force -P N, head at R with
+// +P N; weights peskin(x) peskin(y) peskin(z)), by the GROUP lanes of the
+// swimmer: lane q < 12 forms stencil coordinate q of each end, lane c =
+// 4j + k adds column (j, k) of the four x planes.  Where both ends share
+// their stencil box (|l n| below one site: the usual case) one atomic per
+// site and component adds tail + head.  Atomics make the order of the
+// additions, and so the rounding of the sums, vary from run to run.
+template <typename T>
+__device__ __forceinline__ void scatter_dipole(const double (&R)[3], const double (&N)[3], double P, double l,
+                                               const Geom& g, int c, T* __restrict__ force) {
+  double rp[2][3], F[2][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    rp[0][a] = R[a] - N[a] * l;
+    F[0][a] = -P * N[a];
+    rp[1][a] = R[a];
+    F[1][a] = F[0][a] * -1.0;
+  }
+  const int q = c < 12 ? c : 0;
+  const int axis = q >> 2;
+  const int na = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+  int idx[2];
+  double del[2];
+  bool same = true;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const double ra = axis == 0 ? rp[e][0] : (axis == 1 ? rp[e][1] : rp[e][2]);
+    const double b = ceil(ra - 2.);
+    const double x = b + (q & 3);
+    idx[e] = int(wrap_index(x, na));
+    del[e] = peskin(ra - x);
+    if (e == 1) {
+      const double b0 = ceil((axis == 0 ? rp[0][0] : (axis == 1 ? rp[0][1] : rp[0][2])) - 2.);
+      same = b0 == b;
+    }
+  }
+  // uniform over the group: every axis of the two ends has the same base
+  same = __shfl(same ? 1 : 0, 0, GROUP) && __shfl(same ? 1 : 0, 4, GROUP) && __shfl(same ? 1 : 0, 8, GROUP);
+  const int j = c >> 2, k = c & 3;
+  const size_t n = g.n;
+  double w[2][4];
+  uint32_t ijk[2][4];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const double dy = __shfl(del[e], 4 + j, GROUP), dz = __shfl(del[e], 8 + k, GROUP);
+    const uint32_t iyz =
+        uint32_t(__shfl(idx[e], 4 + j, GROUP)) * uint32_t(g.nz) + uint32_t(__shfl(idx[e], 8 + k, GROUP));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      w[e][i] = __shfl(del[e], i, GROUP) * dy * dz;
+      ijk[e][i] = uint32_t(__shfl(idx[e], i, GROUP)) * g.nyz + iyz;
+    }
+  }
+  if (same) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (w[0][i] == 0.0 && w[1][i] == 0.0) continue;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        spread_atomic(&force[size_t(d) * n + ijk[0][i]], w[0][i] * F[0][d] + w[1][i] * F[1][d]);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (w[e][i] == 0.0) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) spread_atomic(&force[size_t(d) * n + ijk[e][i]], w[e][i] * F[e][d]);
+      }
+  }
+}
+
 // DoSwimmerArrayMove, SwimmerArray.cu:123-210 (norm starts at 0; with
 // rotation off and no tumble, n is kept).  A lane group per swimmer
 // interpolates at r and at r - l n; lane 0 of the group finishes the move.
-template <typename T, bool AOS = false>
+// SCATTER: the group then adds the moved swimmer's dipole into `fnext` (the
+// next step's ZeroForce + AddForces, scatter_dipole).
+template <typename T, bool AOS = false, bool SCATTER = false>
 __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp, double* __restrict__ r,
                                                    double* __restrict__ v, double* __restrict__ nv,
                                                    uint64_t* __restrict__ prng, const int* __restrict__ perm,
                                                    const T* __restrict__ u, const Geom& g,
                                                    unsigned long long* __restrict__ tumbles, uint32_t blk,
-                                                   double* __restrict__ lds) {
+                                                   double* __restrict__ lds, T* __restrict__ fnext = nullptr) {
   const int grp = int(threadIdx.x) / GROUP, c = int(threadIdx.x) % GROUP;
   const int is = int(blk) * GROUPS + grp;
   if (is >= ns) return;  // whole group: the lanes of a group share `is`
@@ -449,48 +526,65 @@ __device__ __forceinline__ void swimmers_move_body(int ns, const SwimParams& cp,
     vel[d] = __shfl(a, d, GROUP);
     vMinus[d] = __shfl(b, d, GROUP);
   }
-  if (c != 0) return;
-  const int size[3] = {g.nx, g.ny, g.nz};
-  const int user = perm[is];  // the random stream belongs to the swimmer, not its storage slot
-  double rDot[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    rDot[d] = cp.mobility * cp.P * N[d];
-    if (!cp.translational_advection_off) rDot[d] += vel[d];
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    v[size_t(ns) * d + is] = rDot[d];
-    const double Rn = R[d] + rDot[d];
-    r[size_t(ns) * d + is] = wrap_position(Rn + size[d], double(size[d]));
-  }
-  const uint64_t ctr = prng[is];
-  const float U = u01f(draw(cp.seed, user, ctr, 0u).x[0]);
-  double newn[3];
-  double norm = 0.0;
-  bool turned = true;
-  if (double(U) < cp.alpha) {
-    normals3(cp.seed, user, ctr, newn);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) norm += newn[d] * newn[d];
-    atomicAdd(tumbles, 1ull);
-  } else if (!cp.rotational_advection_off) {
+  double Rout[3] = {0.0, 0.0, 0.0}, Nout[3] = {0.0, 0.0, 0.0};  // lane 0: the moved swimmer's r and n
+  if (c == 0) {
+    const int size[3] = {g.nx, g.ny, g.nz};
+    const int user = perm[is];  // the random stream belongs to the swimmer, not its storage slot
+    double rDot[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      double nDot = vel[d] - vMinus[d];
-      nDot /= cp.l;
-      newn[d] = N[d] + nDot;
-      norm += newn[d] * newn[d];
+      rDot[d] = cp.mobility * cp.P * N[d];
+      if (!cp.translational_advection_off) rDot[d] += vel[d];
     }
-  } else {
-    turned = false;
-  }
-  if (turned) {
-    norm = sqrt(norm);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) nv[size_t(ns) * d + is] = newn[d] / norm;
+    for (int d = 0; d < 3; ++d) {
+      v[size_t(ns) * d + is] = rDot[d];
+      const double Rn = R[d] + rDot[d];
+      Rout[d] = wrap_position(Rn + size[d], double(size[d]));
+      r[size_t(ns) * d + is] = Rout[d];
+    }
+    const uint64_t ctr = prng[is];
+    const float U = u01f(draw(cp.seed, user, ctr, 0u).x[0]);
+    double newn[3];
+    double norm = 0.0;
+    bool turned = true;
+    if (double(U) < cp.alpha) {
+      normals3(cp.seed, user, ctr, newn);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) norm += newn[d] * newn[d];
+      atomicAdd(tumbles, 1ull);
+    } else if (!cp.rotational_advection_off) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double nDot = vel[d] - vMinus[d];
+        nDot /= cp.l;
+        newn[d] = N[d] + nDot;
+        norm += newn[d] * newn[d];
+      }
+    } else {
+      turned = false;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) Nout[d] = N[d];
+    if (turned) {
+      norm = sqrt(norm);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        Nout[d] = newn[d] / norm;
+        nv[size_t(ns) * d + is] = Nout[d];
+      }
+    }
+    prng[is] = ctr + 1;
   }
-  prng[is] = ctr + 1;
+  if constexpr (SCATTER) {
+    double Rb[3], Nb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      Rb[d] = __shfl(Rout[d], 0, GROUP);
+      Nb[d] = __shfl(Nout[d], 0, GROUP);
+    }
+    scatter_dipole(Rb, Nb, cp.P, cp.l, g, c, fnext);
+  }
 }
 
 // ---- stencil gathers by pairs of columns (tracer moves) ----------------------
@@ -619,16 +713,17 @@ __device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r
 // contiguous range of particle blocks (xcd_block): neighbouring particles
 // share their stencils' lines in one L2.
 
-template <typename T, bool AOS>
+template <typename T, bool AOS, bool SCATTER = false>
 __global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp, double* __restrict__ r,
                                                           double* __restrict__ v, double* __restrict__ nv,
                                                           uint64_t* __restrict__ prng, const int* __restrict__ perm,
                                                           const T* __restrict__ u, Geom g,
-                                                          unsigned long long* __restrict__ tumbles, uint32_t nb) {
+                                                          unsigned long long* __restrict__ tumbles, uint32_t nb,
+                                                          T* __restrict__ fnext) {
   __shared__ __attribute__((aligned(16))) double lds[GROUPS * PLANE_STRIDE];
   const uint32_t blk = xcd_block(round_xcd(nb));
   if (blk >= nb) return;
-  swimmers_move_body<T, AOS>(ns, cp, r, v, nv, prng, perm, u, g, tumbles, blk, lds);
+  swimmers_move_body<T, AOS, SCATTER>(ns, cp, r, v, nv, prng, perm, u, g, tumbles, blk, lds, fnext);
 }
 
 template <typename T, bool AOS>
@@ -1298,15 +1393,25 @@ hipError_t launch_swimmers_add_forces(int ns, const SwimParams& p, const double*
 template <typename T>
 hipError_t launch_swimmers_move(int ns, const SwimParams& p, double* r, double* v, double* n, uint64_t* prng,
                                 const int* perm, const T* u, bool aos, const Geom& g, unsigned long long* tumbles,
-                                hipStream_t s) {
+                                hipStream_t s, T* scatter) {
   if (ns <= 0) return hipSuccess;
   const uint32_t nb = uint32_t((ns + GROUPS - 1) / GROUPS);
-  if (aos)
+  if (scatter) {
+    // the scatter form is built for FP32 storage after a fused step only
+    if constexpr (sizeof(T) == 4) {
+      if (!aos) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_swimmers_move<T, true, true>), dim3(round_xcd(nb)), dim3(PBLOCK), 0, s, ns, p, r, v, n,
+                         prng, perm, u, g, tumbles, nb, scatter);
+    } else {
+      return hipErrorInvalidValue;
+    }
+  } else if (aos) {
     hipLaunchKernelGGL((k_swimmers_move<T, true>), dim3(round_xcd(nb)), dim3(PBLOCK), 0, s, ns, p, r, v, n, prng, perm,
-                       u, g, tumbles, nb);
-  else
+                       u, g, tumbles, nb, nullptr);
+  } else {
     hipLaunchKernelGGL((k_swimmers_move<T, false>), dim3(round_xcd(nb)), dim3(PBLOCK), 0, s, ns, p, r, v, n, prng,
-                       perm, u, g, tumbles, nb);
+                       perm, u, g, tumbles, nb, nullptr);
+  }
   return hipGetLastError();
 }
 
@@ -1328,10 +1433,10 @@ template hipError_t launch_swimmers_add_forces<float>(int, const SwimParams&, co
                                                       float*, const Geom&, hipStream_t);
 template hipError_t launch_swimmers_move<double>(int, const SwimParams&, double*, double*, double*, uint64_t*,
                                                  const int*, const double*, bool, const Geom&, unsigned long long*,
-                                                 hipStream_t);
+                                                 hipStream_t, double*);
 template hipError_t launch_swimmers_move<float>(int, const SwimParams&, double*, double*, double*, uint64_t*,
                                                 const int*, const float*, bool, const Geom&, unsigned long long*,
-                                                hipStream_t);
+                                                hipStream_t, float*);
 template hipError_t launch_tracers_move<double>(int, double*, double*, double*, const double*, bool, const Geom&,
                                                 hipStream_t);
 template hipError_t launch_tracers_move<float>(int, double*, double*, double*, const float*, bool, const Geom&,

@@ -20,7 +20,7 @@ cnp.import_array()
 cdef extern from "lbswim.h" nogil:
     int LBSWIM_OK, LBSWIM_EINVAL, LBSWIM_ENOMEM, LBSWIM_EHIP, LBSWIM_ENODEV
     int LBSWIM_STORE_F64, LBSWIM_STORE_F32DEV
-    int LBSWIM_FORCE_STORED, LBSWIM_FORCE_AUTO, LBSWIM_FORCE_VIRTUAL
+    int LBSWIM_FORCE_STORED, LBSWIM_FORCE_AUTO, LBSWIM_FORCE_VIRTUAL, LBSWIM_FORCE_SCATTER
     int LBSWIM_RHO, LBSWIM_U, LBSWIM_FORCE, LBSWIM_FOLD, LBSWIM_FNEW
     int LBSWIM_PART_R, LBSWIM_PART_V, LBSWIM_PART_N, LBSWIM_PART_S, LBSWIM_PART_PRNG
     int LBSWIM_K_COUNT
@@ -76,6 +76,7 @@ cdef extern from "lbswim.h" nogil:
     int lbswim_lattice_init_from_hydro(lbswim_lattice* lat)
     int lbswim_lattice_zero_force(lbswim_lattice* lat)
     int lbswim_lattice_set_force_mode(lbswim_lattice* lat, int mode)
+    int lbswim_lattice_get_force_mode(lbswim_lattice* lat, int* mode)
     int lbswim_lattice_sync_force(lbswim_lattice* lat)
     int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual)
     int lbswim_lattice_sync_u(lbswim_lattice* lat)
@@ -213,11 +214,12 @@ cdef class Mark:
         return ms
 
 
-FORCE_MODES = ("stored", "auto", "virtual")
+FORCE_MODES = ("stored", "auto", "virtual", "scatter")
 
 
 cdef int force_mode_id(mode) except -1:
-    ids = {"stored": LBSWIM_FORCE_STORED, "auto": LBSWIM_FORCE_AUTO, "virtual": LBSWIM_FORCE_VIRTUAL}
+    ids = {"stored": LBSWIM_FORCE_STORED, "auto": LBSWIM_FORCE_AUTO, "virtual": LBSWIM_FORCE_VIRTUAL,
+           "scatter": LBSWIM_FORCE_SCATTER}
     if mode not in ids:
         raise ValueError("force mode must be one of %s" % (FORCE_MODES,))
     return ids[mode]
@@ -458,8 +460,19 @@ cdef class Lattice:
     def set_force_mode(self, mode):
         """"auto" (default): the coupled step keeps the swimmer force virtual
         where swimmers are sparse (the force array is written on demand);
-        "stored": always write it; "virtual": never write it in the step."""
+        "stored": always write it; "virtual": never write it in the step;
+        "scatter" (FP32 storage): the swimmer moves add the next step's force
+        into a private buffer with atomics, which the fused step reads (no
+        spreading lists; sums rounded in a varying order, so runs agree to
+        FP32 rounding rather than bit for bit)."""
         check(lbswim_lattice_set_force_mode(self.handle(), force_mode_id(mode)))
+
+    @property
+    def force_mode(self):
+        """The force mode in use (System.Autotune may change it)."""
+        cdef int m = 0
+        check(lbswim_lattice_get_force_mode(self.handle(), &m))
+        return FORCE_MODES[m]
 
     def sync_force(self):
         """Write the force array if the last coupled step left it virtual."""

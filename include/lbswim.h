@@ -53,6 +53,7 @@ extern "C" {
 #define LBSWIM_FORCE_STORED 0 /* written to the force array, read back by Step and CalcHydro */
 #define LBSWIM_FORCE_AUTO 1   /* virtual where the swimmers are sparse (the default) */
 #define LBSWIM_FORCE_VIRTUAL 2 /* always virtual */
+#define LBSWIM_FORCE_SCATTER 3 /* FP32 storage: the swimmer moves scatter the next step's force (atomics) */
 
 /* lattice fields (LatticeData members, lbswim/liblbswim/Lattice.h:45-49) */
 #define LBSWIM_RHO 0
@@ -260,8 +261,24 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
  * graphs that captured it) race. */
 int lbswim_system_step_noalloc(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
                                int ntr, int nsteps);
-/* LBSWIM_FORCE_STORED, _AUTO (default) or _VIRTUAL. No reference counterpart. */
+/* LBSWIM_FORCE_STORED, _AUTO (default), _VIRTUAL or (FP32 storage only)
+ * _SCATTER. No reference counterpart.
+ * Scatter force: where the coupled step runs the fused launch, each step's
+ * swimmer moves also add the next step's ZeroForce + AddForces into one of
+ * two private force buffers with hardware FP32 atomics -- the reference's own
+ * scatter (SwimmerArray.cu:42-112) -- and the fused launch reads it there
+ * (clearing the other buffer), so no spreading lists are built.  The order of
+ * the atomic additions varies from run to run, and with it the last bit of
+ * the FP32 force sums: results agree with the other modes to FP32 rounding
+ * (well inside the FP32 form's 1e-5 u error), not bit for bit.  The buffer
+ * is kept between calls while the same swimmer arrays are stepped and none
+ * of them changed (H2D of r or n, common_h2d, a move outside the call, a
+ * device pointer to r or n handed out); otherwise the first step of a call
+ * scatters it anew.  The force array itself is written on demand, as for the
+ * virtual force.  Lattices stepped by the two launches use the lists. */
 int lbswim_lattice_set_force_mode(lbswim_lattice* lat, int mode);
+/* The lattice's force mode (lbswim_system_autotune may change it). */
+int lbswim_lattice_get_force_mode(lbswim_lattice* lat, int* mode);
 /* Writes the force array if the last coupled step left it virtual. No reference counterpart. */
 int lbswim_lattice_sync_force(lbswim_lattice* lat);
 /* *is_virtual = 1 if the last coupled step left the force virtual (not yet
@@ -293,7 +310,9 @@ int lbswim_lattice_u_is_virtual(lbswim_lattice* lat, int* is_virtual);
 int lbswim_lattice_set_fusion(lbswim_lattice* lat, int enable, int rows, int strips);
 /* Chooses how lbswim_system_step runs this system by timing it: the two
  * launches, and the fused launch with tiles of 8, 4 and 2 rows and 1, 2 and
- * 4 times the x strips that give every CU one tile, each over
+ * 4 times the x strips that give every CU one tile (for an FP32 lattice in
+ * LBSWIM_FORCE_AUTO or _SCATTER with swimmers, each fused shape also with
+ * the scatter force, which the lattice then keeps if it wins), each over
  * steps_per_trial (+2 warm-up) real coupled steps, then the three fastest
  * three more times each over 2 steps_per_trial (+2) steps -- the system
  * advances by all those steps exactly as lbswim_system_step advances it --

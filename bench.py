@@ -227,13 +227,16 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True, sequentia
     # members with full 64-site rows, or the batched collide+stream of the
     # other virtual-force members, whichever covers more sites), each phase
     # timed with HIP events in an eager pass
+    timed = 20
     ens.set_timing(True)
-    ens.step(20)
+    ens.step(timed)
     kt = ens.kernel_times()
     ens.set_timing(False)
     vsites, ssites = ens.sites()
     fsites = ens.fused_sites()
-    phases = {k: round(ms / cnt, 4) for k, (ms, cnt) in kt.items() if cnt}
+    # device time of each phase per step (with lanes: summed over the lanes'
+    # launches, which run side by side)
+    phases = {k: round(ms / timed, 4) for k, (ms, cnt) in kt.items() if cnt}
     roof = None
 
     def gbs(byts, ms):
@@ -242,6 +245,7 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True, sequentia
     f_ms, f_n = kt["step_hydro"]
     c_ms, c_n = kt["collide_stream"]
     h_ms, h_n = kt["calc_hydro"]
+    f_n, c_n, h_n = (timed if x else 0 for x in (f_n, c_n, h_n))  # per-step sums below
     if f_n and fsites >= vsites:
         avg = f_ms / f_n
         byts = fsites * fused_bytes(storage, True)
@@ -249,9 +253,9 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True, sequentia
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs(byts, avg) / HBM_PEAK_GBS, 4),
                 "traffic": load_ensemble_traffic(storage, "kb_step_hydro<%s, virtual force>" % storage)
                 if sites == 6069760 else None,
-                "algorithmic_bytes_per_launch": byts, "avg_launch_ms": round(avg, 4)}
+                "algorithmic_bytes_per_step": byts, "launch_ms_per_step": round(avg, 4), "lanes": ens.lanes}
         if c_n:
-            roof["collide_avg_ms"] = round(c_ms / c_n, 4)
+            roof["collide_ms_per_step"] = round(c_ms / c_n, 4)
             roof["collide_gbs"] = gbs(vsites * collide_bytes(storage, True), c_ms / c_n)
     elif c_n:
         avg = c_ms / c_n
@@ -260,8 +264,8 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True, sequentia
         roof = {"kernel": kname, "bound": "hbm", "achieved": gbs(byts, avg), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs(byts, avg) / HBM_PEAK_GBS, 4),
                 "traffic": load_ensemble_traffic(storage, kname) if sites == 6069760 else None,
-                "algorithmic_bytes_per_launch": byts, "avg_launch_ms": round(avg, 4),
-                "calc_hydro_avg_ms": round(h_ms / h_n, 4) if h_n else None,
+                "algorithmic_bytes_per_step": byts, "launch_ms_per_step": round(avg, 4), "lanes": ens.lanes,
+                "calc_hydro_ms_per_step": round(h_ms / h_n, 4) if h_n else None,
                 "calc_hydro_gbs": gbs(vsites * (19 * ELEM[storage]), h_ms / h_n) if h_n else None}
     # main-stream phases of one eager step (the stored-force chain runs beside
     # them on the side stream) against the replayed graphs' time per step
@@ -272,6 +276,7 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True, sequentia
             "sims_per_s_sequential": round(nsims / t_seq, 2) if t_seq else None,
             "speedup_vs_sequential": round(t_seq / t_ens, 2) if t_seq else None,
             "mlups": round(sites * steps / t_ens / 1e6, 1), "order": ORDERS[order], "blocks_per_workgroup": per,
+            "lanes": ens.lanes,
             "fusion_rows_strip": list(ens.fusion), "sites_fused_virtual_stored": [fsites, vsites, ssites],
             "graph": graph, "roofline": roof, "phases_ms": phases,
             "graph_ms_per_step": round(t_ens * 1e3 / steps, 4), "main_phases_ms_per_step": round(main, 4)}

@@ -2019,6 +2019,17 @@ struct lbswim_ensemble {
   std::vector<TimeRec> recs;
   std::vector<hipEvent_t> pool;
   size_t sites_virtual = 0, sites_stored = 0;
+  // Lanes: with lanes > 1 the members are split into that many
+  // sub-ensembles of about equal sites, each stepped by its own batched
+  // launches on a stream of its own, forked from and joined to the caller's
+  // stream once per step (a graph capture records them as parallel
+  // branches): the latency-bound list building and moves of one lane run
+  // beside the lattice sweeps of the others.
+  int lanes = 1;
+  std::vector<lbswim_ensemble*> sub;
+  std::vector<hipStream_t> lane_stream;
+  std::vector<hipEvent_t> lane_join;
+  hipEvent_t lane_fork = nullptr;
 };
 
 namespace {
@@ -2066,9 +2077,14 @@ void ens_drop_graph(lbswim_ensemble* e) {
   }
 }
 
+int ens_build_lanes(lbswim_ensemble* e);
+void ens_drop_lanes(lbswim_ensemble* e);
+
 int ens_build(lbswim_ensemble* e) {
   HIP_TRY(hipStreamSynchronize(stream()));
   ens_drop_graph(e);
+  if (e->lanes > 1) return ens_build_lanes(e);
+  ens_drop_lanes(e);
   std::vector<LatDesc> lat;
   // per member: its lattice blocks, in the stored-force (0), virtual-force
   // two-launch (1) or virtual-force fused (2) group
@@ -2236,6 +2252,75 @@ int ens_build(lbswim_ensemble* e) {
   return LBSWIM_OK;
 }
 
+void ens_drop_lanes(lbswim_ensemble* e) {
+  for (lbswim_ensemble* x : e->sub) lbswim_ensemble_destroy(x);
+  e->sub.clear();
+}
+
+// Splits the members over e->lanes sub-ensembles, largest lattice first to
+// the lane with the fewest sites, and builds each.
+int ens_build_lanes(lbswim_ensemble* e) {
+  std::vector<size_t> live;
+  for (size_t i = 0; i < e->members.size(); ++i)
+    if (e->members[i].lat) live.push_back(i);
+  const size_t nl = std::max<size_t>(1, std::min<size_t>(size_t(e->lanes), live.size()));
+  while (e->sub.size() > nl) {
+    lbswim_ensemble_destroy(e->sub.back());
+    e->sub.pop_back();
+  }
+  while (e->sub.size() < nl) {
+    lbswim_ensemble* x = new (std::nothrow) lbswim_ensemble();
+    if (!x) return fail(LBSWIM_ENOMEM, "ensemble lane allocation failed");
+    e->sub.push_back(x);
+  }
+  if (!e->lane_fork) HIP_TRY(hipEventCreateWithFlags(&e->lane_fork, hipEventDisableTiming));
+  while (e->lane_stream.size() < nl) {
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+    e->lane_stream.push_back(st);
+    hipEvent_t ev = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    e->lane_join.push_back(ev);
+  }
+  std::stable_sort(live.begin(), live.end(),
+                   [&](size_t a, size_t b) { return e->members[a].lat->n > e->members[b].lat->n; });
+  std::vector<size_t> load(nl, 0);
+  for (lbswim_ensemble* x : e->sub) {
+    x->members.clear();
+    x->storage = e->storage;
+    x->order = e->order;
+    x->per = e->per;
+    x->fuse_rows = e->fuse_rows;
+    x->fuse_strip = e->fuse_strip;
+    x->force_mode = e->force_mode;
+    x->timing = e->timing;
+    x->lanes = 1;
+    x->dirty = true;
+  }
+  std::vector<std::vector<size_t>> owner(nl);
+  for (size_t i : live) {
+    const size_t k = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+    load[k] += e->members[i].lat->n;
+    e->sub[k]->members.push_back(e->members[i]);
+    owner[k].push_back(i);
+  }
+  e->sites_virtual = e->sites_stored = e->sites_fused = 0;
+  for (size_t k = 0; k < nl; ++k) {
+    lbswim_ensemble* x = e->sub[k];
+    if (int rc = ens_build(x)) return rc;
+    e->sites_virtual += x->sites_virtual;
+    e->sites_stored += x->sites_stored;
+    e->sites_fused += x->sites_fused;
+    for (size_t j = 0; j < owner[k].size(); ++j) {
+      e->members[owner[k][j]].cur_at_build = x->members[j].cur_at_build;
+      e->members[owner[k][j]].fused = x->members[j].fused;
+    }
+  }
+  e->parity = 0;
+  e->dirty = false;
+  return LBSWIM_OK;
+}
+
 template <typename T>
 int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   hipStream_t s = stream();
@@ -2313,12 +2398,32 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
 }
 
 int ens_enqueue(lbswim_ensemble* e, int parity) {
+  if (e->lanes > 1 && !e->sub.empty()) {
+    // fork every lane from the caller's stream, join them all back
+    hipStream_t s = stream();
+    HIP_TRY(hipEventRecord(e->lane_fork, s));
+    for (size_t k = 0; k < e->sub.size(); ++k) {
+      HIP_TRY(hipStreamWaitEvent(e->lane_stream[k], e->lane_fork, 0));
+      {
+        StreamScope on_lane(e->lane_stream[k]);
+        if (int rc = ens_enqueue(e->sub[k], parity)) return rc;
+      }
+      HIP_TRY(hipEventRecord(e->lane_join[k], e->lane_stream[k]));
+    }
+    for (size_t k = 0; k < e->sub.size(); ++k) HIP_TRY(hipStreamWaitEvent(s, e->lane_join[k], 0));
+    return LBSWIM_OK;
+  }
   return e->storage == LBSWIM_STORE_F64 ? ens_enqueue_step<double>(e, parity) : ens_enqueue_step<float>(e, parity);
 }
 
 // After batched steps with the virtual force: write every member's force
 // array from the last step's lists, as the reference leaves it.
 int ens_materialize_force(lbswim_ensemble* e) {
+  if (e->lanes > 1 && !e->sub.empty()) {
+    for (lbswim_ensemble* x : e->sub)
+      if (int rc = ens_materialize_force(x)) return rc;
+    return LBSWIM_OK;
+  }
   for (int k = 0; k < 2; ++k) {
     const int nw = k ? e->n_fuseblk : e->n_vfwork;
     const WorkItem* w = k ? e->d_fuseblk : e->d_vfwork;
@@ -2357,6 +2462,11 @@ int ens_capture(lbswim_ensemble* e, int p) {
 
 // Host bookkeeping after `k` steps: the reference's pointer swap and counter.
 void ens_advance(lbswim_ensemble* e, int k) {
+  if (e->lanes > 1 && !e->sub.empty()) {  // the lanes own the members' bookkeeping
+    for (lbswim_ensemble* x : e->sub) ens_advance(x, k);
+    if (k & 1) e->parity ^= 1;
+    return;
+  }
   for (auto& m : e->members) {
     if (!m.lat) continue;
     if (k & 1) m.lat->cur ^= 1;
@@ -2405,6 +2515,10 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
     (void)hipEventDestroy(r.stop);
   }
   for (hipEvent_t ev : e->pool) (void)hipEventDestroy(ev);
+  ens_drop_lanes(e);
+  for (hipStream_t st : e->lane_stream) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : e->lane_join) (void)hipEventDestroy(ev);
+  if (e->lane_fork) (void)hipEventDestroy(e->lane_fork);
   if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
   if (e->ev_join) (void)hipEventDestroy(e->ev_join);
   if (e->side) (void)hipStreamDestroy(e->side);
@@ -2537,6 +2651,8 @@ int lbswim_ensemble_set_timing(lbswim_ensemble* e, unsigned mask) {
     e->recs.clear();
   }
   e->timing = mask & ((1u << LBSWIM_K_COUNT) - 1u);
+  for (lbswim_ensemble* x : e->sub)
+    if (int rc = lbswim_ensemble_set_timing(x, mask)) return rc;
   return LBSWIM_OK;
 }
 
@@ -2553,8 +2669,31 @@ int lbswim_ensemble_kernel_time(lbswim_ensemble* e, int kernel, double* total_ms
     tot += ms;
     ++cnt;
   }
+  // lanes: their launches of one step run side by side; the sum over lanes is
+  // their total device time
+  for (lbswim_ensemble* x : e->sub) {
+    double t = 0.0;
+    long c = 0;
+    if (int rc = lbswim_ensemble_kernel_time(x, kernel, &t, &c)) return rc;
+    tot += t;
+    cnt += c;
+  }
   *total_ms = tot;
   *count = cnt;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_set_lanes(lbswim_ensemble* e, int lanes) {
+  CHECK_ARG(e, "ensemble is NULL");
+  CHECK_ARG(lanes >= 1 && lanes <= 8, "lanes must be 1..8 (got %d)", lanes);
+  if (e->lanes != lanes) e->dirty = true;
+  e->lanes = lanes;
+  return LBSWIM_OK;
+}
+
+int lbswim_ensemble_lanes(lbswim_ensemble* e, int* lanes) {
+  CHECK_ARG(e && lanes, "NULL argument");
+  *lanes = e->lanes;
   return LBSWIM_OK;
 }
 
@@ -2638,6 +2777,33 @@ int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choic
       e->fuse_strip = bstrip;
       e->dirty = true;
     }
+  }
+  // then the lanes, with everything else as chosen
+  if (!rc) {
+    int members = 0;
+    for (auto& m : e->members) members += m.lat ? 1 : 0;
+    float lbest = 0.f;
+    int blanes = e->lanes;
+    bool lfirst = true;
+    for (int ln : {1, 2, 4}) {
+      if (ln > 1 && ln > members) break;
+      e->lanes = ln;
+      e->dirty = true;
+      if ((rc = lbswim_ensemble_step(e, 2))) break;
+      (void)hipEventRecord(a, stream());
+      if ((rc = lbswim_ensemble_step(e, steps_per_trial))) break;
+      (void)hipEventRecord(b, stream());
+      (void)hipEventSynchronize(b);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      if (lfirst || ms < lbest) {
+        lbest = ms;
+        blanes = ln;
+        lfirst = false;
+      }
+    }
+    e->lanes = blanes;
+    e->dirty = true;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);

@@ -856,25 +856,46 @@ __device__ __forceinline__ RowLists row_lists(const SpreadPoint& p, int row, con
 // Dense systems: the lanes of a wavefront (points in storage order, so
 // spatial neighbours) often hit the same list; one lane per distinct list
 // adds the wavefront's total with one atomic and hands each lane its slot.
-// Returns the value the atomic returned plus the lane's rank among the lanes
-// with the same list.  Every lane of the wavefront calls it; lanes with
-// `key` ~0u take part without a list.
-__device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t* __restrict__ ctr, uint32_t key) {
+// The distinct lists are found by ballots first (no memory traffic), then
+// every leader lane issues its atomic at once, so the wavefront waits for one
+// round trip instead of one per distinct list.  Returns the value the
+// atomic returned plus the lane's rank among the lanes with the same list.
+// Every lane of the wavefront calls it; lanes with `key` ~0u take part
+// without a list.
+struct WaveGroup {
+  int leader;      // lowest lane with this lane's key
+  uint32_t rank;   // lanes below with the same key
+  uint32_t total;  // lanes with this lane's key (valid on the leader)
+};
+__device__ __forceinline__ WaveGroup wave_group(uint32_t key) {
   const int lane = int(threadIdx.x % WAVE_SITES);
   const uint64_t below = (uint64_t(1) << lane) - 1;
-  uint32_t mine = 0;
+  WaveGroup w{lane, 0u, 0u};
   uint64_t pending = __ballot(key != 0xFFFFFFFFu);
   while (pending) {
     const int leader = __builtin_ctzll(pending);
     const uint32_t lk = uint32_t(__builtin_amdgcn_readlane(int(key), leader));
     const uint64_t same = __ballot(key == lk);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&ctr[lk], uint32_t(__builtin_popcountll(same)));
-    base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
-    if (key == lk) mine = base + uint32_t(__builtin_popcountll(same & below));
+    if (key == lk) {
+      w.leader = leader;
+      w.rank = uint32_t(__builtin_popcountll(same & below));
+      w.total = uint32_t(__builtin_popcountll(same));
+    }
     pending &= ~same;
   }
-  return mine;
+  return w;
+}
+__device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t* __restrict__ ctr, uint32_t key) {
+  const WaveGroup w = wave_group(key);
+  const int lane = int(threadIdx.x % WAVE_SITES);
+  uint32_t base = 0;
+  if (key != 0xFFFFFFFFu && w.leader == lane) base = atomicAdd(&ctr[key], w.total);
+  return uint32_t(__shfl(int(base), w.leader)) + w.rank;
+}
+// The same counts without returning slots (no wait for the atomics).
+__device__ __forceinline__ void wave_aggregated_count(uint32_t* __restrict__ ctr, uint32_t key) {
+  const WaveGroup w = wave_group(key);
+  if (key != 0xFFFFFFFFu && w.leader == int(threadIdx.x % WAVE_SITES)) atomicAdd(&ctr[key], w.total);
 }
 
 // One thread per (point, stencil row), points in storage order (slot_ids):
@@ -901,7 +922,7 @@ __global__ __launch_bounds__(PBLOCK) void k_count_rows(uint32_t npts, Geom g, co
   }
   if constexpr (AGG) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) (void)wave_aggregated_add(counts, k < rl.count ? rl.l[k] : 0xFFFFFFFFu);
+    for (int k = 0; k < 4; ++k) wave_aggregated_count(counts, k < rl.count ? rl.l[k] : 0xFFFFFFFFu);
   } else {
     for (int k = 0; k < rl.count; ++k) atomicAdd(&counts[rl.l[k]], 1u);
   }

@@ -132,6 +132,8 @@ cdef extern from "lbswim.h" nogil:
     int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice)
     int lbswim_ensemble_set_fusion(lbswim_ensemble* e, int rows, int strip)
     int lbswim_ensemble_fusion(lbswim_ensemble* e, int* rows, int* strip)
+    int lbswim_ensemble_set_lanes(lbswim_ensemble* e, int lanes)
+    int lbswim_ensemble_lanes(lbswim_ensemble* e, int* lanes)
     int lbswim_ensemble_fused_sites(lbswim_ensemble* e, size_t* fused_sites)
     int lbswim_lattice_totals(lbswim_lattice* lat, double* out)
     int lbswim_lattice_set_timing(lbswim_lattice* lat, int on)
@@ -1165,6 +1167,18 @@ cdef class Ensemble:
         cdef int r = 0, st = 0
         check(lbswim_ensemble_fusion(self.impl, &r, &st))
         return r, st
+
+    def set_lanes(self, int lanes):
+        """Split the members into `lanes` groups of about equal sites whose
+        batched launches run on streams of their own, side by side (1: one
+        batch).  Results do not depend on it."""
+        check(lbswim_ensemble_set_lanes(self.impl, lanes))
+
+    @property
+    def lanes(self):
+        cdef int k = 0
+        check(lbswim_ensemble_lanes(self.impl, &k))
+        return k
 
     def fused_sites(self):
         """Lattice sites of the members the batched fused step advances."""

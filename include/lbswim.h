@@ -361,8 +361,10 @@ int lbswim_ensemble_set_force_mode(lbswim_ensemble* e, int mode);
  * (256 sites each) over steps_per_trial (+2 warm-up) real steps of the
  * ensemble -- the members advance by those steps -- and keeps the fastest
  * pair, then, with it, the batched fused step's tile shapes
- * (lbswim_ensemble_set_fusion) where some member qualifies; *choice receives
- * the order (lbswim_ensemble_tuning and lbswim_ensemble_fusion give the rest). */
+ * (lbswim_ensemble_set_fusion) where some member qualifies, then the number
+ * of lanes (lbswim_ensemble_set_lanes); *choice receives the order
+ * (lbswim_ensemble_tuning, lbswim_ensemble_fusion and lbswim_ensemble_lanes
+ * give the rest). */
 int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choice);
 /* Lattice blocks of 256 sites per work item (workgroup) of the batched
  * lattice launches, 1..64 (default 1). Results do not depend on it. */
@@ -381,6 +383,16 @@ int lbswim_ensemble_set_fusion(lbswim_ensemble* e, int rows, int strip);
 int lbswim_ensemble_fusion(lbswim_ensemble* e, int* rows, int* strip);
 /* Lattice sites of the members the batched fused step advances. */
 int lbswim_ensemble_fused_sites(lbswim_ensemble* e, size_t* fused_sites);
+/* Lanes (1..8, default 1): with more than one, the members are split into
+ * that many groups of about equal lattice sites (largest lattice first to
+ * the lightest group), and each group's batched launches go to a stream of
+ * its own, forked from the caller's stream and joined back every step (a
+ * captured graph holds them as parallel branches), so that one group's
+ * latency-bound list building and moves run beside the others' lattice
+ * sweeps.  Results do not depend on it.  lbswim_ensemble_autotune also
+ * chooses among 1, 2 and 4 lanes. */
+int lbswim_ensemble_set_lanes(lbswim_ensemble* e, int lanes);
+int lbswim_ensemble_lanes(lbswim_ensemble* e, int* lanes);
 /* Per-phase timing of batched steps with HIP events on the streams the
  * launches go to; while any bit of `mask` (1 << LBSWIM_K_*) is set, steps are
  * launched eagerly instead of from graphs.  Recorded: LBSWIM_K_ADDF (swimmer
@@ -389,7 +401,8 @@ int lbswim_ensemble_fused_sites(lbswim_ensemble* e, size_t* fused_sites);
  * stored-force members' chain), _SMOVE, _TMOVE.  A nonzero mask clears
  * earlier records. */
 int lbswim_ensemble_set_timing(lbswim_ensemble* e, unsigned mask);
-/* Sum of recorded durations (ms) and launch count of one phase. */
+/* Sum of recorded durations (ms) and launch count of one phase (with lanes,
+ * summed over the lanes, whose launches of a step run side by side). */
 int lbswim_ensemble_kernel_time(lbswim_ensemble* e, int kernel, double* total_ms, long* count);
 /* Lattice sites of the members whose swimmer force is virtual and that the
  * batched _COLLIDE/_HYDRO launches advance, and of the stored-force members

@@ -108,13 +108,15 @@ def test_autotune_keeps_results():
         lat, system, sw, tr, things = build_member(c, "f64")
         ens.add(lat, things)
         built.append((lat, sw, tr))
-    choice = ens.autotune(3)          # 3 orders x 3 work-item sizes x (2 + 3) real steps
+    # 3 orders x 3 work-item sizes, then 1, 2 and 4 lanes, x (2 + 3) real
+    # steps (no member of these sizes takes the fused step: no tile trials)
+    choice = ens.autotune(3)
     assert choice in (0, 1, 2)
     order, per = ens.tuning
     assert order == choice and per in (1, 2, 4)
-    assert built[0][0].time_step == 9 * (2 + 3)
+    assert built[0][0].time_step == (9 + 3) * (2 + 3)
     ens.step(5)
-    total = 9 * (2 + 3) + 5
+    total = (9 + 3) * (2 + 3) + 5
     for c, (lat, sw, tr) in zip(cases, built):
         assert lat.time_step == total
         assert_same(collect_lbswim(lat, sw, tr), run_alone(c, "f64", total), "after autotune")
@@ -178,3 +180,52 @@ def test_fused_members_match_alone(storage, graph, fusion):
     lat, system, sw, tr = built[0]
     system.Step(2)
     assert_same(collect_lbswim(lat, sw, tr), run_alone(cases[0], storage, steps + 2), "member 0 continued")
+
+
+@pytest.mark.parametrize("storage,graph,lanes", [("f64", True, 2), ("f64", False, 3), ("f32", True, 2),
+                                                 ("f64", True, 8)])
+def test_lanes_match_alone(storage, graph, lanes):
+    """Members split over lanes (groups stepped on streams of their own) --
+    the fused, two-launch and stored-force members of FUSED_CASES together
+    with uneven ones -- evolve exactly as alone; a membership change and a
+    change of the lane count between calls keep them so."""
+    from lbswim.ensemble import Ensemble
+    cases = [make_case(size, 0, 0.3 + 0.1 * i, ns=ns, nt=nt, alpha=0.02, seed=177 + i)
+             for i, (size, ns, nt) in enumerate(FUSED_CASES)] + uneven_cases(4, seed=5)
+    say("ensemble lanes: %s graph=%s lanes=%d, %d members" % (storage, graph, lanes, len(cases)))
+    ens = Ensemble(storage)
+    ens.set_graph(graph)
+    ens.set_lanes(lanes)
+    built = []
+    for c in cases:
+        lat, system, sw, tr, things = build_member(c, storage)
+        built.append((lat, sw, tr, ens.add(lat, things)))
+    ens.step(5)
+    assert ens.lanes == lanes
+    # drop one member, step the rest with one lane fewer
+    ens.remove(built[2][3])
+    ens.set_lanes(max(1, lanes - 1))
+    ens.step(4)
+    for i, (c, (lat, sw, tr, _)) in enumerate(zip(cases, built)):
+        steps = 5 if i == 2 else 9
+        assert lat.time_step == steps
+        assert_same(collect_lbswim(lat, sw, tr), run_alone(c, storage, steps), "member %d" % i)
+
+
+def test_autotune_chooses_lanes_and_keeps_results():
+    from lbswim.ensemble import Ensemble
+    cases = uneven_cases(6, seed=9)
+    ens = Ensemble("f64")
+    built = []
+    for c in cases:
+        lat, system, sw, tr, things = build_member(c, "f64")
+        ens.add(lat, things)
+        built.append((lat, sw, tr))
+    ens.autotune(2)
+    say("ensemble autotune: order/work %s fusion %s lanes %d" % (ens.tuning, ens.fusion, ens.lanes))
+    assert ens.lanes in (1, 2, 4)
+    ens.step(3)
+    total = built[0][0].time_step
+    for i, (c, (lat, sw, tr)) in enumerate(zip(cases, built)):
+        assert lat.time_step == total
+        assert_same(collect_lbswim(lat, sw, tr), run_alone(c, "f64", total), "member %d" % i)

@@ -75,7 +75,7 @@ def make_workload(size, seed=1234):
     return make_case((size, size, size), 0, 0.25, ns=32768, nt=1_000_000, alpha=0.01, seed=seed)
 
 
-def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, fusion=True, tiles=None):
+def time_gpu(case, storage, steps, warmup, force_mode="auto", state_steps=200, fusion=True, tiles=None):
     from cases import build_lbswim
     import lbswim
 
@@ -138,21 +138,33 @@ def time_gpu(case, storage, steps, warmup, force_mode="auto", total_steps=200, f
     res["step_gbs"] = res["step_bytes"] * n / (ms_step * 1e-3) / 1e9
     res["tumbles"] = sw.tumbles()
     res["autotuned"] = list(tuned) if tuned is not None else None
-    # final state for the FP32-vs-FP64 error report (same inputs, same step
-    # count: untimed steps up to total_steps, the tuner's steps included)
-    done = lat.time_step
-    if total_steps > done:
-        system.Step(total_steps - done)
-    res["state_steps"] = max(total_steps, done)
+    choice = (lat.fusion, lat.force_mode)
+    del lat, system, sw, tr
+    # final state for the FP32-vs-FP64 and oracle error reports: a fresh run of
+    # exactly state_steps coupled steps (the configuration's own count) from
+    # the same inputs with the launch choice the run above used (the tuner's
+    # trial steps differ in number between the forms)
+    res["state_steps"] = state_steps
+    res["state"] = final_state(case, storage, choice, state_steps)
+    return res
+
+
+def final_state(case, storage, choice, steps):
+    from cases import build_lbswim
+    (active, rows, strips), force_mode = choice
+    lat, system, sw, tr = build_lbswim(case, storage, force_mode)
+    lat.set_fusion("always" if active else False, rows if active else 0, strips if active else 0)
+    system.Step(steps)
     for name in ("rho", "u"):
         getattr(lat, name).D2H()
     sw.D2H()
     tr.D2H()
-    res["state"] = {"rho": np.array(lat.rho.data).ravel(), "u": np.array(lat.u.data).ravel(),
-                    "sw_r": np.array(sw.r.data), "sw_n": np.array(sw.n.data), "tr_r": np.array(tr.r.data),
-                    "steps": res["state_steps"]}
+    out = {"rho": np.array(lat.rho.data).ravel(), "u": np.array(lat.u.data).ravel(),
+           "sw_r": np.array(sw.r.data), "sw_n": np.array(sw.n.data), "tr_r": np.array(tr.r.data),
+           "steps": lat.time_step}
+    assert out["steps"] == steps, (out["steps"], steps)
     del lat, system, sw, tr
-    return res
+    return out
 
 
 def precision_report(f64, f32, L):
@@ -168,6 +180,7 @@ def precision_report(f64, f32, L):
         return float(np.max(d))
 
     a, b = f32["state"], f64["state"]
+    assert a["steps"] == b["steps"], (a["steps"], b["steps"])
     return {"u_rel": rel(a["u"], b["u"]), "drho_rel": rel(a["rho"] - 1.0, b["rho"] - 1.0),
             "swimmer_r_abs": traj(a["sw_r"], b["sw_r"], L), "swimmer_n_abs": float(np.max(np.abs(a["sw_n"] - b["sw_n"]))),
             "tracer_r_abs": traj(a["tr_r"], b["tr_r"], L),
@@ -435,6 +448,7 @@ def oracle_legs(case, f64_state, f32_state, f32_small, budget_s=240.0):
         return res
     errs = {}
     target = f64_state["steps"]
+    assert f32_state["steps"] == target, (f32_state["steps"], target)
     left = target - (10 + 3 * k)
     if left >= 0 and left * med / k <= budget_s:
         log("oracle: continuing the 256^3 run to %d steps" % target)

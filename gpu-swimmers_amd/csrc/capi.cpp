@@ -1889,19 +1889,11 @@ struct EnsMember {
 // Workgroup orders the autotuner chooses between (results do not depend on it).
 enum { ORDER_MEMBER_MAJOR = 0, ORDER_ROUND_ROBIN = 1, ORDER_LARGEST_FIRST = 2, ORDER_COUNT = 3 };
 
-template <typename T>
-int upload(T** dev, size_t* cap, const std::vector<T>& host) {
-  if (host.empty()) return LBSWIM_OK;
-  if (*cap < host.size()) {
-    dev_free(*dev);
-    *dev = nullptr;
-    *cap = 0;
-    if (int rc = dev_alloc_zero(reinterpret_cast<void**>(dev), sizeof(T) * host.size())) return rc;
-    *cap = host.size();
-  }
-  HIP_TRY(hipMemcpyAsync(*dev, host.data(), sizeof(T) * host.size(), hipMemcpyHostToDevice, stream()));
-  return LBSWIM_OK;
-}
+// Capacity for `need` elements when `cap` does not suffice: at least double,
+// so that the (device-synchronising) reallocations stay rare as membership
+// grows.
+size_t grown(size_t cap, size_t need) { return std::max(need, 2 * cap); }
+
 
 // Reorders per-member blocks: member-major keeps each member contiguous,
 // round-robin interleaves members block by block, largest-first is
@@ -2030,6 +2022,25 @@ struct lbswim_ensemble {
   std::vector<hipStream_t> lane_stream;
   std::vector<hipEvent_t> lane_join;
   hipEvent_t lane_fork = nullptr;
+  // A rebuild waits for no device work: its table copies go through pinned
+  // staging on the library stream (ordered after the launches already queued,
+  // which still read the old tables), the staging is reused once those copies
+  // have run (event `staged`), and the graphs of the old tables are destroyed
+  // once their last launch has run (event per retired graph).
+  // Two stagings used in turn, so a build waits at most for the copies of
+  // the build before the last one.
+  unsigned char* h_stage[2] = {nullptr, nullptr};
+  size_t cap_stage[2] = {0, 0};
+  hipEvent_t staged[2] = {nullptr, nullptr};
+  bool staged_pending[2] = {false, false};
+  int stage_cur = 0;
+  size_t stage_at = 0;
+  struct Retired {
+    hipGraphExec_t exec;
+    hipGraph_t graph;
+    hipEvent_t done;
+  };
+  std::vector<Retired> retired;
 };
 
 namespace {
@@ -2065,24 +2076,102 @@ struct EnsTimer {
   }
 };
 
-// Destroys the captured graphs once every launch of them has finished.
-void ens_drop_graph(lbswim_ensemble* e) {
-  if (!e->exec[0] && !e->exec[1] && !e->graph[0] && !e->graph[1]) return;
-  (void)hipStreamSynchronize(stream());
+// Copies a host table to the device through the ensemble's pinned staging
+// (ens_stage_begin reserved room for it), on the library stream.
+template <typename T>
+int upload(lbswim_ensemble* e, T** dev, size_t* cap, const std::vector<T>& host) {
+  if (host.empty()) return LBSWIM_OK;
+  if (*cap < host.size()) {
+    dev_free(*dev);
+    *dev = nullptr;
+    const size_t c = grown(*cap, host.size());
+    *cap = 0;
+    if (int rc = dev_alloc_zero(reinterpret_cast<void**>(dev), sizeof(T) * c)) return rc;
+    *cap = c;
+  }
+  const size_t bytes = sizeof(T) * host.size();
+  const int k = e->stage_cur;
+  if (e->stage_at + bytes > e->cap_stage[k]) return fail(LBSWIM_EINVAL, "ensemble table staging overflow");
+  unsigned char* src = e->h_stage[k] + e->stage_at;
+  std::memcpy(src, host.data(), bytes);
+  e->stage_at = (e->stage_at + bytes + 255) / 256 * 256;
+  HIP_TRY(hipMemcpyAsync(*dev, src, bytes, hipMemcpyHostToDevice, stream()));
+  return LBSWIM_OK;
+}
+
+// Room for `bytes` of tables (plus alignment) in the pinned staging, once the
+// copies of the last build have run.
+int ens_stage_begin(lbswim_ensemble* e, size_t bytes) {
+  const int k = e->stage_cur ^= 1;
+  if (e->staged_pending[k]) {
+    HIP_TRY(hipEventSynchronize(e->staged[k]));
+    e->staged_pending[k] = false;
+  }
+  if (e->cap_stage[k] < bytes) {
+    if (e->h_stage[k]) (void)hipHostFree(e->h_stage[k]);
+    e->h_stage[k] = nullptr;
+    e->cap_stage[k] = 0;
+    const size_t c = grown(0, bytes);
+    if (hipHostMalloc(reinterpret_cast<void**>(&e->h_stage[k]), c, hipHostMallocDefault) != hipSuccess)
+      return fail(LBSWIM_ENOMEM, "pinned staging of %zu bytes failed", c);
+    e->cap_stage[k] = c;
+  }
+  if (!e->staged[k]) HIP_TRY(hipEventCreateWithFlags(&e->staged[k], hipEventDisableTiming));
+  e->stage_at = 0;
+  return LBSWIM_OK;
+}
+
+int ens_stage_end(lbswim_ensemble* e) {
+  const int k = e->stage_cur;
+  HIP_TRY(hipEventRecord(e->staged[k], stream()));
+  e->staged_pending[k] = true;
+  return LBSWIM_OK;
+}
+
+// Destroys the retired graphs whose launches have all run (all of them,
+// waiting, with `wait`).
+void ens_reap(lbswim_ensemble* e, bool wait) {
+  std::vector<lbswim_ensemble::Retired> keep;
+  for (auto& r : e->retired) {
+    if (!wait && hipEventQuery(r.done) == hipErrorNotReady) {
+      keep.push_back(r);
+      continue;
+    }
+    (void)hipEventSynchronize(r.done);
+    if (r.exec) (void)hipGraphExecDestroy(r.exec);
+    if (r.graph) (void)hipGraphDestroy(r.graph);
+    (void)hipEventDestroy(r.done);
+  }
+  e->retired.swap(keep);
+}
+
+// Takes the captured graphs out of use; they are destroyed once every launch
+// of them queued so far (on the library stream) has run -- `wait`: before
+// returning.
+void ens_drop_graph(lbswim_ensemble* e, bool wait = true) {
   for (int p = 0; p < 2; ++p) {
-    if (e->exec[p]) (void)hipGraphExecDestroy(e->exec[p]);
-    if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+    if (!e->exec[p] && !e->graph[p]) continue;
+    hipEvent_t done = nullptr;
+    if (hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess &&
+        hipEventRecord(done, stream()) == hipSuccess) {
+      e->retired.push_back(lbswim_ensemble::Retired{e->exec[p], e->graph[p], done});
+    } else {  // no event: wait here instead
+      if (done) (void)hipEventDestroy(done);
+      (void)hipStreamSynchronize(stream());
+      if (e->exec[p]) (void)hipGraphExecDestroy(e->exec[p]);
+      if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+    }
     e->exec[p] = nullptr;
     e->graph[p] = nullptr;
   }
+  ens_reap(e, wait);
 }
 
 int ens_build_lanes(lbswim_ensemble* e);
 void ens_drop_lanes(lbswim_ensemble* e);
 
 int ens_build(lbswim_ensemble* e) {
-  HIP_TRY(hipStreamSynchronize(stream()));
-  ens_drop_graph(e);
+  ens_drop_graph(e, false);
   if (e->lanes > 1) return ens_build_lanes(e);
   ens_drop_lanes(e);
   std::vector<LatDesc> lat;
@@ -2183,13 +2272,23 @@ int ens_build(lbswim_ensemble* e) {
   const std::vector<WorkItem> fuseblk = order_work(latwork[2], e->order, weight[2]);
   const std::vector<WorkItem> fusework = order_work(fusetiles, e->order, weight[2]);
   int rc;
-  if ((rc = upload(&e->d_lat, &e->cap_lat, lat)) || (rc = upload(&e->d_storedwork, &e->cap_storedwork, storedwork)) ||
-      (rc = upload(&e->d_vfwork, &e->cap_vfwork, vfwork)) || (rc = upload(&e->d_sw, &e->cap_sw, sw)) ||
-      (rc = upload(&e->d_swwork, &e->cap_swwork, swwork)) || (rc = upload(&e->d_ptwork, &e->cap_ptwork, ptwork)) ||
-      (rc = upload(&e->d_tr, &e->cap_tr, tr)) || (rc = upload(&e->d_trwork, &e->cap_trwork, trwork)) ||
-      (rc = upload(&e->d_fusework, &e->cap_fusework, fusework)) ||
-      (rc = upload(&e->d_entrywork, &e->cap_entrywork, entrywork)) ||
-      (rc = upload(&e->d_fuseblk, &e->cap_fuseblk, fuseblk)))
+  size_t stage_bytes = 0;
+  auto add = [&](size_t b) { stage_bytes += (b + 255) / 256 * 256; };
+  add(sizeof(LatDesc) * lat.size());
+  add(sizeof(SwimDesc) * sw.size());
+  add(sizeof(TracerDesc) * tr.size());
+  for (const std::vector<WorkItem>* v : std::initializer_list<const std::vector<WorkItem>*>{
+           &storedwork, &vfwork, &swwork, &ptwork, &trwork, &fusework, &entrywork, &fuseblk})
+    add(sizeof(WorkItem) * v->size());
+  if ((rc = ens_stage_begin(e, stage_bytes))) return rc;
+  if ((rc = upload(e, &e->d_lat, &e->cap_lat, lat)) ||
+      (rc = upload(e, &e->d_storedwork, &e->cap_storedwork, storedwork)) ||
+      (rc = upload(e, &e->d_vfwork, &e->cap_vfwork, vfwork)) || (rc = upload(e, &e->d_sw, &e->cap_sw, sw)) ||
+      (rc = upload(e, &e->d_swwork, &e->cap_swwork, swwork)) || (rc = upload(e, &e->d_ptwork, &e->cap_ptwork, ptwork)) ||
+      (rc = upload(e, &e->d_tr, &e->cap_tr, tr)) || (rc = upload(e, &e->d_trwork, &e->cap_trwork, trwork)) ||
+      (rc = upload(e, &e->d_fusework, &e->cap_fusework, fusework)) ||
+      (rc = upload(e, &e->d_entrywork, &e->cap_entrywork, entrywork)) ||
+      (rc = upload(e, &e->d_fuseblk, &e->cap_fuseblk, fuseblk)) || (rc = ens_stage_end(e)))
     return rc;
   e->n_storedwork = int(storedwork.size());
   e->n_vfwork = int(vfwork.size());
@@ -2224,12 +2323,13 @@ int ens_build(lbswim_ensemble* e) {
   for (const auto& a : sw)
     if (lat[a.member].force_path != FORCE_RANGE_STORED) lcap += spread_list_capacity(2 * size_t(a.num));
   auto grow_to = [&](auto** p, size_t* cap, size_t need, size_t elem) -> int {
-    if (*cap >= need) return LBSWIM_OK;
-    dev_free(*p);
+    if (*cap >= need && *p) return LBSWIM_OK;
+    dev_free(*p);  // waits for the device: rare, capacities at least double
     *p = nullptr;
+    const size_t c = std::max<size_t>(1, grown(*cap, need));
     *cap = 0;
-    if (int r = dev_alloc_zero(reinterpret_cast<void**>(p), elem * (need ? need : 1))) return r;
-    *cap = need;
+    if (int r = dev_alloc_zero(reinterpret_cast<void**>(p), elem * c)) return r;
+    *cap = c;
     return LBSWIM_OK;
   };
   const size_t nt = size_t(e->nlists);
@@ -2246,7 +2346,6 @@ int ens_build(lbswim_ensemble* e) {
         (rc = grow_to(&e->edge, &e->cap_edge, 6 * nt, 8)))
       return rc;
   }
-  HIP_TRY(hipStreamSynchronize(stream()));
   e->parity = 0;
   e->dirty = false;
   return LBSWIM_OK;
@@ -2509,7 +2608,11 @@ int lbswim_ensemble_create(int storage, lbswim_ensemble** out) {
 void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   if (!e) return;
   if (g_rt.ready) (void)hipStreamSynchronize(stream());
-  ens_drop_graph(e);
+  ens_drop_graph(e, true);
+  for (int k = 0; k < 2; ++k) {
+    if (e->staged[k]) (void)hipEventDestroy(e->staged[k]);
+    if (e->h_stage[k]) (void)hipHostFree(e->h_stage[k]);
+  }
   for (auto& r : e->recs) {
     (void)hipEventDestroy(r.start);
     (void)hipEventDestroy(r.stop);

@@ -912,6 +912,8 @@ int lbswim_lattice_d2h(lbswim_lattice* lat, int field) {
 
 // ---- particle storage ----------------------------------------------------------
 
+}  // extern "C"
+
 namespace {
 
 // Moves between re-orderings of a particle array's storage.
@@ -1014,6 +1016,7 @@ int pstore_maybe_reorder(PStore& st, const Geom& g) {
 double* fld(PStore& st, int f) { return reinterpret_cast<double*>(st.d[f]); }
 
 }  // namespace
+extern "C" {
 
 // ---- swimmers ---------------------------------------------------------------
 
@@ -1271,6 +1274,8 @@ int lbswim_tracers_slots(lbswim_tracers* tr, const int** device) {
 
 // ---- coupled step -------------------------------------------------------------
 
+}  // extern "C"
+
 namespace {
 
 int grow(void** p, size_t* cap, size_t need, size_t elem) {
@@ -1446,6 +1451,7 @@ int scatter_prepare(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
 }
 
 }  // namespace
+extern "C" {
 
 int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr, int ntr,
                        int nsteps) {
@@ -1633,6 +1639,8 @@ int lbswim_system_autotune(lbswim_lattice* lat, lbswim_swimmers* const* sw, int 
 
 // ---- coupled step without allocation -----------------------------------------
 
+}  // extern "C"
+
 namespace {
 
 constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1671,6 +1679,7 @@ FixedSpread carve_spread(unsigned char* base, uint32_t nl, size_t npts) {
 }
 
 }  // namespace
+extern "C" {
 
 int lbswim_system_step_noalloc(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw, lbswim_tracers* const* tr,
                                int ntr, int nsteps) {

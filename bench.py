@@ -264,8 +264,8 @@ def bench_ensemble(storage, steps, nsims=64, tune_steps=5, graph=True, sequentia
         byts = fsites * fused_bytes(storage, True)
         roof = {"kernel": "kb_step_hydro<%s, virtual force>" % storage, "bound": "hbm", "achieved": gbs(byts, avg),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs(byts, avg) / HBM_PEAK_GBS, 4),
-                "traffic": load_ensemble_traffic(storage, "kb_step_hydro<%s, virtual force>" % storage)
-                if sites == 6069760 else None,
+                "traffic": load_ensemble_traffic(storage, "kb_step_hydro<%s, virtual force>" % storage,
+                                                 list(ens.fusion)) if sites == 6069760 else None,
                 "algorithmic_bytes_per_step": byts, "launch_ms_per_step": round(avg, 4), "lanes": ens.lanes}
         if c_n:
             roof["collide_ms_per_step"] = round(c_ms / c_n, 4)
@@ -474,29 +474,32 @@ def oracle_legs(case, f64_state, f32_state, f32_small, budget_s=240.0):
     return res
 
 
-def load_ensemble_traffic(storage, kernel):
-    """HBM bytes per launch of the ensemble's batched collide from the
-    committed rocprofv3 --pmc summary of it (null if there is none)."""
-    path = os.path.join(ROOT, "profiles", "kb_collide_%s_ensemble_pmc.json" % storage)
+def load_pmc(name, kernel, tiles=None):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary profiles/<name>_pmc.json, or None when there is none or it was
+    measured on another launch shape (tiles: the fused launch's rows and
+    strips, or rows and strip planes in an ensemble)."""
     try:
-        with open(path) as fh:
+        with open(os.path.join(ROOT, "profiles", name + "_pmc.json")) as fh:
             d = json.load(fh)
     except (OSError, ValueError):
         return None
-    return d.get("hbm_bytes_per_launch") if d.get("kernel") == kernel else None
-
-
-def load_traffic(kernel):
-    """HBM bytes per launch of `kernel` (k_<name>_[vf_]<form>) from the
-    committed rocprofv3 --pmc summary of it (null if there is none)."""
-    name = kernel[2:].replace("vf_", "")
-    path = os.path.join(ROOT, "profiles", "%s_256cubed_pmc.json" % name)
-    try:
-        with open(path) as fh:
-            d = json.load(fh)
-    except (OSError, ValueError):
+    if d.get("kernel") != kernel:
         return None
-    return d.get("hbm_bytes_per_launch") if d.get("kernel") == kernel else None
+    if tiles is not None and d.get("tiles") is not None and list(d["tiles"]) != list(tiles):
+        return None
+    return d.get("hbm_bytes_per_launch")
+
+
+def load_ensemble_traffic(storage, kernel, tiles=None):
+    """HBM bytes per launch of the ensemble's batched fused step
+    (kb_step_hydro) or batched collide (kb_collide)."""
+    return load_pmc("%s_%s_ensemble" % (kernel.split("<")[0], storage), kernel, tiles)
+
+
+def load_traffic(kernel, tiles=None):
+    """HBM bytes per launch of `kernel` (k_<name>_[vf_]<form>) at 256^3."""
+    return load_pmc("%s_256cubed" % kernel[2:].replace("vf_", ""), kernel, tiles)
 
 
 def main():
@@ -566,7 +569,7 @@ def main():
     main_form = "f64" if "f64" in res else forms[0]
     m = res[main_form]
     n = a.size ** 3
-    traffic = load_traffic(m["main_kernel"])
+    traffic = load_traffic(m["main_kernel"], m["fusion"][1:] if m["fused"] else None)
     out = {
         "metric": "MLUPS of the coupled step (ZeroForce+AddForces+collide/stream+CalcHydro+Move) at %d^3, "
                   "FP64 drop-in" % a.size,

@@ -686,14 +686,20 @@ __device__ __forceinline__ double pair_sum(const PairGather& G, int c, double* _
 // group of 8 per tracer (pair_gather, pair_sum).
 constexpr int TGROUPS = TRACER_GROUPS;  // tracers per workgroup: 512 threads, contiguous in storage order
 constexpr int TBLOCK = TGROUPS * PAIR;
+// Tracers per workgroup of the single-array move: 256 threads, one wavefront
+// per SIMD, so that at 86 VGPRs five workgroups fill a CU's 5 wavefronts per
+// SIMD (512-thread workgroups leave one slot per SIMD empty).  At 256^3 with
+// 1e6 tracers 2 % faster than 64 tracers; 16 and 8 are slower beside the
+// list building.
+constexpr int TMOVE_GROUPS = 32;
 static_assert(SUM_PLANES == 1, "pair_sum stages one plane at a time");
 
-template <typename T, bool AOS = false>
+template <typename T, bool AOS = false, int TG = TGROUPS>
 __device__ __forceinline__ void tracers_move_body(int nt, double* __restrict__ r, double* __restrict__ sd,
                                                   double* __restrict__ v, const T* __restrict__ u, const Geom& g,
                                                   uint32_t blk, double* __restrict__ lds) {
   const int grp = int(threadIdx.x) / PAIR, c = int(threadIdx.x) % PAIR;
-  const int it = int(blk) * TGROUPS + grp;
+  const int it = int(blk) * TG + grp;
   if (it >= nt) return;  // whole group: the lanes of a group share `it`
   double* terms = lds + grp * PLANE_STRIDE;
   const double R[3] = {r[it], r[size_t(nt) + it], r[2 * size_t(nt) + it]};
@@ -727,13 +733,13 @@ __global__ __launch_bounds__(PBLOCK) void k_swimmers_move(int ns, SwimParams cp,
 }
 
 template <typename T, bool AOS>
-__global__ __launch_bounds__(TBLOCK) void k_tracers_move(int nt, double* __restrict__ r, double* __restrict__ sd,
-                                                         double* __restrict__ v, const T* __restrict__ u, Geom g,
-                                                         uint32_t nb) {
-  __shared__ __attribute__((aligned(16))) double lds[TGROUPS * PLANE_STRIDE];
+__global__ __launch_bounds__(TMOVE_GROUPS * PAIR) void k_tracers_move(int nt, double* __restrict__ r,
+                                                                     double* __restrict__ sd, double* __restrict__ v,
+                                                                     const T* __restrict__ u, Geom g, uint32_t nb) {
+  __shared__ __attribute__((aligned(16))) double lds[TMOVE_GROUPS * PLANE_STRIDE];
   const uint32_t blk = xcd_block(round_xcd(nb));
   if (blk >= nb) return;
-  tracers_move_body<T, AOS>(nt, r, sd, v, u, g, blk, lds);
+  tracers_move_body<T, AOS, TMOVE_GROUPS>(nt, r, sd, v, u, g, blk, lds);
 }
 
 // Batched moves: WorkItem.member indexes the particle-array table, unit is
@@ -1440,11 +1446,12 @@ template <typename T>
 hipError_t launch_tracers_move(int nt, double* r, double* sd, double* v, const T* u, bool aos, const Geom& g,
                                hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  const uint32_t nb = uint32_t((nt + TGROUPS - 1) / TGROUPS);
+  const uint32_t nb = uint32_t((nt + TMOVE_GROUPS - 1) / TMOVE_GROUPS);
+  const dim3 block(TMOVE_GROUPS * PAIR);
   if (aos)
-    hipLaunchKernelGGL((k_tracers_move<T, true>), dim3(round_xcd(nb)), dim3(TBLOCK), 0, s, nt, r, sd, v, u, g, nb);
+    hipLaunchKernelGGL((k_tracers_move<T, true>), dim3(round_xcd(nb)), block, 0, s, nt, r, sd, v, u, g, nb);
   else
-    hipLaunchKernelGGL((k_tracers_move<T, false>), dim3(round_xcd(nb)), dim3(TBLOCK), 0, s, nt, r, sd, v, u, g, nb);
+    hipLaunchKernelGGL((k_tracers_move<T, false>), dim3(round_xcd(nb)), block, 0, s, nt, r, sd, v, u, g, nb);
   return hipGetLastError();
 }
 

@@ -2031,11 +2031,13 @@ struct lbswim_ensemble {
   std::vector<hipStream_t> lane_stream;
   std::vector<hipEvent_t> lane_join;
   hipEvent_t lane_fork = nullptr;
-  // A rebuild waits for no device work: its table copies go through pinned
-  // staging on the library stream (ordered after the launches already queued,
-  // which still read the old tables), the staging is reused once those copies
-  // have run (event `staged`), and the graphs of the old tables are destroyed
-  // once their last launch has run (event per retired graph).
+  // A rebuild waits for no device work unless a table outgrows its device
+  // buffer (hipFree of the old one waits for the device; capacities grow
+  // geometrically): its table copies go through pinned staging on the library
+  // stream (ordered after the launches already queued, which still read the
+  // old tables), the staging is reused once those copies have run (event
+  // `staged`), and the graphs of the old tables are destroyed once their last
+  // launch has run (event per retired graph).
   // Two stagings used in turn, so a build waits at most for the copies of
   // the build before the last one.
   unsigned char* h_stage[2] = {nullptr, nullptr};

@@ -322,7 +322,11 @@ def bench_serving(nreq, rate, seed=1234):
     return {"rate_per_s": rate, "batched": r(batched), "sequential": r(seq),
             "throughput_ratio": round(batched["mlups"] / seq["mlups"], 2),
             "p99_ratio": round(seq["p99_s"] / batched["p99_s"], 2),
-            "mix_resident_mlups": round(mix, 1), "batched_vs_mix_resident": round(batched["mlups"] / mix, 3)}
+            "mix_resident_mlups": round(mix, 1), "batched_vs_mix_resident": round(batched["mlups"] / mix, 3),
+            # the stream's own floor: each request started at its arrival and run
+            # as fast as it runs alone (sequential leg's service times)
+            "makespan_floor_s": round(seq["floor_s"], 4),
+            "batched_wall_vs_floor": round(seq["floor_s"] / batched["wall_s"], 3)}
 
 
 def serving_mix_bound(reqs, steps=40):

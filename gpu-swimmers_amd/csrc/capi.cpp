@@ -200,6 +200,15 @@ struct lbswim_lattice {
   size_t sp_rec_cap = 0;      // list entries sp_rec and sp_wz hold (fused step)
   RunEntry* sp_rec = nullptr;
   int* sp_wz = nullptr;
+  // Stored force by site (launch_site_force): per-site counts, offsets and
+  // cursors, and ss_cap point entries (slot, id, ordered product).
+  uint32_t* ss_counts = nullptr;
+  uint32_t* ss_off = nullptr;
+  uint32_t* ss_cursor = nullptr;
+  size_t ss_cap = 0;
+  uint32_t* ss_local = nullptr;
+  uint32_t* ss_id = nullptr;
+  double* ss_sorted = nullptr;
   // Virtual force: after a fused step the force lives in the sorted point
   // lists above and d_force is stale until materialize_force writes it.
   int force_mode = LBSWIM_FORCE_AUTO;
@@ -529,6 +538,12 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->sp_edge);
   dev_free(lat->sp_rec);
   dev_free(lat->sp_wz);
+  dev_free(lat->ss_counts);
+  dev_free(lat->ss_off);
+  dev_free(lat->ss_cursor);
+  dev_free(lat->ss_local);
+  dev_free(lat->ss_id);
+  dev_free(lat->ss_sorted);
   dev_free(lat->sp_pts);
   dev_free(lat->sp_slot);
   dev_free(lat->sp_list);
@@ -1289,23 +1304,39 @@ int grow(void** p, size_t* cap, size_t need, size_t elem) {
   return LBSWIM_OK;
 }
 
-// Mean entries per list up to which the collide and CalcHydro kernels
-// evaluate the force themselves (each wavefront walks its list twice per
-// step); denser systems write the force array once instead.
-constexpr size_t VF_MAX_MEAN_ENTRIES = 32;
+// The collide and CalcHydro kernels (or the fused step) evaluate the force
+// themselves, walking their lists every step, where the lists are short; a
+// denser system writes the force array once per step instead
+// (launch_site_force and the like), whose cost grows with the lattice.  A
+// single system at 32^3-128^3 with 250-4000 swimmers measured the virtual
+// force faster only up to about 2 (64^3), 3 (86^3), 4 (96^3) and beyond 4
+// (128^3) mean entries per list: the limit taken is 1 + n / 2^18 entries (65
+// at 256^3).  In an ensemble the virtual-force members share the batched
+// lattice launches (the batched fused step among them) and the stored ones a
+// chain beside them; there the limit stays at 32 entries, which measured 10-12 %
+// faster on the 64-member bench ensemble than the single-system limit.
+constexpr size_t VF_BATCH_MAX_MEAN_ENTRIES = 32;
 
-bool use_virtual_force(int mode, size_t npts, size_t lists) {
+bool use_virtual_force(int mode, size_t npts, size_t lists, size_t n, bool batched) {
   // a stencil reaches about 17 lists of 64 sites
-  return mode == LBSWIM_FORCE_VIRTUAL || (mode == LBSWIM_FORCE_AUTO && npts * 17 <= VF_MAX_MEAN_ENTRIES * lists);
+  if (mode != LBSWIM_FORCE_AUTO) return mode == LBSWIM_FORCE_VIRTUAL;
+  if (batched) return npts * 17 <= VF_BATCH_MAX_MEAN_ENTRIES * lists;
+  return npts * 17 * (size_t(1) << 18) <= lists * ((size_t(1) << 18) + n);
 }
 
 // How a system's swimmer force is formed: virtual from its lists where
 // sparse; otherwise stored, from all its points when they are few enough
 // that every run walking them all costs less than building lists that would
 // each hold many of them, else from lists.
-int force_path(int mode, size_t npts, const Geom& g) {
-  if (use_virtual_force(mode, npts, spread_lists(g))) return FORCE_LISTS_VIRTUAL;
+int stored_force_path(size_t npts, const Geom& g) {
+  // each site walks only its own points where every side is >= 4
+  if (site_force_ok(g) && npts > SITE_FORCE_MIN_POINTS) return FORCE_SITES_STORED;
   return npts <= RANGE_FORCE_MAX_POINTS ? FORCE_RANGE_STORED : FORCE_LISTS_STORED;
+}
+
+int force_path(int mode, size_t npts, const Geom& g, bool batched = false) {
+  if (use_virtual_force(mode, npts, spread_lists(g), g.n, batched)) return FORCE_LISTS_VIRTUAL;
+  return stored_force_path(npts, g);
 }
 
 // ZeroForce + AddForces of every swimmer array, as one deterministic gather.
@@ -1335,7 +1366,46 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
   int path = force_path(lat->force_mode, npts, lat->geom);
   // the fused launch walks a virtual force only where each 64-site run is one z-row
   if (path == FORCE_LISTS_VIRTUAL && lat->geom.nz % 64 != 0 && fusion_rows(lat) > 0)
-    path = npts <= RANGE_FORCE_MAX_POINTS ? FORCE_RANGE_STORED : FORCE_LISTS_STORED;
+    path = stored_force_path(npts, lat->geom);
+  if (path == FORCE_SITES_STORED) {
+    // dense: per-site segments of the points' stencil entries
+    const size_t n = lat->geom.n, nent = 64 * npts;
+    int rc;
+    if (!lat->ss_counts &&
+        ((rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->ss_counts), 4 * n)) ||
+         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->ss_off), 4 * spread_off_entries(uint32_t(n)))) ||
+         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->ss_cursor), 4 * n))))
+      return rc;
+    if (lat->ss_cap < nent) {
+      size_t c1 = lat->ss_cap, c2 = lat->ss_cap, c3 = lat->ss_cap;
+      if ((rc = grow(reinterpret_cast<void**>(&lat->ss_local), &c1, nent, 4)) ||
+          (rc = grow(reinterpret_cast<void**>(&lat->ss_id), &c2, nent, 4)) ||
+          (rc = grow(reinterpret_cast<void**>(&lat->ss_sorted), &c3, nent, 3 * sizeof(double))))
+        return rc;
+      lat->ss_cap = nent;
+    }
+    KernelTimer t(lat, LBSWIM_K_ADDF);
+    hipError_t e = hipSuccess;
+    int offset = 0;
+    for (int i = 0; i < nsw && e == hipSuccess; ++i) {
+      PStore& st = sw[i]->st;
+      e = launch_points(st.num, offset, to_kernel(sw[i]->common_dev), fld(st, LBSWIM_PART_R), fld(st, LBSWIM_PART_N),
+                        st.perm, lat->geom, pts, lat->sp_slot, stream());
+      offset += st.num;
+    }
+    if (e == hipSuccess)
+      e = lat->storage == LBSWIM_STORE_F64
+              ? launch_site_force<double>(int(npts), lat->geom, pts, lat->ss_counts, lat->ss_off, lat->ss_cursor,
+                                          lat->ss_local, lat->ss_id, lat->ss_sorted,
+                                          static_cast<double*>(lat->d_force), stream())
+              : launch_site_force<float>(int(npts), lat->geom, pts, lat->ss_counts, lat->ss_off, lat->ss_cursor,
+                                         lat->ss_local, lat->ss_id, lat->ss_sorted,
+                                         static_cast<float*>(lat->d_force), stream());
+    t.done();
+    HIP_TRY(e);
+    lat->force_virtual = false;
+    return LBSWIM_OK;
+  }
   if (path == FORCE_RANGE_STORED) {
     // few points, dense: every run walks all of them, no lists
     KernelTimer t(lat, LBSWIM_K_ADDF);
@@ -2001,6 +2071,20 @@ struct lbswim_ensemble {
   uint32_t* slot_ids = nullptr;  // point ids in storage order
   size_t cap_slot = 0;
   bool dense_lists = false;      // aggregate the list building's atomics
+  // site segments of the FORCE_SITES_STORED members (launch_batch_site_lists)
+  int nsites = 0;
+  uint32_t* ss_counts = nullptr;
+  size_t cap_ss_counts = 0;
+  uint32_t* ss_off = nullptr;
+  size_t cap_ss_off = 0;
+  uint32_t* ss_cursor = nullptr;
+  size_t cap_ss_cursor = 0;
+  uint32_t* ss_local = nullptr;
+  size_t cap_ss_local = 0;
+  uint32_t* ss_eid = nullptr;
+  size_t cap_ss_eid = 0;
+  double* ss_sorted = nullptr;
+  size_t cap_ss_sorted = 0;
   // stored-force members' chain forked to the ensemble's side stream (see
   // ens_enqueue_step)
   bool fork_stored = false;
@@ -2195,7 +2279,8 @@ int ens_build(lbswim_ensemble* e) {
   std::vector<SwimDesc> sw;
   std::vector<WorkItem> swwork, ptwork, trwork;
   std::vector<TracerDesc> tr;
-  uint32_t tile_base = 0, point_base = 0;
+  uint32_t tile_base = 0, point_base = 0, site_base = 0;
+  size_t site_points = 0;  // points of the FORCE_SITES_STORED members
   for (auto& m : e->members) {
     if (!m.lat) continue;
     lbswim_lattice* L = m.lat;
@@ -2214,10 +2299,15 @@ int ens_build(lbswim_ensemble* e) {
     d.cur = L->cur;
     size_t member_pts = 0;
     for (lbswim_swimmers* s : m.sw) member_pts += 2 * size_t(s->st.num);
-    // the same density rule as a single lattice, member by member
-    d.force_path = force_path(e->force_mode, member_pts, L->geom);
-    d.tile_base = d.force_path == FORCE_RANGE_STORED ? 0u : tile_base;
-    if (d.force_path != FORCE_RANGE_STORED) tile_base += spread_lists(L->geom);
+    // the batch's density rule, member by member
+    d.force_path = force_path(e->force_mode, member_pts, L->geom, true);
+    d.tile_base = has_lists(d.force_path) ? tile_base : 0u;
+    if (has_lists(d.force_path)) tile_base += spread_lists(L->geom);
+    d.site_base = d.force_path == FORCE_SITES_STORED ? site_base : 0u;
+    if (d.force_path == FORCE_SITES_STORED) {
+      site_base += L->geom.n;
+      site_points += member_pts;
+    }
     d.point_first = point_base;
     d.point_count = uint32_t(member_pts);
     const bool fused = e->fuse_rows > 0 && d.force_path == FORCE_LISTS_VIRTUAL &&
@@ -2309,7 +2399,7 @@ int ens_build(lbswim_ensemble* e) {
   // aggregated atomics when any member with lists has dense ones
   e->dense_lists = false;
   for (size_t i = 0; i < lat.size(); ++i) {
-    if (lat[i].force_path == FORCE_RANGE_STORED) continue;
+    if (!has_lists(lat[i].force_path)) continue;
     e->dense_lists = e->dense_lists || dense_lists(lat[i].point_count, spread_lists(lat[i].geom));
   }
   e->sites_virtual = e->sites_stored = e->sites_fused = 0;
@@ -2332,7 +2422,7 @@ int ens_build(lbswim_ensemble* e) {
   // spreading workspace
   size_t lcap = 0;  // list entries of the members with lists
   for (const auto& a : sw)
-    if (lat[a.member].force_path != FORCE_RANGE_STORED) lcap += spread_list_capacity(2 * size_t(a.num));
+    if (has_lists(lat[a.member].force_path)) lcap += spread_list_capacity(2 * size_t(a.num));
   auto grow_to = [&](auto** p, size_t* cap, size_t need, size_t elem) -> int {
     if (*cap >= need && *p) return LBSWIM_OK;
     dev_free(*p);  // waits for the device: rare, capacities at least double
@@ -2351,6 +2441,17 @@ int ens_build(lbswim_ensemble* e) {
       (rc = grow_to(&e->slot_ids, &e->cap_slot, size_t(e->npts), 4)) ||
       (rc = grow_to(&e->sorted, &cap_sorted, lcap, 4)) || (rc = grow_to(&e->list, &e->cap_list, lcap, 4)))
     return rc;
+  e->nsites = int(site_base);
+  if (site_base > 0) {
+    const size_t ns = site_base;
+    if ((rc = grow_to(&e->ss_counts, &e->cap_ss_counts, ns, 4)) ||
+        (rc = grow_to(&e->ss_off, &e->cap_ss_off, spread_off_entries(uint32_t(ns)), 4)) ||
+        (rc = grow_to(&e->ss_cursor, &e->cap_ss_cursor, ns, 4)) ||
+        (rc = grow_to(&e->ss_local, &e->cap_ss_local, 64 * size_t(e->npts), 4)) ||
+        (rc = grow_to(&e->ss_eid, &e->cap_ss_eid, 64 * site_points, 4)) ||
+        (rc = grow_to(&e->ss_sorted, &e->cap_ss_sorted, 3 * 64 * site_points, 8)))
+      return rc;
+  }
   if (!fusework.empty()) {  // RunEntry records of every list entry, run-end forces of every list
     size_t cap_wz = e->cap_rec;
     if ((rc = grow_to(&e->rec, &e->cap_rec, lcap, sizeof(RunEntry))) || (rc = grow_to(&e->wz, &cap_wz, lcap, 4)) ||
@@ -2455,8 +2556,10 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
     if (int rc = fork_now()) return rc;
   auto stored_chain = [&]() -> int {
     EnsTimer t(e, LBSWIM_K_STORED, sb);
+    HIP_TRY(launch_batch_site_lists(e->d_lat, e->npts, e->nsites, e->pts, e->ss_counts, e->ss_off, e->ss_cursor,
+                                    e->ss_local, e->ss_eid, e->ss_sorted, sb));
     HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->per, e->pts, e->off,
-                                         e->sorted, sb));
+                                         e->sorted, e->ss_off, e->ss_sorted, sb));
     HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb));
     HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb));
     t.done();
@@ -2538,8 +2641,10 @@ int ens_materialize_force(lbswim_ensemble* e) {
     const int nw = k ? e->n_fuseblk : e->n_vfwork;
     const WorkItem* w = k ? e->d_fuseblk : e->d_vfwork;
     hipError_t r = e->storage == LBSWIM_STORE_F64
-                       ? launch_batch_spread_force<double>(e->d_lat, nw, w, e->per, e->pts, e->off, e->sorted, stream())
-                       : launch_batch_spread_force<float>(e->d_lat, nw, w, e->per, e->pts, e->off, e->sorted, stream());
+                       ? launch_batch_spread_force<double>(e->d_lat, nw, w, e->per, e->pts, e->off, e->sorted,
+                                                           e->ss_off, e->ss_sorted, stream())
+                       : launch_batch_spread_force<float>(e->d_lat, nw, w, e->per, e->pts, e->off, e->sorted,
+                                                          e->ss_off, e->ss_sorted, stream());
     HIP_TRY(r);
   }
   return LBSWIM_OK;
@@ -2650,6 +2755,12 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   dev_free(e->cursor);
   dev_free(e->list);
   dev_free(e->sorted);
+  dev_free(e->ss_counts);
+  dev_free(e->ss_off);
+  dev_free(e->ss_cursor);
+  dev_free(e->ss_local);
+  dev_free(e->ss_eid);
+  dev_free(e->ss_sorted);
   dev_free(e->pts);
   dev_free(e->slot_ids);
   dev_free(e->d_fusework);
@@ -2863,7 +2974,7 @@ int lbswim_ensemble_autotune(lbswim_ensemble* e, int steps_per_trial, int* choic
       if (!m.lat) continue;
       size_t npts = 0;
       for (lbswim_swimmers* sw : m.sw) npts += 2 * size_t(sw->st.num);
-      any = any || (force_path(e->force_mode, npts, m.lat->geom) == FORCE_LISTS_VIRTUAL &&
+      any = any || (force_path(e->force_mode, npts, m.lat->geom, true) == FORCE_LISTS_VIRTUAL &&
                     batch_fused_member(m.lat->geom, 4, m.lat->elem));
     }
     if (any) {

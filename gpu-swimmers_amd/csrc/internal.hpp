@@ -146,6 +146,16 @@ hipError_t launch_range_force(const Geom& g, const SpreadPoint* pts, uint32_t np
 template <typename T>
 hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
                                T* force, hipStream_t s);
+// Writes the force of every site from per-site segments of the points' 64
+// stencil entries, for lattices with every side >= 4 (site_force_ok): the
+// sum the sorted lists give.  counts[g.n] (zero on entry, left zero),
+// off[spread_off_entries(g.n)], cursor[g.n]; scratch local, eid[64 npts]
+// (local 16-byte aligned) and sorted[3 * 64 npts].
+inline bool site_force_ok(const Geom& g) { return g.nx >= 4 && g.ny >= 4 && g.nz >= 4; }
+template <typename T>
+hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, uint32_t* counts, uint32_t* off,
+                             uint32_t* cursor, uint32_t* local, uint32_t* eid, double* sorted, T* force,
+                             hipStream_t s);
 
 // Force at the first and last site of every list's 64-site run:
 // edge[6 l + 3 e + a] (e = 0 first, 1 last), 6 spread_lists(g) doubles; and,
@@ -168,10 +178,12 @@ struct LatDesc {
   uint32_t tile_base;    // first spreading list of this member in the batch
   uint32_t point_first;  // first point id of this member (its arrays' points are consecutive)
   uint32_t point_count;
+  uint32_t site_base;    // first site of this member in the batch's site segments (FORCE_SITES_STORED)
   // how the member's swimmer force reaches its collide and CalcHydro:
   // FORCE_LISTS_VIRTUAL (evaluated from its lists there), FORCE_LISTS_STORED
   // (written from its lists to the force array first), FORCE_RANGE_STORED
-  // (written from all its points in id order, no lists)
+  // (written from all its points in id order, no lists), FORCE_SITES_STORED
+  // (written from per-site segments of its points' stencil entries, no lists)
   int force_path;
   // the interleaved u of a member the batched fused step advances (it then
   // writes u only there, and the batched moves gather from it); null for the
@@ -181,10 +193,17 @@ struct LatDesc {
 constexpr int FORCE_LISTS_VIRTUAL = 0;
 constexpr int FORCE_LISTS_STORED = 1;
 constexpr int FORCE_RANGE_STORED = 2;
+constexpr int FORCE_SITES_STORED = 3;
+__host__ __device__ inline bool has_lists(int force_path) {
+  return force_path == FORCE_LISTS_VIRTUAL || force_path == FORCE_LISTS_STORED;
+}
 // Points up to which a dense (stored-force) system takes FORCE_RANGE_STORED:
 // every 64-site run walks all of them instead of building lists that would
 // hold a large share of them anyway.
 constexpr size_t RANGE_FORCE_MAX_POINTS = 4096;
+// On lattices with every side >= 4, dense systems with more points than this
+// take FORCE_SITES_STORED instead.
+constexpr size_t SITE_FORCE_MIN_POINTS = 1024;
 // One workgroup of a batched launch: member index and the block (or tile,
 // or particle group) of that member it processes.
 struct WorkItem {
@@ -251,12 +270,21 @@ hipError_t launch_batch_points(const LatDesc* ldesc, const SwimDesc* sdesc, cons
 hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const SpreadPoint* pts,
                               const uint32_t* slot_ids, bool dense, uint32_t* counts, uint32_t* off, uint32_t* cursor,
                               uint32_t* list, uint32_t* sorted, hipStream_t s);
+// Per-site segments of the FORCE_SITES_STORED members' points (see
+// launch_site_force), nsites = their sites (site_base numbering), npts all
+// the batch's points: counts[nsites] (zero on entry, left zero),
+// off[spread_off_entries(nsites)], cursor[nsites], local[64 npts] (16-byte
+// aligned), eid[64 points of those members], sorted[3 * 64 of them].
+hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, const SpreadPoint* pts,
+                                   uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* local, uint32_t* eid,
+                                   double* sorted, hipStream_t s);
 // Writes the force arrays of the lattice blocks in block_work: from the lists
-// (FORCE_LISTS_STORED) or from all the member's points (FORCE_RANGE_STORED).
+// (FORCE_LISTS_STORED), from all the member's points (FORCE_RANGE_STORED) or
+// from its site segments soff/ssorted (FORCE_SITES_STORED).
 template <typename T>
 hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nwork, const WorkItem* block_work, int per,
                                      const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
-                                     hipStream_t s);
+                                     const uint32_t* soff, const double* ssorted, hipStream_t s);
 template <typename T>
 hipError_t launch_batch_swimmers_move(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,
                                       hipStream_t s);

@@ -920,7 +920,7 @@ __global__ __launch_bounds__(PBLOCK) void k_count_rows(uint32_t npts, Geom g, co
     bool has = true;
     if constexpr (Batched) {
       const LatDesc& m = ldesc[p.pad[0]];
-      has = m.force_path != FORCE_RANGE_STORED;  // a member without lists
+      has = has_lists(m.force_path);  // a member without lists
       g = m.geom;
       base = m.tile_base;
     }
@@ -1012,7 +1012,7 @@ __global__ __launch_bounds__(PBLOCK) void k_fill_rows(uint32_t npts, Geom g, con
     bool has = true;
     if constexpr (Batched) {
       const LatDesc& m = ldesc[p.pad[0]];
-      has = m.force_path != FORCE_RANGE_STORED;
+      has = has_lists(m.force_path);
       g = m.geom;
       base = m.tile_base;
     }
@@ -1216,6 +1216,142 @@ __global__ __launch_bounds__(SITE_BLOCK) void k_spread_blocks(Geom g, const Spre
   spread_block_body(g, pts, off, sorted, force, blockIdx.x, blockIdx.x * LISTS_PER_BLOCK, stage);
 }
 
+// ---- stored force by site (dense systems, every lattice side >= 4) ----------
+// With every side >= 4 a point's 4x4x4 stencil covers 64 distinct sites, and
+// add_point gives site (wc[i], wc[4+j], wc[8+k]) the product
+// (del[i] * del[4+j]) * del[8+k] * F.  The points' entries are counted per
+// site (each entry keeps the slot its atomic returned), the counts scanned
+// into segments, the point ids written to their slots, every product placed
+// at the rank of its point id within its segment, and a thread per site adds
+// its segment in order from zero: the products and the order of the sorted
+// 64-site lists (wave_force), which follow the sequential AccumulateDeltaForce
+// order (SwimmerArray.cu:42-79) -- but each site walks only the points that
+// reach it instead of every point of its 64-site run.  Entry e = 16 id + row
+// holds the 4 sites of stencil row `row` = 4 i + j.
+
+// Linear site of row `row` = 4 i + j of point p's stencil at z offset 0.
+__device__ __forceinline__ uint32_t stencil_row_site(const SpreadPoint& p, int row, const Geom& g) {
+  return (uint32_t(p.wc[row >> 2]) * uint32_t(g.ny) + uint32_t(p.wc[4 + (row & 3)])) * uint32_t(g.nz);
+}
+
+// The point's member (Batched: geometry and first site from its LatDesc);
+// false for a batched point whose member takes another force path.
+template <bool Batched>
+__device__ __forceinline__ bool site_member(const SpreadPoint& p, const LatDesc* __restrict__ ldesc, Geom& g,
+                                            uint32_t& base) {
+  base = 0;
+  if constexpr (Batched) {
+    const LatDesc& m = ldesc[p.pad[0]];
+    if (m.force_path != FORCE_SITES_STORED) return false;
+    g = m.geom;
+    base = m.site_base;
+  }
+  return true;
+}
+
+// One thread per (point, stencil row): counts the row's 4 sites and keeps
+// each site's slot within its segment.
+template <bool Batched>
+__global__ __launch_bounds__(PBLOCK) void k_site_count(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
+                                                      const SpreadPoint* __restrict__ pts,
+                                                      uint32_t* __restrict__ counts, uint4* __restrict__ local) {
+  const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
+  if (t >= 16 * npts) return;
+  const SpreadPoint& p = pts[t >> 4];
+  uint32_t base;
+  if (!site_member<Batched>(p, ldesc, g, base)) return;
+  const uint32_t row = base + stencil_row_site(p, int(t & 15), g);
+  uint4 q;
+  q.x = atomicAdd(&counts[row + uint32_t(p.wc[8])], 1u);
+  q.y = atomicAdd(&counts[row + uint32_t(p.wc[9])], 1u);
+  q.z = atomicAdd(&counts[row + uint32_t(p.wc[10])], 1u);
+  q.w = atomicAdd(&counts[row + uint32_t(p.wc[11])], 1u);
+  local[t] = q;
+}
+
+// One thread per (point, stencil row): the point id into its 4 slots.
+template <bool Batched>
+__global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
+                                                     const SpreadPoint* __restrict__ pts,
+                                                     const uint32_t* __restrict__ off,
+                                                     const uint4* __restrict__ local, uint32_t* __restrict__ eid) {
+  const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
+  if (t >= 16 * npts) return;
+  const SpreadPoint& p = pts[t >> 4];
+  uint32_t base;
+  if (!site_member<Batched>(p, ldesc, g, base)) return;
+  const uint32_t row = base + stencil_row_site(p, int(t & 15), g);
+  const uint4 q = local[t];
+  const uint32_t slot[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) eid[off[row + uint32_t(p.wc[8 + k])] + slot[k]] = t >> 4;
+}
+
+// One thread per (point, stencil row): each of the row's products goes to
+// the rank of the point id within its site's segment (the ids of a segment
+// are distinct).
+template <bool Batched>
+__global__ __launch_bounds__(PBLOCK) void k_site_place(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
+                                                      const SpreadPoint* __restrict__ pts,
+                                                      const uint32_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ eid, double* __restrict__ sorted) {
+  const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
+  if (t >= 16 * npts) return;
+  const uint32_t id = t >> 4;
+  const int i = int(t & 15) >> 2, j = int(t & 3);
+  const SpreadPoint& p = pts[id];
+  uint32_t base;
+  if (!site_member<Batched>(p, ldesc, g, base)) return;
+  const uint32_t row = base + stencil_row_site(p, int(t & 15), g);
+  const double dxy = p.del[i] * p.del[4 + j];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t site = row + uint32_t(p.wc[8 + k]);
+    const uint32_t o = off[site], end = off[site + 1];
+    uint32_t rank = 0;
+    for (uint32_t e = o; e < end; ++e) rank += eid[e] < id ? 1u : 0u;
+    const double d3 = dxy * p.del[8 + k];
+    double* __restrict__ dst = sorted + 3 * size_t(o + rank);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dst[a] = d3 * p.F[a];
+  }
+}
+
+// Force at site s (segment gs) of a lattice of n sites: its segment's
+// products added in order (loaded four entries ahead of the adds).
+template <typename T>
+__device__ __forceinline__ void site_sum_body(uint32_t n, uint32_t s, uint32_t gs, const uint32_t* __restrict__ off,
+                                              const double* __restrict__ sorted, T* __restrict__ force) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  uint32_t e = off[gs];
+  const uint32_t end = off[gs + 1];
+  for (; e + 4 <= end; e += 4) {
+    double v[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[q][a] = sorted[3 * size_t(e + q) + a];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) acc[a] += v[q][a];
+  }
+  for (; e < end; ++e) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] += sorted[3 * size_t(e) + a];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) force[size_t(a) * n + s] = T(acc[a]);
+}
+
+// One thread per site of a single lattice.
+template <typename T>
+__global__ __launch_bounds__(PBLOCK) void k_site_sum(uint32_t n, const uint32_t* __restrict__ off,
+                                                    const double* __restrict__ sorted, T* __restrict__ force) {
+  const uint32_t s = blockIdx.x * PBLOCK + threadIdx.x;
+  if (s < n) site_sum_body(n, s, s, off, sorted, force);
+}
+
 // Force at the first and the last site of every 64-site run (edge[6 l + 3 e
 // + a]: list l, e = 0 first, 1 last), the sum wave_force gives there.  The
 // fused step's z-halo sites are such sites of neighbouring runs; they load
@@ -1367,7 +1503,9 @@ __global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __
                                                               const WorkItem* __restrict__ block_work,
                                                               const SpreadPoint* __restrict__ pts,
                                                               const uint32_t* __restrict__ off,
-                                                              const uint32_t* __restrict__ sorted, int per) {
+                                                              const uint32_t* __restrict__ sorted, int per,
+                                                              const uint32_t* __restrict__ soff,
+                                                              const double* __restrict__ ssorted) {
   __shared__ union {
     WaveStage list[LISTS_PER_BLOCK];
     RangeStage range;
@@ -1378,9 +1516,12 @@ __global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __
   for (int b = 0; b < per; ++b) {  // the work item's `per` consecutive blocks
     const uint32_t blk = w.unit + uint32_t(b);
     if (blk >= nb) break;
-    if (m.force_path == FORCE_RANGE_STORED)
+    if (m.force_path == FORCE_SITES_STORED) {
+      const uint32_t site = blk * SITE_BLOCK + threadIdx.x;
+      if (site < m.geom.n) site_sum_body(m.geom.n, site, m.site_base + site, soff, ssorted, static_cast<T*>(m.force));
+    } else if (m.force_path == FORCE_RANGE_STORED) {
       range_block_body(m.geom, pts, m.point_first, m.point_count, static_cast<T*>(m.force), blk, &stage.range);
-    else
+    } else
       spread_block_body(m.geom, pts, off, sorted, static_cast<T*>(m.force), blk, m.tile_base + blk * LISTS_PER_BLOCK,
                         stage.list);
   }
@@ -1551,6 +1692,60 @@ template hipError_t launch_range_force<double>(const Geom&, const SpreadPoint*, 
 template hipError_t launch_range_force<float>(const Geom&, const SpreadPoint*, uint32_t, float*, hipStream_t);
 
 template <typename T>
+hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, uint32_t* counts, uint32_t* off,
+                             uint32_t* cursor, uint32_t* local, uint32_t* eid, double* sorted, T* force,
+                             hipStream_t s) {
+  if (g.n == 0) return hipSuccess;
+  const uint32_t nt = g.n;
+  const unsigned rows = blocks_for(16 * size_t(npts));
+  uint4* loc = reinterpret_cast<uint4*>(local);
+  if (npts > 0)
+    hipLaunchKernelGGL(k_site_count<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, counts,
+                       loc);
+  // per-workgroup totals of the scan after the nt + 1 offsets
+  const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  uint32_t* sums = off + nt + 1;
+  hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, cursor);
+  if (npts > 0) {
+    hipLaunchKernelGGL(k_site_fill<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, off, loc,
+                       eid);
+    hipLaunchKernelGGL(k_site_place<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, off, eid,
+                       sorted);
+  }
+  hipLaunchKernelGGL(k_site_sum<T>, dim3(blocks_for(nt)), dim3(PBLOCK), 0, s, nt, off, sorted, force);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, const SpreadPoint* pts,
+                                   uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* local, uint32_t* eid,
+                                   double* sorted, hipStream_t s) {
+  if (nsites <= 0) return hipSuccess;
+  const uint32_t nt = uint32_t(nsites);
+  const unsigned rows = blocks_for(16 * size_t(npts));
+  uint4* loc = reinterpret_cast<uint4*>(local);
+  const Geom none{};
+  if (npts > 0)
+    hipLaunchKernelGGL(k_site_count<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, counts,
+                       loc);
+  const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  uint32_t* sums = off + nt + 1;
+  hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, cursor);
+  if (npts > 0) {
+    hipLaunchKernelGGL(k_site_fill<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, off, loc,
+                       eid);
+    hipLaunchKernelGGL(k_site_place<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, off,
+                       eid, sorted);
+  }
+  return hipGetLastError();
+}
+template hipError_t launch_site_force<double>(int, const Geom&, const SpreadPoint*, uint32_t*, uint32_t*, uint32_t*,
+                                              uint32_t*, uint32_t*, double*, double*, hipStream_t);
+template hipError_t launch_site_force<float>(int, const Geom&, const SpreadPoint*, uint32_t*, uint32_t*, uint32_t*,
+                                             uint32_t*, uint32_t*, double*, float*, hipStream_t);
+
+template <typename T>
 hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
                                T* force, hipStream_t s) {
   if (g.n == 0) return hipSuccess;
@@ -1611,16 +1806,18 @@ hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const 
 template <typename T>
 hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nwork, const WorkItem* block_work, int per,
                                      const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
-                                     hipStream_t s) {
+                                     const uint32_t* soff, const double* ssorted, hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
   hipLaunchKernelGGL(kb_spread_blocks<T>, dim3(unsigned(nwork)), dim3(SITE_BLOCK), 0, s, ldesc, block_work, pts, off,
-                     sorted, per);
+                     sorted, per, soff, ssorted);
   return hipGetLastError();
 }
 template hipError_t launch_batch_spread_force<double>(const LatDesc*, int, const WorkItem*, int, const SpreadPoint*,
-                                                      const uint32_t*, const uint32_t*, hipStream_t);
+                                                      const uint32_t*, const uint32_t*, const uint32_t*, const double*,
+                                                      hipStream_t);
 template hipError_t launch_batch_spread_force<float>(const LatDesc*, int, const WorkItem*, int, const SpreadPoint*,
-                                                     const uint32_t*, const uint32_t*, hipStream_t);
+                                                     const uint32_t*, const uint32_t*, const uint32_t*, const double*,
+                                                     hipStream_t);
 
 template <typename T>
 hipError_t launch_batch_swimmers_move(const LatDesc* ldesc, const SwimDesc* sdesc, const WorkItem* work, int nwork,

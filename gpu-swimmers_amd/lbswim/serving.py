@@ -338,17 +338,25 @@ class Server:
 
 def run_sequential(requests, realtime=True, on_done=None):
     """Baseline: one request at a time in arrival order, each alone with the
-    plain per-system step (no batching); on_done(request) after each."""
+    plain per-system step (no batching); on_done(request) after each.
+
+    Also returns busy_s, the summed service times (build + steps + sync of
+    each request alone), and floor_s, the latest arrival + service time over
+    the requests: no schedule of the same stream can finish sooner if a
+    request runs no faster beside others than alone."""
     from .System import System
 
     t0 = time.perf_counter()
     lat_s = []
     updates = 0
+    busy = 0.0
+    floor = 0.0
     for req in sorted(requests, key=lambda r: r.arrival):
         if realtime:
             wait = req.arrival - (time.perf_counter() - t0)
             if wait > 0:
                 time.sleep(wait)
+        start = time.perf_counter() - t0
         lat, things = build(req)
         system = System(lat)
         for th in things:
@@ -356,6 +364,8 @@ def run_sequential(requests, realtime=True, on_done=None):
         system.Step(req.steps)
         _core.synchronize()
         done = time.perf_counter() - t0
+        busy += done - start
+        floor = max(floor, (req.arrival if realtime else 0.0) + done - start)
         lat_s.append(done - (req.arrival if realtime else 0.0))
         updates += req.sites * req.steps
         if on_done is not None:
@@ -364,4 +374,5 @@ def run_sequential(requests, realtime=True, on_done=None):
     wall = time.perf_counter() - t0
     lat_s = np.array(lat_s)
     return {"requests": len(requests), "wall_s": wall, "p50_s": float(np.percentile(lat_s, 50)),
-            "p99_s": float(np.percentile(lat_s, 99)), "mean_s": float(lat_s.mean()), "mlups": updates / wall / 1e6}
+            "p99_s": float(np.percentile(lat_s, 99)), "mean_s": float(lat_s.mean()), "mlups": updates / wall / 1e6,
+            "busy_s": busy, "floor_s": floor}

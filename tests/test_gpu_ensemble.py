@@ -229,3 +229,42 @@ def test_autotune_chooses_lanes_and_keeps_results():
     for i, (c, (lat, sw, tr)) in enumerate(zip(cases, built)):
         assert lat.time_step == total
         assert_same(collect_lbswim(lat, sw, tr), run_alone(c, "f64", total), "member %d" % i)
+
+
+@pytest.mark.parametrize("storage,graph,force_mode", [("f64", True, "auto"), ("f32", False, "auto"),
+                                                      ("f64", False, "stored")])
+def test_dense_members_by_site_match_alone_and_oracle(storage, graph, force_mode):
+    """Dense members (more than 1024 swimmer ends on lattices with every side
+    >= 4) write their force from per-site segments numbered across the batch,
+    beside sparse members with lists and a dense member of a thin lattice
+    (side 3) that keeps the all-points or list path: every member equals its
+    run alone, and in FP64 the dense ones equal the oracle bit for bit."""
+    import os
+
+    import lbswim_oracle as O
+    from cases import run_oracle
+    from lbswim.ensemble import Ensemble
+    O.set_threads(min(16, os.cpu_count() or 1))
+    cases = [make_case((24, 24, 24), 4, 0.3, ns=2000, nt=100, seed=11, drho=1e-2, du=1e-2, P=1e-2, l=0.3),
+             make_case((20, 26, 17), 4, 0.4, ns=700, nt=0, seed=12, drho=1e-2, du=1e-2, P=1e-2, l=1.2),
+             make_case((32, 32, 32), 4, 0.5, ns=20, nt=200, seed=13),
+             make_case((3, 30, 28), 4, 0.3, ns=2500, nt=0, seed=14, drho=1e-2, du=1e-2, P=1e-2, l=0.2),
+             make_case((16, 16, 16), 4, 0.3, ns=1500, nt=50, seed=15, drho=1e-2, du=1e-2, P=1e-2, l=0.5)]
+    steps = 4
+    say("dense members by site: %s graph=%s force %s" % (storage, graph, force_mode))
+    ens = Ensemble(storage)
+    ens.set_graph(graph)
+    ens.set_force_mode(force_mode)
+    built = []
+    for c in cases:
+        lat, system, sw, tr, things = build_member(c, storage)
+        ens.add(lat, things)
+        built.append((lat, sw, tr))
+    ens.step(steps)
+    for i, (c, (lat, sw, tr)) in enumerate(zip(cases, built)):
+        got = collect_lbswim(lat, sw, tr)
+        assert_same(got, run_alone(c, storage, steps), "member %d" % i)
+        if storage == "f64" and i in (0, 1, 4):
+            ref, _ = run_oracle(O, c, steps)
+            for k in ("force", "fOld", "rho", "u", "sw_r", "sw_n", "tr_r"):
+                assert np.array_equal(got[k], ref[k]), ("oracle", i, k)

@@ -151,6 +151,42 @@ def test_spreading_exact_on_awkward_shapes(size, ns, l, force_mode):
         assert np.array_equal(got[k], ref[k]), k
 
 
+DENSE_SITE_CASES = [((24, 24, 24), 3700, 0.1, False), ((4, 5, 6), 2100, 0.3, False),
+                    ((7, 33, 20), 2500, 2.5, False), ((16, 16, 16), 3000, 0.2, True)]
+
+
+@pytest.mark.parametrize("size,ns,l,clustered", DENSE_SITE_CASES)
+def test_dense_stored_force_by_site(size, ns, l, clustered):
+    """Dense systems (more than 4096 swimmer ends, every side >= 4) form the
+    stored force from per-site segments of the stencil entries, ordered by
+    point id: the force, state and swimmers equal the oracle bit for bit over
+    three steps -- also on a 4x5x6 lattice, where each stencil covers most of
+    it, with swimmers longer than two sites, and with every swimmer inside one
+    site (segments of about 6000 entries)."""
+    case = make_case(size, 3, 0.3, ns=ns, nt=0, alpha=0.0, seed=ns + size[0], drho=1e-2, du=1e-2, P=1e-2, l=l)
+    if clustered:
+        rng = np.random.default_rng(9)
+        case.sw_r[:] = 7.3 + rng.uniform(0.0, 1.0, case.sw_r.shape)
+    say("dense stored force by site: %s ns=%d l=%g clustered=%s" % (size, ns, l, clustered))
+    ref, _ = run_oracle(O, case)
+    got, _ = run_lbswim(case, force_mode="stored")
+    for k in ("force", "fOld", "rho", "u", "sw_r", "sw_n"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
+@pytest.mark.parametrize("size,ns,l,clustered", DENSE_SITE_CASES[:2])
+def test_dense_stored_force_by_site_fp32(size, ns, l, clustered):
+    """The FP32 form stores the same double sums rounded once to float: its
+    first step's force equals the oracle's rounded to FP32."""
+    case = make_case(size, 1, 0.3, ns=ns, nt=0, alpha=0.0, seed=ns + size[0], drho=1e-2, du=1e-2, P=1e-2, l=l)
+    say("dense stored force by site, FP32: %s ns=%d" % (size, ns))
+    ref, _ = run_oracle(O, case)
+    got, _ = run_lbswim(case, storage="f32", force_mode="stored")
+    want = ref["force"].astype(np.float32).astype(np.float64)
+    assert np.array_equal(got["force"], want)
+    assert np.abs(want).max() > 0
+
+
 @pytest.mark.parametrize("force_mode", ["stored", "virtual"])
 def test_two_swimmer_arrays_in_order(force_mode):
     """Two swimmer arrays: contributions are added array by array, as System.Step does."""

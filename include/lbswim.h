@@ -232,8 +232,10 @@ int lbswim_tracers_slots(lbswim_tracers* tr, const int** device);
 /* `nsteps` repetitions of System.Step (lbswim/System.py:9-16): ZeroForce;
  * AddForces of every swimmer array; Step; CalcHydro; Move of every swimmer
  * array, then of every tracer array.
- * Virtual force: with LBSWIM_FORCE_AUTO and at most ~4 swimmer ends per 256
- * sites, ZeroForce + AddForces only build sorted per-block point lists and
+ * Virtual force: with LBSWIM_FORCE_AUTO where the swimmers are sparse (at
+ * most about 0.24 (1 + n / 2^18) swimmer ends per 256 of the lattice's n
+ * sites; for an ensemble member about 7.5), ZeroForce + AddForces only
+ * build sorted per-block point lists and
  * Step and CalcHydro evaluate each site's force from them, bit-identical to
  * the stored force, without writing or reading the force array.  The force
  * array is then written on demand: by lbswim_lattice_d2h(LBSWIM_FORCE),

@@ -478,6 +478,37 @@ def oracle_legs(case, f64_state, f32_state, f32_small, budget_s=240.0):
     return res
 
 
+def probe_dense(steps=40):
+    """Coupled steps of dense small systems, the kind the serving mix holds
+    (cube side, swimmers; FP64, no tracers, the serving leg's synthetic state):
+    ms per step and of its swimmer-force phase (HIP events)."""
+    from lbswim.serving import Request, build
+    from lbswim.System import System
+    import lbswim
+    out = []
+    for size, ns in ((24, 3710), (32, 3710), (58, 3547)):
+        lat, things = build(Request(rid=0, size=size, steps=steps, swimmers=ns, tracers=0, storage="f64", seed=7))
+        system = System(lat)
+        for t in things:
+            system.AddThing(t)
+        system.Step(5)
+        lbswim.synchronize()
+        t0 = time.perf_counter()
+        system.Step(steps)
+        lbswim.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        lat.set_timing(["add_forces"])
+        system.Step(10)
+        lbswim.synchronize()
+        fms, cnt = lat.kernel_times()["add_forces"]
+        lat.set_timing(False)
+        out.append({"side": size, "swimmers": ns, "force_mode": lat.force_mode, "ms_per_step": round(ms, 4),
+                    "force_ms_per_step": round(fms / max(cnt, 1), 4)})
+        log("dense %d^3 with %d swimmers: %.4f ms/step" % (size, ns, ms))
+        del lat, things, system
+    return out
+
+
 def load_pmc(name, kernel, tiles=None):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary profiles/<name>_pmc.json, or None when there is none or it was
@@ -521,7 +552,8 @@ def main():
     ap.add_argument("--tiles", default="auto",
                     help="fused tile shape 'rows,strips' of the coupled step (rows 0: two launches), or auto "
                          "(autotuned on real steps)")
-    ap.add_argument("--probe", choices=["copy"], help="run the streaming-copy bandwidth probe instead")
+    ap.add_argument("--probe", choices=["copy", "dense"],
+                    help="run the streaming-copy bandwidth probe, or time dense small systems' coupled steps, instead")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-sim ensemble leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving leg (0: skip)")
     ap.add_argument("--ensemble-eager", action="store_true",
@@ -546,6 +578,9 @@ def main():
         gbs = 2 * 8 * count / (ms * 1e-3) / 1e9
         print(json.dumps({"probe": "copy", "doubles": count, "bytes_per_copy": 16 * count, "avg_ms": round(ms, 4),
                           "GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+        return
+    if a.probe == "dense":
+        print(json.dumps({"probe": "dense", "systems": probe_dense()}), flush=True)
         return
     legs = {x for x in a.legs.split(",") if x}
     if "main" not in legs:  # partial runs (profiling): no headline value

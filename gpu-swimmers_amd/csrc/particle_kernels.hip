@@ -1287,8 +1287,8 @@ __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, con
   for (int k = 0; k < 4; ++k) eid[off[row + uint32_t(p.wc[8 + k])] + slot[k]] = t >> 4;
 }
 
-// One thread per (point, stencil row): each of the row's products goes to
-// the rank of the point id within its site's segment (the ids of a segment
+// One thread per entry (point, stencil row, z offset k): its product goes to
+// the rank of its point id within its site's segment (the ids of a segment
 // are distinct).
 template <bool Batched>
 __global__ __launch_bounds__(PBLOCK) void k_site_place(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
@@ -1296,25 +1296,20 @@ __global__ __launch_bounds__(PBLOCK) void k_site_place(uint32_t npts, Geom g, co
                                                       const uint32_t* __restrict__ off,
                                                       const uint32_t* __restrict__ eid, double* __restrict__ sorted) {
   const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
-  if (t >= 16 * npts) return;
-  const uint32_t id = t >> 4;
-  const int i = int(t & 15) >> 2, j = int(t & 3);
+  if (t >= 64 * npts) return;
+  const uint32_t id = t >> 6;
+  const int r = int(t >> 2) & 15, k = int(t & 3);
   const SpreadPoint& p = pts[id];
   uint32_t base;
   if (!site_member<Batched>(p, ldesc, g, base)) return;
-  const uint32_t row = base + stencil_row_site(p, int(t & 15), g);
-  const double dxy = p.del[i] * p.del[4 + j];
+  const uint32_t site = base + stencil_row_site(p, r, g) + uint32_t(p.wc[8 + k]);
+  const uint32_t o = off[site], end = off[site + 1];
+  uint32_t rank = 0;
+  for (uint32_t e = o; e < end; ++e) rank += eid[e] < id ? 1u : 0u;
+  const double d3 = p.del[r >> 2] * p.del[4 + (r & 3)] * p.del[8 + k];
+  double* __restrict__ dst = sorted + 3 * size_t(o + rank);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t site = row + uint32_t(p.wc[8 + k]);
-    const uint32_t o = off[site], end = off[site + 1];
-    uint32_t rank = 0;
-    for (uint32_t e = o; e < end; ++e) rank += eid[e] < id ? 1u : 0u;
-    const double d3 = dxy * p.del[8 + k];
-    double* __restrict__ dst = sorted + 3 * size_t(o + rank);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) dst[a] = d3 * p.F[a];
-  }
+  for (int a = 0; a < 3; ++a) dst[a] = d3 * p.F[a];
 }
 
 // Force at site s (segment gs) of a lattice of n sites: its segment's
@@ -1710,8 +1705,8 @@ hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, ui
   if (npts > 0) {
     hipLaunchKernelGGL(k_site_fill<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, off, loc,
                        eid);
-    hipLaunchKernelGGL(k_site_place<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, off, eid,
-                       sorted);
+    hipLaunchKernelGGL(k_site_place<false>, dim3(blocks_for(64 * size_t(npts))), dim3(PBLOCK), 0, s, uint32_t(npts),
+                       g, nullptr, pts, off, eid, sorted);
   }
   hipLaunchKernelGGL(k_site_sum<T>, dim3(blocks_for(nt)), dim3(PBLOCK), 0, s, nt, off, sorted, force);
   return hipGetLastError();
@@ -1735,8 +1730,8 @@ hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, c
   if (npts > 0) {
     hipLaunchKernelGGL(k_site_fill<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, off, loc,
                        eid);
-    hipLaunchKernelGGL(k_site_place<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, off,
-                       eid, sorted);
+    hipLaunchKernelGGL(k_site_place<true>, dim3(blocks_for(64 * size_t(npts))), dim3(PBLOCK), 0, s, uint32_t(npts),
+                       none, ldesc, pts, off, eid, sorted);
   }
   return hipGetLastError();
 }

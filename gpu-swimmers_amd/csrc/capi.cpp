@@ -201,13 +201,14 @@ struct lbswim_lattice {
   RunEntry* sp_rec = nullptr;
   int* sp_wz = nullptr;
   // Stored force by site (launch_site_force): per-site counts, offsets and
-  // cursors, and ss_cap point entries (slot, id, ordered product).
+  // cursors, and ss_cap point entries (slot, id, site, ordered product).
   uint32_t* ss_counts = nullptr;
   uint32_t* ss_off = nullptr;
   uint32_t* ss_cursor = nullptr;
   size_t ss_cap = 0;
   uint32_t* ss_local = nullptr;
   uint32_t* ss_id = nullptr;
+  uint32_t* ss_site = nullptr;
   double* ss_sorted = nullptr;
   // Virtual force: after a fused step the force lives in the sorted point
   // lists above and d_force is stale until materialize_force writes it.
@@ -543,6 +544,7 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->ss_cursor);
   dev_free(lat->ss_local);
   dev_free(lat->ss_id);
+  dev_free(lat->ss_site);
   dev_free(lat->ss_sorted);
   dev_free(lat->sp_pts);
   dev_free(lat->sp_slot);
@@ -1377,10 +1379,11 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
          (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->ss_cursor), 4 * n))))
       return rc;
     if (lat->ss_cap < nent) {
-      size_t c1 = lat->ss_cap, c2 = lat->ss_cap, c3 = lat->ss_cap;
+      size_t c1 = lat->ss_cap, c2 = lat->ss_cap, c3 = lat->ss_cap, c4 = lat->ss_cap;
       if ((rc = grow(reinterpret_cast<void**>(&lat->ss_local), &c1, nent, 4)) ||
           (rc = grow(reinterpret_cast<void**>(&lat->ss_id), &c2, nent, 4)) ||
-          (rc = grow(reinterpret_cast<void**>(&lat->ss_sorted), &c3, nent, 3 * sizeof(double))))
+          (rc = grow(reinterpret_cast<void**>(&lat->ss_site), &c3, nent, 4)) ||
+          (rc = grow(reinterpret_cast<void**>(&lat->ss_sorted), &c4, nent, 3 * sizeof(double))))
         return rc;
       lat->ss_cap = nent;
     }
@@ -1396,10 +1399,10 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     if (e == hipSuccess)
       e = lat->storage == LBSWIM_STORE_F64
               ? launch_site_force<double>(int(npts), lat->geom, pts, lat->ss_counts, lat->ss_off, lat->ss_cursor,
-                                          lat->ss_local, lat->ss_id, lat->ss_sorted,
+                                          lat->ss_local, lat->ss_id, lat->ss_site, lat->ss_sorted,
                                           static_cast<double*>(lat->d_force), stream())
               : launch_site_force<float>(int(npts), lat->geom, pts, lat->ss_counts, lat->ss_off, lat->ss_cursor,
-                                         lat->ss_local, lat->ss_id, lat->ss_sorted,
+                                         lat->ss_local, lat->ss_id, lat->ss_site, lat->ss_sorted,
                                          static_cast<float*>(lat->d_force), stream());
     t.done();
     HIP_TRY(e);
@@ -2083,6 +2086,9 @@ struct lbswim_ensemble {
   size_t cap_ss_local = 0;
   uint32_t* ss_eid = nullptr;
   size_t cap_ss_eid = 0;
+  uint32_t* ss_site = nullptr;
+  size_t cap_ss_site = 0;
+  int ss_nent = 0;  // 64 points of those members
   double* ss_sorted = nullptr;
   size_t cap_ss_sorted = 0;
   // stored-force members' chain forked to the ensemble's side stream (see
@@ -2442,6 +2448,7 @@ int ens_build(lbswim_ensemble* e) {
       (rc = grow_to(&e->sorted, &cap_sorted, lcap, 4)) || (rc = grow_to(&e->list, &e->cap_list, lcap, 4)))
     return rc;
   e->nsites = int(site_base);
+  e->ss_nent = int(64 * site_points);
   if (site_base > 0) {
     const size_t ns = site_base;
     if ((rc = grow_to(&e->ss_counts, &e->cap_ss_counts, ns, 4)) ||
@@ -2449,6 +2456,7 @@ int ens_build(lbswim_ensemble* e) {
         (rc = grow_to(&e->ss_cursor, &e->cap_ss_cursor, ns, 4)) ||
         (rc = grow_to(&e->ss_local, &e->cap_ss_local, 64 * size_t(e->npts), 4)) ||
         (rc = grow_to(&e->ss_eid, &e->cap_ss_eid, 64 * site_points, 4)) ||
+        (rc = grow_to(&e->ss_site, &e->cap_ss_site, 64 * site_points, 4)) ||
         (rc = grow_to(&e->ss_sorted, &e->cap_ss_sorted, 3 * 64 * site_points, 8)))
       return rc;
   }
@@ -2556,8 +2564,8 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
     if (int rc = fork_now()) return rc;
   auto stored_chain = [&]() -> int {
     EnsTimer t(e, LBSWIM_K_STORED, sb);
-    HIP_TRY(launch_batch_site_lists(e->d_lat, e->npts, e->nsites, e->pts, e->ss_counts, e->ss_off, e->ss_cursor,
-                                    e->ss_local, e->ss_eid, e->ss_sorted, sb));
+    HIP_TRY(launch_batch_site_lists(e->d_lat, e->npts, e->nsites, e->ss_nent, e->pts, e->ss_counts, e->ss_off,
+                                    e->ss_cursor, e->ss_local, e->ss_eid, e->ss_site, e->ss_sorted, sb));
     HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->per, e->pts, e->off,
                                          e->sorted, e->ss_off, e->ss_sorted, sb));
     HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb));
@@ -2760,6 +2768,7 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   dev_free(e->ss_cursor);
   dev_free(e->ss_local);
   dev_free(e->ss_eid);
+  dev_free(e->ss_site);
   dev_free(e->ss_sorted);
   dev_free(e->pts);
   dev_free(e->slot_ids);

@@ -1269,12 +1269,14 @@ __global__ __launch_bounds__(PBLOCK) void k_site_count(uint32_t npts, Geom g, co
   local[t] = q;
 }
 
-// One thread per (point, stencil row): the point id into its 4 slots.
+// One thread per (point, stencil row): the point id and the site into the
+// row's 4 slots.
 template <bool Batched>
 __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
                                                      const SpreadPoint* __restrict__ pts,
                                                      const uint32_t* __restrict__ off,
-                                                     const uint4* __restrict__ local, uint32_t* __restrict__ eid) {
+                                                     const uint4* __restrict__ local, uint32_t* __restrict__ eid,
+                                                     uint32_t* __restrict__ esite) {
   const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
   if (t >= 16 * npts) return;
   const SpreadPoint& p = pts[t >> 4];
@@ -1284,29 +1286,42 @@ __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, con
   const uint4 q = local[t];
   const uint32_t slot[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-  for (int k = 0; k < 4; ++k) eid[off[row + uint32_t(p.wc[8 + k])] + slot[k]] = t >> 4;
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t site = row + uint32_t(p.wc[8 + k]);
+    const uint32_t e = off[site] + slot[k];
+    eid[e] = t >> 4;
+    esite[e] = site;
+  }
 }
 
-// One thread per entry (point, stencil row, z offset k): its product goes to
-// the rank of its point id within its site's segment (the ids of a segment
-// are distinct).
+// One thread per slot, in slot order (so a wavefront's lanes mostly scan
+// the same segment): the entry's product, formed from the site and the point
+// as add_point forms it, goes to the rank of its point id within the segment
+// (the ids of a segment are distinct).  nent = 64 points of the members
+// concerned.
 template <bool Batched>
-__global__ __launch_bounds__(PBLOCK) void k_site_place(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
+__global__ __launch_bounds__(PBLOCK) void k_site_place(uint32_t nent, Geom g, const LatDesc* __restrict__ ldesc,
                                                       const SpreadPoint* __restrict__ pts,
                                                       const uint32_t* __restrict__ off,
-                                                      const uint32_t* __restrict__ eid, double* __restrict__ sorted) {
-  const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
-  if (t >= 64 * npts) return;
-  const uint32_t id = t >> 6;
-  const int r = int(t >> 2) & 15, k = int(t & 3);
+                                                      const uint32_t* __restrict__ eid,
+                                                      const uint32_t* __restrict__ esite, double* __restrict__ sorted) {
+  const uint32_t e = blockIdx.x * PBLOCK + threadIdx.x;
+  if (e >= nent) return;
+  const uint32_t id = eid[e], site = esite[e];
   const SpreadPoint& p = pts[id];
   uint32_t base;
-  if (!site_member<Batched>(p, ldesc, g, base)) return;
-  const uint32_t site = base + stencil_row_site(p, r, g) + uint32_t(p.wc[8 + k]);
+  site_member<Batched>(p, ldesc, g, base);
+  const SiteXYZ c = site_xyz(site - base, g);
+  int di = c.x - p.wc[0];
+  di += di < 0 ? g.nx : 0;
+  int dj = c.y - p.wc[4];
+  dj += dj < 0 ? g.ny : 0;
+  int dk = c.z - p.wc[8];
+  dk += dk < 0 ? g.nz : 0;
   const uint32_t o = off[site], end = off[site + 1];
   uint32_t rank = 0;
-  for (uint32_t e = o; e < end; ++e) rank += eid[e] < id ? 1u : 0u;
-  const double d3 = p.del[r >> 2] * p.del[4 + (r & 3)] * p.del[8 + k];
+  for (uint32_t q = o; q < end; ++q) rank += eid[q] < id ? 1u : 0u;
+  const double d3 = p.del[di] * p.del[4 + dj] * p.del[8 + dk];
   double* __restrict__ dst = sorted + 3 * size_t(o + rank);
 #pragma unroll
   for (int a = 0; a < 3; ++a) dst[a] = d3 * p.F[a];
@@ -1688,8 +1703,8 @@ template hipError_t launch_range_force<float>(const Geom&, const SpreadPoint*, u
 
 template <typename T>
 hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, uint32_t* counts, uint32_t* off,
-                             uint32_t* cursor, uint32_t* local, uint32_t* eid, double* sorted, T* force,
-                             hipStream_t s) {
+                             uint32_t* cursor, uint32_t* local, uint32_t* eid, uint32_t* esite, double* sorted,
+                             T* force, hipStream_t s) {
   if (g.n == 0) return hipSuccess;
   const uint32_t nt = g.n;
   const unsigned rows = blocks_for(16 * size_t(npts));
@@ -1704,17 +1719,18 @@ hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, ui
   hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, cursor);
   if (npts > 0) {
     hipLaunchKernelGGL(k_site_fill<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, off, loc,
-                       eid);
-    hipLaunchKernelGGL(k_site_place<false>, dim3(blocks_for(64 * size_t(npts))), dim3(PBLOCK), 0, s, uint32_t(npts),
-                       g, nullptr, pts, off, eid, sorted);
+                       eid, esite);
+    const uint32_t nent = 64u * uint32_t(npts);
+    hipLaunchKernelGGL(k_site_place<false>, dim3(blocks_for(nent)), dim3(PBLOCK), 0, s, nent, g, nullptr, pts, off,
+                       eid, esite, sorted);
   }
   hipLaunchKernelGGL(k_site_sum<T>, dim3(blocks_for(nt)), dim3(PBLOCK), 0, s, nt, off, sorted, force);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, const SpreadPoint* pts,
+hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, int nent, const SpreadPoint* pts,
                                    uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* local, uint32_t* eid,
-                                   double* sorted, hipStream_t s) {
+                                   uint32_t* esite, double* sorted, hipStream_t s) {
   if (nsites <= 0) return hipSuccess;
   const uint32_t nt = uint32_t(nsites);
   const unsigned rows = blocks_for(16 * size_t(npts));
@@ -1729,16 +1745,17 @@ hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, c
   hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, cursor);
   if (npts > 0) {
     hipLaunchKernelGGL(k_site_fill<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, off, loc,
-                       eid);
-    hipLaunchKernelGGL(k_site_place<true>, dim3(blocks_for(64 * size_t(npts))), dim3(PBLOCK), 0, s, uint32_t(npts),
-                       none, ldesc, pts, off, eid, sorted);
+                       eid, esite);
+    if (nent > 0)
+      hipLaunchKernelGGL(k_site_place<true>, dim3(blocks_for(size_t(nent))), dim3(PBLOCK), 0, s, uint32_t(nent), none,
+                         ldesc, pts, off, eid, esite, sorted);
   }
   return hipGetLastError();
 }
 template hipError_t launch_site_force<double>(int, const Geom&, const SpreadPoint*, uint32_t*, uint32_t*, uint32_t*,
-                                              uint32_t*, uint32_t*, double*, double*, hipStream_t);
+                                              uint32_t*, uint32_t*, uint32_t*, double*, double*, hipStream_t);
 template hipError_t launch_site_force<float>(int, const Geom&, const SpreadPoint*, uint32_t*, uint32_t*, uint32_t*,
-                                             uint32_t*, uint32_t*, double*, float*, hipStream_t);
+                                             uint32_t*, uint32_t*, uint32_t*, double*, float*, hipStream_t);
 
 template <typename T>
 hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,

@@ -223,6 +223,45 @@ def test_two_swimmer_arrays_in_order(force_mode):
     assert np.array_equal(sw2.r.data, s2.r)
 
 
+def test_two_dense_swimmer_arrays_by_site():
+    """Two swimmer arrays on a dense system (3000 swimmer ends on 16^3: the
+    per-site segments): the first array's points come first in every site's
+    sum, as System.Step adds them array by array."""
+    from lbswim import swimmers
+    from lbswim.System import System
+    size = (16, 16, 16)
+    c1 = make_case(size, 0, 0.3, ns=800, nt=0, seed=3, drho=1e-2, du=1e-2, P=1e-2, l=0.4)
+    c2 = make_case(size, 0, 0.3, ns=700, nt=0, seed=4, drho=1e-2, du=1e-2, P=2e-2, l=0.9)
+    say("two dense swimmer arrays by site")
+    lat, system, sw1, _ = build_lbswim(c1)
+    cp = swimmers.CommonParams()
+    for k, v in c2.common.items():
+        setattr(cp, k, v)
+    sw2 = swimmers.Array(700, cp)
+    sw2.r[:] = c2.sw_r
+    sw2.n[:] = c2.sw_n
+    sw2.H2D()
+    system.AddThing(sw2)
+    system.Step(3)
+    assert lat.force_mode == "auto" and not lat.force_is_virtual
+    lat.force.D2H()
+    lat.fOld.D2H()
+    sw1.D2H()
+    sw2.D2H()
+    ol = O.Lattice(*size, 0.3, 0.3)
+    ol.rho[:], ol.u[:] = c1.rho, c1.u
+    ol.InitFromHydro()
+    s1, s2 = O.Swimmers(800, O.Common(**c1.common)), O.Swimmers(700, O.Common(**c2.common))
+    s1.r[:], s1.n[:] = c1.sw_r, c1.sw_n
+    s2.r[:], s2.n[:] = c2.sw_r, c2.sw_n
+    for _ in range(3):
+        O.system_step(ol, [s1, s2])
+    assert np.array_equal(lat.force.data.ravel(), ol.force)
+    assert np.array_equal(lat.fOld.data.ravel(), ol.fOld)
+    assert np.array_equal(sw1.r.data, s1.r)
+    assert np.array_equal(sw2.n.data, s2.n)
+
+
 def test_virtual_force_written_on_demand():
     """After fused steps with a virtual force, every path that reads or
     replaces the force array sees what the stored form holds: D2H, a

@@ -268,3 +268,35 @@ def test_dense_members_by_site_match_alone_and_oracle(storage, graph, force_mode
             ref, _ = run_oracle(O, c, steps)
             for k in ("force", "fOld", "rho", "u", "sw_r", "sw_n", "tr_r"):
                 assert np.array_equal(got[k], ref[k]), ("oracle", i, k)
+
+
+def test_dense_members_by_site_with_lanes_and_membership_changes():
+    """Site segments numbered across members survive two lanes (each lane
+    numbers its own members' segments) and membership changes (a dense
+    member leaves, another joins, so every later member's first site moves):
+    every member equals its run alone, step counts included."""
+    from lbswim.ensemble import Ensemble
+    mk = lambda size, ns, seed, nt=0: make_case(size, 0, 0.3, ns=ns, nt=nt, seed=seed, drho=1e-2, du=1e-2,  # noqa: E731
+                                                P=1e-2, l=0.3)
+    cases = [mk((24, 24, 24), 1800, 21, 50), mk((16, 18, 20), 900, 22), mk((40, 40, 40), 30, 23, 100),
+             mk((20, 20, 20), 1200, 24), mk((12, 16, 28), 700, 25)]
+    say("dense members by site: lanes 2, membership changes")
+    ens = Ensemble("f64")
+    ens.set_lanes(2)
+    built = []
+    for c in cases[:4]:
+        lat, system, sw, tr, things = build_member(c, "f64")
+        built.append([ens.add(lat, things), lat, sw, tr, 0])
+    ens.step(2)
+    for b in built:
+        b[4] += 2
+    ens.remove(built[0][0])            # the first dense member leaves
+    lat, system, sw, tr, things = build_member(cases[4], "f64")
+    built.append([ens.add(lat, things), lat, sw, tr, 0])
+    ens.step(3)
+    for b in built[1:]:
+        b[4] += 3
+    for i, (c, b) in enumerate(zip(cases, built)):
+        _, lat, sw, tr, steps = b
+        assert lat.time_step == steps, (i, lat.time_step, steps)
+        assert_same(collect_lbswim(lat, sw, tr), run_alone(c, "f64", steps), "member %d" % i)

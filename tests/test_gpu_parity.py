@@ -141,8 +141,9 @@ def test_spreading_exact_on_awkward_shapes(size, ns, l, force_mode):
     """Stencils wrapping several times onto lattices thinner than 4, partial
     blocks, z-runs crossing block boundaries and the periodic face, dense
     overlapping swimmers, long swimmers spanning blocks.  With the stored
-    force, systems of up to 4096 swimmer ends walk all points per run and
-    denser ones (2600 swimmers) build sorted lists."""
+    force, systems of up to 1024 swimmer ends (4096 on lattices thinner than
+    4) walk all points per run, denser ones with every side >= 4 (2000 and
+    2600 swimmers) use per-site segments, and the others sorted lists."""
     case = make_case(size, 3, 0.3, ns=ns, nt=0, alpha=0.0, seed=ns, drho=1e-2, du=1e-2, P=1e-2, l=l)
     say("spreading exact: %s ns=%d l=%g force %s" % (size, ns, l, force_mode))
     ref, _ = run_oracle(O, case)
@@ -157,7 +158,7 @@ DENSE_SITE_CASES = [((24, 24, 24), 3700, 0.1, False), ((4, 5, 6), 2100, 0.3, Fal
 
 @pytest.mark.parametrize("size,ns,l,clustered", DENSE_SITE_CASES)
 def test_dense_stored_force_by_site(size, ns, l, clustered):
-    """Dense systems (more than 4096 swimmer ends, every side >= 4) form the
+    """Dense systems (more than 1024 swimmer ends, every side >= 4) form the
     stored force from per-site segments of the stencil entries, ordered by
     point id: the force, state and swimmers equal the oracle bit for bit over
     three steps -- also on a 4x5x6 lattice, where each stencil covers most of
@@ -469,3 +470,28 @@ def test_empty_particle_arrays_and_python_driver():
     sw.D2H()
     tr.D2H()
     assert sw.r.data.shape == (3, 0)
+
+
+@pytest.mark.parametrize("size,ns,virtual", [((128, 128, 128), 250, True), ((96, 96, 96), 300, True),
+                                             ((96, 96, 96), 4000, False), ((48, 48, 48), 250, False),
+                                             ((24, 24, 24), 2000, False)])
+def test_auto_force_mode_density_rule(size, ns, virtual):
+    """Force mode auto, a single system: the force stays virtual up to
+    1 + n / 2^18 mean entries per 64-site list (about 17 per swimmer end)
+    and is stored beyond; as an ensemble member the same system keeps the
+    virtual force up to 32 entries."""
+    from lbswim.ensemble import Ensemble
+    case = make_case(size, 1, 0.3, ns=ns, nt=0, seed=ns)
+    n = size[0] * size[1] * size[2]
+    mean_entries = 2 * ns * 17 / (4 * ((n + 255) // 256))
+    say("auto force mode: %s ns=%d (%.1f entries per list)" % (size, ns, mean_entries))
+    lat, system, sw, tr = build_lbswim(case)
+    system.Step(1)
+    assert lat.force_is_virtual == virtual
+    ens = Ensemble("f64")
+    lat2, _, sw2, tr2 = build_lbswim(case)
+    ens.add(lat2, [x for x in (sw2, tr2) if x is not None])
+    ens.step(1)
+    vsites, ssites = ens.sites()
+    member_virtual = vsites + ens.fused_sites() == n
+    assert member_virtual == (mean_entries <= 32), (vsites, ssites)

@@ -627,9 +627,15 @@ static int lattice_step_impl(lbswim_lattice* lat) {
   return LBSWIM_OK;
 }
 
-static int lattice_hydro_impl(lbswim_lattice* lat) {
-  lat->u_aos_valid = false;
-  lat->u_virtual = false;  // the two-launch CalcHydro writes the reference layout
+// aos (the coupled step): u is written only component-interleaved, as the
+// fused step writes it, for the particle moves to gather from; d_u is then
+// stale until materialize_u writes it.  Otherwise the reference layout.
+static int lattice_hydro_impl(lbswim_lattice* lat, bool aos = false) {
+  if (aos && !lat->d_u_aos)
+    if (int rc = dev_alloc_zero(&lat->d_u_aos, 3 * lat->elem * lat->n)) return rc;
+  lat->u_aos_valid = aos;
+  lat->u_virtual = aos;
+  void* u = aos ? lat->d_u_aos : lat->d_u;
   const VForce vf = virtual_force(lat);
   const VForce* pvf = lat->force_virtual ? &vf : nullptr;
   KernelTimer t(lat, LBSWIM_K_HYDRO);
@@ -637,11 +643,11 @@ static int lattice_hydro_impl(lbswim_lattice* lat) {
   if (lat->storage == LBSWIM_STORE_F64)
     e = launch_calc_hydro<double>(static_cast<const double*>(lat->d_f[lat->cur]),
                                   static_cast<const double*>(lat->d_force), static_cast<double*>(lat->d_rho),
-                                  static_cast<double*>(lat->d_u), lat->geom, pvf, stream());
+                                  static_cast<double*>(u), lat->geom, pvf, stream(), aos);
   else
     e = launch_calc_hydro<float>(static_cast<const float*>(lat->d_f[lat->cur]), static_cast<const float*>(lat->d_force),
-                                 static_cast<float*>(lat->d_rho), static_cast<float*>(lat->d_u), lat->geom, pvf,
-                                 stream());
+                                 static_cast<float*>(lat->d_rho), static_cast<float*>(u), lat->geom, pvf, stream(),
+                                 aos);
   t.done();
   HIP_TRY(e);
   return LBSWIM_OK;
@@ -1571,7 +1577,7 @@ int lbswim_system_step(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw,
         if ((rc = lattice_step_hydro_impl(lat, rows))) break;
       } else {
         if ((rc = lattice_step_impl(lat))) break;
-        if ((rc = lattice_hydro_impl(lat))) break;
+        if ((rc = lattice_hydro_impl(lat, true))) break;
       }
     }
     if (overlap) {
@@ -1811,7 +1817,8 @@ int lbswim_system_step_noalloc(lbswim_lattice* lat, lbswim_swimmers* const* sw, 
       if ((rc = lattice_step_hydro_impl(lat, fusion_rows(lat)))) break;
     } else {
       if ((rc = lattice_step_impl(lat))) break;
-      if ((rc = lattice_hydro_impl(lat))) break;
+      // interleaved u only where its buffer exists: this entry point allocates nothing
+      if ((rc = lattice_hydro_impl(lat, lat->d_u_aos != nullptr))) break;
     }
     for (int i = 0; i < nsw && !rc; ++i) rc = swimmers_move_impl(sw[i], lat);
     for (int i = 0; i < ntr && !rc; ++i) rc = tracers_move_impl(tr[i], lat);
@@ -2276,6 +2283,7 @@ int ens_build(lbswim_ensemble* e) {
   if (e->lanes > 1) return ens_build_lanes(e);
   ens_drop_lanes(e);
   std::vector<LatDesc> lat;
+  std::vector<char> fused_member;  // lat[i] is advanced by the batched fused step
   // per member: its lattice blocks, in the stored-force (0), virtual-force
   // two-launch (1) or virtual-force fused (2) group
   std::vector<std::vector<WorkItem>> latwork[3];
@@ -2318,11 +2326,14 @@ int ens_build(lbswim_ensemble* e) {
     d.point_count = uint32_t(member_pts);
     const bool fused = e->fuse_rows > 0 && d.force_path == FORCE_LISTS_VIRTUAL &&
                        batch_fused_member(L->geom, e->fuse_rows, L->elem);
-    if (fused && !L->d_u_aos)
+    // every member's CalcHydro (or fused step) writes u only interleaved, the
+    // copy the batched moves gather from
+    if (!L->d_u_aos)
       if (int rc = dev_alloc_zero(&L->d_u_aos, 3 * L->elem * L->n)) return rc;
-    d.u_aos = fused ? L->d_u_aos : nullptr;
+    d.u_aos = L->d_u_aos;
     m.fused = fused;
     lat.push_back(d);
+    fused_member.push_back(fused);
     std::vector<WorkItem> lw;
     for (uint32_t b = 0; b < lattice_blocks(L->geom); b += uint32_t(e->per)) lw.push_back(WorkItem{mi, b});
     const int group = d.force_path != FORCE_LISTS_VIRTUAL ? 0 : (fused ? 2 : 1);
@@ -2410,7 +2421,8 @@ int ens_build(lbswim_ensemble* e) {
   }
   e->sites_virtual = e->sites_stored = e->sites_fused = 0;
   for (const LatDesc& d : lat)
-    (d.force_path != FORCE_LISTS_VIRTUAL ? e->sites_stored : (d.u_aos ? e->sites_fused : e->sites_virtual)) +=
+    (d.force_path != FORCE_LISTS_VIRTUAL ? e->sites_stored
+                                         : (fused_member[size_t(&d - lat.data())] ? e->sites_fused : e->sites_virtual)) +=
         d.geom.n;
   e->stored_lists = false;
   for (const LatDesc& d : lat) e->stored_lists = e->stored_lists || d.force_path == FORCE_LISTS_STORED;
@@ -2695,9 +2707,9 @@ void ens_advance(lbswim_ensemble* e, int k) {
     if (k & 1) m.lat->cur ^= 1;
     m.lat->time_step += k;
     m.lat->force_virtual = false;  // the batched step spreads into the force array
-    // and writes u in the reference layout, or (fused) only interleaved
-    m.lat->u_aos_valid = m.fused;
-    m.lat->u_virtual = m.fused;
+    // and writes u only interleaved
+    m.lat->u_aos_valid = true;
+    m.lat->u_virtual = true;
     for (auto* s : m.sw) {
       s->st.moves_since_order += k;
       s->version = next_version();

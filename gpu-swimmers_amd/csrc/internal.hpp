@@ -53,9 +53,10 @@ struct VForce {
 template <typename T>
 hipError_t launch_collide_stream(const T* fOld, const T* force, T* fNew, const Geom& g, double tau_s,
                                  double tau_b, const VForce* vf, hipStream_t s);
+// aos: u written component-interleaved (u[3 ijk + a], 3 n values).
 template <typename T>
 hipError_t launch_calc_hydro(const T* f, const T* force, T* rho, T* u, const Geom& g, const VForce* vf,
-                             hipStream_t s);
+                             hipStream_t s, bool aos = false);
 // Collide+stream followed by CalcHydro in one launch, with the results of
 // launch_collide_stream + launch_calc_hydro bit for bit, for lattices where
 // fused_step_supported(g, ty, sizeof(T)): ny >= ty, ty in {2, 4, 8} (rows

@@ -255,11 +255,13 @@ __device__ __forceinline__ void collide_f64_body(const double* __restrict__ gOld
   store_all(gNew, n, s, mode, std::make_integer_sequence<int, Q>{});
 }
 
-// lb.cu:250-295: only the density and momentum modes are needed.
+// lb.cu:250-295: only the density and momentum modes are needed.  aos: u
+// goes component-interleaved, u[3 s + a] (the copy the particle moves gather
+// from), instead of u[a n + s].
 template <bool VF>
 __device__ __forceinline__ void hydro_f64_body(const double* __restrict__ gCur, const double* __restrict__ force,
                                                double* __restrict__ rho, double* __restrict__ u, Geom g, uint32_t blk,
-                                               const VForce& vf, uint32_t vlist) {
+                                               const VForce& vf, uint32_t vlist, bool aos) {
   const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
@@ -278,11 +280,17 @@ __device__ __forceinline__ void hydro_f64_body(const double* __restrict__ gCur, 
     F[1] = force[size_t(n) + s];
     F[2] = force[2 * size_t(n) + s];
   }
-  // streaming stores: the fields are far larger than the caches
+  const double uv[3] = {(m1 + 0.5 * F[0]) / m0, (m2 + 0.5 * F[1]) / m0, (m3 + 0.5 * F[2]) / m0};
+  // streaming stores: the fields are far larger than the caches (the
+  // interleaved u through the caches, where L2 merges the strided stores)
   store_stream(&rho[s], m0);
-  store_stream(&u[s], (m1 + 0.5 * F[0]) / m0);
-  store_stream(&u[size_t(n) + s], (m2 + 0.5 * F[1]) / m0);
-  store_stream(&u[2 * size_t(n) + s], (m3 + 0.5 * F[2]) / m0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (aos)
+      u[3 * size_t(s) + a] = uv[a];
+    else
+      store_stream(&u[size_t(a) * n + s], uv[a]);
+  }
 }
 
 // lb.cu:314-371, including the reference's SXZ = rho*ux*uy (lb.cu:356).
@@ -448,7 +456,7 @@ __device__ __forceinline__ void collide_f32_body(const float* __restrict__ fOld,
 template <bool VF>
 __device__ __forceinline__ void hydro_f32_body(const float* __restrict__ f, const float* __restrict__ force,
                                                float* __restrict__ drho_out, float* __restrict__ u, Geom g,
-                                               uint32_t blk, const VForce& vf, uint32_t vlist) {
+                                               uint32_t blk, const VForce& vf, uint32_t vlist, bool aos) {
   const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
@@ -470,7 +478,13 @@ __device__ __forceinline__ void hydro_f32_body(const float* __restrict__ f, cons
   const float inv_rho = 1.0f / (1.0f + m.drho);
   store_stream(&drho_out[s], m.drho);
 #pragma unroll
-  for (int a = 0; a < 3; ++a) store_stream(&u[size_t(a) * n + s], __builtin_fmaf(0.5f, F[a], m.j[a]) * inv_rho);
+  for (int a = 0; a < 3; ++a) {
+    const float ua = __builtin_fmaf(0.5f, F[a], m.j[a]) * inv_rho;
+    if (aos)
+      u[3 * size_t(s) + a] = ua;
+    else
+      store_stream(&u[size_t(a) * n + s], ua);
+  }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_init_from_hydro_f32(const float* __restrict__ drho,
@@ -1025,22 +1039,23 @@ __global__ __launch_bounds__(BLOCK) void k_collide_stream_vf_f32(const float* __
 }
 __global__ __launch_bounds__(BLOCK) void k_calc_hydro_f64(const double* __restrict__ gCur,
                                                           const double* __restrict__ force, double* __restrict__ rho,
-                                                          double* __restrict__ u, Geom g) {
-  hydro_f64_body<false>(gCur, force, rho, u, g, blockIdx.x, VForce{}, 0);
+                                                          double* __restrict__ u, Geom g, bool aos) {
+  hydro_f64_body<false>(gCur, force, rho, u, g, blockIdx.x, VForce{}, 0, aos);
 }
 __global__ __launch_bounds__(BLOCK) void k_calc_hydro_vf_f64(const double* __restrict__ gCur, VForce vf,
                                                              double* __restrict__ rho, double* __restrict__ u,
-                                                             Geom g) {
-  hydro_f64_body<true>(gCur, nullptr, rho, u, g, blockIdx.x, vf, blockIdx.x * LISTS_PER_BLOCK);
+                                                             Geom g, bool aos) {
+  hydro_f64_body<true>(gCur, nullptr, rho, u, g, blockIdx.x, vf, blockIdx.x * LISTS_PER_BLOCK, aos);
 }
 __global__ __launch_bounds__(BLOCK) void k_calc_hydro_f32(const float* __restrict__ gCur,
                                                           const float* __restrict__ force, float* __restrict__ rho,
-                                                          float* __restrict__ u, Geom g) {
-  hydro_f32_body<false>(gCur, force, rho, u, g, blockIdx.x, VForce{}, 0);
+                                                          float* __restrict__ u, Geom g, bool aos) {
+  hydro_f32_body<false>(gCur, force, rho, u, g, blockIdx.x, VForce{}, 0, aos);
 }
 __global__ __launch_bounds__(BLOCK) void k_calc_hydro_vf_f32(const float* __restrict__ gCur, VForce vf,
-                                                             float* __restrict__ rho, float* __restrict__ u, Geom g) {
-  hydro_f32_body<true>(gCur, nullptr, rho, u, g, blockIdx.x, vf, blockIdx.x * LISTS_PER_BLOCK);
+                                                             float* __restrict__ rho, float* __restrict__ u, Geom g,
+                                                             bool aos) {
+  hydro_f32_body<true>(gCur, nullptr, rho, u, g, blockIdx.x, vf, blockIdx.x * LISTS_PER_BLOCK, aos);
 }
 
 // At the start of a batched step with `parity` p the state of member m is in
@@ -1084,12 +1099,16 @@ __global__ __launch_bounds__(BLOCK) void kb_hydro(const LatDesc* __restrict__ de
     const uint32_t blk = w.unit + uint32_t(b);
     if (blk >= nb) break;
     const uint32_t vlist = m.tile_base + blk * LISTS_PER_BLOCK;
+    // u interleaved where the member has that copy (the batched moves gather from it)
+    const bool aos = m.u_aos != nullptr;
     if constexpr (sizeof(T) == 8)
       hydro_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
-                         static_cast<double*>(m.rho), static_cast<double*>(m.u), m.geom, blk, vf, vlist);
+                         static_cast<double*>(m.rho), static_cast<double*>(aos ? m.u_aos : m.u), m.geom, blk, vf,
+                         vlist, aos);
     else
       hydro_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
-                         static_cast<float*>(m.rho), static_cast<float*>(m.u), m.geom, blk, vf, vlist);
+                         static_cast<float*>(m.rho), static_cast<float*>(aos ? m.u_aos : m.u), m.geom, blk, vf,
+                         vlist, aos);
   }
 }
 
@@ -1281,23 +1300,23 @@ template hipError_t launch_u_from_aos<float>(const float*, float*, const Geom&, 
 
 template <>
 hipError_t launch_calc_hydro<double>(const double* f, const double* force, double* rho, double* u, const Geom& g,
-                                     const VForce* vf, hipStream_t s) {
+                                     const VForce* vf, hipStream_t s, bool aos) {
   if (g.n == 0) return hipSuccess;
   if (vf)
-    hipLaunchKernelGGL(k_calc_hydro_vf_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, *vf, rho, u, g);
+    hipLaunchKernelGGL(k_calc_hydro_vf_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, *vf, rho, u, g, aos);
   else
-    hipLaunchKernelGGL(k_calc_hydro_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g);
+    hipLaunchKernelGGL(k_calc_hydro_f64, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g, aos);
   return hipGetLastError();
 }
 
 template <>
 hipError_t launch_calc_hydro<float>(const float* f, const float* force, float* rho, float* u, const Geom& g,
-                                    const VForce* vf, hipStream_t s) {
+                                    const VForce* vf, hipStream_t s, bool aos) {
   if (g.n == 0) return hipSuccess;
   if (vf)
-    hipLaunchKernelGGL(k_calc_hydro_vf_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, *vf, rho, u, g);
+    hipLaunchKernelGGL(k_calc_hydro_vf_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, *vf, rho, u, g, aos);
   else
-    hipLaunchKernelGGL(k_calc_hydro_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g);
+    hipLaunchKernelGGL(k_calc_hydro_f32, dim3(grid_for(g.n)), dim3(BLOCK), 0, s, f, force, rho, u, g, aos);
   return hipGetLastError();
 }
 

@@ -73,11 +73,12 @@ def poisson_requests(n, rate, seed=1234, sizes=(24, 96), steps=(200, 5000), swim
 
 
 def estimate_bytes(req):
-    """Device memory of a request: two distribution buffers, rho, u and
-    force, the swimmer and tracer stores (fields, staging and sort scratch),
-    plus the spreading workspace it adds to its ensemble."""
+    """Device memory of a request: two distribution buffers, rho, u (in the
+    reference layout and interleaved) and force, the swimmer and tracer
+    stores (fields, staging and sort scratch), plus the spreading workspace
+    it adds to its ensemble."""
     e = 8 if req.storage == "f64" else 4
-    lattice = req.sites * (2 * 15 + 1 + 3 + 3) * e + 64 * 1024
+    lattice = req.sites * (2 * 15 + 1 + 3 + 3 + 3) * e + 64 * 1024
     swim = req.swimmers * (5 * 24 + 6 * 4 + 8) + 2 * req.swimmers * (176 + 64 * 4 * 2)
     trac = req.tracers * (4 * 24 + 6 * 4 + 8)
     tiles = req.sites // 512 + 1

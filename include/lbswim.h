@@ -286,8 +286,8 @@ int lbswim_lattice_sync_force(lbswim_lattice* lat);
 /* *is_virtual = 1 if the last coupled step left the force virtual (not yet
  * written to the force array). No reference counterpart. */
 int lbswim_lattice_force_is_virtual(lbswim_lattice* lat, int* is_virtual);
-/* After fused coupled steps (below) u lives only in a component-interleaved
- * copy that the particle moves read; the reference-layout u array is written
+/* After coupled steps and ensemble steps (below) u lives only in a
+ * component-interleaved copy that the particle moves read; the reference-layout u array is written
  * on demand: by lbswim_lattice_d2h(LBSWIM_U), lbswim_lattice_field(LBSWIM_U)
  * with a device pointer requested, lbswim_lattice_init_from_hydro and
  * lbswim_lattice_sync_u.  A caller that reads u on the device by a pointer

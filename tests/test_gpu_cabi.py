@@ -1,9 +1,9 @@
 """The C ABI driven directly, as a binding in another language would drive it.
 
-After fused coupled steps u lives only in the particle moves' interleaved
-copy (lbswim_lattice_u_is_virtual); a device pointer to u obtained from
-lbswim_lattice_field must then hold u in the reference layout -- the values
-lbswim_lattice_d2h returns, and the values the two-launch step writes.
+After coupled steps, fused or in two launches, u lives only in the particle
+moves' interleaved copy (lbswim_lattice_u_is_virtual); a device pointer to u
+obtained from lbswim_lattice_field must then hold u in the reference layout --
+the values lbswim_lattice_d2h returns, the same from both forms.
 """
 import ctypes
 import os
@@ -87,8 +87,10 @@ def test_device_pointer_to_u_after_fused_steps(lib):
     print("C ABI: 4x8x64, 300 tracers, 3 coupled steps, fused and two-launch", flush=True)
     fused = run(lib, True)
     plain = run(lib, False)
-    assert fused[:2] == (1, 0)   # virtual until the device pointer was asked for
-    assert plain[:2] == (0, 0)
+    # both forms of the coupled step write u only interleaved: virtual until
+    # the device pointer was asked for
+    assert fused[:2] == (1, 0)
+    assert plain[:2] == (1, 0)
     assert np.array_equal(fused[2], fused[3])
     assert np.array_equal(fused[2], plain[2])
     assert np.array_equal(fused[4], plain[4])

@@ -176,15 +176,16 @@ def test_moves_use_u_written_after_fused_steps():
 
 
 def test_u_written_on_demand_after_fused_steps():
-    """Fused steps leave u only in the moves' interleaved copy; InitFromHydro,
-    D2H and sync_u must see the u the two launches write."""
+    """Coupled steps, fused or in two launches, leave u only in the moves'
+    interleaved copy; InitFromHydro, D2H and sync_u must see the same u
+    from both."""
     case = swimmer_case((4, 8, 64), seed=91)
     out = []
     for fusion in (None, (0, 0)):
         lat, system, sw, tr = build_lbswim(case, "f64", "virtual")
         lat.set_fusion(fusion is not None)
         system.Step(3)
-        assert lat.u_is_virtual == (fusion is not None)
+        assert lat.u_is_virtual
         lat.InitFromHydro()  # rebuilds f from rho and u
         assert not lat.u_is_virtual
         system.Step(2)

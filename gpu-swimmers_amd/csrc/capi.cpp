@@ -1321,9 +1321,10 @@ int grow(void** p, size_t* cap, size_t need, size_t elem) {
 // (128^3) mean entries per list: the limit taken is 1 + n / 2^18 entries (65
 // at 256^3).  In an ensemble the virtual-force members share the batched
 // lattice launches (the batched fused step among them) and the stored ones a
-// chain beside them; there the limit stays at 32 entries, which measured 10-12 %
-// faster on the 64-member bench ensemble than the single-system limit.
-constexpr size_t VF_BATCH_MAX_MEAN_ENTRIES = 32;
+// chain beside them; there the limit is 8 entries: on the 64-member bench
+// ensemble and the serving mix it measured 4-8 % faster than 32 and than
+// 4 or 2, and 10-12 % faster than the single-system limit.
+constexpr size_t VF_BATCH_MAX_MEAN_ENTRIES = 8;
 
 bool use_virtual_force(int mode, size_t npts, size_t lists, size_t n, bool batched) {
   // a stencil reaches about 17 lists of 64 sites

@@ -234,7 +234,7 @@ int lbswim_tracers_slots(lbswim_tracers* tr, const int** device);
  * array, then of every tracer array.
  * Virtual force: with LBSWIM_FORCE_AUTO where the swimmers are sparse (at
  * most about 0.24 (1 + n / 2^18) swimmer ends per 256 of the lattice's n
- * sites; for an ensemble member about 7.5), ZeroForce + AddForces only
+ * sites; for an ensemble member about 1.9), ZeroForce + AddForces only
  * build sorted per-block point lists and
  * Step and CalcHydro evaluate each site's force from them, bit-identical to
  * the stored force, without writing or reading the force array.  The force

@@ -146,8 +146,11 @@ def test_work_items_of_several_blocks(per):
 # Members the batched fused step takes (virtual force, nz % 64 == 0, rows
 # dividing ny) beside members it does not (nz not a multiple of 64, a dense
 # stored-force member, ny not a multiple of 8).
-FUSED_CASES = [((8, 16, 64), 30, 200), ((5, 8, 128), 40, 0), ((6, 24, 64), 10, 100), ((4, 12, 64), 12, 50),
-               ((16, 16, 16), 8, 60), ((12, 12, 24), 200, 0)]
+# four members the batched fused step advances (virtual force: at most 8
+# mean list entries), a virtual-force member of the two launches (16^3) and a
+# stored-force one
+FUSED_CASES = [((8, 16, 64), 30, 200), ((5, 8, 128), 16, 0), ((6, 24, 64), 10, 100), ((4, 12, 64), 10, 50),
+               ((16, 16, 16), 3, 60), ((12, 12, 24), 200, 0)]
 
 
 @pytest.mark.parametrize("storage,graph,fusion", [("f64", True, (8, 32)), ("f64", False, (8, 3)),

@@ -479,7 +479,7 @@ def test_auto_force_mode_density_rule(size, ns, virtual):
     """Force mode auto, a single system: the force stays virtual up to
     1 + n / 2^18 mean entries per 64-site list (about 17 per swimmer end)
     and is stored beyond; as an ensemble member the same system keeps the
-    virtual force up to 32 entries."""
+    virtual force up to 8 entries."""
     from lbswim.ensemble import Ensemble
     case = make_case(size, 1, 0.3, ns=ns, nt=0, seed=ns)
     n = size[0] * size[1] * size[2]
@@ -494,4 +494,4 @@ def test_auto_force_mode_density_rule(size, ns, virtual):
     ens.step(1)
     vsites, ssites = ens.sites()
     member_virtual = vsites + ens.fused_sites() == n
-    assert member_virtual == (mean_entries <= 32), (vsites, ssites)
+    assert member_virtual == (mean_entries <= 8), (vsites, ssites)

@@ -218,10 +218,11 @@ struct lbswim_lattice {
   // lattice qualifies); rows/strips 0 = chosen from the lattice size
   int fuse = LBSWIM_FUSION_AUTO;
   int fuse_rows = 0;
-  // u with a site's components together, written by the fused step (the
-  // particle moves gather from it); current only while nothing else has
-  // written u since.  u_virtual: the fused step wrote u only there, and d_u
-  // is stale until materialize_u writes it (u_virtual implies u_aos_valid).
+  // u with a site's components together, written by the coupled step's
+  // CalcHydro (fused or not) and by ensemble steps (the particle moves gather
+  // from it); current only while nothing else has written u since.
+  // u_virtual: u was written only there, and d_u is stale until materialize_u
+  // writes it (u_virtual implies u_aos_valid).
   // tracers_reading: tracer moves on the side stream may still read d_u_aos;
   // they are done once tracers_done has fired.
   void* d_u_aos = nullptr;

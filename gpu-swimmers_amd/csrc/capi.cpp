@@ -1406,10 +1406,12 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     }
     if (e == hipSuccess)
       e = lat->storage == LBSWIM_STORE_F64
-              ? launch_site_force<double>(int(npts), lat->geom, pts, lat->ss_counts, lat->ss_off, lat->ss_cursor,
+              ? launch_site_force<double>(int(npts), lat->geom, pts, lat->sp_slot, lat->ss_counts, lat->ss_off,
+                                          lat->ss_cursor,
                                           lat->ss_local, lat->ss_id, lat->ss_site, lat->ss_sorted,
                                           static_cast<double*>(lat->d_force), stream())
-              : launch_site_force<float>(int(npts), lat->geom, pts, lat->ss_counts, lat->ss_off, lat->ss_cursor,
+              : launch_site_force<float>(int(npts), lat->geom, pts, lat->sp_slot, lat->ss_counts, lat->ss_off,
+                                         lat->ss_cursor,
                                          lat->ss_local, lat->ss_id, lat->ss_site, lat->ss_sorted,
                                          static_cast<float*>(lat->d_force), stream());
     t.done();
@@ -2578,8 +2580,8 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
     if (int rc = fork_now()) return rc;
   auto stored_chain = [&]() -> int {
     EnsTimer t(e, LBSWIM_K_STORED, sb);
-    HIP_TRY(launch_batch_site_lists(e->d_lat, e->npts, e->nsites, e->ss_nent, e->pts, e->ss_counts, e->ss_off,
-                                    e->ss_cursor, e->ss_local, e->ss_eid, e->ss_site, e->ss_sorted, sb));
+    HIP_TRY(launch_batch_site_lists(e->d_lat, e->npts, e->nsites, e->ss_nent, e->pts, e->slot_ids, e->ss_counts,
+                                    e->ss_off, e->ss_cursor, e->ss_local, e->ss_eid, e->ss_site, e->ss_sorted, sb));
     HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->per, e->pts, e->off,
                                          e->sorted, e->ss_off, e->ss_sorted, sb));
     HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb));

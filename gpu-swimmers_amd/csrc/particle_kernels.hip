@@ -966,18 +966,38 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(int nt, const uint32_t
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
+// The nsb workgroup totals replaced by their exclusive prefix sums, by one
+// workgroup (SCAN_PER consecutive totals per thread, in chunks).
+constexpr int SCAN_PER = 16;
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_block_sums(int nsb, uint32_t* __restrict__ sums) {
+  __shared__ uint32_t part[SCAN_BLOCK];
+  uint32_t carry = 0;
+  for (int base = 0; base < nsb; base += SCAN_BLOCK * SCAN_PER) {
+    const int i0 = base + int(threadIdx.x) * SCAN_PER;
+    uint32_t v[SCAN_PER];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      v[k] = i0 + k < nsb ? sums[i0 + k] : 0u;
+      mine += v[k];
+    }
+    uint32_t total = 0;
+    uint32_t run = carry + block_exclusive_scan(mine, part, &total);
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      if (i0 + k < nsb) sums[i0 + k] = run;
+      run += v[k];
+    }
+    carry += total;
+  }
+}
+
+// `sums`: the workgroup totals after k_scan_block_sums (exclusive prefixes).
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(int nt, uint32_t* __restrict__ counts,
                                                            const uint32_t* __restrict__ sums,
                                                            uint32_t* __restrict__ off, uint32_t* __restrict__ cursor) {
   __shared__ uint32_t part[SCAN_BLOCK];
-  __shared__ uint32_t base;
-  // offset of this workgroup: sum of the earlier workgroups' totals
-  uint32_t mine = 0;
-  for (int b = int(threadIdx.x); b < int(blockIdx.x); b += SCAN_BLOCK) mine += sums[b];
-  uint32_t all = 0;
-  block_exclusive_scan(mine, part, &all);
-  if (threadIdx.x == 0) base = all;
-  __syncthreads();
+  const uint32_t base = sums[blockIdx.x];  // offset of this workgroup
   const int i = int(blockIdx.x) * SCAN_BLOCK + int(threadIdx.x);
   const uint32_t v = i < nt ? counts[i] : 0u;
   uint32_t total = 0;
@@ -1249,15 +1269,17 @@ __device__ __forceinline__ bool site_member(const SpreadPoint& p, const LatDesc*
   return true;
 }
 
-// One thread per (point, stencil row): counts the row's 4 sites and keeps
-// each site's slot within its segment.
+// One thread per (point, stencil row), points in storage order (slot_ids:
+// spatial neighbours, so the atomics of a workgroup stay in a few lines):
+// counts the row's 4 sites and keeps each site's slot within its segment.
 template <bool Batched>
 __global__ __launch_bounds__(PBLOCK) void k_site_count(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
                                                       const SpreadPoint* __restrict__ pts,
+                                                      const uint32_t* __restrict__ slot_ids,
                                                       uint32_t* __restrict__ counts, uint4* __restrict__ local) {
   const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
   if (t >= 16 * npts) return;
-  const SpreadPoint& p = pts[t >> 4];
+  const SpreadPoint& p = pts[slot_ids[t >> 4]];
   uint32_t base;
   if (!site_member<Batched>(p, ldesc, g, base)) return;
   const uint32_t row = base + stencil_row_site(p, int(t & 15), g);
@@ -1274,12 +1296,14 @@ __global__ __launch_bounds__(PBLOCK) void k_site_count(uint32_t npts, Geom g, co
 template <bool Batched>
 __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
                                                      const SpreadPoint* __restrict__ pts,
+                                                     const uint32_t* __restrict__ slot_ids,
                                                      const uint32_t* __restrict__ off,
                                                      const uint4* __restrict__ local, uint32_t* __restrict__ eid,
                                                      uint32_t* __restrict__ esite) {
   const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
   if (t >= 16 * npts) return;
-  const SpreadPoint& p = pts[t >> 4];
+  const uint32_t id = slot_ids[t >> 4];
+  const SpreadPoint& p = pts[id];
   uint32_t base;
   if (!site_member<Batched>(p, ldesc, g, base)) return;
   const uint32_t row = base + stencil_row_site(p, int(t & 15), g);
@@ -1289,7 +1313,7 @@ __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, con
   for (int k = 0; k < 4; ++k) {
     const uint32_t site = row + uint32_t(p.wc[8 + k]);
     const uint32_t e = off[site] + slot[k];
-    eid[e] = t >> 4;
+    eid[e] = id;
     esite[e] = site;
   }
 }
@@ -1663,6 +1687,7 @@ static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ld
   const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
   uint32_t* sums = off + nt + 1;
   hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, nt, counts, sums);
+  hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, int(nsb), sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, nt, counts, sums, off, cursor);
   if (npts > 0) {
     if (dense)
@@ -1702,24 +1727,25 @@ template hipError_t launch_range_force<double>(const Geom&, const SpreadPoint*, 
 template hipError_t launch_range_force<float>(const Geom&, const SpreadPoint*, uint32_t, float*, hipStream_t);
 
 template <typename T>
-hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, uint32_t* counts, uint32_t* off,
-                             uint32_t* cursor, uint32_t* local, uint32_t* eid, uint32_t* esite, double* sorted,
-                             T* force, hipStream_t s) {
+hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, const uint32_t* slot_ids,
+                             uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* local, uint32_t* eid,
+                             uint32_t* esite, double* sorted, T* force, hipStream_t s) {
   if (g.n == 0) return hipSuccess;
   const uint32_t nt = g.n;
   const unsigned rows = blocks_for(16 * size_t(npts));
   uint4* loc = reinterpret_cast<uint4*>(local);
   if (npts > 0)
-    hipLaunchKernelGGL(k_site_count<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, counts,
-                       loc);
+    hipLaunchKernelGGL(k_site_count<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, slot_ids,
+                       counts, loc);
   // per-workgroup totals of the scan after the nt + 1 offsets
   const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
   uint32_t* sums = off + nt + 1;
   hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums);
+  hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, int(nsb), sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, cursor);
   if (npts > 0) {
-    hipLaunchKernelGGL(k_site_fill<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, off, loc,
-                       eid, esite);
+    hipLaunchKernelGGL(k_site_fill<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, slot_ids,
+                       off, loc, eid, esite);
     const uint32_t nent = 64u * uint32_t(npts);
     hipLaunchKernelGGL(k_site_place<false>, dim3(blocks_for(nent)), dim3(PBLOCK), 0, s, nent, g, nullptr, pts, off,
                        eid, esite, sorted);
@@ -1729,33 +1755,36 @@ hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, ui
 }
 
 hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, int nent, const SpreadPoint* pts,
-                                   uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* local, uint32_t* eid,
-                                   uint32_t* esite, double* sorted, hipStream_t s) {
+                                   const uint32_t* slot_ids, uint32_t* counts, uint32_t* off, uint32_t* cursor,
+                                   uint32_t* local, uint32_t* eid, uint32_t* esite, double* sorted, hipStream_t s) {
   if (nsites <= 0) return hipSuccess;
   const uint32_t nt = uint32_t(nsites);
   const unsigned rows = blocks_for(16 * size_t(npts));
   uint4* loc = reinterpret_cast<uint4*>(local);
   const Geom none{};
   if (npts > 0)
-    hipLaunchKernelGGL(k_site_count<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, counts,
-                       loc);
+    hipLaunchKernelGGL(k_site_count<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, slot_ids,
+                       counts, loc);
   const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
   uint32_t* sums = off + nt + 1;
   hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums);
+  hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, int(nsb), sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, cursor);
   if (npts > 0) {
-    hipLaunchKernelGGL(k_site_fill<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, off, loc,
-                       eid, esite);
+    hipLaunchKernelGGL(k_site_fill<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, slot_ids,
+                       off, loc, eid, esite);
     if (nent > 0)
       hipLaunchKernelGGL(k_site_place<true>, dim3(blocks_for(size_t(nent))), dim3(PBLOCK), 0, s, uint32_t(nent), none,
                          ldesc, pts, off, eid, esite, sorted);
   }
   return hipGetLastError();
 }
-template hipError_t launch_site_force<double>(int, const Geom&, const SpreadPoint*, uint32_t*, uint32_t*, uint32_t*,
-                                              uint32_t*, uint32_t*, uint32_t*, double*, double*, hipStream_t);
-template hipError_t launch_site_force<float>(int, const Geom&, const SpreadPoint*, uint32_t*, uint32_t*, uint32_t*,
-                                             uint32_t*, uint32_t*, uint32_t*, double*, float*, hipStream_t);
+template hipError_t launch_site_force<double>(int, const Geom&, const SpreadPoint*, const uint32_t*, uint32_t*,
+                                              uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, double*, double*,
+                                              hipStream_t);
+template hipError_t launch_site_force<float>(int, const Geom&, const SpreadPoint*, const uint32_t*, uint32_t*,
+                                             uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, double*, float*,
+                                             hipStream_t);
 
 template <typename T>
 hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,

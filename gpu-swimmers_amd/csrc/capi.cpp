@@ -200,11 +200,10 @@ struct lbswim_lattice {
   size_t sp_rec_cap = 0;      // list entries sp_rec and sp_wz hold (fused step)
   RunEntry* sp_rec = nullptr;
   int* sp_wz = nullptr;
-  // Stored force by site (launch_site_force): per-site counts, offsets and
-  // cursors, and ss_cap point entries (slot, id, site, ordered product).
+  // Stored force by site (launch_site_force): per-site counts and offsets,
+  // and ss_cap point entries (slot, id, site, ordered product).
   uint32_t* ss_counts = nullptr;
   uint32_t* ss_off = nullptr;
-  uint32_t* ss_cursor = nullptr;
   size_t ss_cap = 0;
   uint32_t* ss_local = nullptr;
   uint32_t* ss_id = nullptr;
@@ -542,7 +541,6 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->sp_wz);
   dev_free(lat->ss_counts);
   dev_free(lat->ss_off);
-  dev_free(lat->ss_cursor);
   dev_free(lat->ss_local);
   dev_free(lat->ss_id);
   dev_free(lat->ss_site);
@@ -1383,8 +1381,7 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     int rc;
     if (!lat->ss_counts &&
         ((rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->ss_counts), 4 * n)) ||
-         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->ss_off), 4 * spread_off_entries(uint32_t(n)))) ||
-         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->ss_cursor), 4 * n))))
+         (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->ss_off), 4 * spread_off_entries(uint32_t(n))))))
       return rc;
     if (lat->ss_cap < nent) {
       size_t c1 = lat->ss_cap, c2 = lat->ss_cap, c3 = lat->ss_cap, c4 = lat->ss_cap;
@@ -1407,11 +1404,9 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     if (e == hipSuccess)
       e = lat->storage == LBSWIM_STORE_F64
               ? launch_site_force<double>(int(npts), lat->geom, pts, lat->sp_slot, lat->ss_counts, lat->ss_off,
-                                          lat->ss_cursor,
                                           lat->ss_local, lat->ss_id, lat->ss_site, lat->ss_sorted,
                                           static_cast<double*>(lat->d_force), stream())
               : launch_site_force<float>(int(npts), lat->geom, pts, lat->sp_slot, lat->ss_counts, lat->ss_off,
-                                         lat->ss_cursor,
                                          lat->ss_local, lat->ss_id, lat->ss_site, lat->ss_sorted,
                                          static_cast<float*>(lat->d_force), stream());
     t.done();
@@ -2091,8 +2086,6 @@ struct lbswim_ensemble {
   size_t cap_ss_counts = 0;
   uint32_t* ss_off = nullptr;
   size_t cap_ss_off = 0;
-  uint32_t* ss_cursor = nullptr;
-  size_t cap_ss_cursor = 0;
   uint32_t* ss_local = nullptr;
   size_t cap_ss_local = 0;
   uint32_t* ss_eid = nullptr;
@@ -2469,7 +2462,6 @@ int ens_build(lbswim_ensemble* e) {
     const size_t ns = site_base;
     if ((rc = grow_to(&e->ss_counts, &e->cap_ss_counts, ns, 4)) ||
         (rc = grow_to(&e->ss_off, &e->cap_ss_off, spread_off_entries(uint32_t(ns)), 4)) ||
-        (rc = grow_to(&e->ss_cursor, &e->cap_ss_cursor, ns, 4)) ||
         (rc = grow_to(&e->ss_local, &e->cap_ss_local, 64 * size_t(e->npts), 4)) ||
         (rc = grow_to(&e->ss_eid, &e->cap_ss_eid, 64 * site_points, 4)) ||
         (rc = grow_to(&e->ss_site, &e->cap_ss_site, 64 * site_points, 4)) ||
@@ -2581,7 +2573,7 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   auto stored_chain = [&]() -> int {
     EnsTimer t(e, LBSWIM_K_STORED, sb);
     HIP_TRY(launch_batch_site_lists(e->d_lat, e->npts, e->nsites, e->ss_nent, e->pts, e->slot_ids, e->ss_counts,
-                                    e->ss_off, e->ss_cursor, e->ss_local, e->ss_eid, e->ss_site, e->ss_sorted, sb));
+                                    e->ss_off, e->ss_local, e->ss_eid, e->ss_site, e->ss_sorted, sb));
     HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->per, e->pts, e->off,
                                          e->sorted, e->ss_off, e->ss_sorted, sb));
     HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb));
@@ -2781,7 +2773,6 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   dev_free(e->sorted);
   dev_free(e->ss_counts);
   dev_free(e->ss_off);
-  dev_free(e->ss_cursor);
   dev_free(e->ss_local);
   dev_free(e->ss_eid);
   dev_free(e->ss_site);

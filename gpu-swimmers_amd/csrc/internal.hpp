@@ -155,8 +155,8 @@ hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint
 inline bool site_force_ok(const Geom& g) { return g.nx >= 4 && g.ny >= 4 && g.nz >= 4; }
 template <typename T>
 hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, const uint32_t* slot_ids,
-                             uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* local, uint32_t* eid,
-                             uint32_t* esite, double* sorted, T* force, hipStream_t s);
+                             uint32_t* counts, uint32_t* off, uint32_t* local, uint32_t* eid, uint32_t* esite,
+                             double* sorted, T* force, hipStream_t s);
 
 // Force at the first and last site of every list's 64-site run:
 // edge[6 l + 3 e + a] (e = 0 first, 1 last), 6 spread_lists(g) doubles; and,
@@ -278,8 +278,8 @@ hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const 
 // cursor[nsites], local[64 npts] (16-byte aligned), eid, esite[nent],
 // sorted[3 nent].
 hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, int nent, const SpreadPoint* pts,
-                                   const uint32_t* slot_ids, uint32_t* counts, uint32_t* off, uint32_t* cursor,
-                                   uint32_t* local, uint32_t* eid, uint32_t* esite, double* sorted, hipStream_t s);
+                                   const uint32_t* slot_ids, uint32_t* counts, uint32_t* off, uint32_t* local,
+                                   uint32_t* eid, uint32_t* esite, double* sorted, hipStream_t s);
 // Writes the force arrays of the lattice blocks in block_work: from the lists
 // (FORCE_LISTS_STORED), from all the member's points (FORCE_RANGE_STORED) or
 // from its site segments soff/ssorted (FORCE_SITES_STORED).

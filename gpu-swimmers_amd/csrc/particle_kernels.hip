@@ -934,12 +934,18 @@ __global__ __launch_bounds__(PBLOCK) void k_count_rows(uint32_t npts, Geom g, co
   }
 }
 
-// Exclusive scan of counts[0..nt) into off[0..nt] in two coalesced passes
-// over SCAN_BLOCK lists per workgroup; the second pass also clears cursor and
-// resets counts to zero for the next build (its last reader).
+// Exclusive scan of counts[0..nt) into off[0..nt] in three passes: the
+// total of every tile of SCAN_TILE counts, an exclusive scan of those totals
+// by one workgroup, and the scan within each tile from its offset.  Each
+// thread holds SCAN_ITEMS consecutive counts (two 16-byte accesses; the
+// arrays are allocation bases).  The last pass also resets counts to zero for
+// the next build (its last reader) and clears cursor where one is given.
 // Small workgroups: the scans run beside the tracer moves, and a workgroup
 // has to wait for a CU with room for all of its wavefronts.
 constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+inline unsigned scan_blocks(size_t nt) { return unsigned((nt + SCAN_TILE - 1) / SCAN_TILE); }
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* part, uint32_t* total) {
   const int t = int(threadIdx.x);
@@ -957,16 +963,43 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* p
   return incl - v;
 }
 
+__device__ __forceinline__ void scan_load(const uint32_t* __restrict__ a, int i0, int nt, uint32_t (&v)[SCAN_ITEMS]) {
+  if (i0 + SCAN_ITEMS <= nt) {
+    const uint4 x = *reinterpret_cast<const uint4*>(a + i0), y = *reinterpret_cast<const uint4*>(a + i0 + 4);
+    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+    v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = i0 + k < nt ? a[i0 + k] : 0u;
+  }
+}
+
+__device__ __forceinline__ void scan_store(uint32_t* __restrict__ a, int i0, int nt, const uint32_t (&v)[SCAN_ITEMS]) {
+  if (i0 + SCAN_ITEMS <= nt) {
+    *reinterpret_cast<uint4*>(a + i0) = make_uint4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<uint4*>(a + i0 + 4) = make_uint4(v[4], v[5], v[6], v[7]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+      if (i0 + k < nt) a[i0 + k] = v[k];
+  }
+}
+
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(int nt, const uint32_t* __restrict__ counts,
                                                           uint32_t* __restrict__ sums) {
   __shared__ uint32_t part[SCAN_BLOCK];
-  const int i = int(blockIdx.x) * SCAN_BLOCK + int(threadIdx.x);
+  const int i0 = (int(blockIdx.x) * SCAN_BLOCK + int(threadIdx.x)) * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  scan_load(counts, i0, nt, v);
+  uint32_t mine = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) mine += v[k];
   uint32_t total = 0;
-  block_exclusive_scan(i < nt ? counts[i] : 0u, part, &total);
+  block_exclusive_scan(mine, part, &total);
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// The nsb workgroup totals replaced by their exclusive prefix sums, by one
+// The nsb tile totals replaced by their exclusive prefix sums, by one
 // workgroup (SCAN_PER consecutive totals per thread, in chunks).
 constexpr int SCAN_PER = 16;
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_block_sums(int nsb, uint32_t* __restrict__ sums) {
@@ -992,21 +1025,31 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_block_sums(int nsb, uint32_
   }
 }
 
-// `sums`: the workgroup totals after k_scan_block_sums (exclusive prefixes).
+// `sums`: the tile totals after k_scan_block_sums (exclusive prefixes).
+// `cursor` may be null.
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(int nt, uint32_t* __restrict__ counts,
                                                            const uint32_t* __restrict__ sums,
                                                            uint32_t* __restrict__ off, uint32_t* __restrict__ cursor) {
   __shared__ uint32_t part[SCAN_BLOCK];
-  const uint32_t base = sums[blockIdx.x];  // offset of this workgroup
-  const int i = int(blockIdx.x) * SCAN_BLOCK + int(threadIdx.x);
-  const uint32_t v = i < nt ? counts[i] : 0u;
+  const uint32_t base = sums[blockIdx.x];  // offset of this tile
+  const int i0 = (int(blockIdx.x) * SCAN_BLOCK + int(threadIdx.x)) * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  scan_load(counts, i0, nt, v);
+  uint32_t mine = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) mine += v[k];
   uint32_t total = 0;
-  const uint32_t ex = block_exclusive_scan(v, part, &total);
-  if (i < nt) {
-    off[i] = base + ex;
-    cursor[i] = 0u;
-    counts[i] = 0u;
+  uint32_t run = base + block_exclusive_scan(mine, part, &total);
+  uint32_t o[SCAN_ITEMS], zero[SCAN_ITEMS];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    o[k] = run;
+    run += v[k];
+    zero[k] = 0u;
   }
+  scan_store(off, i0, nt, o);
+  scan_store(counts, i0, nt, zero);
+  if (cursor) scan_store(cursor, i0, nt, zero);
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
     off[nt] = base + total;
     off[nt + 1 + gridDim.x] = 0u;  // queue counter of the sort, after the scan's partial sums
@@ -1272,23 +1315,42 @@ __device__ __forceinline__ bool site_member(const SpreadPoint& p, const LatDesc*
 // One thread per (point, stencil row), points in storage order (slot_ids:
 // spatial neighbours, so the atomics of a workgroup stay in a few lines):
 // counts the row's 4 sites and keeps each site's slot within its segment.
+// Lanes l and l ^ 16 of a wavefront hold the same row of storage-adjacent
+// points 2m and 2m + 1 -- the tail and head of one swimmer, whose stencils
+// mostly coincide (l is a fraction of a site) -- so where their sites are
+// equal the even lane adds 2 for both and the odd lane takes the next slot.
 template <bool Batched>
 __global__ __launch_bounds__(PBLOCK) void k_site_count(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
                                                       const SpreadPoint* __restrict__ pts,
                                                       const uint32_t* __restrict__ slot_ids,
                                                       uint32_t* __restrict__ counts, uint4* __restrict__ local) {
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
   const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
-  if (t >= 16 * npts) return;
-  const SpreadPoint& p = pts[slot_ids[t >> 4]];
-  uint32_t base;
-  if (!site_member<Batched>(p, ldesc, g, base)) return;
-  const uint32_t row = base + stencil_row_site(p, int(t & 15), g);
-  uint4 q;
-  q.x = atomicAdd(&counts[row + uint32_t(p.wc[8])], 1u);
-  q.y = atomicAdd(&counts[row + uint32_t(p.wc[9])], 1u);
-  q.z = atomicAdd(&counts[row + uint32_t(p.wc[10])], 1u);
-  q.w = atomicAdd(&counts[row + uint32_t(p.wc[11])], 1u);
-  local[t] = q;
+  uint32_t site[4] = {NONE, NONE, NONE, NONE};
+  if (t < 16 * npts) {
+    const SpreadPoint& p = pts[slot_ids[t >> 4]];
+    uint32_t base;
+    if (site_member<Batched>(p, ldesc, g, base)) {
+      const uint32_t row = base + stencil_row_site(p, int(t & 15), g);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) site[k] = row + uint32_t(p.wc[8 + k]);
+    }
+  }
+  const bool odd = (threadIdx.x & 16u) != 0;
+  bool shared[4];
+  uint32_t slot[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    shared[k] = site[k] != NONE && uint32_t(__shfl_xor(int(site[k]), 16)) == site[k];
+    const uint32_t add = shared[k] ? (odd ? 0u : 2u) : (site[k] != NONE ? 1u : 0u);
+    slot[k] = add ? atomicAdd(&counts[site[k]], add) : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t even_slot = uint32_t(__shfl_xor(int(slot[k]), 16));
+    if (shared[k] && odd) slot[k] = even_slot + 1u;
+  }
+  if (site[0] != NONE) local[t] = make_uint4(slot[0], slot[1], slot[2], slot[3]);
 }
 
 // One thread per (point, stencil row): the point id and the site into the
@@ -1648,7 +1710,7 @@ template hipError_t launch_tracers_move<float>(int, double*, double*, double*, c
 
 size_t spread_off_entries(uint32_t nlists) {
   // nlists + 1 offsets, the scan's per-workgroup sums, the sort's queue counter
-  return size_t(nlists) + 2 + (size_t(nlists) + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  return size_t(nlists) + 2 + scan_blocks(nlists);
 }
 
 size_t spread_list_capacity(size_t npts) {
@@ -1684,7 +1746,7 @@ static hipError_t build_lists(int npts, int nt, const Geom& g, const LatDesc* ld
                          pts, slot_ids, counts);
   }
   // the per-workgroup totals of the scan live after the nt+1 offsets
-  const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  const unsigned nsb = scan_blocks(size_t(nt));
   uint32_t* sums = off + nt + 1;
   hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, nt, counts, sums);
   hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, int(nsb), sums);
@@ -1728,8 +1790,8 @@ template hipError_t launch_range_force<float>(const Geom&, const SpreadPoint*, u
 
 template <typename T>
 hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, const uint32_t* slot_ids,
-                             uint32_t* counts, uint32_t* off, uint32_t* cursor, uint32_t* local, uint32_t* eid,
-                             uint32_t* esite, double* sorted, T* force, hipStream_t s) {
+                             uint32_t* counts, uint32_t* off, uint32_t* local, uint32_t* eid, uint32_t* esite,
+                             double* sorted, T* force, hipStream_t s) {
   if (g.n == 0) return hipSuccess;
   const uint32_t nt = g.n;
   const unsigned rows = blocks_for(16 * size_t(npts));
@@ -1738,11 +1800,11 @@ hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, co
     hipLaunchKernelGGL(k_site_count<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, slot_ids,
                        counts, loc);
   // per-workgroup totals of the scan after the nt + 1 offsets
-  const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  const unsigned nsb = scan_blocks(size_t(nt));
   uint32_t* sums = off + nt + 1;
   hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums);
   hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, int(nsb), sums);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, cursor);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, nullptr);
   if (npts > 0) {
     hipLaunchKernelGGL(k_site_fill<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, slot_ids,
                        off, loc, eid, esite);
@@ -1755,8 +1817,8 @@ hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, co
 }
 
 hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, int nent, const SpreadPoint* pts,
-                                   const uint32_t* slot_ids, uint32_t* counts, uint32_t* off, uint32_t* cursor,
-                                   uint32_t* local, uint32_t* eid, uint32_t* esite, double* sorted, hipStream_t s) {
+                                   const uint32_t* slot_ids, uint32_t* counts, uint32_t* off, uint32_t* local,
+                                   uint32_t* eid, uint32_t* esite, double* sorted, hipStream_t s) {
   if (nsites <= 0) return hipSuccess;
   const uint32_t nt = uint32_t(nsites);
   const unsigned rows = blocks_for(16 * size_t(npts));
@@ -1765,11 +1827,11 @@ hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, i
   if (npts > 0)
     hipLaunchKernelGGL(k_site_count<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, slot_ids,
                        counts, loc);
-  const unsigned nsb = unsigned((nt + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  const unsigned nsb = scan_blocks(size_t(nt));
   uint32_t* sums = off + nt + 1;
   hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums);
   hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, int(nsb), sums);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, cursor);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, nullptr);
   if (npts > 0) {
     hipLaunchKernelGGL(k_site_fill<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, slot_ids,
                        off, loc, eid, esite);
@@ -1780,11 +1842,10 @@ hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, i
   return hipGetLastError();
 }
 template hipError_t launch_site_force<double>(int, const Geom&, const SpreadPoint*, const uint32_t*, uint32_t*,
-                                              uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, double*, double*,
+                                              uint32_t*, uint32_t*, uint32_t*, uint32_t*, double*, double*,
                                               hipStream_t);
 template hipError_t launch_site_force<float>(int, const Geom&, const SpreadPoint*, const uint32_t*, uint32_t*,
-                                             uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, double*, float*,
-                                             hipStream_t);
+                                             uint32_t*, uint32_t*, uint32_t*, uint32_t*, double*, float*, hipStream_t);
 
 template <typename T>
 hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,

@@ -21,11 +21,22 @@ def small_requests(n, seed):
 
 def test_results_same_alone_and_in_the_mix():
     from lbswim.serving import Server, estimate_bytes, run_alone
+
+    class AdmitTogether(Server):
+        """Admits only once every request being prepared is built, so the
+        requests submitted together run together whatever the build times
+        (a short request may otherwise finish before the next is built)."""
+
+        def admit_ready(self):
+            for _, fut in self.preparing.values():
+                fut.result()
+            super().admit_ready()
+
     reqs = small_requests(10, seed=7)
     # room for about three requests at a time: the rest queue and are admitted as others finish
     budget = 3 * max(estimate_bytes(r) for r in reqs)
     say("serving mix: %d requests, budget %d bytes, storages %s" % (len(reqs), budget, [r.storage for r in reqs]))
-    srv = Server(hbm_budget=budget, chunk=16)
+    srv = AdmitTogether(hbm_budget=budget, chunk=16)
     peak = []
     stats = srv.run(reqs, realtime=False, on_chunk=lambda s: peak.append(len(s.active)))
     say("stats %s, peak concurrent %d" % (stats, max(peak)))

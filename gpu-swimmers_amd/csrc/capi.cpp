@@ -2574,9 +2574,11 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
     EnsTimer t(e, LBSWIM_K_STORED, sb);
     HIP_TRY(launch_batch_site_lists(e->d_lat, e->npts, e->nsites, e->ss_nent, e->pts, e->slot_ids, e->ss_counts,
                                     e->ss_off, e->ss_local, e->ss_eid, e->ss_site, e->ss_sorted, sb));
+    // the site members' force is summed and stored by the collide
     HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->per, e->pts, e->off,
-                                         e->sorted, e->ss_off, e->ss_sorted, sb));
-    HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb));
+                                         e->sorted, nullptr, nullptr, sb));
+    HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb,
+                                    e->ss_off, e->ss_sorted));
     HIP_TRY(launch_batch_hydro<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb));
     t.done();
     return LBSWIM_OK;

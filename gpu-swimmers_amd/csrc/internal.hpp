@@ -238,9 +238,13 @@ constexpr int TRACER_GROUPS = 64;        // tracers per workgroup of the tracer 
 // b comes from lists tile_base_m + 4 b ...; null: members that read their
 // force array.
 // Each work item covers `per` consecutive 256-site blocks of its member.
+// Without vf, the members with FORCE_SITES_STORED take their force from the
+// site segments soff/ssorted (the sum launch_batch_spread_force forms) and
+// store it in their force arrays for the CalcHydro.
 template <typename T>
 hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int per, int parity,
-                                const VForce* vf, hipStream_t s);
+                                const VForce* vf, hipStream_t s, const uint32_t* soff = nullptr,
+                                const double* ssorted = nullptr);
 template <typename T>
 hipError_t launch_batch_hydro(const LatDesc* desc, const WorkItem* work, int nwork, int per, int parity,
                               const VForce* vf, hipStream_t s);
@@ -282,7 +286,8 @@ hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, i
                                    uint32_t* eid, uint32_t* esite, double* sorted, hipStream_t s);
 // Writes the force arrays of the lattice blocks in block_work: from the lists
 // (FORCE_LISTS_STORED), from all the member's points (FORCE_RANGE_STORED) or
-// from its site segments soff/ssorted (FORCE_SITES_STORED).
+// from its site segments soff/ssorted (FORCE_SITES_STORED; none where soff is
+// null, for launch_batch_collide to write from the segments).
 template <typename T>
 hipError_t launch_batch_spread_force(const LatDesc* ldesc, int nwork, const WorkItem* block_work, int per,
                                      const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,

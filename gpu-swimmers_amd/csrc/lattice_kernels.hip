@@ -224,6 +224,16 @@ __device__ __forceinline__ void store_all(double* __restrict__ gNew, uint32_t n,
   (store_stream(&gNew[size_t(P) * n + s], inv<P>(mode)), ...);
 }
 
+// Stored force from per-site segments of ordered products (segment_force):
+// the collide sums its site's segment (site sbase + s) and stores the force
+// where the CalcHydro reads it.  off null: the force array is read.
+struct SegForce {
+  const uint32_t* off = nullptr;
+  const double* sorted = nullptr;
+  uint32_t sbase = 0;
+  void* out = nullptr;  // the member's force array
+};
+
 // DoStep (lb.cu:72-218): pull-stream, then collide into the site's own slots.
 // VF: the force comes from the wavefront's point list (wave_force; lists
 // vlist.. of the block) instead of the force array.  No barrier is involved,
@@ -232,7 +242,7 @@ template <bool VF>
 __device__ __forceinline__ void collide_f64_body(const double* __restrict__ gOld, const double* __restrict__ force,
                                                  double* __restrict__ gNew, Geom g, double tau_s, double tau_b,
                                                  double omega_s, double omega_b, uint32_t blk, const VForce& vf,
-                                                 uint32_t vlist) {
+                                                 uint32_t vlist, const SegForce& sf = SegForce{}) {
   const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
@@ -244,7 +254,11 @@ __device__ __forceinline__ void collide_f64_body(const double* __restrict__ gOld
   // the list walk uses scalar loads only, so it overlaps the loads of f
   if constexpr (VF)
     wave_force(vf.pts, vf.off, vf.sorted, vlist + threadIdx.x / WAVE_SITES, true, int(c.i), int(c.j), int(c.k), g, F);
-  else {
+  else if (sf.off) {
+    segment_force(sf.off, sf.sorted, sf.sbase + s, F);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) static_cast<double*>(sf.out)[size_t(a) * n + s] = F[a];
+  } else {
     F[0] = force[s];
     F[1] = force[size_t(n) + s];
     F[2] = force[2 * size_t(n) + s];
@@ -427,7 +441,7 @@ template <bool VF>
 __device__ __forceinline__ void collide_f32_body(const float* __restrict__ fOld, const float* __restrict__ force,
                                                  float* __restrict__ fNew, Geom g, float tau_s, float tau_b,
                                                  float omega_s, float omega_b, uint32_t blk, const VForce& vf,
-                                                 uint32_t vlist) {
+                                                 uint32_t vlist, const SegForce& sf = SegForce{}) {
   const uint32_t s = blk * BLOCK + threadIdx.x;
   if (s >= g.n) return;
   const uint32_t n = g.n;
@@ -441,6 +455,14 @@ __device__ __forceinline__ void collide_f32_body(const float* __restrict__ fOld,
                acc);
 #pragma unroll
     for (int a = 0; a < 3; ++a) F[a] = float(acc[a]);
+  } else if (sf.off) {
+    double acc[3];
+    segment_force(sf.off, sf.sorted, sf.sbase + s, acc);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      F[a] = float(acc[a]);
+      static_cast<float*>(sf.out)[size_t(a) * n + s] = F[a];
+    }
   } else {
 #pragma unroll
     for (int a = 0; a < 3; ++a) F[a] = force[size_t(a) * n + s];
@@ -1068,11 +1090,14 @@ __global__ __launch_bounds__(BLOCK) void k_calc_hydro_vf_f32(const float* __rest
 // work-unit size the ensemble autotuner chooses).
 template <typename T, bool VF>
 __global__ __launch_bounds__(BLOCK) void kb_collide(const LatDesc* __restrict__ desc, const WorkItem* __restrict__ work,
-                                                    int parity, VForce vf, int per) {
+                                                    int parity, VForce vf, int per, const uint32_t* __restrict__ soff,
+                                                    const double* __restrict__ ssorted) {
   const WorkItem w = work[blockIdx.x];
   const LatDesc& m = desc[w.member];
   const int c = m.cur ^ parity;
   const uint32_t nb = grid_blocks(m.geom.n);
+  SegForce sf;
+  if (!VF && soff && m.force_path == FORCE_SITES_STORED) sf = SegForce{soff, ssorted, m.site_base, m.force};
   for (int b = 0; b < per; ++b) {
     const uint32_t blk = w.unit + uint32_t(b);
     if (blk >= nb) break;
@@ -1080,11 +1105,11 @@ __global__ __launch_bounds__(BLOCK) void kb_collide(const LatDesc* __restrict__ 
     if constexpr (sizeof(T) == 8)
       collide_f64_body<VF>(static_cast<const double*>(m.g[c]), static_cast<const double*>(m.force),
                            static_cast<double*>(m.g[c ^ 1]), m.geom, m.tau_s, m.tau_b, m.omega_s, m.omega_b, blk, vf,
-                           vlist);
+                           vlist, sf);
     else
       collide_f32_body<VF>(static_cast<const float*>(m.g[c]), static_cast<const float*>(m.force),
                            static_cast<float*>(m.g[c ^ 1]), m.geom, float(m.tau_s), float(m.tau_b), float(m.omega_s),
-                           float(m.omega_b), blk, vf, vlist);
+                           float(m.omega_b), blk, vf, vlist, sf);
   }
 }
 
@@ -1338,12 +1363,14 @@ hipError_t launch_init_from_hydro<float>(const float* rho, const float* u, float
 
 template <typename T>
 hipError_t launch_batch_collide(const LatDesc* desc, const WorkItem* work, int nwork, int per, int parity,
-                                const VForce* vf, hipStream_t s) {
+                                const VForce* vf, hipStream_t s, const uint32_t* soff, const double* ssorted) {
   if (nwork <= 0) return hipSuccess;
   if (vf)
-    hipLaunchKernelGGL((kb_collide<T, true>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf, per);
+    hipLaunchKernelGGL((kb_collide<T, true>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, *vf, per, nullptr,
+                       nullptr);
   else
-    hipLaunchKernelGGL((kb_collide<T, false>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, VForce{}, per);
+    hipLaunchKernelGGL((kb_collide<T, false>), dim3(nwork), dim3(BLOCK), 0, s, desc, work, parity, VForce{}, per,
+                       soff, ssorted);
   return hipGetLastError();
 }
 template <typename T>
@@ -1383,9 +1410,9 @@ bool batch_fused_member(const Geom& g, int rows, size_t elem) {
 }
 
 template hipError_t launch_batch_collide<double>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,
-                                                 hipStream_t);
+                                                 hipStream_t, const uint32_t*, const double*);
 template hipError_t launch_batch_collide<float>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,
-                                                hipStream_t);
+                                                hipStream_t, const uint32_t*, const double*);
 template hipError_t launch_batch_hydro<double>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,
                                                hipStream_t);
 template hipError_t launch_batch_hydro<float>(const LatDesc*, const WorkItem*, int, int, int, const VForce*,

@@ -1414,28 +1414,12 @@ __global__ __launch_bounds__(PBLOCK) void k_site_place(uint32_t nent, Geom g, co
 }
 
 // Force at site s (segment gs) of a lattice of n sites: its segment's
-// products added in order (loaded four entries ahead of the adds).
+// products added in order (segment_force).
 template <typename T>
 __device__ __forceinline__ void site_sum_body(uint32_t n, uint32_t s, uint32_t gs, const uint32_t* __restrict__ off,
                                               const double* __restrict__ sorted, T* __restrict__ force) {
-  double acc[3] = {0.0, 0.0, 0.0};
-  uint32_t e = off[gs];
-  const uint32_t end = off[gs + 1];
-  for (; e + 4 <= end; e += 4) {
-    double v[4][3];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) v[q][a] = sorted[3 * size_t(e + q) + a];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) acc[a] += v[q][a];
-  }
-  for (; e < end; ++e) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) acc[a] += sorted[3 * size_t(e) + a];
-  }
+  double acc[3];
+  segment_force(off, sorted, gs, acc);
 #pragma unroll
   for (int a = 0; a < 3; ++a) force[size_t(a) * n + s] = T(acc[a]);
 }
@@ -1613,8 +1597,12 @@ __global__ __launch_bounds__(SITE_BLOCK) void kb_spread_blocks(const LatDesc* __
     const uint32_t blk = w.unit + uint32_t(b);
     if (blk >= nb) break;
     if (m.force_path == FORCE_SITES_STORED) {
-      const uint32_t site = blk * SITE_BLOCK + threadIdx.x;
-      if (site < m.geom.n) site_sum_body(m.geom.n, site, m.site_base + site, soff, ssorted, static_cast<T*>(m.force));
+      // soff null: the batched collide sums these members' segments itself
+      if (soff) {
+        const uint32_t site = blk * SITE_BLOCK + threadIdx.x;
+        if (site < m.geom.n)
+          site_sum_body(m.geom.n, site, m.site_base + site, soff, ssorted, static_cast<T*>(m.force));
+      }
     } else if (m.force_path == FORCE_RANGE_STORED) {
       range_block_body(m.geom, pts, m.point_first, m.point_count, static_cast<T*>(m.force), blk, &stage.range);
     } else

@@ -114,6 +114,31 @@ __device__ __forceinline__ void add_point_lds(const SpreadPoint* __restrict__ p,
   acc[2] += delta3d * p->F[2];
 }
 
+// Force of the site whose segment is gs in per-site segments of ordered
+// products (off: segment bounds, sorted: 3 doubles per entry): the products
+// added in segment order from zero (loaded four entries ahead of the adds).
+__device__ __forceinline__ void segment_force(const uint32_t* __restrict__ off, const double* __restrict__ sorted,
+                                              uint32_t gs, double (&acc)[3]) {
+  acc[0] = acc[1] = acc[2] = 0.0;
+  uint32_t e = off[gs];
+  const uint32_t end = off[gs + 1];
+  for (; e + 4 <= end; e += 4) {
+    double v[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[q][a] = sorted[3 * size_t(e + q) + a];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) acc[a] += v[q][a];
+  }
+  for (; e < end; ++e) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] += sorted[3 * size_t(e) + a];
+  }
+}
+
 // Force at this lane's site (X, Y, Z) from list `wl`, the list of the
 // wavefront's 64 sites (ok = false for lanes past the lattice end).  The list
 // index is wavefront-uniform, so ids and point records are read through the

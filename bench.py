@@ -509,6 +509,40 @@ def probe_dense(steps=40):
     return out
 
 
+def probe_mix(nreq=48, rate=16.0, steps=40):
+    """The serving leg's requests (same seed) all resident in their two
+    ensembles: ms per graph-replayed step of each ensemble, then each phase's
+    device time per step in an eagerly launched, event-timed pass."""
+    from lbswim.ensemble import Ensemble
+    from lbswim.serving import build, poisson_requests
+    import lbswim
+    reqs = poisson_requests(nreq, rate, 1234)
+    ens = {"f64": Ensemble("f64"), "f32": Ensemble("f32")}
+    keep = []
+    for req in reqs:
+        lat, things = build(req)
+        ens[req.storage].add(lat, things)
+        keep.append((lat, things))
+    out = {}
+    for form, e in ens.items():
+        e.step(2)
+        lbswim.synchronize()
+        t0 = time.perf_counter()
+        e.step(steps)
+        lbswim.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        e.set_timing(True)
+        e.step(20)
+        kt = e.kernel_times()
+        e.set_timing(False)
+        vs, ss = e.sites()
+        out[form] = {"members": sum(1 for r in reqs if r.storage == form), "sites_virtual_stored": [vs, ss],
+                     "fused_sites": e.fused_sites(), "graph_ms_per_step": round(ms, 4),
+                     "phases_ms": {k: round(v / 20, 4) for k, (v, c) in kt.items() if c}}
+        log("mix %s: %.4f ms per graph step" % (form, ms))
+    return out
+
+
 def load_pmc(name, kernel, tiles=None):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary profiles/<name>_pmc.json, or None when there is none or it was
@@ -552,8 +586,9 @@ def main():
     ap.add_argument("--tiles", default="auto",
                     help="fused tile shape 'rows,strips' of the coupled step (rows 0: two launches), or auto "
                          "(autotuned on real steps)")
-    ap.add_argument("--probe", choices=["copy", "dense"],
-                    help="run the streaming-copy bandwidth probe, or time dense small systems' coupled steps, instead")
+    ap.add_argument("--probe", choices=["copy", "dense", "mix"],
+                    help="run the streaming-copy bandwidth probe, time dense small systems' coupled steps, or time "
+                         "the serving leg's requests all resident, instead")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-sim ensemble leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving leg (0: skip)")
     ap.add_argument("--ensemble-eager", action="store_true",
@@ -581,6 +616,9 @@ def main():
         return
     if a.probe == "dense":
         print(json.dumps({"probe": "dense", "systems": probe_dense()}), flush=True)
+        return
+    if a.probe == "mix":
+        print(json.dumps({"probe": "mix", "ensembles": probe_mix(a.serve_requests, a.serve_rate)}), flush=True)
         return
     legs = {x for x in a.legs.split(",") if x}
     if "main" not in legs:  # partial runs (profiling): no headline value

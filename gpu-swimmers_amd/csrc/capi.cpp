@@ -2025,6 +2025,11 @@ struct lbswim_ensemble {
   WorkItem* d_storedwork = nullptr;
   size_t cap_storedwork = 0;
   int n_storedwork = 0;
+  // of those, the blocks whose force a separate pass writes (the lists and
+  // range members; the collide sums the site members' segments itself)
+  WorkItem* d_spreadwork = nullptr;
+  size_t cap_spreadwork = 0;
+  int n_spreadwork = 0;
   WorkItem* d_vfwork = nullptr;
   size_t cap_vfwork = 0;
   int n_vfwork = 0;
@@ -2282,9 +2287,10 @@ int ens_build(lbswim_ensemble* e) {
   std::vector<LatDesc> lat;
   std::vector<char> fused_member;  // lat[i] is advanced by the batched fused step
   // per member: its lattice blocks, in the stored-force (0), virtual-force
-  // two-launch (1) or virtual-force fused (2) group
-  std::vector<std::vector<WorkItem>> latwork[3];
-  std::vector<size_t> weight[3];
+  // two-launch (1) or virtual-force fused (2) group, and (3) the stored-force
+  // members without site segments again
+  std::vector<std::vector<WorkItem>> latwork[4];
+  std::vector<size_t> weight[4];
   std::vector<std::vector<WorkItem>> fusetiles;  // per fused member: its tile-strips
   std::vector<WorkItem> entrywork;
   std::vector<SwimDesc> sw;
@@ -2336,6 +2342,10 @@ int ens_build(lbswim_ensemble* e) {
     const int group = d.force_path != FORCE_LISTS_VIRTUAL ? 0 : (fused ? 2 : 1);
     latwork[group].push_back(lw);
     weight[group].push_back(L->n);
+    if (group == 0 && d.force_path != FORCE_SITES_STORED) {
+      latwork[3].push_back(lw);
+      weight[3].push_back(L->n);
+    }
     if (fused) {
       const uint32_t tiles = uint32_t(L->geom.ny / e->fuse_rows) * uint32_t(L->geom.nz / 64);
       const uint32_t strips = uint32_t((L->geom.nx + e->fuse_strip - 1) / e->fuse_strip);
@@ -2386,6 +2396,7 @@ int ens_build(lbswim_ensemble* e) {
   const std::vector<WorkItem> vfwork = order_work(latwork[1], e->order, weight[1]);
   const std::vector<WorkItem> fuseblk = order_work(latwork[2], e->order, weight[2]);
   const std::vector<WorkItem> fusework = order_work(fusetiles, e->order, weight[2]);
+  const std::vector<WorkItem> spreadwork = order_work(latwork[3], e->order, weight[3]);
   int rc;
   size_t stage_bytes = 0;
   auto add = [&](size_t b) { stage_bytes += (b + 255) / 256 * 256; };
@@ -2393,7 +2404,7 @@ int ens_build(lbswim_ensemble* e) {
   add(sizeof(SwimDesc) * sw.size());
   add(sizeof(TracerDesc) * tr.size());
   for (const std::vector<WorkItem>* v : std::initializer_list<const std::vector<WorkItem>*>{
-           &storedwork, &vfwork, &swwork, &ptwork, &trwork, &fusework, &entrywork, &fuseblk})
+           &storedwork, &vfwork, &swwork, &ptwork, &trwork, &fusework, &entrywork, &fuseblk, &spreadwork})
     add(sizeof(WorkItem) * v->size());
   if ((rc = ens_stage_begin(e, stage_bytes))) return rc;
   if ((rc = upload(e, &e->d_lat, &e->cap_lat, lat)) ||
@@ -2403,9 +2414,11 @@ int ens_build(lbswim_ensemble* e) {
       (rc = upload(e, &e->d_tr, &e->cap_tr, tr)) || (rc = upload(e, &e->d_trwork, &e->cap_trwork, trwork)) ||
       (rc = upload(e, &e->d_fusework, &e->cap_fusework, fusework)) ||
       (rc = upload(e, &e->d_entrywork, &e->cap_entrywork, entrywork)) ||
-      (rc = upload(e, &e->d_fuseblk, &e->cap_fuseblk, fuseblk)) || (rc = ens_stage_end(e)))
+      (rc = upload(e, &e->d_fuseblk, &e->cap_fuseblk, fuseblk)) ||
+      (rc = upload(e, &e->d_spreadwork, &e->cap_spreadwork, spreadwork)) || (rc = ens_stage_end(e)))
     return rc;
   e->n_storedwork = int(storedwork.size());
+  e->n_spreadwork = int(spreadwork.size());
   e->n_vfwork = int(vfwork.size());
   e->n_fusework = int(fusework.size());
   e->n_entrywork = int(entrywork.size());
@@ -2574,8 +2587,9 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
     EnsTimer t(e, LBSWIM_K_STORED, sb);
     HIP_TRY(launch_batch_site_lists(e->d_lat, e->npts, e->nsites, e->ss_nent, e->pts, e->slot_ids, e->ss_counts,
                                     e->ss_off, e->ss_local, e->ss_eid, e->ss_site, e->ss_sorted, sb));
-    // the site members' force is summed and stored by the collide
-    HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_storedwork, e->d_storedwork, e->per, e->pts, e->off,
+    // the site members' force is summed and stored by the collide; a pass
+    // over the other stored members' blocks writes theirs
+    HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_spreadwork, e->d_spreadwork, e->per, e->pts, e->off,
                                          e->sorted, nullptr, nullptr, sb));
     HIP_TRY(launch_batch_collide<T>(e->d_lat, e->d_storedwork, e->n_storedwork, e->per, parity, nullptr, sb,
                                     e->ss_off, e->ss_sorted));
@@ -2762,6 +2776,7 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   if (e->cap) (void)hipStreamDestroy(e->cap);
   dev_free(e->d_lat);
   dev_free(e->d_storedwork);
+  dev_free(e->d_spreadwork);
   dev_free(e->d_vfwork);
   dev_free(e->d_sw);
   dev_free(e->d_swwork);

@@ -1313,18 +1313,27 @@ __device__ __forceinline__ bool site_member(const SpreadPoint& p, const LatDesc*
 }
 
 // One thread per (point, stencil row), points in storage order (slot_ids:
-// spatial neighbours, so the atomics of a workgroup stay in a few lines):
-// counts the row's 4 sites and keeps each site's slot within its segment.
-// Lanes l and l ^ 16 of a wavefront hold the same row of storage-adjacent
-// points 2m and 2m + 1 -- the tail and head of one swimmer, whose stencils
-// mostly coincide (l is a fraction of a site) -- so where their sites are
-// equal the even lane adds 2 for both and the odd lane takes the next slot.
+// spatial neighbours): counts the row's 4 sites and keeps each site's slot
+// within its segment.  A workgroup's 16 points share most of their sites (a
+// swimmer's tail and head lie a fraction of a site apart), so the workgroup
+// counts its entries per site in an LDS hash table first (linear probing;
+// at most 1024 distinct sites in SITE_HASH slots, so the table stays at most
+// half full and every probe ends), then reserves each distinct site's slots
+// with one global atomic, and every entry takes its site's reserved base
+// plus its rank from the LDS count.
+constexpr int SITE_HASH = 2048;
 template <bool Batched>
 __global__ __launch_bounds__(PBLOCK) void k_site_count(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
                                                       const SpreadPoint* __restrict__ pts,
                                                       const uint32_t* __restrict__ slot_ids,
                                                       uint32_t* __restrict__ counts, uint4* __restrict__ local) {
   constexpr uint32_t NONE = 0xFFFFFFFFu;
+  __shared__ uint32_t key[SITE_HASH];
+  __shared__ uint32_t num[SITE_HASH];
+  for (int i = int(threadIdx.x); i < SITE_HASH; i += PBLOCK) {
+    key[i] = NONE;
+    num[i] = 0u;
+  }
   const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
   uint32_t site[4] = {NONE, NONE, NONE, NONE};
   if (t < 16 * npts) {
@@ -1336,21 +1345,28 @@ __global__ __launch_bounds__(PBLOCK) void k_site_count(uint32_t npts, Geom g, co
       for (int k = 0; k < 4; ++k) site[k] = row + uint32_t(p.wc[8 + k]);
     }
   }
-  const bool odd = (threadIdx.x & 16u) != 0;
-  bool shared[4];
-  uint32_t slot[4];
+  __syncthreads();
+  uint32_t h[4] = {0u, 0u, 0u, 0u}, rank[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    shared[k] = site[k] != NONE && uint32_t(__shfl_xor(int(site[k]), 16)) == site[k];
-    const uint32_t add = shared[k] ? (odd ? 0u : 2u) : (site[k] != NONE ? 1u : 0u);
-    slot[k] = add ? atomicAdd(&counts[site[k]], add) : 0u;
+    if (site[k] == NONE) continue;
+    uint32_t x = (site[k] * 2654435761u) >> 21;  // 11 bits: SITE_HASH slots
+    for (;;) {
+      const uint32_t old = atomicCAS(&key[x], NONE, site[k]);
+      if (old == NONE || old == site[k]) break;
+      x = (x + 1u) & uint32_t(SITE_HASH - 1);
+    }
+    h[k] = x;
+    rank[k] = atomicAdd(&num[x], 1u);
   }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t even_slot = uint32_t(__shfl_xor(int(slot[k]), 16));
-    if (shared[k] && odd) slot[k] = even_slot + 1u;
+  __syncthreads();
+  for (int i = int(threadIdx.x); i < SITE_HASH; i += PBLOCK) {
+    const uint32_t sk = key[i];
+    if (sk != NONE) num[i] = atomicAdd(&counts[sk], num[i]);  // the workgroup's first slot of site sk
   }
-  if (site[0] != NONE) local[t] = make_uint4(slot[0], slot[1], slot[2], slot[3]);
+  __syncthreads();
+  if (site[0] != NONE)
+    local[t] = make_uint4(num[h[0]] + rank[0], num[h[1]] + rank[1], num[h[2]] + rank[2], num[h[3]] + rank[3]);
 }
 
 // One thread per (point, stencil row): the point id and the site into the

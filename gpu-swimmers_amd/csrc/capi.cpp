@@ -1338,7 +1338,7 @@ bool use_virtual_force(int mode, size_t npts, size_t lists, size_t n, bool batch
 // each hold many of them, else from lists.
 int stored_force_path(size_t npts, const Geom& g) {
   // each site walks only its own points where every side is >= 4
-  if (site_force_ok(g) && npts > SITE_FORCE_MIN_POINTS) return FORCE_SITES_STORED;
+  if (site_force_ok(g) && npts > SITE_FORCE_MIN_POINTS && npts < SITE_FORCE_MAX_POINTS) return FORCE_SITES_STORED;
   return npts <= RANGE_FORCE_MAX_POINTS ? FORCE_RANGE_STORED : FORCE_LISTS_STORED;
 }
 
@@ -2469,6 +2469,9 @@ int ens_build(lbswim_ensemble* e) {
       (rc = grow_to(&e->slot_ids, &e->cap_slot, size_t(e->npts), 4)) ||
       (rc = grow_to(&e->sorted, &cap_sorted, lcap, 4)) || (rc = grow_to(&e->list, &e->cap_list, lcap, 4)))
     return rc;
+  if (site_base > 0 && point_base >= SITE_FORCE_MAX_POINTS)
+    return fail(LBSWIM_EINVAL, "an ensemble with dense members holds at most %zu swimmer ends (it has %u)",
+                SITE_FORCE_MAX_POINTS - 1, point_base);
   e->nsites = int(site_base);
   e->ss_nent = int(64 * site_points);
   if (site_base > 0) {

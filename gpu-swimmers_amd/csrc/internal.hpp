@@ -205,6 +205,9 @@ constexpr size_t RANGE_FORCE_MAX_POINTS = 4096;
 // On lattices with every side >= 4, dense systems with more points than this
 // take FORCE_SITES_STORED instead.
 constexpr size_t SITE_FORCE_MIN_POINTS = 1024;
+// Site segments key their entries by point id * 64 + stencil index: point
+// ids (numbered across an ensemble) below 2^26.
+constexpr size_t SITE_FORCE_MAX_POINTS = size_t(1) << 26;
 // One workgroup of a batched launch: member index and the block (or tile,
 // or particle group) of that member it processes.
 struct WorkItem {

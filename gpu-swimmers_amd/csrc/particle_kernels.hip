@@ -1369,7 +1369,8 @@ __global__ __launch_bounds__(PBLOCK) void k_site_count(uint32_t npts, Geom g, co
     local[t] = make_uint4(num[h[0]] + rank[0], num[h[1]] + rank[1], num[h[2]] + rank[2], num[h[3]] + rank[3]);
 }
 
-// One thread per (point, stencil row): the point id and the site into the
+// One thread per (point, stencil row): the entry key -- point id * 64 +
+// stencil index 16 i + 4 j + k (ids below 2^26) -- and the site into the
 // row's 4 slots.
 template <bool Batched>
 __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
@@ -1391,38 +1392,28 @@ __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, con
   for (int k = 0; k < 4; ++k) {
     const uint32_t site = row + uint32_t(p.wc[8 + k]);
     const uint32_t e = off[site] + slot[k];
-    eid[e] = id;
+    eid[e] = (id << 6) | (uint32_t(t & 15) << 2) | uint32_t(k);
     esite[e] = site;
   }
 }
 
 // One thread per slot, in slot order (so a wavefront's lanes mostly scan
-// the same segment): the entry's product, formed from the site and the point
-// as add_point forms it, goes to the rank of its point id within the segment
-// (the ids of a segment are distinct).  nent = 64 points of the members
-// concerned.
-template <bool Batched>
-__global__ __launch_bounds__(PBLOCK) void k_site_place(uint32_t nent, Geom g, const LatDesc* __restrict__ ldesc,
-                                                      const SpreadPoint* __restrict__ pts,
+// the same segment): the entry's product, formed from its point and stencil
+// index as add_point forms it, goes to the rank of its point id within the
+// segment (the points of a segment are distinct, so keys rank as ids do).
+// nent = 64 points of the members concerned.
+__global__ __launch_bounds__(PBLOCK) void k_site_place(uint32_t nent, const SpreadPoint* __restrict__ pts,
                                                       const uint32_t* __restrict__ off,
                                                       const uint32_t* __restrict__ eid,
                                                       const uint32_t* __restrict__ esite, double* __restrict__ sorted) {
   const uint32_t e = blockIdx.x * PBLOCK + threadIdx.x;
   if (e >= nent) return;
-  const uint32_t id = eid[e], site = esite[e];
-  const SpreadPoint& p = pts[id];
-  uint32_t base;
-  site_member<Batched>(p, ldesc, g, base);
-  const SiteXYZ c = site_xyz(site - base, g);
-  int di = c.x - p.wc[0];
-  di += di < 0 ? g.nx : 0;
-  int dj = c.y - p.wc[4];
-  dj += dj < 0 ? g.ny : 0;
-  int dk = c.z - p.wc[8];
-  dk += dk < 0 ? g.nz : 0;
+  const uint32_t key = eid[e], site = esite[e];
+  const SpreadPoint& p = pts[key >> 6];
+  const int di = int(key >> 4) & 3, dj = int(key >> 2) & 3, dk = int(key) & 3;
   const uint32_t o = off[site], end = off[site + 1];
   uint32_t rank = 0;
-  for (uint32_t q = o; q < end; ++q) rank += eid[q] < id ? 1u : 0u;
+  for (uint32_t q = o; q < end; ++q) rank += eid[q] < key ? 1u : 0u;
   const double d3 = p.del[di] * p.del[4 + dj] * p.del[8 + dk];
   double* __restrict__ dst = sorted + 3 * size_t(o + rank);
 #pragma unroll
@@ -1813,8 +1804,7 @@ hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, co
     hipLaunchKernelGGL(k_site_fill<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, slot_ids,
                        off, loc, eid, esite);
     const uint32_t nent = 64u * uint32_t(npts);
-    hipLaunchKernelGGL(k_site_place<false>, dim3(blocks_for(nent)), dim3(PBLOCK), 0, s, nent, g, nullptr, pts, off,
-                       eid, esite, sorted);
+    hipLaunchKernelGGL(k_site_place, dim3(blocks_for(nent)), dim3(PBLOCK), 0, s, nent, pts, off, eid, esite, sorted);
   }
   hipLaunchKernelGGL(k_site_sum<T>, dim3(blocks_for(nt)), dim3(PBLOCK), 0, s, nt, off, sorted, force);
   return hipGetLastError();
@@ -1840,8 +1830,8 @@ hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, i
     hipLaunchKernelGGL(k_site_fill<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, slot_ids,
                        off, loc, eid, esite);
     if (nent > 0)
-      hipLaunchKernelGGL(k_site_place<true>, dim3(blocks_for(size_t(nent))), dim3(PBLOCK), 0, s, uint32_t(nent), none,
-                         ldesc, pts, off, eid, esite, sorted);
+      hipLaunchKernelGGL(k_site_place, dim3(blocks_for(size_t(nent))), dim3(PBLOCK), 0, s, uint32_t(nent), pts, off,
+                         eid, esite, sorted);
   }
   return hipGetLastError();
 }

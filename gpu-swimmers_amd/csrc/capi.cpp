@@ -206,8 +206,7 @@ struct lbswim_lattice {
   uint32_t* ss_off = nullptr;
   size_t ss_cap = 0;
   uint32_t* ss_local = nullptr;
-  uint32_t* ss_id = nullptr;
-  uint32_t* ss_site = nullptr;
+  uint32_t* ss_ent = nullptr;  // key and site of each entry
   double* ss_sorted = nullptr;
   // Virtual force: after a fused step the force lives in the sorted point
   // lists above and d_force is stale until materialize_force writes it.
@@ -542,8 +541,7 @@ void lbswim_lattice_destroy(lbswim_lattice* lat) {
   dev_free(lat->ss_counts);
   dev_free(lat->ss_off);
   dev_free(lat->ss_local);
-  dev_free(lat->ss_id);
-  dev_free(lat->ss_site);
+  dev_free(lat->ss_ent);
   dev_free(lat->ss_sorted);
   dev_free(lat->sp_pts);
   dev_free(lat->sp_slot);
@@ -1384,10 +1382,9 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
          (rc = dev_alloc_zero(reinterpret_cast<void**>(&lat->ss_off), 4 * spread_off_entries(uint32_t(n))))))
       return rc;
     if (lat->ss_cap < nent) {
-      size_t c1 = lat->ss_cap, c2 = lat->ss_cap, c3 = lat->ss_cap, c4 = lat->ss_cap;
+      size_t c1 = lat->ss_cap, c2 = lat->ss_cap, c4 = lat->ss_cap;
       if ((rc = grow(reinterpret_cast<void**>(&lat->ss_local), &c1, nent, 4)) ||
-          (rc = grow(reinterpret_cast<void**>(&lat->ss_id), &c2, nent, 4)) ||
-          (rc = grow(reinterpret_cast<void**>(&lat->ss_site), &c3, nent, 4)) ||
+          (rc = grow(reinterpret_cast<void**>(&lat->ss_ent), &c2, nent, 8)) ||
           (rc = grow(reinterpret_cast<void**>(&lat->ss_sorted), &c4, nent, 3 * sizeof(double))))
         return rc;
       lat->ss_cap = nent;
@@ -1404,10 +1401,10 @@ int spread_forces(lbswim_lattice* lat, lbswim_swimmers* const* sw, int nsw) {
     if (e == hipSuccess)
       e = lat->storage == LBSWIM_STORE_F64
               ? launch_site_force<double>(int(npts), lat->geom, pts, lat->sp_slot, lat->ss_counts, lat->ss_off,
-                                          lat->ss_local, lat->ss_id, lat->ss_site, lat->ss_sorted,
+                                          lat->ss_local, lat->ss_ent, lat->ss_sorted,
                                           static_cast<double*>(lat->d_force), stream())
               : launch_site_force<float>(int(npts), lat->geom, pts, lat->sp_slot, lat->ss_counts, lat->ss_off,
-                                         lat->ss_local, lat->ss_id, lat->ss_site, lat->ss_sorted,
+                                         lat->ss_local, lat->ss_ent, lat->ss_sorted,
                                          static_cast<float*>(lat->d_force), stream());
     t.done();
     HIP_TRY(e);
@@ -2093,10 +2090,8 @@ struct lbswim_ensemble {
   size_t cap_ss_off = 0;
   uint32_t* ss_local = nullptr;
   size_t cap_ss_local = 0;
-  uint32_t* ss_eid = nullptr;
-  size_t cap_ss_eid = 0;
-  uint32_t* ss_site = nullptr;
-  size_t cap_ss_site = 0;
+  uint32_t* ss_ent = nullptr;  // key and site of each entry
+  size_t cap_ss_ent = 0;
   int ss_nent = 0;  // 64 points of those members
   double* ss_sorted = nullptr;
   size_t cap_ss_sorted = 0;
@@ -2479,8 +2474,7 @@ int ens_build(lbswim_ensemble* e) {
     if ((rc = grow_to(&e->ss_counts, &e->cap_ss_counts, ns, 4)) ||
         (rc = grow_to(&e->ss_off, &e->cap_ss_off, spread_off_entries(uint32_t(ns)), 4)) ||
         (rc = grow_to(&e->ss_local, &e->cap_ss_local, 64 * size_t(e->npts), 4)) ||
-        (rc = grow_to(&e->ss_eid, &e->cap_ss_eid, 64 * site_points, 4)) ||
-        (rc = grow_to(&e->ss_site, &e->cap_ss_site, 64 * site_points, 4)) ||
+        (rc = grow_to(&e->ss_ent, &e->cap_ss_ent, 2 * 64 * site_points, 4)) ||
         (rc = grow_to(&e->ss_sorted, &e->cap_ss_sorted, 3 * 64 * site_points, 8)))
       return rc;
   }
@@ -2589,7 +2583,7 @@ int ens_enqueue_step(lbswim_ensemble* e, int parity) {
   auto stored_chain = [&]() -> int {
     EnsTimer t(e, LBSWIM_K_STORED, sb);
     HIP_TRY(launch_batch_site_lists(e->d_lat, e->npts, e->nsites, e->ss_nent, e->pts, e->slot_ids, e->ss_counts,
-                                    e->ss_off, e->ss_local, e->ss_eid, e->ss_site, e->ss_sorted, sb));
+                                    e->ss_off, e->ss_local, e->ss_ent, e->ss_sorted, sb));
     // the site members' force is summed and stored by the collide; a pass
     // over the other stored members' blocks writes theirs
     HIP_TRY(launch_batch_spread_force<T>(e->d_lat, e->n_spreadwork, e->d_spreadwork, e->per, e->pts, e->off,
@@ -2794,8 +2788,7 @@ void lbswim_ensemble_destroy(lbswim_ensemble* e) {
   dev_free(e->ss_counts);
   dev_free(e->ss_off);
   dev_free(e->ss_local);
-  dev_free(e->ss_eid);
-  dev_free(e->ss_site);
+  dev_free(e->ss_ent);
   dev_free(e->ss_sorted);
   dev_free(e->pts);
   dev_free(e->slot_ids);

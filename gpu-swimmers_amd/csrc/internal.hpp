@@ -150,13 +150,14 @@ hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint
 // Writes the force of every site from per-site segments of the points' 64
 // stencil entries, for lattices with every side >= 4 (site_force_ok): the
 // sum the sorted lists give.  counts[g.n] (zero on entry, left zero),
-// off[spread_off_entries(g.n)], cursor[g.n]; scratch local, eid,
-// esite[64 npts] (local 16-byte aligned) and sorted[3 * 64 npts].
+// off[spread_off_entries(g.n)]; scratch local[64 npts] (16-byte aligned),
+// entries[2 * 64 npts] (key and site of each entry, 8-byte aligned) and
+// sorted[3 * 64 npts].
 inline bool site_force_ok(const Geom& g) { return g.nx >= 4 && g.ny >= 4 && g.nz >= 4; }
 template <typename T>
 hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, const uint32_t* slot_ids,
-                             uint32_t* counts, uint32_t* off, uint32_t* local, uint32_t* eid, uint32_t* esite,
-                             double* sorted, T* force, hipStream_t s);
+                             uint32_t* counts, uint32_t* off, uint32_t* local, uint32_t* entries, double* sorted,
+                             T* force, hipStream_t s);
 
 // Force at the first and last site of every list's 64-site run:
 // edge[6 l + 3 e + a] (e = 0 first, 1 last), 6 spread_lists(g) doubles; and,
@@ -282,11 +283,11 @@ hipError_t launch_batch_lists(const LatDesc* ldesc, int npts, int nlists, const 
 // launch_site_force), nsites = their sites (site_base numbering), npts all
 // the batch's points, nent = 64 points of those members: counts[nsites]
 // (zero on entry, left zero), off[spread_off_entries(nsites)],
-// cursor[nsites], local[64 npts] (16-byte aligned), eid, esite[nent],
+// local[64 npts] (16-byte aligned), entries[2 nent] (8-byte aligned),
 // sorted[3 nent].
 hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, int nent, const SpreadPoint* pts,
                                    const uint32_t* slot_ids, uint32_t* counts, uint32_t* off, uint32_t* local,
-                                   uint32_t* eid, uint32_t* esite, double* sorted, hipStream_t s);
+                                   uint32_t* entries, double* sorted, hipStream_t s);
 // Writes the force arrays of the lattice blocks in block_work: from the lists
 // (FORCE_LISTS_STORED), from all the member's points (FORCE_RANGE_STORED) or
 // from its site segments soff/ssorted (FORCE_SITES_STORED; none where soff is

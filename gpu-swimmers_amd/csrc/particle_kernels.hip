@@ -1371,14 +1371,13 @@ __global__ __launch_bounds__(PBLOCK) void k_site_count(uint32_t npts, Geom g, co
 
 // One thread per (point, stencil row): the entry key -- point id * 64 +
 // stencil index 16 i + 4 j + k (ids below 2^26) -- and the site into the
-// row's 4 slots.
+// row's 4 slots, one 8-byte store per entry.
 template <bool Batched>
 __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, const LatDesc* __restrict__ ldesc,
                                                      const SpreadPoint* __restrict__ pts,
                                                      const uint32_t* __restrict__ slot_ids,
                                                      const uint32_t* __restrict__ off,
-                                                     const uint4* __restrict__ local, uint32_t* __restrict__ eid,
-                                                     uint32_t* __restrict__ esite) {
+                                                     const uint4* __restrict__ local, uint2* __restrict__ ent) {
   const uint32_t t = blockIdx.x * PBLOCK + threadIdx.x;
   if (t >= 16 * npts) return;
   const uint32_t id = slot_ids[t >> 4];
@@ -1392,8 +1391,7 @@ __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, con
   for (int k = 0; k < 4; ++k) {
     const uint32_t site = row + uint32_t(p.wc[8 + k]);
     const uint32_t e = off[site] + slot[k];
-    eid[e] = (id << 6) | (uint32_t(t & 15) << 2) | uint32_t(k);
-    esite[e] = site;
+    ent[e] = make_uint2((id << 6) | (uint32_t(t & 15) << 2) | uint32_t(k), site);
   }
 }
 
@@ -1404,16 +1402,16 @@ __global__ __launch_bounds__(PBLOCK) void k_site_fill(uint32_t npts, Geom g, con
 // nent = 64 points of the members concerned.
 __global__ __launch_bounds__(PBLOCK) void k_site_place(uint32_t nent, const SpreadPoint* __restrict__ pts,
                                                       const uint32_t* __restrict__ off,
-                                                      const uint32_t* __restrict__ eid,
-                                                      const uint32_t* __restrict__ esite, double* __restrict__ sorted) {
+                                                      const uint2* __restrict__ ent, double* __restrict__ sorted) {
   const uint32_t e = blockIdx.x * PBLOCK + threadIdx.x;
   if (e >= nent) return;
-  const uint32_t key = eid[e], site = esite[e];
+  const uint2 me = ent[e];
+  const uint32_t key = me.x, site = me.y;
   const SpreadPoint& p = pts[key >> 6];
   const int di = int(key >> 4) & 3, dj = int(key >> 2) & 3, dk = int(key) & 3;
   const uint32_t o = off[site], end = off[site + 1];
   uint32_t rank = 0;
-  for (uint32_t q = o; q < end; ++q) rank += eid[q] < key ? 1u : 0u;
+  for (uint32_t q = o; q < end; ++q) rank += ent[q].x < key ? 1u : 0u;
   const double d3 = p.del[di] * p.del[4 + dj] * p.del[8 + dk];
   double* __restrict__ dst = sorted + 3 * size_t(o + rank);
 #pragma unroll
@@ -1785,8 +1783,8 @@ template hipError_t launch_range_force<float>(const Geom&, const SpreadPoint*, u
 
 template <typename T>
 hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, const uint32_t* slot_ids,
-                             uint32_t* counts, uint32_t* off, uint32_t* local, uint32_t* eid, uint32_t* esite,
-                             double* sorted, T* force, hipStream_t s) {
+                             uint32_t* counts, uint32_t* off, uint32_t* local, uint32_t* entries, double* sorted,
+                             T* force, hipStream_t s) {
   if (g.n == 0) return hipSuccess;
   const uint32_t nt = g.n;
   const unsigned rows = blocks_for(16 * size_t(npts));
@@ -1801,10 +1799,11 @@ hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, co
   hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, int(nsb), sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, nullptr);
   if (npts > 0) {
+    uint2* ent = reinterpret_cast<uint2*>(entries);
     hipLaunchKernelGGL(k_site_fill<false>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), g, nullptr, pts, slot_ids,
-                       off, loc, eid, esite);
+                       off, loc, ent);
     const uint32_t nent = 64u * uint32_t(npts);
-    hipLaunchKernelGGL(k_site_place, dim3(blocks_for(nent)), dim3(PBLOCK), 0, s, nent, pts, off, eid, esite, sorted);
+    hipLaunchKernelGGL(k_site_place, dim3(blocks_for(nent)), dim3(PBLOCK), 0, s, nent, pts, off, ent, sorted);
   }
   hipLaunchKernelGGL(k_site_sum<T>, dim3(blocks_for(nt)), dim3(PBLOCK), 0, s, nt, off, sorted, force);
   return hipGetLastError();
@@ -1812,7 +1811,7 @@ hipError_t launch_site_force(int npts, const Geom& g, const SpreadPoint* pts, co
 
 hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, int nent, const SpreadPoint* pts,
                                    const uint32_t* slot_ids, uint32_t* counts, uint32_t* off, uint32_t* local,
-                                   uint32_t* eid, uint32_t* esite, double* sorted, hipStream_t s) {
+                                   uint32_t* entries, double* sorted, hipStream_t s) {
   if (nsites <= 0) return hipSuccess;
   const uint32_t nt = uint32_t(nsites);
   const unsigned rows = blocks_for(16 * size_t(npts));
@@ -1827,19 +1826,19 @@ hipError_t launch_batch_site_lists(const LatDesc* ldesc, int npts, int nsites, i
   hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, int(nsb), sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_BLOCK), 0, s, int(nt), counts, sums, off, nullptr);
   if (npts > 0) {
+    uint2* ent = reinterpret_cast<uint2*>(entries);
     hipLaunchKernelGGL(k_site_fill<true>, dim3(rows), dim3(PBLOCK), 0, s, uint32_t(npts), none, ldesc, pts, slot_ids,
-                       off, loc, eid, esite);
+                       off, loc, ent);
     if (nent > 0)
       hipLaunchKernelGGL(k_site_place, dim3(blocks_for(size_t(nent))), dim3(PBLOCK), 0, s, uint32_t(nent), pts, off,
-                         eid, esite, sorted);
+                         ent, sorted);
   }
   return hipGetLastError();
 }
 template hipError_t launch_site_force<double>(int, const Geom&, const SpreadPoint*, const uint32_t*, uint32_t*,
-                                              uint32_t*, uint32_t*, uint32_t*, uint32_t*, double*, double*,
-                                              hipStream_t);
+                                              uint32_t*, uint32_t*, uint32_t*, double*, double*, hipStream_t);
 template hipError_t launch_site_force<float>(int, const Geom&, const SpreadPoint*, const uint32_t*, uint32_t*,
-                                             uint32_t*, uint32_t*, uint32_t*, uint32_t*, double*, float*, hipStream_t);
+                                             uint32_t*, uint32_t*, uint32_t*, double*, float*, hipStream_t);
 
 template <typename T>
 hipError_t launch_spread_force(const Geom& g, const SpreadPoint* pts, const uint32_t* off, const uint32_t* sorted,
